@@ -1,0 +1,220 @@
+/*
+ * td3.h - the C ABI of libtd3_hip.so, the gfx950 (MI355X) backend of the
+ * field-evaluation path of tree-d-3d (direct sums, flat quadtree, Barnes-Hut walk, FMM).
+ *
+ * Conventions (shared by every entry point):
+ *  - Array arguments are DEVICE pointers (HIP global memory), e.g. torch tensors'
+ *    data_ptr(), in the reference's layouts: points (2, N) component-major float64,
+ *    per-node arrays node-major, indices int64 unless stated.
+ *  - `stream` is a hipStream_t passed as void*; every kernel is enqueued on it.  Entry
+ *    points that must size outputs read a few counters back and synchronise the stream
+ *    (stated per entry).
+ *  - The caller allocates every output and workspace; the library never allocates or
+ *    frees device memory.  `*_workspace_bytes` queries give workspace sizes.
+ *  - Return codes: TD3_OK (0); TD3_ERR_CAPACITY (-2, a caller buffer is too small: the
+ *    needed sizes come back through the entry's out-parameter); TD3_ERR_ARG (-3);
+ *    TD3_ERR_HIP (-4).  td3_last_error() describes the last failure of this thread.
+ *  - Kernel failures are reported in a device `status` int64[1] (zero on entry), as the
+ *    reference's kernels do: > 0 is 1 + the smallest index of a target that met an
+ *    unsoftened coincident charge (the reference raises ValueError), < 0 a traversal
+ *    stack or depth overflow (RuntimeError).
+ *  - Reference interfaces cited as src/tree_d_3d/<file>:<line>.
+ */
+#ifndef TD3_H
+#define TD3_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TD3_ABI_VERSION 1
+
+#define TD3_OK 0
+#define TD3_ERR_CAPACITY (-2)
+#define TD3_ERR_ARG (-3)
+#define TD3_ERR_HIP (-4)
+
+/* kernel */
+#define TD3_COULOMB 0
+#define TD3_LOG 1
+
+/* precision: all-FP64, or FP32 arithmetic with FP64 accumulation and FP64 tree data */
+#define TD3_FP64 0
+#define TD3_MIXED 1
+
+int td3_abi_version(void);
+const char *td3_last_error(void);
+
+/*
+ * Direct (all-pairs) potential and field at M targets from N sources, without k_e.
+ * Replaces the kernels direct_sum / direct_log_sum
+ * (kernels/direct_sum/direct_sum_cuda.cu:7-44, kernels/direct_log_sum/direct_log_sum_cuda.cu:7-42,
+ * called by flat_tree.py:153-158): coulomb phi = sum q/sqrt(r^2+eps^2), E = sum q d/(r^2+eps^2)^1.5;
+ * log phi = -sum q ln(r^2+eps^2), E = 2 sum q d/(r^2+eps^2).  Source exclude[i] and zero
+ * charges are skipped.  targets (2, M), exclude (M,), pos (2, N), charge (N,),
+ * phi (M,), field (2, M) written (not accumulated).  workspace: td3_direct_workspace_bytes.
+ */
+size_t td3_direct_workspace_bytes(int64_t m, int64_t n);
+int td3_direct(int kernel, int precision, const double *targets, int64_t m,
+               const int64_t *exclude, const double *pos, int64_t n, const double *charge,
+               double softening, double *phi, double *field, int64_t *status, void *workspace,
+               size_t workspace_bytes, void *stream);
+
+/*
+ * Flat quadtree build, bit-identical to the reference's (the arrays of
+ * kernels/quadtree_build/quadtree_build_pyccel.py:12-137 and of the per-level GPU build
+ * flat_tree.py:532-625 via kernels/quadtree_quadrants, kernels/quadtree_scatter).
+ * pos (2, N) in input order; root (xmid, ymid, half).  Node arrays hold `capacity`
+ * entries: node_center (C, 2), node_half, node_start, node_end, node_child, node_depth;
+ * levels int64[67] (levels[d]:levels[d+1] are the nodes of depth d); perm (N,) int64;
+ * sorted_pos (2, N) = pos[:, perm] (may be NULL); node_info (C,) int4 {start, end, child,
+ * depth} for the other kernels (may be NULL).  out_sizes (HOST int64[3]) gets
+ * {node_count, n_levels, needed capacity}.  Returns TD3_ERR_CAPACITY when capacity <
+ * node_count (call again with out_sizes[2]).  Synchronises the stream once (one
+ * read-back of the split counts).  Supports N < 2^31 and trees of depth <= 64.
+ */
+size_t td3_tree_workspace_bytes(int64_t n, int64_t capacity);
+int td3_tree_build(const double *pos, int64_t n, double xmid, double ymid, double half,
+                   int64_t leaf_capacity, int64_t max_depth, int64_t capacity, double *node_center,
+                   double *node_half, int64_t *node_start, int64_t *node_end, int64_t *node_child,
+                   int64_t *node_depth, int64_t *levels, int64_t *perm, double *sorted_pos,
+                   void *node_info, void *workspace, size_t workspace_bytes, int64_t *out_sizes,
+                   void *stream);
+
+/* dst[r, k] = src[r * ld_src + perm[k]] for r < rows (<= 8), k < n: tree-order copies
+ * (flat_tree.py:879-889 _sorted_sources). */
+int td3_permute(const double *src, int64_t rows, int64_t ld_src, const int64_t *perm, int64_t n,
+                double *dst, void *stream);
+
+/*
+ * Cell moments from the sources in tree order (sorted_pos (2, N), sorted_charge (N,)).
+ * Replaces kernels/quadtree_moments, kernels/log_quadtree_moments (flat_tree.py:663-744)
+ * with the upward pass of kernels/quadtree_moments_upward/..._pyccel.py:13-83 and
+ * kernels/log_moments_upward/..._pyccel.py:14-93.  coulomb: node_q, node_qabs (C,),
+ * node_p (C, 2), node_s (C, 3) raw (xx, xy, yy) about the cell centre.  log: node_qabs,
+ * moment_re/moment_im (C, order+1) with a_n = sum q (z - c)^n (node_q/node_p/node_s
+ * untouched).  levels_host: HOST copy of levels (n_levels + 1 entries); binomial (bsz,
+ * bsz) device table C(n, k) with bsz >= order + 1.
+ */
+int td3_moments(int kernel, int64_t order, const double *sorted_pos, const double *sorted_charge,
+                int64_t n, const double *node_center, const void *node_info, int64_t node_count,
+                const int64_t *levels_host, int64_t n_levels, const double *binomial, int64_t bsz,
+                double *node_q, double *node_qabs, double *node_p, double *node_s,
+                double *moment_re, double *moment_im, void *stream);
+
+/*
+ * Locality order of M targets (2, M): a stable sort by the reference's 16-bit-per-axis
+ * Morton key ((t - lo) * scale, x in the high bit of each pair; flat_tree.py:891-912).
+ * order (M,) int64 receives the target indices.
+ */
+size_t td3_target_order_workspace_bytes(int64_t m);
+int td3_target_order(const double *targets, int64_t m, double lox, double loy, double scale,
+                     int64_t *order, void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * Barnes-Hut walk at M targets, without k_e.  Replaces kernels/evaluate_quadtree
+ * (evaluate_quadtree_cuda.cu:9-107) and kernels/evaluate_log_quadtree
+ * (evaluate_log_quadtree_cuda.cu:9-104) behind flat_tree.py:988-1061, with the per-target
+ * semantics of kernels/evaluate_groups/evaluate_groups_pyccel.py:16-297: cells with
+ * node_qabs == 0 skipped; a cell not containing the target is accepted when
+ * 2h < theta |r - c| (coulomb, quadrupole term with rho = sqrt(D^2 + eps^2)) or
+ * 4h^2 < theta^2 |r - c|^2 (log, order-p series, unsoftened); leaves sum their particles
+ * directly (softened), skipping exclude[i] and zero charges.  order (M,) is the target
+ * order of the wavefronts (td3_target_order); sorted_pos / sorted_charge / perm are the
+ * sources in tree order.  phi (M,), field (2, M) written in target order.
+ */
+int td3_walk(int kernel, int precision, const double *targets, int64_t m, const int64_t *order,
+             const int64_t *exclude, const double *sorted_pos, const double *sorted_charge,
+             const int64_t *perm, int64_t n, const double *node_center, const double *node_half,
+             const void *node_info, const double *node_qabs, const double *node_q,
+             const double *node_p, const double *node_s, const double *moment_re,
+             const double *moment_im, int64_t order_p, int64_t tree_depth, double theta,
+             double softening, int with_potential, double *phi, double *field, int64_t *status,
+             void *stream);
+
+/*
+ * Dual-tree traversal of one tree with itself (the FMM's interaction lists).  Replaces
+ * flat_tree.py:1318-1358 and kernels/fmm_traverse (fmm_traverse_cuda.cu:14-97) with the
+ * rules of kernels/fmm_traverse/fmm_traverse_pyccel.py:15-106.  Writes the mutual M2L
+ * pairs m2l (cap_m2l, 2) int32 and the P2P leaf pairs p2p (cap_p2p, 2) int32, in no
+ * particular order (the pair SETS equal the reference's).  out_counts (HOST int64[4]) gets
+ * {n_m2l, n_p2p, largest frontier, overflow}; TD3_ERR_CAPACITY when a buffer was too small
+ * (retry with out_counts[0..2] or larger).  Synchronises the stream once.
+ */
+size_t td3_traverse_workspace_bytes(int64_t cap_front);
+int td3_traverse(const double *node_center, const double *node_half, const void *node_info,
+                 int64_t node_count, int64_t tree_depth, double theta, int64_t cap_m2l,
+                 int64_t cap_p2p, int64_t cap_front, int32_t *m2l, int32_t *p2p, void *workspace,
+                 size_t workspace_bytes, int64_t *out_counts, void *stream);
+
+/*
+ * Rows by node of a pair list (kernels/pair_csr/pair_csr_pyccel.py:12-45, mutual): each
+ * pair (a, b) lists b under a and a under b (a self pair once when self_once).  row_start
+ * (node_count + 1) uint32, col (2 count) int32 sorted ascending within each row, dst (may be
+ * NULL) the row of each entry.
+ */
+size_t td3_lists_workspace_bytes(int64_t node_count);
+int td3_lists(const int32_t *pairs, int64_t count, int self_once, int64_t node_count,
+              uint32_t *row_start, int32_t *col, int32_t *dst, void *workspace,
+              size_t workspace_bytes, void *stream);
+
+/*
+ * Radius-scaled moments A_k = a_k / r^k, r = sqrt(2) node_half (FP64 -> double or float
+ * by precision), for the M2L.
+ */
+int td3_scale_moments(int precision, const double *moment_re, const double *moment_im,
+                      const double *node_half, int64_t node_count, int64_t order, void *out_re,
+                      void *out_im, void *stream);
+
+/*
+ * M2L into the scaled locals B_l = b_l r^l of the destination rows [node_lo, node_hi).
+ * Replaces kernels/fmm_m2l (fmm_m2l_cuda.cu:12-80) / kernels/fmm_m2l_host
+ * (fmm_m2l_host_pyccel.py:16-105): b_0 += a_0 ln t - s_0, b_l += (-1)^(l+1) s_l w^l with
+ * s_l = sum_k coeff[k, l] a_k w^k, w = 1/t, t = c_dst - c_src (ln t principal).  col/dst/
+ * row_start are td3_lists rows of the M2L pairs; coeff (order+1)^2 device table
+ * (flat_tree.py:234-245, either truncation); scaled_re/im from td3_scale_moments of the
+ * same precision.  Rows of the range are WRITTEN (not accumulated); rows outside it are
+ * untouched.  Reads two row offsets back (synchronises the stream).
+ */
+size_t td3_m2l_workspace_bytes(int64_t pair_count, int64_t order, int precision);
+int td3_m2l(int precision, int64_t order, const int32_t *col, const int32_t *dst,
+            const uint32_t *row_start, int64_t node_lo, int64_t node_hi,
+            const double *node_center, const double *node_half, const void *scaled_re,
+            const void *scaled_im, const double *coeff, double *local_re, double *local_im,
+            void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * L2L down the tree on scaled locals, level by level from the root (flat_tree.py:1540-1552,
+ * kernels/fmm_downward/fmm_downward_cuda.cu:10-51).  levels_host: HOST levels array.
+ */
+int td3_l2l(int64_t order, const double *node_center, const double *node_half,
+            const void *node_info, const int64_t *levels_host, int64_t n_levels,
+            const double *binomial, int64_t bsz, double *local_re, double *local_im,
+            void *stream);
+
+/* b_l = B_l / r^l: the reference's unscaled locals, in place. */
+int td3_unscale_locals(const double *node_half, int64_t node_count, int64_t order,
+                       double *local_re, double *local_im, void *stream);
+
+/*
+ * L2P + P2P at every source (flat_tree.py:1580-1600, kernels/fmm_evaluate
+ * fmm_evaluate_cuda.cu:11-87; kernels/fmm_evaluate_host/fmm_evaluate_host_pyccel.py:14-100):
+ * one wavefront per leaf, owner-computes over the leaf's P2P rows (td3_lists of the P2P
+ * pairs with self_once).  Own charge excluded; softening on the near pairs only.  phi (N,),
+ * field (2, N) written in input order (phi zero when with_potential is 0).
+ */
+int td3_fmm_eval(int precision, int64_t order, const double *sorted_pos,
+                 const double *sorted_charge, const int64_t *perm, int64_t n,
+                 const double *node_center, const double *node_half, const void *node_info,
+                 int64_t node_count, const double *local_re, const double *local_im,
+                 const uint32_t *p2p_row, const int32_t *p2p_col, double softening,
+                 int with_potential, double *phi, double *field, int64_t *status, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TD3_H */

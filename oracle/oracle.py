@@ -1,0 +1,414 @@
+"""CPU oracle (test infrastructure only): the reference's algorithm restated in C.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, as the checker or as the timed CPU baseline. The
+product path (``tree-d-3d_amd/``) never imports it.
+
+The kernels live in ``td3_oracle.c`` (built by ``oracle/Makefile`` into
+``libtd3_oracle.so``); this module holds the orchestration of the reference's
+``flat_tree.py`` that calls them:
+
+* `direct_field`            flat_tree.py:125-160
+* `OracleTree.__init__`     flat_tree.py:746-809 (checks), `_build_host` :483-530
+* `OracleTree._moments`     flat_tree.py:663-744 (the CPU upward pass)
+* `OracleTree.evaluate`     flat_tree.py:988-1061 with `_evaluate_groups` :914-963 and
+                            `_target_order` :891-912
+* `OracleTree.traverse`     flat_tree.py:1318-1358
+* `OracleTree.fmm`          flat_tree.py:1461-1634 (log kernel, one process)
+
+Pinning: ``tests/test_oracle_*.py`` checks every function here against closed forms,
+brute force on tiny inputs and the invariants that fix the tree uniquely.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libtd3_oracle.so")
+_LIB = None
+
+MAX_ORDER = 64
+GROUP_SIZE = 32
+_KERNELS = ("coulomb", "log")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int64
+_D = ctypes.c_double
+_C = ctypes.c_int
+
+
+def build() -> str:
+    """Compile ``libtd3_oracle.so`` with ``make`` (gcc -O3 -fopenmp)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _LIB = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "oracle_direct": [_C, _P, _I, _I, _P, _P, _I, _I, _P, _D, _P, _P, _I, _P],
+            "oracle_quadtree_build": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P],
+            "oracle_quadtree_moments_upward": [_P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
+            "oracle_binomial_table": [_I, _P],
+            "oracle_m2l_coefficients": [_I, _C, _P],
+            "oracle_log_moments_upward": [_P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P],
+            "oracle_evaluate_groups": [_P, _I, _P, _P, _I, _C, _C, _P, _I, _P, _P, _P, _P, _P, _P,
+                                       _P, _P, _P, _P, _P, _P, _P, _I, _D, _D, _I, _P, _P, _I],
+            "oracle_fmm_traverse": [_P, _P, _P, _P, _P, _D, _P, _I, _P, _I, _P, _I, _P, _I, _P],
+            "oracle_pair_csr": [_P, _I, _C, _I, _P, _P],
+            "oracle_fmm_m2l_host": [_P, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P],
+            "oracle_fmm_downward": [_I, _I, _P, _P, _P, _I, _I, _P, _P],
+            "oracle_fmm_evaluate_host": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _I, _D, _C,
+                                         _P, _P, _I, _P],
+        }
+        for name, args in sig.items():
+            fn = getattr(_LIB, name)
+            fn.argtypes = args
+            fn.restype = None
+    return _LIB
+
+
+def _ptr(a: np.ndarray) -> int:
+    assert a.flags.c_contiguous, "oracle arrays must be C-contiguous"
+    return a.ctypes.data
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _i64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+def _coincident() -> ValueError:
+    return ValueError("Coincident charges require positive softening")
+
+
+def _check(status) -> None:
+    """_utils.py:26-35."""
+    code = int(status[0])
+    if code > 0:
+        raise _coincident()
+    if code < 0:
+        raise RuntimeError("the quadtree is too deep for the traversal stack")
+
+
+def _points(array, name: str) -> np.ndarray:
+    """_utils.py:51-63: a contiguous (2, N) float64 array."""
+    points = _f64(array)
+    if points.ndim != 2 or points.shape[0] != 2:
+        raise ValueError(f"{name} must have shape (2, N) (component-major), got {points.shape}")
+    return points.copy(order="C")
+
+
+def binomial_table(size: int) -> np.ndarray:
+    out = np.zeros((size, size))
+    lib().oracle_binomial_table(size, _ptr(out))
+    return out
+
+
+def m2l_coefficients(order: int, symmetric: bool) -> np.ndarray:
+    out = np.zeros((order + 1, order + 1))
+    lib().oracle_m2l_coefficients(order, 1 if symmetric else 0, _ptr(out))
+    return out
+
+
+def direct_field(targets, pos, charge, *, softening=0.0, exclude=None, kernel="coulomb"):
+    """flat_tree.py:125-160 on the CPU oracle."""
+    if kernel not in _KERNELS:
+        raise ValueError(f"kernel must be one of {_KERNELS}")
+    targets = _points(targets, "targets")
+    pos = _points(pos, "pos")
+    charge = _f64(charge).reshape(-1)
+    m, n = targets.shape[1], pos.shape[1]
+    exclude = np.full(m, -1, dtype=np.int64) if exclude is None else _i64(exclude).reshape(-1)
+    if not m or not n:
+        return np.zeros(m), np.zeros((2, m))
+    phi, field = np.zeros(m), np.zeros((2, m))
+    status = np.zeros(1, dtype=np.int64)
+    lib().oracle_direct(1 if kernel == "log" else 0, _ptr(targets), m, m, _ptr(exclude),
+                        _ptr(pos), n, n, _ptr(charge), float(softening), _ptr(phi), _ptr(field),
+                        m, _ptr(status))
+    _check(status)
+    return phi, field
+
+
+class OracleTree:
+    """The reference's FlatQuadtree on the CPU oracle (one process, FP64)."""
+
+    def __init__(self, positions, charges, root, *, leaf_capacity=8, max_depth=48,
+                 kernel="coulomb", order=None, symmetric=False):
+        if kernel not in _KERNELS:
+            raise ValueError(f"kernel must be one of {_KERNELS}")
+        if order is None:
+            order = 12 if kernel == "log" else 4
+        if int(order) != order or order < 1 or order > MAX_ORDER:
+            raise ValueError("bad order")
+        self.kernel, self.order, self.symmetric = kernel, int(order), bool(symmetric)
+        self.leaf_capacity, self.max_depth = int(leaf_capacity), int(max_depth)
+        xmid, ymid, half = (float(v) for v in root)
+        self.pos = _points(positions, "positions")
+        self.charge = _f64(charges).reshape(-1).copy()
+        n = self.pos.shape[1]
+        if len(self.charge) != n:
+            raise ValueError("positions and charges must have the same length")
+        if not np.all(np.isfinite(self.pos)):
+            raise ValueError("Particle position must be finite")
+        if np.any(self.pos[0] < xmid - half) or np.any(self.pos[0] > xmid + half) or \
+           np.any(self.pos[1] < ymid - half) or np.any(self.pos[1] > ymid + half):
+            raise ValueError("Particle lies outside tree bounds")
+        self._traversals = {}
+        if not n:
+            self.perm = np.empty(0, dtype=np.int64)
+            self.node_count = 1
+            self.node_center = np.array([[xmid, ymid]])
+            self.node_half = np.array([half])
+            self.node_start = np.zeros(1, dtype=np.int64)
+            self.node_end = np.zeros(1, dtype=np.int64)
+            self.node_child = np.full(1, -1, dtype=np.int64)
+            self.node_depth = np.zeros(1, dtype=np.int64)
+            self.levels = np.array([0, 1])
+            self.node_q = np.zeros(1)
+            self.node_qabs = np.zeros(1)
+            self.node_p = np.zeros((1, 2))
+            self.node_s = np.zeros((1, 3))
+            self.moment_re = np.zeros((1, self.order + 1))
+            self.moment_im = np.zeros((1, self.order + 1))
+            return
+        self._build(xmid, ymid, half, 4 * (2 * n // self.leaf_capacity + 16))
+        self._moments()
+
+    # flat_tree.py:483-530
+    def _build(self, xmid, ymid, half, capacity):
+        n = self.pos.shape[1]
+        sizes = np.zeros(3, dtype=np.int64)
+        self.build_calls = 0
+        while True:
+            self.build_calls += 1
+            sorted_pos = np.zeros((2, n + 1))
+            sorted_pos[:, :n] = self.pos
+            center = np.empty((capacity, 2))
+            half_arr = np.empty(capacity)
+            start = np.empty(capacity, dtype=np.int64)
+            end = np.empty(capacity, dtype=np.int64)
+            child = np.empty(capacity, dtype=np.int64)
+            depth = np.empty(capacity, dtype=np.int64)
+            center[0] = (xmid, ymid)
+            half_arr[0] = half
+            perm = np.empty(n, dtype=np.int64)
+            levels = np.empty(capacity + 1, dtype=np.int64)
+            lib().oracle_quadtree_build(_ptr(sorted_pos), n + 1, n, self.leaf_capacity,
+                                        self.max_depth, _ptr(center), _ptr(half_arr), _ptr(start),
+                                        _ptr(end), _ptr(child), _ptr(depth), _ptr(perm),
+                                        _ptr(levels), capacity, _ptr(sizes))
+            if not sizes[2]:
+                break
+            capacity *= 2
+        count = int(sizes[0])
+        self.node_count = count
+        self.node_center = center[:count].copy()
+        self.node_half = half_arr[:count].copy()
+        self.node_start = start[:count].copy()
+        self.node_end = end[:count].copy()
+        self.node_child = child[:count].copy()
+        self.node_depth = depth[:count].copy()
+        self.perm = perm
+        self.levels = levels[: sizes[1] + 1].copy()
+        self._sorted_pos = sorted_pos
+
+    # flat_tree.py:663-744 (CPU, one process: the upward pass)
+    def _moments(self):
+        c = self.node_count
+        lev = _i64(self.levels)
+        nl = len(lev) - 1
+        n = self.pos.shape[1]
+        self.node_q = np.empty(c)
+        self.node_qabs = np.empty(c)
+        self.node_p = np.empty((c, 2))
+        self.node_s = np.empty((c, 3))
+        if self.kernel == "log":
+            p1 = self.order + 1
+            self.moment_re = np.empty((c, p1))
+            self.moment_im = np.empty((c, p1))
+            bsz = 2 * self.order + 1
+            binom = binomial_table(bsz)
+            lib().oracle_log_moments_upward(
+                _ptr(self.pos), n, _ptr(self.charge), _ptr(self.perm), _ptr(lev), nl,
+                _ptr(self.node_center), _ptr(self.node_start), _ptr(self.node_end),
+                _ptr(self.node_child), _ptr(binom), bsz, self.order, _ptr(self.node_qabs),
+                _ptr(self.moment_re), _ptr(self.moment_im))
+            self.node_q[:] = self.moment_re[:, 0]
+            self.node_p[:, 0] = self.moment_re[:, 1]
+            self.node_p[:, 1] = self.moment_im[:, 1]
+            return
+        lib().oracle_quadtree_moments_upward(
+            _ptr(self.pos), n, _ptr(self.charge), _ptr(self.perm), _ptr(lev), nl,
+            _ptr(self.node_center), _ptr(self.node_start), _ptr(self.node_end),
+            _ptr(self.node_child), _ptr(self.node_q), _ptr(self.node_qabs), _ptr(self.node_p),
+            _ptr(self.node_s))
+        self.moment_re = np.zeros((1, 1))
+        self.moment_im = np.zeros((1, 1))
+
+    def set_charges(self, charges):
+        """flat_tree.py:853-868."""
+        charges = _f64(charges).reshape(-1)
+        if charges.shape[0] != self.pos.shape[1]:
+            raise ValueError("charges must have one entry per particle")
+        self.charge[...] = charges
+        if self.pos.shape[1]:
+            self._moments()
+
+    def _sorted_sources(self):
+        """flat_tree.py:879-889."""
+        n = self.pos.shape[1]
+        spos = np.zeros((2, n + 1))
+        spos[:, :n] = self.pos[:, self.perm]
+        return spos, np.ascontiguousarray(self.charge[self.perm])
+
+    def _target_order(self, targets):
+        """flat_tree.py:891-912: 16-bit-per-axis Morton order (locality only)."""
+        if targets.shape == self.pos.shape and np.array_equal(targets, self.pos):
+            return self.perm
+        lo = targets.min(axis=1, keepdims=True)
+        span = float(np.max(targets.max(axis=1, keepdims=True) - lo))
+        cells = ((targets - lo) * ((65535.0 / span) if span > 0 else 0.0)).astype(np.uint64)
+        key = np.zeros(targets.shape[1], dtype=np.uint64)
+        for axis, shift in ((0, 1), (1, 0)):
+            v = cells[axis]
+            v = (v | (v << np.uint64(8))) & np.uint64(0x00FF00FF)
+            v = (v | (v << np.uint64(4))) & np.uint64(0x0F0F0F0F)
+            v = (v | (v << np.uint64(2))) & np.uint64(0x33333333)
+            v = (v | (v << np.uint64(1))) & np.uint64(0x55555555)
+            key |= v << np.uint64(shift)
+        return np.argsort(key, kind="stable")
+
+    def evaluate(self, targets, *, exclude=None, theta=0.5, softening=0.0, potential=True):
+        """flat_tree.py:988-1061 with `_evaluate_groups` :914-963."""
+        targets = _points(targets, "targets")
+        m = targets.shape[1]
+        exclude = np.full(m, -1, dtype=np.int64) if exclude is None else _i64(exclude).reshape(-1)
+        phi, field = np.zeros(m), np.zeros((2, m))
+        n = self.pos.shape[1]
+        if not (m and n):
+            return phi, field
+        order = self._target_order(targets)
+        spos, scharge = self._sorted_sources()
+        log_kernel = self.kernel == "log"
+        st = np.empty((2, m + 1))
+        st[:, :m] = targets[:, order]
+        phi_s = np.empty(m)
+        field_s = np.empty((2, m + 1))
+        groups = _i64(np.append(np.arange(0, m, GROUP_SIZE), m))
+        excl = _i64(exclude[order])
+        stack_size = 3 * int(self.node_depth.max()) + 8
+        mre = self.moment_re if log_kernel else np.zeros((1, 1))
+        mim = self.moment_im if log_kernel else np.zeros((1, 1))
+        lib().oracle_evaluate_groups(
+            _ptr(st), m + 1, _ptr(excl), _ptr(groups), len(groups) - 1, 1 if log_kernel else 0,
+            1 if potential else 0, _ptr(spos), n + 1, _ptr(scharge), _ptr(self.perm),
+            _ptr(self.node_center), _ptr(self.node_half), _ptr(self.node_start),
+            _ptr(self.node_end), _ptr(self.node_child), _ptr(self.node_q), _ptr(self.node_qabs),
+            _ptr(self.node_p), _ptr(self.node_s), _ptr(_f64(mre)), _ptr(_f64(mim)),
+            self.order if log_kernel else 0, float(theta), float(softening), stack_size,
+            _ptr(phi_s), _ptr(field_s), m + 1)
+        phi[order] = phi_s
+        field[:, order] = field_s[:, :m]
+        bad = np.flatnonzero(phi == 1.0e300)
+        if bad.size:
+            raise _coincident()
+        if not potential:
+            phi[...] = 0.0
+        return phi, field
+
+    def traverse(self, theta):
+        """flat_tree.py:1318-1358: generations of fmm_traverse from (0, 0)."""
+        if theta in self._traversals:
+            return self._traversals[theta]
+        c = self.node_count
+        capacity = np.array([16, 16, 10]) * c + 64
+        self.traverse_restarts = 0
+        while True:
+            m2l = np.empty((capacity[0], 2), dtype=np.int64)
+            p2p = np.empty((capacity[1], 2), dtype=np.int64)
+            frontier = np.zeros((capacity[2], 2), dtype=np.int64)
+            nxt = np.empty((capacity[2], 2), dtype=np.int64)
+            counts = np.zeros(4, dtype=np.int64)
+            n_frontier = 1
+            while n_frontier:
+                counts[2] = 0
+                lib().oracle_fmm_traverse(
+                    _ptr(self.node_center), _ptr(self.node_half), _ptr(self.node_start),
+                    _ptr(self.node_end), _ptr(self.node_child), float(theta), _ptr(frontier),
+                    n_frontier, _ptr(nxt), capacity[2], _ptr(m2l), capacity[0], _ptr(p2p),
+                    capacity[1], _ptr(counts))
+                if counts[3]:
+                    break
+                n_frontier = int(counts[2])
+                frontier, nxt = nxt, frontier
+            if not counts[3]:
+                out = (m2l[: counts[0]].copy(), p2p[: counts[1]].copy())
+                self._traversals[theta] = out
+                return out
+            self.traverse_restarts += 1
+            capacity = np.maximum(2 * capacity, counts[:3])
+
+    def fmm(self, *, theta=0.5, softening=0.0, potential=True):
+        """flat_tree.py:1461-1634 for the log kernel, one process (CPU branch)."""
+        if not 0 < theta < 1:
+            raise ValueError("FMM theta must lie in (0, 1)")
+        if self.kernel != "log":
+            raise NotImplementedError("the oracle's FMM covers the log kernel")
+        n = self.pos.shape[1]
+        phi, field = np.zeros(n), np.zeros((2, n))
+        if not n:
+            return phi, field
+        m2l, p2p = self.traverse(theta)
+        c, order = self.node_count, self.order
+        p1 = order + 1
+        local_re = np.zeros((c, p1))
+        local_im = np.zeros((c, p1))
+        if len(m2l):
+            rows = np.zeros(c + 1, dtype=np.int64)
+            partners = np.empty(max(2 * len(m2l), 1), dtype=np.int64)
+            lib().oracle_pair_csr(_ptr(_i64(m2l)), len(m2l), 1, c, _ptr(rows), _ptr(partners))
+            coeff = m2l_coefficients(order, self.symmetric)
+            lib().oracle_fmm_m2l_host(_ptr(rows), _ptr(partners), c, _ptr(self.node_center),
+                                      _ptr(self.node_center), _ptr(self.moment_re),
+                                      _ptr(self.moment_im), _ptr(coeff), order, _ptr(local_re),
+                                      _ptr(local_im))
+        bsz = 2 * order + 1
+        binom = binomial_table(bsz)
+        for lo, hi in zip(self.levels[:-2], self.levels[1:-1]):
+            lib().oracle_fmm_downward(int(lo), int(hi), _ptr(self.node_center),
+                                      _ptr(self.node_child), _ptr(binom), bsz, order,
+                                      _ptr(local_re), _ptr(local_im))
+        self.local_re, self.local_im = local_re, local_im
+        spos, scharge = self._sorted_sources()
+        phi_s = np.zeros(n)
+        field_s = np.zeros((2, n + 1))
+        status = np.zeros(1, dtype=np.int64)
+        pairs = _i64(p2p) if len(p2p) else np.zeros((1, 2), dtype=np.int64)
+        lib().oracle_fmm_evaluate_host(
+            _ptr(spos), n + 1, _ptr(scharge), c, _ptr(self.node_center), _ptr(self.node_start),
+            _ptr(self.node_end), _ptr(self.node_child), _ptr(local_re), _ptr(local_im), order,
+            _ptr(pairs), len(p2p), float(softening), 1 if potential else 0, _ptr(phi_s),
+            _ptr(field_s), n + 1, _ptr(status))
+        if int(status[0]):
+            status[0] = self.perm[int(status[0]) - 1] + 1
+        _check(status)
+        phi[self.perm] = phi_s
+        field[:, self.perm] = field_s[:, :n]
+        self.fmm_pairs = (len(m2l), len(p2p))
+        if not potential:
+            phi[:] = 0.0
+        return phi, field

@@ -1,0 +1,148 @@
+"""GPU log-kernel FMM against the oracle (lists, locals, fields) and direct sums."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def relative(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def cloud(n=3000, seed=0):
+    """test_fmm.py:17-27."""
+    rng = np.random.default_rng(seed)
+    pos = np.concatenate(
+        (rng.normal([[-0.3], [0.1]], 0.1, (2, n // 2)),
+         rng.normal([[0.4], [-0.2]], 0.25, (2, n - n // 2))), axis=1).clip(-0.99, 0.99)
+    return pos, rng.normal(0.3, 1, n)
+
+
+def config2(n, seed=7):
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    centres = rng.uniform(-0.8, 0.8, (2, 16))
+    per = (n - half) // 16
+    clusters = (centres[:, :, None] + rng.normal(0, 0.02, (2, 16, per))).reshape(2, -1)
+    pos = np.concatenate((rng.uniform(-1, 1, (2, half)), clusters), axis=1).clip(-1, 1)
+    return pos, rng.choice([-1.0, 1.0], pos.shape[1])
+
+
+def directed(rows, cols):
+    row = np.asarray(rows.cpu()).astype(np.int64)
+    col = np.asarray(cols.cpu()).astype(np.int64)[: row[-1]]
+    dst = np.repeat(np.arange(len(row) - 1), np.diff(row))
+    return np.stack((dst, col), axis=1)
+
+
+def as_directed(pairs, self_once):
+    pairs = np.asarray(pairs)
+    back = pairs[:, ::-1]
+    if self_once:
+        back = back[pairs[:, 0] != pairs[:, 1]]
+    both = np.concatenate((pairs, back))
+    return both[np.lexsort((both[:, 1], both[:, 0]))]
+
+
+@pytest.mark.parametrize("leaf", [1, 8, 64])
+def test_lists_equal_reference_traversal(td3, oracle, leaf):
+    pos, q = cloud(3000)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", leaf_capacity=leaf)
+    ref = oracle.OracleTree(pos, q, (0, 0, 1), kernel="log", leaf_capacity=leaf)
+    (mrow, mcol, _), (prow, pcol, _), counts = tree._lists(0.5)
+    m2l, p2p = ref.traverse(0.5)
+    assert counts == (len(m2l), len(p2p))
+    np.testing.assert_array_equal(directed(mrow, mcol), as_directed(m2l, False))
+    np.testing.assert_array_equal(directed(prow, pcol), as_directed(p2p, True))
+
+
+@pytest.mark.parametrize("order", [2, 6, 12, 18])
+def test_fmm_fp64_matches_oracle(td3, oracle, order):
+    pos, q = cloud(3000)
+    print(f"fmm order={order}", flush=True)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=order)
+    ref = oracle.OracleTree(pos, q, (0, 0, 1), kernel="log", order=order)
+    phi, field = tree.fmm(theta=0.5)
+    rphi, rfield = ref.fmm(theta=0.5)
+    assert tree.fmm_pairs == ref.fmm_pairs
+    assert relative(phi, rphi) < 1e-12
+    assert relative(field, rfield) < 1e-12
+    lre, lim = tree.local_expansions(0.5)
+    # compare the radius-scaled locals b_l r^l (r the cell's half-diagonal): all O(1)
+    r = np.sqrt(2.0) * ref.node_half[:, None] ** np.arange(order + 1)[None, :]
+    r = (np.sqrt(2.0) * ref.node_half[:, None]) ** np.arange(order + 1)[None, :]
+    for got, want in ((lre, ref.local_re), (lim, ref.local_im)):
+        scaled_want = want * r
+        assert np.max(np.abs(got * r - scaled_want)) < 1e-11 * np.max(np.abs(scaled_want))
+
+
+@pytest.mark.parametrize("order,tolerance", [(6, 1e-3), (10, 1e-5), (18, 1e-8)])
+def test_fmm_converges_with_expansion_order(td3, oracle, order, tolerance):
+    # test_fmm.py:38-48
+    pos, q = cloud()
+    ref_phi, ref_field = oracle.direct_field(pos, pos, q, exclude=np.arange(3000), kernel="log")
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=order)
+    phi, field = tree.fmm(theta=0.5)
+    assert relative(phi, ref_phi) < tolerance
+    assert relative(field, ref_field) < 10 * tolerance
+
+
+def test_fmm_mixed_precision_error(td3, oracle):
+    pos, q = config2(200000)
+    print("fmm mixed n=200000", flush=True)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=12)
+    p64, f64 = tree.fmm(theta=0.5, softening=1e-4, precision="fp64")
+    pmx, fmx = tree.fmm(theta=0.5, softening=1e-4, precision="mixed")
+    assert relative(pmx, p64) < 1e-6
+    assert relative(fmx, f64) < 1e-6
+    idx = np.random.default_rng(0).choice(pos.shape[1], 512, replace=False)
+    dphi, dfield = oracle.direct_field(pos[:, idx], pos, q, exclude=idx, softening=1e-4,
+                                       kernel="log")
+    assert relative(f64[:, idx], dfield) < 1e-2  # far cells are unsoftened (test_fmm.py:66-83)
+
+
+def test_fmm_edges(td3):
+    pos, q = cloud(400, seed=2)
+    q[::7] = 0.0
+    pos[:, 2] = pos[:, 1]
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log")
+    tree.fmm(softening=0.01)
+    with pytest.raises(ValueError, match="Coincident"):
+        tree.fmm()
+    with pytest.raises(ValueError, match="theta"):
+        tree.fmm(theta=1.0)
+    nophi, field = tree.fmm(softening=0.01, potential=False)
+    assert not np.any(nophi)
+    one = td3.FlatQuadtree([[0.1], [0.2]], [1.0], td3.Quad(0, 0, 1), kernel="log")
+    phi, field = one.fmm()
+    assert phi[0] == 0.0 and not np.any(field)
+
+
+def test_fmm_net_force_is_at_the_truncation_level(td3):
+    # test_fmm.py:85-91
+    pos, q = cloud(6000, seed=3)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=18)
+    _, field = tree.fmm(theta=0.5)
+    force = q * field
+    assert np.abs(force.sum(axis=1)).max() < 1e-11 * np.abs(force).sum()
+
+
+def test_fmm_set_charges_keeps_lists(td3):
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(-1, 1, (2, 3000))
+    first, second = rng.normal(size=3000), rng.normal(size=3000)
+    tree = td3.FlatQuadtree(pos, first, td3.Quad(0, 0, 1), kernel="log", leaf_capacity=16)
+    tree.fmm()
+    tree.set_charges(second)
+    fresh = td3.FlatQuadtree(pos, second, td3.Quad(0, 0, 1), kernel="log", leaf_capacity=16)
+    np.testing.assert_allclose(tree.fmm()[1], fresh.fmm()[1], rtol=0, atol=1e-10)
+
+
+def test_fmm_is_deterministic(td3):
+    pos, q = config2(50000, seed=3)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log")
+    a = tree.fmm(softening=1e-4)
+    b = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log").fmm(softening=1e-4)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])

@@ -1,0 +1,78 @@
+// Shared device helpers of the gfx950 field-evaluation kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "td3.h"
+
+#define TD3_WAVE 64
+
+typedef int64_t i64;
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+// Error state of the library (host side, one string per thread).
+void td3_set_error(const char* fmt, ...);
+
+#define TD3_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (call);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            td3_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(_e)); \
+            return TD3_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+#define TD3_LAUNCHED()                                                                  \
+    do {                                                                                \
+        hipError_t _e = hipGetLastError();                                              \
+        if (_e != hipSuccess) {                                                         \
+            td3_set_error("%s:%d launch failed: %s", __FILE__, __LINE__, hipGetErrorString(_e)); \
+            return TD3_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+#define TD3_ARG(cond, msg)                                                              \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            td3_set_error("bad argument: %s", msg);                                     \
+            return TD3_ERR_ARG;                                                         \
+        }                                                                               \
+    } while (0)
+
+static inline unsigned td3_blocks(i64 items, int threads) {
+    i64 b = (items + threads - 1) / threads;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+// status[0] keeps the smallest positive report (1 + an index), so the result does not
+// depend on which wave reports first; a negative report (stack or depth overflow)
+// wins over every positive one.
+__device__ __forceinline__ void td3_report(i64* status, i64 value) {
+    unsigned long long* s = reinterpret_cast<unsigned long long*>(status);
+    unsigned long long old = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const i64 o = (i64)old;
+        if (o < 0) return;
+        if (value > 0 && o > 0 && o <= value) return;
+        unsigned long long assumed = old;
+        old = atomicCAS(s, assumed, (unsigned long long)value);
+        if (old == assumed) return;
+    }
+}
+
+// Sum of a double over the 64 lanes of a wave (every lane gets the total).
+__device__ __forceinline__ double td3_wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, TD3_WAVE);
+    return v;
+}
+
+__device__ __forceinline__ float td3_wave_sumf(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, TD3_WAVE);
+    return v;
+}
+
+__device__ __forceinline__ int td3_lane() { return (int)(threadIdx.x & (TD3_WAVE - 1)); }

@@ -1,0 +1,521 @@
+// FMM translations and evaluation for the log (line-charge) kernel.
+//
+// Replaces kernels/fmm_m2l/fmm_m2l_cuda.cu:12-80 (one thread per pair, 2 x 130 local
+// doubles, 2(p+1) FP64 atomics per direction), kernels/fmm_downward/fmm_downward_cuda.cu:10-51
+// and kernels/fmm_evaluate/fmm_evaluate_cuda.cu:11-87 (6 FP64 atomics per near pair).
+// Everything here is owner-computes and deterministic: no float atomics.
+//
+// Expansions are stored radius-scaled, with r = sqrt(2) h the half-diagonal of a cell:
+//   moments  A_k = a_k / r^k,   locals  B_l = b_l r^l,
+// so every coefficient of every operator is bounded (|u| + |v| < theta below).  This is
+// what lets the deep levels translate in FP32; the FP64 path uses the same code in FP64.
+//
+// M2L (the reference's factorisation, kernels/fmm_m2l_host/fmm_m2l_host_pyccel.py:16-105):
+// for a directed pair, t = c_dst - c_src, u = r_src / t, v = r_dst / t,
+//   S_l = sum_k coeff[k, l] A_k u^k,   B_0 += A_0 ln t - S_0,   B_l += (-1)^(l+1) v^l S_l,
+// coeff the constant real matrix of flat_tree.py:234-245.  One lane per directed pair over
+// the destination-sorted pair list (CSR); lanes of one destination are summed by a
+// segmented wave reduction, and destinations whose rows cross a wave boundary are
+// finished by a fixed-order pass over per-wave partials.
+
+#include "td3_common.h"
+
+namespace {
+
+template <typename T>
+struct Cx {
+    T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+__device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
+__device__ __forceinline__ float shfl_up_d(float v, int off) { return __shfl_up(v, off, 64); }
+
+// ------------------------------------------------------------------------------ M2L
+template <int P, typename T>
+__global__ __launch_bounds__(256) void m2l_kernel(
+    const int* __restrict__ col, const int* __restrict__ dst, const u32* __restrict__ row_start,
+    u32 e_lo, u32 e_hi, const double* __restrict__ center, const double* __restrict__ half,
+    const T* __restrict__ mre, const T* __restrict__ mim, int p1, const double* __restrict__ coeff,
+    double* __restrict__ lre, double* __restrict__ lim, T* __restrict__ side) {
+    __shared__ T s_coeff[(P + 1) * (P + 1)];
+    for (int i = threadIdx.x; i < (P + 1) * (P + 1); i += blockDim.x) {
+        const int k = i / (P + 1), l = i % (P + 1);
+        s_coeff[i] = (k < p1 && l < p1) ? (T)coeff[k * p1 + l] : (T)0;
+    }
+    __syncthreads();
+    const int lane = td3_lane();
+    const u32 wave = (u32)((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+    const u32 w0 = e_lo + wave * 64;
+    if (w0 >= e_hi) return;
+    const u32 e = w0 + lane;
+    const bool valid = e < e_hi;
+    const int d = valid ? dst[e] : -1 - lane;
+    Cx<T> b[P + 1];
+#pragma unroll
+    for (int l = 0; l <= P; ++l) b[l] = {(T)0, (T)0};
+    if (valid) {
+        const int src = col[e];
+        const double tr = center[2 * d] - center[2 * src];
+        const double ti = center[2 * d + 1] - center[2 * src + 1];
+        const double t2 = tr * tr + ti * ti;
+        const double wr = tr / t2, wi = -ti / t2;
+        const double rs = 1.4142135623730951 * half[src];
+        const double rt = 1.4142135623730951 * half[d];
+        const Cx<T> u = {(T)(rs * wr), (T)(rs * wi)};
+        const Cx<T> v = {(T)(rt * wr), (T)(rt * wi)};
+        // A_k u^k
+        Cx<T> a[P + 1];
+        Cx<T> pw = {(T)1, (T)0};
+#pragma unroll
+        for (int k = 0; k <= P; ++k) {
+            if (k < p1) {
+                const Cx<T> ak = {mre[(i64)src * p1 + k], mim[(i64)src * p1 + k]};
+                a[k] = cmul(ak, pw);
+            } else {
+                a[k] = {(T)0, (T)0};
+            }
+            pw = cmul(pw, u);
+        }
+        // S_l = sum_k coeff[k, l] a_k
+#pragma unroll
+        for (int k = 0; k <= P; ++k) {
+#pragma unroll
+            for (int l = 0; l <= P; ++l) {
+                const T c = s_coeff[k * (P + 1) + l];
+                b[l].re += c * a[k].re;
+                b[l].im += c * a[k].im;
+            }
+        }
+        // B_0 = A_0 ln t - S_0 ; B_l = (-1)^(l+1) v^l S_l
+        T lnr, lni;
+        if (sizeof(T) == 8) {
+            lnr = (T)(0.5 * log(t2));
+            lni = (T)atan2(ti, tr);
+        } else {
+            lnr = (T)(0.5f * __logf((float)t2));
+            lni = (T)atan2f((float)ti, (float)tr);
+        }
+        b[0].re = a[0].re * lnr - a[0].im * lni - b[0].re;
+        b[0].im = a[0].re * lni + a[0].im * lnr - b[0].im;
+        Cx<T> vp = v;
+        T sign = (T)1;
+#pragma unroll
+        for (int l = 1; l <= P; ++l) {
+            const Cx<T> r = cmul(b[l], vp);
+            b[l] = {sign * r.re, sign * r.im};
+            sign = -sign;
+            vp = cmul(vp, v);
+        }
+    }
+    // segmented inclusive scan over lanes of equal destination
+    const int prev_d = __shfl_up(d, 1, 64);
+    const bool head = lane == 0 || prev_d != d;
+    const u64 heads = __ballot(head);
+    const u64 le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int seg_start = 63 - __clzll(heads & le);
+#pragma unroll
+    for (int l = 0; l <= P; ++l) {
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const T r = shfl_up_d(b[l].re, off);
+            const T i = shfl_up_d(b[l].im, off);
+            if (lane - off >= seg_start) {
+                b[l].re += r;
+                b[l].im += i;
+            }
+        }
+    }
+    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+    if (!valid || !tail) return;
+    const u32 rs0 = row_start[d], rs1 = row_start[d + 1];
+    const bool before = rs0 < w0;
+    const bool after = rs1 > w0 + 64;
+    if (!before && !after) {
+        for (int l = 0; l < p1; ++l) {
+            lre[(i64)d * p1 + l] = (double)b[l].re;
+            lim[(i64)d * p1 + l] = (double)b[l].im;
+        }
+    } else {
+        const i64 slot = ((i64)((w0 - e_lo) / 64) * 2 + (before ? 0 : 1)) * 2 * p1;
+        for (int l = 0; l < p1; ++l) {
+            side[slot + 2 * l] = b[l].re;
+            side[slot + 2 * l + 1] = b[l].im;
+        }
+    }
+}
+
+// Destinations whose row crosses a wave boundary: last-segment partial of the first wave
+// plus the first-segment partials of the following waves, in order.
+template <typename T>
+__global__ void m2l_fixup_kernel(const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
+                                 u32 e_lo, int p1, const T* __restrict__ side,
+                                 double* __restrict__ lre, double* __restrict__ lim) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    const i64 node = node_lo + t / (2 * p1);
+    const int j = (int)(t % (2 * p1));
+    if (node >= node_hi) return;
+    const u32 rs0 = row_start[node], rs1 = row_start[node + 1];
+    if (rs1 == rs0) return;
+    const u32 wa = (rs0 - e_lo) / 64, wb = (rs1 - 1 - e_lo) / 64;
+    if (wa == wb) return;
+    double v = (double)side[((i64)wa * 2 + 1) * 2 * p1 + j];
+    for (u32 w = wa + 1; w <= wb; ++w) v += (double)side[((i64)w * 2) * 2 * p1 + j];
+    const int l = j / 2;
+    if (j & 1)
+        lim[node * p1 + l] = v;
+    else
+        lre[node * p1 + l] = v;
+}
+
+// ------------------------------------------------------------------------------ scaling
+// A_k = a_k / r^k (r = sqrt(2) h), as T.
+template <typename T>
+__global__ void scale_moments_kernel(const double* __restrict__ mre, const double* __restrict__ mim,
+                                     const double* __restrict__ half, i64 count, int p1,
+                                     T* __restrict__ ore, T* __restrict__ oim) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * p1) return;
+    const i64 node = t / p1;
+    const int k = (int)(t % p1);
+    const double inv = 1.0 / (1.4142135623730951 * half[node]);
+    double s = 1.0;
+    for (int i = 0; i < k; ++i) s *= inv;
+    ore[t] = (T)(mre[t] * s);
+    oim[t] = (T)(mim[t] * s);
+}
+
+// b_l = B_l / r^l (the reference's unscaled locals), in place.
+__global__ void unscale_locals_kernel(double* __restrict__ lre, double* __restrict__ lim,
+                                      const double* __restrict__ half, i64 count, int p1) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * p1) return;
+    const i64 node = t / p1;
+    const int l = (int)(t % p1);
+    const double inv = 1.0 / (1.4142135623730951 * half[node]);
+    double s = 1.0;
+    for (int i = 0; i < l; ++i) s *= inv;
+    lre[t] *= s;
+    lim[t] *= s;
+}
+
+// ------------------------------------------------------------------------------ L2L
+// kernels/fmm_downward/fmm_downward_pyccel.py:15-64 in scaled form: with s = c_child - c_parent,
+// B^child_m += sum_{l >= m} C(l, m) B_l (r_c / r_p)^m (s / r_p)^(l - m).  One thread per
+// (child, m) of the children of the nodes [lo, hi).
+__global__ void l2l_kernel(const double* __restrict__ center, const double* __restrict__ half,
+                           const int4* __restrict__ info, i64 lo, i64 hi, int p1,
+                           const double* __restrict__ binomial, int bsz, double* __restrict__ lre,
+                           double* __restrict__ lim) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    const i64 parent = lo + t / (4 * p1);
+    if (parent >= hi) return;
+    const int4 pf = info[parent];
+    if (pf.z < 0) return;
+    const int rem = (int)(t % (4 * p1));
+    const i64 child = pf.z + rem / p1;
+    const int m = rem % p1;
+    const double rp = 1.4142135623730951 * half[parent];
+    const double rc = 1.4142135623730951 * half[child];
+    const double sr = (center[2 * child] - center[2 * parent]) / rp;
+    const double si = (center[2 * child + 1] - center[2 * parent + 1]) / rp;
+    double acc_r = 0.0, acc_i = 0.0, pr = 1.0, pi = 0.0;
+    for (int l = m; l < p1; ++l) {
+        const double c = binomial[l * bsz + m];
+        const double br = lre[parent * p1 + l] * c, bi = lim[parent * p1 + l] * c;
+        acc_r += br * pr - bi * pi;
+        acc_i += br * pi + bi * pr;
+        const double tr = pr * sr - pi * si;
+        pi = pr * si + pi * sr;
+        pr = tr;
+    }
+    double f = 1.0;
+    const double ratio = rc / rp;
+    for (int i = 0; i < m; ++i) f *= ratio;
+    lre[child * p1 + m] += acc_r * f;
+    lim[child * p1 + m] += acc_i * f;
+}
+
+// ------------------------------------------------------------------------------ L2P + P2P
+// One wavefront per target leaf.  kernels/fmm_evaluate_host/fmm_evaluate_host_pyccel.py:14-100:
+// L2P W = sum_l b_l u^l, phi += -2 Re W, E_x - i E_y += 2 W'; near field
+// phi -= q ln(r^2 + eps^2), E += 2 q d / (r^2 + eps^2) over the P2P partner leaves, each
+// particle's own charge excluded; a charged coincident pair sets status.  Lanes run over
+// the partners' particles (64 at a time), each accumulating every target of the leaf
+// (kChunk at a time); a wave reduction finishes each target.
+constexpr int kChunk = 8;
+
+template <int PREC>
+__global__ __launch_bounds__(256) void eval_kernel(
+    const double* __restrict__ spos, const double* __restrict__ scharge, const i64* __restrict__ perm,
+    i64 n, const double* __restrict__ center, const double* __restrict__ half,
+    const int4* __restrict__ info, i64 node_count, const double* __restrict__ lre,
+    const double* __restrict__ lim, int p1, const u32* __restrict__ prow,
+    const int* __restrict__ pcol, double eps2, int with_potential, double* __restrict__ phi,
+    double* __restrict__ field, i64* __restrict__ status) {
+    const int lane = td3_lane();
+    const i64 leaf = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    if (leaf >= node_count) return;
+    const int4 lf = info[leaf];
+    if (lf.z >= 0 || lf.y == lf.x) return;
+    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
+    const double cx = center[2 * leaf], cy = center[2 * leaf + 1];
+    const double inv_r = 1.0 / (1.4142135623730951 * half[leaf]);
+    for (int t0 = lf.x; t0 < lf.y; t0 += kChunk) {
+        const int nt = (lf.y - t0) < kChunk ? (lf.y - t0) : kChunk;
+        double acc_p[kChunk], acc_x[kChunk], acc_y[kChunk];
+        double tx[kChunk], ty[kChunk], tq[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            acc_p[j] = acc_x[j] = acc_y[j] = 0.0;
+            const int k = t0 + (j < nt ? j : 0);
+            tx[j] = spos[k];
+            ty[j] = spos[n + k];
+            tq[j] = scharge[k];
+        }
+        int bad_slot = -1;
+        // source leaves in batches of 64 lanes
+        for (u32 rb = r0; rb < r1; rb += 64) {
+            const u32 rr = rb + lane;
+            int sstart = 0, scount = 0;
+            if (rr < r1) {
+                const int4 sf = info[pcol[rr]];
+                sstart = sf.x;
+                scount = sf.y - sf.x;
+            }
+            // inclusive prefix of counts over lanes
+            int incl = scount;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += v;
+            }
+            const int total = __shfl(incl, 63, 64);
+            for (int base = 0; base < total; base += 64) {
+                const int idx = base + lane;
+                // owning lane: first lane whose inclusive prefix exceeds idx
+                int lo_l = 0, hi_l = 63;
+                while (lo_l < hi_l) {
+                    const int mid = (lo_l + hi_l) >> 1;
+                    const int pv = __shfl(incl, mid, 64);
+                    if (pv > idx) hi_l = mid; else lo_l = mid + 1;
+                }
+                const int owner = lo_l;
+                const int ostart = __shfl(sstart, owner, 64);
+                const int oincl = __shfl(incl, owner, 64);
+                const int ocount = __shfl(scount, owner, 64);
+                if (idx >= total) continue;
+                const int ks = ostart + (idx - (oincl - ocount));
+                const double sx = spos[ks], sy = spos[n + ks], sq = scharge[ks];
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                    if (j >= nt) break;
+                    if (ks == t0 + j) continue;
+                    const double dx = tx[j] - sx, dy = ty[j] - sy;
+                    const double r2 = dx * dx + dy * dy + eps2;
+                    if (r2 == 0.0) {
+                        if (tq[j] != 0.0 && sq != 0.0) bad_slot = t0 + j;
+                        continue;
+                    }
+                    if (PREC == TD3_FP64) {
+                        if (with_potential) acc_p[j] -= sq * log(r2);
+                        const double inv = 2.0 * sq / r2;
+                        acc_x[j] += inv * dx;
+                        acc_y[j] += inv * dy;
+                    } else {
+                        const float r2f = (float)r2;
+                        if (with_potential) acc_p[j] -= (double)((float)sq * __logf(r2f));
+                        const float inv = 2.0f * (float)sq * __frcp_rn(r2f);
+                        acc_x[j] += (double)(inv * (float)dx);
+                        acc_y[j] += (double)(inv * (float)dy);
+                    }
+                }
+            }
+        }
+        if (bad_slot >= 0) td3_report(status, perm[bad_slot] + 1);
+        // reduce and L2P
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            if (j >= nt) break;
+            acc_p[j] = td3_wave_sum(acc_p[j]);
+            acc_x[j] = td3_wave_sum(acc_x[j]);
+            acc_y[j] = td3_wave_sum(acc_y[j]);
+        }
+        if (lane < nt) {
+            const int k = t0 + lane;
+            double p = 0, fx = 0, fy = 0;
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j)
+                if (j == lane) {
+                    p = acc_p[j];
+                    fx = acc_x[j];
+                    fy = acc_y[j];
+                }
+            // Horner on the scaled local expansion at u' = (z - c) / r
+            const double ur = (spos[k] - cx) * inv_r, ui = (spos[n + k] - cy) * inv_r;
+            double w_re = lre[leaf * p1 + p1 - 1], w_im = lim[leaf * p1 + p1 - 1];
+            double d_re = 0.0, d_im = 0.0;
+            for (int l = p1 - 2; l >= 0; --l) {
+                const double nd = d_re * ur - d_im * ui + w_re;
+                d_im = d_re * ui + d_im * ur + w_im;
+                d_re = nd;
+                const double nw = w_re * ur - w_im * ui + lre[leaf * p1 + l];
+                w_im = w_re * ui + w_im * ur + lim[leaf * p1 + l];
+                w_re = nw;
+            }
+            const i64 i = perm[k];
+            phi[i] = with_potential ? p - 2.0 * w_re : 0.0;
+            field[i] = fx + 2.0 * d_re * inv_r;
+            field[n + i] = fy - 2.0 * d_im * inv_r;
+        }
+    }
+}
+
+template <int P, typename T>
+int launch_m2l(hipStream_t s, const int* col, const int* dst, const u32* row_start, u32 e_lo,
+               u32 e_hi, i64 node_lo, i64 node_hi, const double* center, const double* half,
+               const T* mre, const T* mim, int p1, const double* coeff, double* lre, double* lim,
+               T* side) {
+    if (e_hi <= e_lo) return TD3_OK;
+    const i64 waves = ((i64)(e_hi - e_lo) + 63) / 64;
+    hipLaunchKernelGGL((m2l_kernel<P, T>), dim3(td3_blocks(waves * 64, 256)), dim3(256), 0, s, col, dst,
+                       row_start, e_lo, e_hi, center, half, mre, mim, p1, coeff, lre, lim, side);
+    TD3_LAUNCHED();
+    const i64 items = (node_hi - node_lo) * 2 * p1;
+    hipLaunchKernelGGL((m2l_fixup_kernel<T>), dim3(td3_blocks(items, 256)), dim3(256), 0, s, row_start,
+                       node_lo, node_hi, e_lo, p1, side, lre, lim);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+template <typename T>
+int dispatch_m2l(hipStream_t s, int order, const int* col, const int* dst, const u32* row_start,
+                 u32 e_lo, u32 e_hi, i64 node_lo, i64 node_hi, const double* center,
+                 const double* half, const T* mre, const T* mim, const double* coeff, double* lre,
+                 double* lim, T* side) {
+    const int p1 = order + 1;
+#define TD3_M2L_CASE(PP)                                                                          \
+    if (order <= PP)                                                                             \
+        return launch_m2l<PP, T>(s, col, dst, row_start, e_lo, e_hi, node_lo, node_hi, center, half, \
+                                 mre, mim, p1, coeff, lre, lim, side);
+    TD3_M2L_CASE(4)
+    TD3_M2L_CASE(8)
+    TD3_M2L_CASE(10)
+    TD3_M2L_CASE(12)
+    TD3_M2L_CASE(16)
+    TD3_M2L_CASE(20)
+    TD3_M2L_CASE(32)
+    TD3_M2L_CASE(64)
+#undef TD3_M2L_CASE
+    td3_set_error("order %d above 64", order);
+    return TD3_ERR_ARG;
+}
+
+}  // namespace
+
+extern "C" size_t td3_m2l_workspace_bytes(int64_t pair_count, int64_t order, int precision) {
+    const i64 waves = (pair_count + 63) / 64 + 1;
+    const size_t word = precision == TD3_FP64 ? 8 : 4;
+    return (size_t)waves * 2 * 2 * (size_t)(order + 1) * word + 256;
+}
+
+extern "C" int td3_scale_moments(int precision, const double* moment_re, const double* moment_im,
+                                 const double* node_half, int64_t node_count, int64_t order,
+                                 void* out_re, void* out_im, void* stream) {
+    TD3_ARG(node_count >= 1 && order >= 1 && order <= 64, "sizes");
+    const int p1 = (int)order + 1;
+    const i64 items = node_count * p1;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == TD3_FP64)
+        hipLaunchKernelGGL(scale_moments_kernel<double>, dim3(td3_blocks(items, 256)), dim3(256), 0, s,
+                           moment_re, moment_im, node_half, node_count, p1, (double*)out_re,
+                           (double*)out_im);
+    else
+        hipLaunchKernelGGL(scale_moments_kernel<float>, dim3(td3_blocks(items, 256)), dim3(256), 0, s,
+                           moment_re, moment_im, node_half, node_count, p1, (float*)out_re,
+                           (float*)out_im);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const int32_t* dst,
+                       const uint32_t* row_start, int64_t node_lo, int64_t node_hi,
+                       const double* node_center, const double* node_half, const void* scaled_re,
+                       const void* scaled_im, const double* coeff, double* local_re,
+                       double* local_im, void* workspace, size_t workspace_bytes, void* stream) {
+    TD3_ARG(order >= 1 && order <= 64, "order");
+    TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
+    hipStream_t s = (hipStream_t)stream;
+    if (node_hi == node_lo) return TD3_OK;
+    u32 bounds[2];
+    TD3_HIP(hipMemcpyAsync(&bounds[0], row_start + node_lo, sizeof(u32), hipMemcpyDeviceToHost, s));
+    TD3_HIP(hipMemcpyAsync(&bounds[1], row_start + node_hi, sizeof(u32), hipMemcpyDeviceToHost, s));
+    TD3_HIP(hipStreamSynchronize(s));
+    const i64 pairs = (i64)bounds[1] - (i64)bounds[0];
+    if (pairs <= 0) return TD3_OK;
+    TD3_ARG(workspace_bytes >= td3_m2l_workspace_bytes(pairs, order, precision), "workspace too small");
+    if (precision == TD3_FP64)
+        return dispatch_m2l<double>(s, (int)order, col, dst, row_start, bounds[0], bounds[1], node_lo,
+                                    node_hi, node_center, node_half, (const double*)scaled_re,
+                                    (const double*)scaled_im, coeff, local_re, local_im,
+                                    (double*)workspace);
+    return dispatch_m2l<float>(s, (int)order, col, dst, row_start, bounds[0], bounds[1], node_lo,
+                               node_hi, node_center, node_half, (const float*)scaled_re,
+                               (const float*)scaled_im, coeff, local_re, local_im, (float*)workspace);
+}
+
+extern "C" int td3_l2l(int64_t order, const double* node_center, const double* node_half,
+                       const void* node_info, const int64_t* levels_host, int64_t n_levels,
+                       const double* binomial, int64_t bsz, double* local_re, double* local_im,
+                       void* stream) {
+    TD3_ARG(order >= 1 && order <= 64 && bsz >= order + 1, "order");
+    hipStream_t s = (hipStream_t)stream;
+    const int p1 = (int)order + 1;
+    for (i64 d = 0; d + 1 < n_levels; ++d) {
+        const i64 lo = levels_host[d], hi = levels_host[d + 1];
+        hipLaunchKernelGGL(l2l_kernel, dim3(td3_blocks((hi - lo) * 4 * p1, 256)), dim3(256), 0, s,
+                           node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
+                           local_re, local_im);
+        TD3_LAUNCHED();
+    }
+    return TD3_OK;
+}
+
+extern "C" int td3_unscale_locals(const double* node_half, int64_t node_count, int64_t order,
+                                  double* local_re, double* local_im, void* stream) {
+    const int p1 = (int)order + 1;
+    hipLaunchKernelGGL(unscale_locals_kernel, dim3(td3_blocks(node_count * p1, 256)), dim3(256), 0,
+                       (hipStream_t)stream, local_re, local_im, node_half, node_count, p1);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+extern "C" int td3_fmm_eval(int precision, int64_t order, const double* sorted_pos,
+                            const double* sorted_charge, const int64_t* perm, int64_t n,
+                            const double* node_center, const double* node_half,
+                            const void* node_info, int64_t node_count, const double* local_re,
+                            const double* local_im, const uint32_t* p2p_row, const int32_t* p2p_col,
+                            double softening, int with_potential, double* phi, double* field,
+                            int64_t* status, void* stream) {
+    TD3_ARG(order >= 1 && order <= 64, "order");
+    TD3_ARG(n >= 1 && node_count >= 1, "sizes");
+    const int p1 = (int)order + 1;
+    hipStream_t s = (hipStream_t)stream;
+    const double eps2 = softening * softening;
+    const unsigned blocks = td3_blocks(node_count * 64, 256);
+    if (precision == TD3_FP64)
+        hipLaunchKernelGGL(eval_kernel<TD3_FP64>, dim3(blocks), dim3(256), 0, s, sorted_pos, sorted_charge,
+                           perm, n, node_center, node_half, (const int4*)node_info, node_count, local_re,
+                           local_im, p1, p2p_row, p2p_col, eps2, with_potential, phi, field, status);
+    else
+        hipLaunchKernelGGL(eval_kernel<TD3_MIXED>, dim3(blocks), dim3(256), 0, s, sorted_pos,
+                           sorted_charge, perm, n, node_center, node_half, (const int4*)node_info,
+                           node_count, local_re, local_im, p1, p2p_row, p2p_col, eps2, with_potential,
+                           phi, field, status);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}

@@ -1,0 +1,201 @@
+// Cell moments: P2M at the leaves, M2M up the tree, FP64.
+//
+// Replaces the reference's GPU moments (kernels/log_quadtree_moments/log_quadtree_moments_cuda.cu:7-46
+// and kernels/quadtree_moments/quadtree_moments_cuda.cu:7-46: one thread per node summing
+// all of its particles, the root thread looping over all N) with the upward pass of the
+// reference's CPU kernels (kernels/log_moments_upward/log_moments_upward_pyccel.py:14-93,
+// kernels/quadtree_moments_upward/quadtree_moments_upward_pyccel.py:13-83): every leaf is
+// summed from its particles in one launch, then one launch per level shifts the four
+// children into each parent.  Log moments use one thread per (node, coefficient n).
+
+#include "td3_common.h"
+
+namespace {
+
+// a_n = sum_j q_j (z_j - c)^n for every leaf; qabs = sum |q_j|.
+__global__ void log_p2m_kernel(const double* __restrict__ spos, const double* __restrict__ scharge,
+                               i64 n, const double* __restrict__ center,
+                               const int4* __restrict__ info, i64 count, int p1,
+                               double* __restrict__ qabs, double* __restrict__ mre,
+                               double* __restrict__ mim) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * p1) return;
+    const i64 node = t / p1;
+    const int k = (int)(t - node * p1);
+    const int4 nf = info[node];
+    if (nf.z >= 0) return;
+    const double cx = center[2 * node], cy = center[2 * node + 1];
+    double are = 0.0, aim = 0.0, qa = 0.0;
+    for (int j = nf.x; j < nf.y; ++j) {
+        const double q = scharge[j];
+        const double dx = spos[j] - cx, dy = spos[n + j] - cy;
+        // q (dx + i dy)^k by binary powering
+        double pr = q, pi = 0.0, br = dx, bi = dy;
+        int e = k;
+        while (e) {
+            if (e & 1) {
+                const double tr = pr * br - pi * bi;
+                pi = pr * bi + pi * br;
+                pr = tr;
+            }
+            e >>= 1;
+            if (e) {
+                const double tr = br * br - bi * bi;
+                bi = 2.0 * br * bi;
+                br = tr;
+            }
+        }
+        are += pr;
+        aim += pi;
+        qa += fabs(q);
+    }
+    mre[node * p1 + k] = are;
+    mim[node * p1 + k] = aim;
+    if (k == 0) qabs[node] = qa;
+}
+
+// a_n += sum_m C(n, m) a_m(child) s^(n-m), s = c_child - c, for the nodes [lo, hi).
+__global__ void log_m2m_kernel(const double* __restrict__ center, const int4* __restrict__ info,
+                               i64 lo, i64 hi, int p1, const double* __restrict__ binomial,
+                               int bsz, double* __restrict__ qabs, double* __restrict__ mre,
+                               double* __restrict__ mim) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (hi - lo) * p1) return;
+    const i64 node = lo + t / p1;
+    const int k = (int)(t % p1);
+    const int4 nf = info[node];
+    if (nf.z < 0) return;
+    const double cx = center[2 * node], cy = center[2 * node + 1];
+    double are = 0.0, aim = 0.0, qa = 0.0;
+    for (int c = nf.z; c < nf.z + 4; ++c) {
+        qa += qabs[c];
+        const int4 cf = info[c];
+        if (cf.y == cf.x) continue;
+        const double sr = center[2 * c] - cx, si = center[2 * c + 1] - cy;
+        // sum over m = k .. 0 with s^(k-m) built upward
+        double pr = 1.0, pi = 0.0, acc_r = 0.0, acc_i = 0.0;
+        for (int m = k; m >= 0; --m) {
+            const double b = binomial[k * bsz + m];
+            const double ar = mre[(i64)c * p1 + m] * b, ai = mim[(i64)c * p1 + m] * b;
+            acc_r += ar * pr - ai * pi;
+            acc_i += ar * pi + ai * pr;
+            const double tr = pr * sr - pi * si;
+            pi = pr * si + pi * sr;
+            pr = tr;
+        }
+        are += acc_r;
+        aim += acc_i;
+    }
+    mre[node * p1 + k] = are;
+    mim[node * p1 + k] = aim;
+    if (k == 0) qabs[node] = qa;
+}
+
+// Coulomb moments (q, |q|, p, raw S) of the nodes [lo, hi): leaves from their particles
+// (leaves_only) or parents from their children.
+__global__ void coulomb_kernel(const double* __restrict__ spos, const double* __restrict__ scharge,
+                               i64 n, const double* __restrict__ center,
+                               const int4* __restrict__ info, i64 lo, i64 hi, int leaves_only,
+                               double* __restrict__ nq, double* __restrict__ nqabs,
+                               double* __restrict__ np, double* __restrict__ ns) {
+    const i64 node = lo + (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= hi) return;
+    const int4 nf = info[node];
+    const bool leaf = nf.z < 0;
+    if (leaf != (leaves_only != 0)) return;
+    const double xmid = center[2 * node], ymid = center[2 * node + 1];
+    double q = 0, qa = 0, px = 0, py = 0, sxx = 0, sxy = 0, syy = 0;
+    if (leaf) {
+        for (int j = nf.x; j < nf.y; ++j) {
+            const double c = scharge[j];
+            const double dx = spos[j] - xmid, dy = spos[n + j] - ymid;
+            q += c;
+            qa += fabs(c);
+            px += c * dx;
+            py += c * dy;
+            sxx += c * dx * dx;
+            sxy += c * dx * dy;
+            syy += c * dy * dy;
+        }
+    } else {
+        for (int ch = nf.z; ch < nf.z + 4; ++ch) {
+            const double dx = center[2 * ch] - xmid, dy = center[2 * ch + 1] - ymid;
+            const double cq = nq[ch], cpx = np[2 * ch], cpy = np[2 * ch + 1];
+            q += cq;
+            qa += nqabs[ch];
+            px += cpx + cq * dx;
+            py += cpy + cq * dy;
+            sxx += ns[3 * ch] + 2.0 * cpx * dx + cq * dx * dx;
+            sxy += ns[3 * ch + 1] + cpx * dy + cpy * dx + cq * dx * dy;
+            syy += ns[3 * ch + 2] + 2.0 * cpy * dy + cq * dy * dy;
+        }
+    }
+    nq[node] = q;
+    nqabs[node] = qa;
+    np[2 * node] = px;
+    np[2 * node + 1] = py;
+    ns[3 * node] = sxx;
+    ns[3 * node + 1] = sxy;
+    ns[3 * node + 2] = syy;
+}
+
+__global__ void permute_kernel(const double* __restrict__ src, const i64* __restrict__ perm, i64 n,
+                               int rows, i64 ld_src, double* __restrict__ dst) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const i64 j = perm[k];
+    for (int r = 0; r < rows; ++r) dst[r * n + k] = src[r * ld_src + j];
+}
+
+}  // namespace
+
+extern "C" int td3_permute(const double* src, int64_t rows, int64_t ld_src, const int64_t* perm,
+                           int64_t n, double* dst, void* stream) {
+    TD3_ARG(n >= 0 && rows >= 1 && rows <= 8, "sizes");
+    if (n == 0) return TD3_OK;
+    hipLaunchKernelGGL(permute_kernel, dim3(td3_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       src, perm, n, (int)rows, ld_src, dst);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+extern "C" int td3_moments(int kernel, int64_t order, const double* sorted_pos,
+                           const double* sorted_charge, int64_t n, const double* node_center,
+                           const void* node_info, int64_t node_count, const int64_t* levels_host,
+                           int64_t n_levels, const double* binomial, int64_t bsz, double* node_q,
+                           double* node_qabs, double* node_p, double* node_s, double* moment_re,
+                           double* moment_im, void* stream) {
+    TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
+    TD3_ARG(n >= 1 && node_count >= 1 && n_levels >= 1, "sizes");
+    TD3_ARG(levels_host != nullptr && levels_host[n_levels] == node_count, "levels");
+    hipStream_t s = (hipStream_t)stream;
+    const int4* info = (const int4*)node_info;
+    if (kernel == TD3_LOG) {
+        TD3_ARG(order >= 1 && order <= 64 && bsz >= order + 1, "order");
+        const int p1 = (int)order + 1;
+        hipLaunchKernelGGL(log_p2m_kernel, dim3(td3_blocks(node_count * p1, 256)), dim3(256), 0, s,
+                           sorted_pos, sorted_charge, n, node_center, info, node_count, p1, node_qabs,
+                           moment_re, moment_im);
+        TD3_LAUNCHED();
+        for (i64 d = n_levels - 2; d >= 0; --d) {
+            const i64 lo = levels_host[d], hi = levels_host[d + 1];
+            hipLaunchKernelGGL(log_m2m_kernel, dim3(td3_blocks((hi - lo) * p1, 256)), dim3(256), 0, s,
+                               node_center, info, lo, hi, p1, binomial, (int)bsz, node_qabs, moment_re,
+                               moment_im);
+            TD3_LAUNCHED();
+        }
+        return TD3_OK;
+    }
+    hipLaunchKernelGGL(coulomb_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0, s, sorted_pos,
+                       sorted_charge, n, node_center, info, (i64)0, (i64)node_count, 1, node_q,
+                       node_qabs, node_p, node_s);
+    TD3_LAUNCHED();
+    for (i64 d = n_levels - 2; d >= 0; --d) {
+        const i64 lo = levels_host[d], hi = levels_host[d + 1];
+        hipLaunchKernelGGL(coulomb_kernel, dim3(td3_blocks(hi - lo, 256)), dim3(256), 0, s, sorted_pos,
+                           sorted_charge, n, node_center, info, lo, hi, 0, node_q, node_qabs, node_p,
+                           node_s);
+        TD3_LAUNCHED();
+    }
+    return TD3_OK;
+}

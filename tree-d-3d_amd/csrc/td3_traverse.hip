@@ -1,0 +1,298 @@
+// Dual-tree traversal and the interaction lists of the FMM.
+//
+// Replaces flat_tree.py:1318-1358 (_traverse: one fmm_traverse launch per generation,
+// each followed by a host read of the counters) and kernels/fmm_traverse/fmm_traverse_cuda.cu:14-97.
+// The rules are those of kernels/fmm_traverse/fmm_traverse_pyccel.py:15-106, evaluated in
+// FP64 without contraction so the pair sets equal the reference's:
+//   empty cell -> dropped; (a, a) leaf -> P2P; (a, a) internal -> the 10 child pairs c <= d;
+//   sqrt(2) (h_a + h_b) < theta d -> M2L; two leaves -> P2P; else split a if b is a leaf or
+//   (a is not a leaf and h_a >= h_b), else split b.
+// Every generation is one fixed-size grid that reads its frontier size from device
+// memory, so the whole traversal is enqueued without a host sync (2 depth + 1 launches).
+// Appends use one atomic per wavefront.  The lists are then turned into rows by node
+// (td3_fmm_lists) and every row is sorted, so the later sums are deterministic.
+
+#include "td3_common.h"
+
+namespace {
+
+constexpr int kTravThreads = 256;
+constexpr unsigned kTravBlocks = 2048;
+
+// ctr: [0] m2l count, [1] p2p count, [2] overflow, [3 + g] frontier size of generation g.
+__device__ __forceinline__ u32 wave_append(bool want, u32 count_each, u32* counter, u32* base_out) {
+    // exclusive prefix of count_each over the lanes that want, one atomic per wave
+    const int lane = td3_lane();
+    u32 v = want ? count_each : 0u;
+    u32 incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const u32 t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    const u32 total = __shfl(incl, 63, 64);
+    u32 base = 0;
+    if (lane == 0 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 0, 64);
+    *base_out = base + incl - v;
+    return total;
+}
+
+__global__ __launch_bounds__(kTravThreads) void traverse_kernel(
+    const double* __restrict__ center, const double* __restrict__ half,
+    const int4* __restrict__ info, double theta, int g, const int2* __restrict__ frontier,
+    u32 cap_front, int2* __restrict__ next, int2* __restrict__ m2l, u32 cap_m2l,
+    int2* __restrict__ p2p, u32 cap_p2p, u32* __restrict__ ctr) {
+    // a list or frontier overflowed earlier: the caller retries with larger buffers
+    if (__hip_atomic_load(&ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const u32 nf = __hip_atomic_load(&ctr[3 + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const u32 n_front = nf < cap_front ? nf : cap_front;
+    const u32 stride = gridDim.x * kTravThreads;
+    const int lane = td3_lane();
+    for (u32 base = blockIdx.x * kTravThreads + (threadIdx.x & ~63u); base < n_front; base += stride) {
+        const u32 k = base + lane;
+        const bool valid = k < n_front;
+        int a = 0, b = 0;
+        int kind = 0;  // 0 none, 1 m2l, 2 p2p, 3 split a (4), 4 split b (4), 5 self split (10)
+        int4 fa = make_int4(0, 0, -1, 0), fb = fa;
+        if (valid) {
+            const int2 pr = frontier[k];
+            a = pr.x;
+            b = pr.y;
+            fa = info[a];
+            fb = info[b];
+            if (fa.y != fa.x && fb.y != fb.x) {
+                const bool leaf_a = fa.z < 0, leaf_b = fb.z < 0;
+                if (a == b) {
+                    kind = leaf_a ? 2 : 5;
+                } else {
+                    const double dx = __dadd_rn(center[2 * a], -center[2 * b]);
+                    const double dy = __dadd_rn(center[2 * a + 1], -center[2 * b + 1]);
+                    const double distance = __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
+                    const double radii = __dmul_rn(1.4142135623730951, __dadd_rn(half[a], half[b]));
+                    if (radii < __dmul_rn(theta, distance))
+                        kind = 1;
+                    else if (leaf_a && leaf_b)
+                        kind = 2;
+                    else if (leaf_b || (!leaf_a && half[a] >= half[b]))
+                        kind = 3;
+                    else
+                        kind = 4;
+                }
+            }
+        }
+        u32 pos;
+        wave_append(kind == 1, 1u, &ctr[0], &pos);
+        if (kind == 1) {
+            if (pos < cap_m2l)
+                m2l[pos] = make_int2(a, b);
+            else
+                ctr[2] = 1u;
+        }
+        wave_append(kind == 2, 1u, &ctr[1], &pos);
+        if (kind == 2) {
+            if (pos < cap_p2p)
+                p2p[pos] = make_int2(a, b);
+            else
+                ctr[2] = 1u;
+        }
+        const u32 cnt = kind == 5 ? 10u : (kind == 3 || kind == 4) ? 4u : 0u;
+        wave_append(cnt != 0, cnt, &ctr[3 + g + 1], &pos);
+        if (cnt) {
+            if (pos + cnt > cap_front) {
+                ctr[2] = 1u;
+            } else if (kind == 5) {
+                const int first = fa.z;
+                u32 o = pos;
+                for (int c = 0; c < 4; ++c)
+                    for (int d = c; d < 4; ++d) next[o++] = make_int2(first + c, first + d);
+            } else if (kind == 3) {
+                for (int c = 0; c < 4; ++c) next[pos + c] = make_int2(fa.z + c, b);
+            } else {
+                for (int c = 0; c < 4; ++c) next[pos + c] = make_int2(a, fb.z + c);
+            }
+        }
+    }
+}
+
+__global__ void traverse_init_kernel(int2* frontier, u32* ctr) {
+    frontier[0] = make_int2(0, 0);
+    ctr[3] = 1u;
+}
+
+// Row lengths of the directed lists: M2L both ways; P2P both ways, a self pair once.
+__global__ void degree_kernel(const int2* __restrict__ pairs, u32 count, int self_once,
+                              u32* __restrict__ deg) {
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int2 p = pairs[k];
+    atomicAdd(&deg[p.x], 1u);
+    if (!(self_once && p.x == p.y)) atomicAdd(&deg[p.y], 1u);
+}
+
+__global__ void fill_kernel(const int2* __restrict__ pairs, u32 count, int self_once,
+                            u32* __restrict__ cursor, int* __restrict__ col,
+                            int* __restrict__ dst) {
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int2 p = pairs[k];
+    u32 o = atomicAdd(&cursor[p.x], 1u);
+    col[o] = p.y;
+    if (dst) dst[o] = p.x;
+    if (!(self_once && p.x == p.y)) {
+        o = atomicAdd(&cursor[p.y], 1u);
+        col[o] = p.x;
+        if (dst) dst[o] = p.y;
+    }
+}
+
+// Sorts every row of col ascending: one wavefront per row, bitonic in LDS.
+constexpr int kSortMax = 4096;
+__global__ __launch_bounds__(256) void row_sort_kernel(const u32* __restrict__ row_start, i64 rows,
+                                                       int* __restrict__ col, u32* __restrict__ err) {
+    __shared__ int s_buf[4][kSortMax];
+    const int wave = threadIdx.x / 64, lane = td3_lane();
+    int* buf = s_buf[wave];
+    for (i64 r = (i64)blockIdx.x * 4 + wave; r < rows; r += (i64)gridDim.x * 4) {
+        const u32 s = row_start[r], e = row_start[r + 1];
+        const int len = (int)(e - s);
+        if (len <= 1) continue;
+        if (len > kSortMax) {
+            if (lane == 0) atomicOr(err, 1u);
+            continue;
+        }
+        int pw = 2;
+        while (pw < len) pw <<= 1;
+        for (int i = lane; i < pw; i += 64) buf[i] = i < len ? col[s + i] : 0x7fffffff;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int size = 2; size <= pw; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = lane; i < pw / 2; i += 64) {
+                    const int lo = 2 * i - (i & (stride - 1));
+                    const int hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    const int x = buf[lo], y = buf[hi];
+                    if ((x > y) == up) {
+                        buf[lo] = y;
+                        buf[hi] = x;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+        }
+        for (int i = lane; i < len; i += 64) col[s + i] = buf[i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// dst of each directed pair after the row sort (rows keep their node).
+__global__ void row_dst_kernel(const u32* __restrict__ row_start, i64 rows, int* __restrict__ dst) {
+    const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    for (u32 k = row_start[r]; k < row_start[r + 1]; ++k) dst[k] = (int)r;
+}
+
+}  // namespace
+
+#include "td3_sort.h"
+
+extern "C" size_t td3_traverse_workspace_bytes(int64_t cap_front) {
+    return (size_t)(2 * cap_front) * sizeof(int2) + 4096;
+}
+
+extern "C" int td3_traverse(const double* node_center, const double* node_half,
+                            const void* node_info, int64_t node_count, int64_t tree_depth,
+                            double theta, int64_t cap_m2l, int64_t cap_p2p, int64_t cap_front,
+                            int32_t* m2l, int32_t* p2p, void* workspace, size_t workspace_bytes,
+                            int64_t* out_counts, void* stream) {
+    TD3_ARG(node_count >= 1 && node_count < 0x7fffffff, "node_count");
+    TD3_ARG(tree_depth >= 0 && tree_depth <= 64, "tree_depth");
+    TD3_ARG(cap_front >= 16 && cap_front < 0x7fffffff && cap_m2l < 0xffffffffll &&
+                cap_p2p < 0xffffffffll, "capacities");
+    TD3_ARG(workspace_bytes >= td3_traverse_workspace_bytes(cap_front), "workspace too small");
+    TD3_ARG(out_counts != nullptr, "out_counts");
+    hipStream_t s = (hipStream_t)stream;
+    int2* fa = (int2*)workspace;
+    int2* fb = fa + cap_front;
+    u32* ctr = (u32*)(fb + cap_front);
+    const int gens = (int)(2 * tree_depth + 1);
+    TD3_HIP(hipMemsetAsync(ctr, 0, sizeof(u32) * (size_t)(gens + 8), s));
+    hipLaunchKernelGGL(traverse_init_kernel, dim3(1), dim3(1), 0, s, fa, ctr);
+    TD3_LAUNCHED();
+    for (int g = 0; g < gens; ++g) {
+        hipLaunchKernelGGL(traverse_kernel, dim3(kTravBlocks), dim3(kTravThreads), 0, s, node_center,
+                           node_half, (const int4*)node_info, theta, g, (g & 1) ? fb : fa,
+                           (u32)cap_front, (g & 1) ? fa : fb, (int2*)m2l, (u32)cap_m2l, (int2*)p2p,
+                           (u32)cap_p2p, ctr);
+        TD3_LAUNCHED();
+    }
+    u32 host[4 + 130];
+    TD3_HIP(hipMemcpyAsync(host, ctr, sizeof(u32) * (size_t)(gens + 4), hipMemcpyDeviceToHost, s));
+    TD3_HIP(hipStreamSynchronize(s));
+    u32 max_front = 0;
+    for (int g = 0; g <= gens; ++g) max_front = host[3 + g] > max_front ? host[3 + g] : max_front;
+    out_counts[0] = host[0];
+    out_counts[1] = host[1];
+    out_counts[2] = max_front;
+    out_counts[3] = host[2] ? 1 : 0;
+    if (host[2] || host[3 + gens] != 0) {
+        if (!host[2]) {
+            td3_set_error("traversal did not finish in %d generations", gens);
+            return TD3_ERR_ARG;
+        }
+        td3_set_error("traversal lists overflowed (m2l %u, p2p %u, frontier %u)", host[0], host[1],
+                      max_front);
+        return TD3_ERR_CAPACITY;
+    }
+    return TD3_OK;
+}
+
+extern "C" size_t td3_lists_workspace_bytes(int64_t node_count) {
+    return (size_t)(node_count + 1) * sizeof(u32) * 2 + td3sort::scratch_bytes(node_count + 1) + 1024;
+}
+
+extern "C" int td3_lists(const int32_t* pairs, int64_t count, int self_once, int64_t node_count,
+                         uint32_t* row_start, int32_t* col, int32_t* dst, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+    TD3_ARG(count >= 0 && node_count >= 1, "sizes");
+    TD3_ARG(workspace_bytes >= td3_lists_workspace_bytes(node_count), "workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    u32* deg = (u32*)workspace;
+    u32* cursor = deg + (node_count + 1);
+    u32* scratch = (u32*)(((uintptr_t)(cursor + node_count + 1) + 255) & ~(uintptr_t)255);
+    TD3_HIP(hipMemsetAsync(deg, 0, sizeof(u32) * (size_t)(node_count + 1), s));
+    if (count > 0) {
+        hipLaunchKernelGGL(degree_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, (const int2*)pairs,
+                           (u32)count, self_once, deg);
+        TD3_LAUNCHED();
+    }
+    if (int rc = td3sort::exclusive_scan_u32(deg, row_start, node_count + 1, nullptr, scratch, s)) return rc;
+    if (count > 0) {
+        TD3_HIP(hipMemcpyAsync(cursor, row_start, sizeof(u32) * (size_t)node_count, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(fill_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, (const int2*)pairs,
+                           (u32)count, self_once, cursor, col, (int*)nullptr);
+        TD3_LAUNCHED();
+        u32* err = deg;  // reuse: zero it first
+        TD3_HIP(hipMemsetAsync(err, 0, sizeof(u32), s));
+        unsigned blocks = td3_blocks(node_count, 4);
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(row_sort_kernel, dim3(blocks), dim3(256), 0, s, row_start, node_count, col, err);
+        TD3_LAUNCHED();
+        if (dst) {
+            hipLaunchKernelGGL(row_dst_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0, s, row_start,
+                               node_count, dst);
+            TD3_LAUNCHED();
+        }
+        u32 herr = 0;
+        TD3_HIP(hipMemcpyAsync(&herr, err, sizeof(u32), hipMemcpyDeviceToHost, s));
+        TD3_HIP(hipStreamSynchronize(s));
+        if (herr) {
+            td3_set_error("an interaction row holds more than %d cells", kSortMax);
+            return TD3_ERR_ARG;
+        }
+    }
+    return TD3_OK;
+}

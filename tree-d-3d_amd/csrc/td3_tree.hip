@@ -1,0 +1,409 @@
+// Bit-exact flat quadtree build.
+//
+// Replaces the reference's per-level build (flat_tree.py:532-625: quadtree_quadrants +
+// a (N, 4) cumsum + quadtree_scatter per level, with a host sync per level) and its
+// serial CPU build (kernels/quadtree_build/quadtree_build_pyccel.py:12-137), producing
+// identical arrays.
+//
+// 1. keys: every particle descends from the root, taking at level L the digit
+//    q = [x > cx] + 2 [y < cy] of the FP64 comparison against the centre of its
+//    depth-(L-1) cell, then moving to the child centre c +- h/2 - the same FP64
+//    operations the reference does per level, so the digit path is exact.  Digits are
+//    packed most significant first: levels 1-32 in `hi`, 33-64 in `lo`.
+// 2. a stable LSD radix sort by key (values = input index): every cell is a contiguous
+//    run, cells in breadth-first order within each depth.
+// 3. topology without a level loop: with L_i the common-prefix depth of sorted keys
+//    i and i+1 and M_i that of keys i and i+cap, particle i starts a splitting node at
+//    every depth d with L_{i-1} < d <= min(M_i, max_depth-1).  Those (d, i) entries,
+//    bucketed by d, number the splitting nodes of each depth in order; node k's four
+//    children are levels[d+1] + 4 rank + q.  One thread per splitting node writes its
+//    children (centres by the same exact descent, ranges by binary search on the
+//    digit of level d+1, child index by binary search in the next bucket).
+// 4. particles keep input order inside each leaf: a stable radix sort of the input
+//    indices by the start of their leaf gives `perm`.
+// One host read-back (the splitting-node count per depth) sizes the node arrays.
+
+#include "td3_sort.h"
+
+namespace {
+
+constexpr int kMaxLevels = 64;
+
+__device__ __forceinline__ int lcp_levels(u64 ahi, u64 alo, u64 bhi, u64 blo, int nkey) {
+    u64 x = ahi ^ bhi;
+    int l;
+    if (x)
+        l = __clzll(x) >> 1;
+    else if ((x = alo ^ blo))
+        l = 32 + (__clzll(x) >> 1);
+    else
+        l = 64;
+    return l < nkey ? l : nkey;
+}
+
+__device__ __forceinline__ int digit_at(u64 hi, u64 lo, int level) {
+    return level <= 32 ? (int)((hi >> (64 - 2 * level)) & 3ull)
+                       : (int)((lo >> (64 - 2 * (level - 32))) & 3ull);
+}
+
+__global__ void keys_kernel(const double* __restrict__ pos, i64 n, double cx0, double cy0,
+                            double h0, int nlev, u64* __restrict__ key_hi,
+                            u64* __restrict__ key_lo) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[i];
+    const double y = pos[n + i];
+    double cx = cx0, cy = cy0, h = h0;
+    u64 hi = 0, lo = 0;
+    for (int level = 1; level <= nlev; ++level) {
+        const int q = (x > cx ? 1 : 0) + (y < cy ? 2 : 0);
+        h = 0.5 * h;
+        cx = (q & 1) ? cx + h : cx - h;
+        cy = (q & 2) ? cy - h : cy + h;
+        if (level <= 32)
+            hi |= (u64)q << (64 - 2 * level);
+        else
+            lo |= (u64)q << (64 - 2 * (level - 32));
+    }
+    key_hi[i] = hi;
+    if (key_lo != nullptr) key_lo[i] = lo;
+}
+
+// Splitting-node starts per sorted particle, and their count per depth.
+__global__ __launch_bounds__(256) void count_kernel(const u64* __restrict__ khi,
+                                                    const u64* __restrict__ klo, i64 n, i64 cap,
+                                                    int max_split_depth, int nkey,
+                                                    u32* __restrict__ cnt,
+                                                    u32* __restrict__ depth_count,
+                                                    u32* __restrict__ flag) {
+    __shared__ u32 s_depth[kMaxLevels + 1];
+    if (threadIdx.x <= kMaxLevels) s_depth[threadIdx.x] = 0;
+    __syncthreads();
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const u64 ahi = khi[i], alo = klo ? klo[i] : 0ull;
+        int prev = -1;
+        if (i > 0) prev = lcp_levels(khi[i - 1], klo ? klo[i - 1] : 0ull, ahi, alo, nkey);
+        int m = -1;
+        if (i + cap < n) m = lcp_levels(ahi, alo, khi[i + cap], klo ? klo[i + cap] : 0ull, nkey);
+        const int top = m < max_split_depth ? m : max_split_depth;
+        const int c = top > prev ? top - prev : 0;
+        cnt[i] = (u32)c;
+        for (int d = prev + 1; d <= top; ++d) atomicAdd(&s_depth[d], 1u);
+        // a common prefix as long as the key while cells that deep may still split:
+        // the key must resolve more levels
+        if (m >= nkey && nkey <= max_split_depth) atomicOr(flag, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x <= kMaxLevels && s_depth[threadIdx.x])
+        atomicAdd(&depth_count[threadIdx.x], s_depth[threadIdx.x]);
+}
+
+// One (depth, particle) entry per splitting node, in particle order.
+__global__ void entries_kernel(const u64* __restrict__ khi, const u64* __restrict__ klo, i64 n,
+                               int nkey, const u32* __restrict__ cnt, const u32* __restrict__ off,
+                               u64* __restrict__ ekey, u32* __restrict__ eval) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u32 c = cnt[i];
+    if (!c) return;
+    int prev = -1;
+    if (i > 0) prev = lcp_levels(khi[i - 1], klo ? klo[i - 1] : 0ull, khi[i], klo ? klo[i] : 0ull, nkey);
+    const u32 o = off[i];
+    for (u32 j = 0; j < c; ++j) {
+        ekey[o + j] = (u64)(prev + 1 + (int)j);
+        eval[o + j] = (u32)i;
+    }
+}
+
+// Level offsets (int64, reference layout) and per-depth bucket starts from the counts of
+// splitting nodes per depth; also writes the root node.
+__global__ void levels_kernel(const u32* __restrict__ depth_count, i64 n, i64 cap, int max_depth,
+                              double cx0, double cy0, double h0, i64* __restrict__ levels,
+                              u32* __restrict__ bstart, double* __restrict__ node_center,
+                              double* __restrict__ node_half, i64* __restrict__ node_start,
+                              i64* __restrict__ node_end, i64* __restrict__ node_child,
+                              i64* __restrict__ node_depth, int4* __restrict__ node_info) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    levels[0] = 0;
+    levels[1] = 1;
+    i64 lv = 1;
+    u32 b = 0;
+    for (int d = 0; d <= kMaxLevels; ++d) {
+        bstart[d] = b;
+        const u32 c = depth_count[d];
+        b += c;
+        if (c) {
+            lv += 4 * (i64)c;
+            levels[d + 2] = lv;
+        }
+    }
+    bstart[kMaxLevels + 1] = b;
+    const bool split = n > cap && max_depth > 0;
+    node_center[0] = cx0;
+    node_center[1] = cy0;
+    node_half[0] = h0;
+    node_start[0] = 0;
+    node_end[0] = n;
+    node_child[0] = split ? 1 : -1;
+    node_depth[0] = 0;
+    if (node_info) node_info[0] = make_int4(0, (int)n, split ? 1 : -1, 0);
+}
+
+// One thread per splitting node: writes its four children.
+__global__ void nodes_kernel(const u64* __restrict__ khi, const u64* __restrict__ klo, i64 n,
+                             i64 cap, int max_depth, int nkey, double cx0, double cy0, double h0,
+                             const u64* __restrict__ edepth, const u32* __restrict__ epos, i64 s_total,
+                             const u32* __restrict__ bstart, const u32* __restrict__ depth_count,
+                             const i64* __restrict__ levels, double* __restrict__ node_center,
+                             double* __restrict__ node_half, i64* __restrict__ node_start,
+                             i64* __restrict__ node_end, i64* __restrict__ node_child,
+                             i64* __restrict__ node_depth, int4* __restrict__ node_info) {
+    const i64 g = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= s_total) return;
+    const int d = (int)edepth[g];
+    const i64 i = (i64)epos[g];
+    const i64 r = g - (i64)bstart[d];
+    const u64 ihi = khi[i], ilo = klo ? klo[i] : 0ull;
+    // Centre and half-width of the depth-d cell of particle i: the exact descent.
+    double cx = cx0, cy = cy0, h = h0;
+    for (int level = 1; level <= d; ++level) {
+        const int q = digit_at(ihi, ilo, level);
+        h = 0.5 * h;
+        cx = (q & 1) ? cx + h : cx - h;
+        cy = (q & 2) ? cy - h : cy + h;
+    }
+    // End of the cell: first j > i whose key leaves the depth-d prefix.
+    i64 lo_j = i + 1, hi_j = n;
+    while (lo_j < hi_j) {
+        const i64 mid = (lo_j + hi_j) >> 1;
+        if (lcp_levels(ihi, ilo, khi[mid], klo ? klo[mid] : 0ull, nkey) >= d)
+            lo_j = mid + 1;
+        else
+            hi_j = mid;
+    }
+    const i64 e = lo_j;
+    i64 bound[5];
+    bound[0] = i;
+    bound[4] = e;
+    for (int q = 1; q < 4; ++q) {
+        i64 a = bound[q - 1], z = e;
+        while (a < z) {
+            const i64 mid = (a + z) >> 1;
+            if (digit_at(khi[mid], klo ? klo[mid] : 0ull, d + 1) < q)
+                a = mid + 1;
+            else
+                z = mid;
+        }
+        bound[q] = a;
+    }
+    const double hc = 0.5 * h;
+    const i64 base = levels[d + 1] + 4 * r;
+    for (int q = 0; q < 4; ++q) {
+        const i64 node = base + q;
+        const i64 start = bound[q], end = bound[q + 1];
+        i64 child = -1;
+        if (end - start > cap && d + 1 < max_depth) {
+            // rank of the entry (d + 1, start) among the splitting nodes of depth d + 1
+            i64 a = bstart[d + 1], z = a + depth_count[d + 1];
+            while (a < z) {
+                const i64 mid = (a + z) >> 1;
+                if ((i64)epos[mid] < start)
+                    a = mid + 1;
+                else
+                    z = mid;
+            }
+            child = levels[d + 2] + 4 * (a - (i64)bstart[d + 1]);
+        }
+        node_center[2 * node] = (q & 1) ? cx + hc : cx - hc;
+        node_center[2 * node + 1] = (q & 2) ? cy - hc : cy + hc;
+        node_half[node] = hc;
+        node_start[node] = start;
+        node_end[node] = end;
+        node_child[node] = child;
+        node_depth[node] = d + 1;
+        if (node_info) node_info[node] = make_int4((int)start, (int)end, (int)child, d + 1);
+    }
+}
+
+// For every leaf, its start position, scattered to the input order of its particles.
+__global__ void leaf_key_kernel(const i64* __restrict__ node_start, const i64* __restrict__ node_end,
+                                const i64* __restrict__ node_child, i64 count,
+                                const u32* __restrict__ sorted_idx, u64* __restrict__ key2) {
+    const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= count || node_child[node] >= 0) return;
+    const i64 s = node_start[node], e = node_end[node];
+    for (i64 k = s; k < e; ++k) key2[sorted_idx[k]] = (u64)s;
+}
+
+__global__ void gather_kernel(const u32* __restrict__ order, const double* __restrict__ pos, i64 n,
+                              i64* __restrict__ perm, double* __restrict__ sorted_pos) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const u32 j = order[k];
+    perm[k] = (i64)j;
+    if (sorted_pos) {
+        sorted_pos[k] = pos[j];
+        sorted_pos[n + k] = pos[n + j];
+    }
+}
+
+struct TreeWs {
+    u64 *khi, *khi_alt, *klo, *klo_alt, *ekey, *ekey_alt;
+    u32 *idx, *idx_alt, *cnt, *eval, *eval_alt, *scratch, *small;
+};
+
+i64 entries_cap(i64 capacity) { return (capacity - 1) / 4 + 1; }
+
+size_t carve(TreeWs* w, void* base, i64 n, i64 capacity) {
+    const i64 ne = entries_cap(capacity);
+    const i64 big = n > ne ? n : ne;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return base ? (void*)((char*)base + o) : nullptr;
+    };
+    w->khi = (u64*)take(sizeof(u64) * (size_t)big);
+    w->khi_alt = (u64*)take(sizeof(u64) * (size_t)big);
+    w->klo = (u64*)take(sizeof(u64) * (size_t)n);
+    w->klo_alt = (u64*)take(sizeof(u64) * (size_t)n);
+    w->idx = (u32*)take(sizeof(u32) * (size_t)big);
+    w->idx_alt = (u32*)take(sizeof(u32) * (size_t)big);
+    w->cnt = (u32*)take(sizeof(u32) * (size_t)(n + 1));
+    w->ekey = (u64*)take(sizeof(u64) * (size_t)ne);
+    w->ekey_alt = (u64*)take(sizeof(u64) * (size_t)ne);
+    w->eval = (u32*)take(sizeof(u32) * (size_t)ne);
+    w->eval_alt = (u32*)take(sizeof(u32) * (size_t)ne);
+    w->scratch = (u32*)take(td3sort::scratch_bytes(big));
+    w->small = (u32*)take(sizeof(u32) * 256);
+    return off;
+}
+
+int bits_for(i64 v) {
+    int b = 0;
+    while (b < 63 && ((i64)1 << b) <= v) ++b;
+    return b;
+}
+
+}  // namespace
+
+extern "C" size_t td3_tree_workspace_bytes(int64_t n, int64_t capacity) {
+    TreeWs w;
+    return carve(&w, nullptr, n < 1 ? 1 : n, capacity < 1 ? 1 : capacity);
+}
+
+extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double ymid, double half,
+                              int64_t leaf_capacity, int64_t max_depth, int64_t capacity,
+                              double* node_center, double* node_half, int64_t* node_start,
+                              int64_t* node_end, int64_t* node_child, int64_t* node_depth,
+                              int64_t* levels, int64_t* perm, double* sorted_pos, void* node_info,
+                              void* workspace, size_t workspace_bytes, int64_t* out_sizes,
+                              void* stream) {
+    TD3_ARG(n > 0 && n < (int64_t)0x7fffffff, "n must be in [1, 2^31)");
+    TD3_ARG(leaf_capacity >= 1 && max_depth >= 1, "leaf_capacity and max_depth must be positive");
+    TD3_ARG(capacity >= 1, "capacity");
+    TD3_ARG(out_sizes != nullptr, "out_sizes");
+    TD3_ARG(levels != nullptr, "levels (int64[67])");
+    TreeWs w;
+    const size_t need = carve(&w, nullptr, n, capacity);
+    TD3_ARG(workspace != nullptr && workspace_bytes >= need, "workspace too small");
+    carve(&w, workspace, n, capacity);
+    hipStream_t s = (hipStream_t)stream;
+    const int nlev = (int)(max_depth < kMaxLevels ? max_depth : kMaxLevels);
+    const int max_split = (int)(max_depth - 1 < kMaxLevels ? max_depth - 1 : kMaxLevels);
+    u32* depth_count = w.small;          // [65]
+    u32* flag = w.small + 80;            // [1]
+    u32* bstart = w.small + 96;          // [66]
+    u32* s_total = w.small + 176;        // [1]
+    const unsigned nb = td3_blocks(n, 256);
+
+    // 1-2. keys and sort by the first min(32, nlev) levels
+    int nkey = nlev < 32 ? nlev : 32;
+    bool deep = false;
+    u32 host_small[81];
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (!deep) {
+            hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
+                               w.khi, (u64*)nullptr);
+            TD3_LAUNCHED();
+            if (int rc = td3sort::sort_pairs(w.khi, w.idx, nullptr, w.khi_alt, w.idx_alt, nullptr, n,
+                                             64 - 2 * nkey, 64, true, w.scratch, s))
+                return rc;
+        } else {
+            hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
+                               w.khi, w.klo);
+            TD3_LAUNCHED();
+            if (int rc = td3sort::sort_pairs(w.klo, w.idx, w.khi, w.klo_alt, w.idx_alt, w.khi_alt, n,
+                                             64 - 2 * (nkey - 32), 64, true, w.scratch, s))
+                return rc;
+            if (int rc = td3sort::sort_pairs(w.khi, w.idx, w.klo, w.khi_alt, w.idx_alt, w.klo_alt, n,
+                                             0, 64, false, w.scratch, s))
+                return rc;
+        }
+        // 3. splitting-node starts
+        TD3_HIP(hipMemsetAsync(w.small, 0, sizeof(u32) * 96, s));
+        hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
+                           (i64)leaf_capacity, max_split, nkey, w.cnt, depth_count, flag);
+        TD3_LAUNCHED();
+        TD3_HIP(hipMemcpyAsync(host_small, w.small, sizeof(u32) * 81, hipMemcpyDeviceToHost, s));
+        TD3_HIP(hipStreamSynchronize(s));
+        if (!host_small[80]) break;
+        if (deep || nlev <= 32) {
+            td3_set_error("the tree needs more than %d levels (max_depth %lld)", kMaxLevels,
+                          (long long)max_depth);
+            return TD3_ERR_ARG;
+        }
+        deep = true;
+        nkey = nlev;
+    }
+    // node count and level count
+    i64 count = 1, s_sum = 0;
+    int n_levels = 1;
+    for (int d = 0; d <= kMaxLevels; ++d) {
+        if (host_small[d]) {
+            count += 4 * (i64)host_small[d];
+            s_sum += host_small[d];
+            n_levels = d + 2;
+        }
+    }
+    out_sizes[0] = count;
+    out_sizes[1] = n_levels;
+    out_sizes[2] = count;
+    if (count > capacity) {
+        td3_set_error("node capacity %lld < %lld", (long long)capacity, (long long)count);
+        return TD3_ERR_CAPACITY;
+    }
+    const u64* klo = deep ? w.klo : nullptr;
+    // 4. entries (depth, particle) in particle order, then bucketed by depth (stable)
+    if (s_sum > 0) {
+        if (int rc = td3sort::exclusive_scan_u32(w.cnt, w.cnt, n, s_total, w.scratch, s)) return rc;
+        hipLaunchKernelGGL(entries_kernel, dim3(nb), dim3(256), 0, s, w.khi, klo, n, nkey, w.cnt,
+                           w.cnt, w.ekey, w.eval);
+        TD3_LAUNCHED();
+        if (int rc = td3sort::sort_pairs(w.ekey, w.eval, nullptr, w.ekey_alt, w.eval_alt, nullptr,
+                                         s_sum, 0, 8, false, w.scratch, s))
+            return rc;
+    }
+    hipLaunchKernelGGL(levels_kernel, dim3(1), dim3(64), 0, s, depth_count, n, (i64)leaf_capacity,
+                       (int)max_depth, xmid, ymid, half, levels, bstart, node_center, node_half,
+                       node_start, node_end, node_child, node_depth, (int4*)node_info);
+    TD3_LAUNCHED();
+    if (s_sum > 0) {
+        hipLaunchKernelGGL(nodes_kernel, dim3(td3_blocks(s_sum, 256)), dim3(256), 0, s, w.khi, klo, n,
+                           (i64)leaf_capacity, (int)max_depth, nkey, xmid, ymid, half, w.ekey, w.eval,
+                           s_sum, bstart, depth_count, levels, node_center, node_half, node_start,
+                           node_end, node_child, node_depth, (int4*)node_info);
+        TD3_LAUNCHED();
+    }
+    // 5. input order inside each leaf: sort the input indices by their leaf's start
+    hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_start,
+                       node_end, node_child, count, w.idx, w.khi_alt);
+    TD3_LAUNCHED();
+    if (int rc = td3sort::sort_pairs(w.khi_alt, w.idx_alt, nullptr, w.khi, w.idx, nullptr, n, 0,
+                                     bits_for(n), true, w.scratch, s))
+        return rc;
+    hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, w.idx_alt, pos, n, perm, sorted_pos);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}

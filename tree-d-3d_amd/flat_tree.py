@@ -1,0 +1,671 @@
+"""Flat quadtree and direct sums on the GPU: the reference's flat_tree.py API, run by the
+gfx950 kernels of libtd3_hip.so.
+
+Two interaction kernels, as in the reference (flat_tree.py:1-27): ``"coulomb"``, the
+softened 3D point charge ``q / sqrt(r^2 + eps^2)``, and ``"log"``, the 2D line charge
+``-q ln(r^2 + eps^2)``.  Positions, targets and fields are component-major ``(2, N)``;
+per-particle scalars are ``(N,)``; per-node arrays are node-major.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._utils import _check, _device, _is_torch, _out, _points
+
+_KERNELS = ("coulomb", "log")
+_PRECISIONS = {"fp64": _abi.FP64, "mixed": _abi.MIXED}
+
+
+def _precision(precision: str) -> int:
+    if precision not in _PRECISIONS:
+        raise ValueError(f"precision must be one of {tuple(_PRECISIONS)}")
+    return _PRECISIONS[precision]
+
+
+def direct_field(
+    targets,
+    pos,
+    charge,
+    *,
+    softening=0.0,
+    exclude=None,
+    compiled=True,
+    kernel="coulomb",
+    precision="fp64",
+):
+    """Exact potential ``(M,)`` and field ``(2, M)`` at ``targets`` ``(2, M)``, without ``k_e``.
+
+    flat_tree.py:125-160.  ``pos`` ``(2, N)`` and ``charge`` ``(N,)`` are the sources;
+    ``exclude[i]`` is a source skipped for target ``i`` (-1 for none).  ``compiled`` is
+    accepted for compatibility (every call runs the GPU kernels).  ``precision="mixed"``
+    evaluates logarithms and reciprocals in FP32 (sums stay FP64).
+    """
+    if kernel not in _KERNELS:
+        raise ValueError(f"kernel must be one of {_KERNELS}")
+    like_torch = _is_torch(targets) or _is_torch(pos)
+    t = _points(targets, "targets")
+    p = _points(pos, "pos")
+    q = _device(charge).reshape(-1)
+    m, n = t.shape[1], p.shape[1]
+    if q.shape[0] != n:
+        raise ValueError("pos and charge must have the same length")
+    if exclude is None:
+        ex = torch.full((m,), -1, dtype=torch.int64, device=t.device)
+    else:
+        ex = _device(exclude, torch.int64).reshape(-1)
+    phi = torch.zeros(m, dtype=torch.float64, device=t.device)
+    field = torch.zeros((2, m), dtype=torch.float64, device=t.device)
+    if not m or not n:
+        return _out(phi, like_torch), _out(field, like_torch)
+    status = torch.zeros(1, dtype=torch.int64, device=t.device)
+    nbytes = _abi.lib().td3_direct_workspace_bytes(m, n)
+    ws = _abi.workspace(nbytes)
+    _abi.call(
+        "td3_direct",
+        _abi.LOG if kernel == "log" else _abi.COULOMB,
+        _precision(precision),
+        _abi.ptr(t),
+        m,
+        _abi.ptr(ex),
+        _abi.ptr(p),
+        n,
+        _abi.ptr(q),
+        float(softening),
+        _abi.ptr(phi),
+        _abi.ptr(field),
+        _abi.ptr(status),
+        _abi.ptr(ws),
+        nbytes,
+        _abi.stream(),
+    )
+    _check(status)
+    return _out(phi, like_torch), _out(field, like_torch)
+
+
+#: Highest expansion order of the kernels (flat_tree.py:46, TREE_D_3D_MAX_ORDER).
+MAX_ORDER = 64
+#: Default orders (flat_tree.py:50-51).
+LOG_ORDER = 12
+COULOMB_ORDER = 4
+
+#: Device arrays of the tree, exposed in the caller's array type.
+_PUBLIC = (
+    "pos",
+    "charge",
+    "perm",
+    "node_center",
+    "node_half",
+    "node_start",
+    "node_end",
+    "node_child",
+    "node_depth",
+    "node_q",
+    "node_qabs",
+    "node_p",
+    "node_s",
+    "moment_re",
+    "moment_im",
+)
+
+_CACHE: dict = {}
+
+
+def _binomial_table(size: int) -> np.ndarray:
+    """C(n, k) for 0 <= n, k < size, as floats (flat_tree.py:224-231)."""
+    table = np.zeros((size, size))
+    for row in range(size):
+        table[row, 0] = 1.0
+        for col in range(1, row + 1):
+            table[row, col] = table[row - 1, col - 1] + table[row - 1, col]
+    return table
+
+
+def _device_table(name: str, builder, *key) -> torch.Tensor:
+    """A small constant table on the current device, built once."""
+    dev = _abi.device()
+    k = (name, dev.index, *key)
+    if k not in _CACHE:
+        _CACHE[k] = torch.from_numpy(np.ascontiguousarray(builder(*key))).to(dev)
+    return _CACHE[k]
+
+
+class FlatNode:
+    """Read-only view of one node of a `FlatQuadtree` (flat_tree.py:163-211)."""
+
+    def __init__(self, tree: "FlatQuadtree", index: int) -> None:
+        self.tree = tree
+        self.index = index
+
+    @property
+    def quad(self):
+        from .quad import Quad
+
+        x, y = (float(v) for v in self.tree._d["node_center"][self.index].tolist())
+        return Quad(x, y, float(self.tree._d["node_half"][self.index]))
+
+    @property
+    def count(self) -> int:
+        d = self.tree._d
+        return int(d["node_end"][self.index] - d["node_start"][self.index])
+
+    @property
+    def total_charge(self) -> float:
+        return float(self.tree._d["node_q"][self.index])
+
+    @property
+    def total_abs_charge(self) -> float:
+        return float(self.tree._d["node_qabs"][self.index])
+
+    @property
+    def dipole(self) -> np.ndarray:
+        return self.tree._d["node_p"][self.index].cpu().numpy().copy()
+
+    @property
+    def second_moment(self) -> np.ndarray:
+        sxx, sxy, syy = (float(v) for v in self.tree._d["node_s"][self.index].tolist())
+        return np.array([[sxx, sxy], [sxy, syy]])
+
+    @property
+    def particles(self) -> np.ndarray:
+        d = self.tree._d
+        start, end = int(d["node_start"][self.index]), int(d["node_end"][self.index])
+        return d["perm"][start:end].cpu().numpy()
+
+    def is_external(self) -> bool:
+        return int(self.tree._d["node_child"][self.index]) < 0
+
+    @property
+    def children(self):
+        first = int(self.tree._d["node_child"][self.index])
+        if first < 0:
+            return (None, None, None, None)
+        return tuple(FlatNode(self.tree, first + k) for k in range(4))
+
+
+class FlatQuadtree:
+    """Array quadtree for point charges in a plane or for line charges, on the GPU.
+
+    The reference's FlatQuadtree (flat_tree.py:437-1676): same constructor, same tree
+    (bit-identical arrays), same ``evaluate`` / ``fmm`` / ``fmm_at`` / ``field_at`` /
+    ``set_charges``.  ``precision`` selects all-FP64 (``"fp64"``, the reference's
+    arithmetic) or ``"mixed"`` (FP32 far field and near field on FP64 tree data, FP64
+    sums); every call may override it.
+    """
+
+    def __init__(
+        self,
+        positions,
+        charges,
+        root_quad,
+        *,
+        leaf_capacity: int = 8,
+        max_depth: int = 48,
+        compiled: bool = True,
+        kernel: str = "coulomb",
+        order: int | None = None,
+        symmetric: bool = False,
+        comm=None,
+        precision: str = "fp64",
+    ) -> None:
+        from ._utils import _finite
+
+        if kernel not in _KERNELS:
+            raise ValueError(f"kernel must be one of {_KERNELS}")
+        if order is None:
+            order = LOG_ORDER if kernel == "log" else COULOMB_ORDER
+        if isinstance(order, bool) or int(order) != order or order < 1:
+            raise ValueError("order must be a positive integer")
+        if order > MAX_ORDER:
+            raise ValueError(f"order must be at most {MAX_ORDER}, got {order}")
+        if comm is not None:
+            raise NotImplementedError("tree-d-3d_amd runs on one GPU (comm must be None)")
+        self.kernel = kernel
+        self.order = int(order)
+        self.symmetric = bool(symmetric)
+        self.comm = None
+        self.precision = precision
+        _precision(precision)
+        _finite("half_size", root_quad.half, positive=True)
+        _finite("xmid", root_quad.xmid)
+        _finite("ymid", root_quad.ymid)
+        for name, value in (("leaf_capacity", leaf_capacity), ("max_depth", max_depth)):
+            if isinstance(value, bool) or int(value) != value or value < 1:
+                raise ValueError(f"{name} must be a positive integer")
+        self._like_torch = _is_torch(positions)
+        self._host_cache: dict = {}
+        self._d: dict = {}
+        pos = _points(positions, "positions").clone()
+        charge = _device(charges).reshape(-1).clone()
+        n = pos.shape[1]
+        if charge.shape[0] != n:
+            raise ValueError("positions and charges must have the same length")
+        self._d["pos"] = pos
+        self._d["charge"] = charge
+        if n and not bool(torch.isfinite(pos).all()):
+            raise ValueError("Particle position must be finite")
+        lo_x, hi_x = root_quad.xmid - root_quad.half, root_quad.xmid + root_quad.half
+        lo_y, hi_y = root_quad.ymid - root_quad.half, root_quad.ymid + root_quad.half
+        if n and bool(
+            ((pos[0] < lo_x) | (pos[0] > hi_x) | (pos[1] < lo_y) | (pos[1] > hi_y)).any()
+        ):
+            raise ValueError("Particle lies outside tree bounds")
+        self.leaf_capacity = int(leaf_capacity)
+        self.max_depth = int(max_depth)
+        self._root = (float(root_quad.xmid), float(root_quad.ymid), float(root_quad.half))
+        self._traversals: dict = {}
+        self.fmm_pairs = (0, 0)
+        if not n:
+            self._empty()
+            return
+        self._build()
+        self._sorted_charge()
+        self._moments()
+
+    # ---------------------------------------------------------------- arrays
+    def __getattr__(self, name):
+        if name in _PUBLIC:
+            d = self.__dict__.get("_d", {})
+            if name not in d:
+                raise AttributeError(name)
+            t = d[name]
+            if self.__dict__.get("_like_torch", True):
+                return t
+            cache = self.__dict__["_host_cache"]
+            if name not in cache:
+                cache[name] = t.cpu().numpy()
+            return cache[name]
+        raise AttributeError(name)
+
+    def _invalidate_host(self, *names):
+        for name in names or tuple(self._host_cache):
+            self._host_cache.pop(name, None)
+
+    def _empty(self) -> None:
+        """flat_tree.py:466-481: the lone root of an empty tree."""
+        dev = self._d["pos"].device
+        xmid, ymid, half = self._root
+        p1 = self.order + 1
+        self.node_count = 1
+        self.levels = np.array([0, 1])
+        self._d.update(
+            perm=torch.empty(0, dtype=torch.int64, device=dev),
+            node_center=torch.tensor([[xmid, ymid]], dtype=torch.float64, device=dev),
+            node_half=torch.tensor([half], dtype=torch.float64, device=dev),
+            node_start=torch.zeros(1, dtype=torch.int64, device=dev),
+            node_end=torch.zeros(1, dtype=torch.int64, device=dev),
+            node_child=torch.full((1,), -1, dtype=torch.int64, device=dev),
+            node_depth=torch.zeros(1, dtype=torch.int64, device=dev),
+            node_q=torch.zeros(1, dtype=torch.float64, device=dev),
+            node_qabs=torch.zeros(1, dtype=torch.float64, device=dev),
+            node_p=torch.zeros((1, 2), dtype=torch.float64, device=dev),
+            node_s=torch.zeros((1, 3), dtype=torch.float64, device=dev),
+            moment_re=torch.zeros((1, p1), dtype=torch.float64, device=dev),
+            moment_im=torch.zeros((1, p1), dtype=torch.float64, device=dev),
+        )
+
+    def _build(self) -> None:
+        """flat_tree.py:532-625 / :483-530: one td3_tree_build call, regrown on overflow."""
+        pos = self._d["pos"]
+        dev = pos.device
+        n = pos.shape[1]
+        capacity = 4 * (2 * n // self.leaf_capacity + 16)
+        xmid, ymid, half = self._root
+        out = np.zeros(3, dtype=np.int64)
+        while True:
+            f64 = dict(dtype=torch.float64, device=dev)
+            i64 = dict(dtype=torch.int64, device=dev)
+            center = torch.empty((capacity, 2), **f64)
+            half_arr = torch.empty(capacity, **f64)
+            start = torch.empty(capacity, **i64)
+            end = torch.empty(capacity, **i64)
+            child = torch.empty(capacity, **i64)
+            depth = torch.empty(capacity, **i64)
+            levels = torch.zeros(67, **i64)
+            perm = torch.empty(n, **i64)
+            sorted_pos = torch.empty((2, n), **f64)
+            info = torch.empty((capacity, 4), dtype=torch.int32, device=dev)
+            nbytes = _abi.lib().td3_tree_workspace_bytes(n, capacity)
+            ws = _abi.workspace(nbytes)
+            code = _abi.lib().td3_tree_build(
+                _abi.ptr(pos), n, xmid, ymid, half, self.leaf_capacity, self.max_depth, capacity,
+                _abi.ptr(center), _abi.ptr(half_arr), _abi.ptr(start), _abi.ptr(end),
+                _abi.ptr(child), _abi.ptr(depth), _abi.ptr(levels), _abi.ptr(perm),
+                _abi.ptr(sorted_pos), _abi.ptr(info), _abi.ptr(ws), nbytes,
+                out.ctypes.data, _abi.stream(),
+            )
+            if code == _abi.TD3_ERR_CAPACITY:
+                capacity = max(2 * capacity, int(out[2]))
+                continue
+            _abi.check(code, "td3_tree_build")
+            break
+        count, n_levels = int(out[0]), int(out[1])
+        self.node_count = count
+        self.levels = levels[: n_levels + 1].cpu().numpy()
+        self._d.update(
+            perm=perm,
+            node_center=center[:count],
+            node_half=half_arr[:count],
+            node_start=start[:count],
+            node_end=end[:count],
+            node_child=child[:count],
+            node_depth=depth[:count],
+        )
+        self._sorted_pos = sorted_pos
+        self._node_info = info[:count]
+        self._depth = n_levels - 1
+
+    def _sorted_charge(self) -> None:
+        n = self._d["pos"].shape[1]
+        out = torch.empty(n, dtype=torch.float64, device=self._d["pos"].device)
+        _abi.call("td3_permute", _abi.ptr(self._d["charge"]), 1, n, _abi.ptr(self._d["perm"]), n,
+                  _abi.ptr(out), _abi.stream())
+        self._scharge = out
+
+    def _moments(self) -> None:
+        """flat_tree.py:663-744: the cells' charges and multipole moments."""
+        dev = self._d["pos"].device
+        c, n = self.node_count, self._d["pos"].shape[1]
+        f64 = dict(dtype=torch.float64, device=dev)
+        node_q = torch.zeros(c, **f64)
+        node_qabs = torch.zeros(c, **f64)
+        node_p = torch.zeros((c, 2), **f64)
+        node_s = torch.zeros((c, 3), **f64)
+        p1 = self.order + 1
+        if self.kernel == "log":
+            mre = torch.empty((c, p1), **f64)
+            mim = torch.empty((c, p1), **f64)
+        else:
+            mre = torch.zeros((1, p1), **f64)
+            mim = torch.zeros((1, p1), **f64)
+        bsz = 2 * self.order + 1
+        binom = _device_table("binomial", _binomial_table, bsz)
+        levels = np.ascontiguousarray(self.levels, dtype=np.int64)
+        _abi.call(
+            "td3_moments", _abi.LOG if self.kernel == "log" else _abi.COULOMB, self.order,
+            _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge), n,
+            _abi.ptr(self._d["node_center"]), _abi.ptr(self._node_info), c, levels.ctypes.data,
+            len(levels) - 1, _abi.ptr(binom), bsz, _abi.ptr(node_q), _abi.ptr(node_qabs),
+            _abi.ptr(node_p), _abi.ptr(node_s), _abi.ptr(mre), _abi.ptr(mim), _abi.stream(),
+        )
+        if self.kernel == "log":
+            node_q.copy_(mre[:, 0])
+            node_p[:, 0] = mre[:, 1]
+            node_p[:, 1] = mim[:, 1]
+        self._d.update(node_q=node_q, node_qabs=node_qabs, node_p=node_p, node_s=node_s,
+                       moment_re=mre, moment_im=mim)
+        self._invalidate_host("node_q", "node_qabs", "node_p", "node_s", "moment_re",
+                              "moment_im")
+
+    def set_charges(self, charges) -> None:
+        """flat_tree.py:853-868: new charges for the same particles; keeps cells and lists."""
+        charges = _device(charges).reshape(-1)
+        if charges.shape[0] != self._d["pos"].shape[1]:
+            raise ValueError("charges must have one entry per particle")
+        self._d["charge"].copy_(charges)
+        self._invalidate_host("charge")
+        if self._d["pos"].shape[1]:
+            self._sorted_charge()
+            self._moments()
+
+    @property
+    def root(self) -> FlatNode:
+        return FlatNode(self, 0)
+
+    def collect_nodes(self) -> list:
+        """Views of every node that holds at least one particle."""
+        occupied = torch.nonzero(self._d["node_end"] > self._d["node_start"]).reshape(-1)
+        return [FlatNode(self, int(i)) for i in occupied.cpu().tolist()]
+
+    # ---------------------------------------------------------------- the walk
+    def _target_order(self, targets: torch.Tensor) -> torch.Tensor:
+        """flat_tree.py:891-912: tree order for the sources themselves, else Morton order."""
+        pos = self._d["pos"]
+        if targets.shape == pos.shape and bool(torch.equal(targets, pos)):
+            return self._d["perm"]
+        m = targets.shape[1]
+        lo = targets.min(dim=1).values
+        span = float((targets.max(dim=1).values - lo).max())
+        scale = (65535.0 / span) if span > 0 else 0.0
+        order = torch.empty(m, dtype=torch.int64, device=targets.device)
+        nbytes = _abi.lib().td3_target_order_workspace_bytes(m)
+        ws = _abi.workspace(nbytes)
+        lox, loy = (float(v) for v in lo.tolist())
+        _abi.call("td3_target_order", _abi.ptr(targets), m, lox, loy, scale, _abi.ptr(order),
+                  _abi.ptr(ws), nbytes, _abi.stream())
+        return order
+
+    def evaluate(self, targets, *, exclude=None, theta=0.5, softening=0.0, potential=True,
+                 precision=None):
+        """Potential ``(M,)`` and field ``(2, M)`` at ``targets`` ``(2, M)``, without ``k_e``.
+
+        flat_tree.py:988-1061: the Barnes-Hut walk.  ``exclude[i]`` is a source index
+        skipped for target ``i`` (-1: none); ``potential=False`` returns the potential as
+        zeros (and skips the line-charge logarithms).
+        """
+        like_torch = _is_torch(targets) or self._like_torch
+        t = _points(targets, "targets")
+        m = t.shape[1]
+        if exclude is None:
+            ex = torch.full((m,), -1, dtype=torch.int64, device=t.device)
+        else:
+            ex = _device(exclude, torch.int64).reshape(-1)
+        phi = torch.zeros(m, dtype=torch.float64, device=t.device)
+        field = torch.zeros((2, m), dtype=torch.float64, device=t.device)
+        n = self._d["pos"].shape[1]
+        if n and m:
+            status = torch.zeros(1, dtype=torch.int64, device=t.device)
+            order = self._target_order(t)
+            d = self._d
+            _abi.call(
+                "td3_walk", _abi.LOG if self.kernel == "log" else _abi.COULOMB,
+                _precision(precision or self.precision), _abi.ptr(t), m, _abi.ptr(order),
+                _abi.ptr(ex), _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
+                _abi.ptr(d["perm"]), n, _abi.ptr(d["node_center"]), _abi.ptr(d["node_half"]),
+                _abi.ptr(self._node_info), _abi.ptr(d["node_qabs"]), _abi.ptr(d["node_q"]),
+                _abi.ptr(d["node_p"]), _abi.ptr(d["node_s"]), _abi.ptr(d["moment_re"]),
+                _abi.ptr(d["moment_im"]), self.order, self._depth, float(theta),
+                float(softening), 1 if potential else 0, _abi.ptr(phi), _abi.ptr(field),
+                _abi.ptr(status), _abi.stream(),
+            )
+            _check(status)
+        if not potential:
+            phi.zero_()
+        return _out(phi, like_torch), _out(field, like_torch)
+
+    def field_at(self, positions, *, theta=0.5, k_e=None, softening=0.0, reference_radius=1.0):
+        """Potential ``(n,)`` and field ``(2, n)`` at probes, with ``k_e`` (flat_tree.py:1648-1676).
+
+        For line charges the potential is ``-2 k_e sum q ln(r / reference_radius)``.
+        """
+        from ._utils import _finite
+        from .constants import K_E
+
+        k_e = K_E if k_e is None else k_e
+        theta = _finite("theta", theta, minimum=0)
+        reference_radius = _finite("reference_radius", reference_radius, positive=True)
+        k_e = _finite("k_e", k_e, minimum=0)
+        softening = _finite("softening", softening, minimum=0)
+        like_torch = _is_torch(positions) or self._like_torch
+        p = _points(positions, "positions")
+        if not bool(torch.isfinite(p).all()):
+            raise ValueError("positions must be finite")
+        if k_e == 0:
+            m = p.shape[1]
+            z = torch.zeros(m, dtype=torch.float64, device=p.device)
+            return _out(z, like_torch), _out(torch.zeros((2, m), dtype=torch.float64,
+                                                         device=p.device), like_torch)
+        phi, field = self.evaluate(p, theta=theta, softening=softening)
+        if self.kernel == "log":
+            phi = phi + 2 * float(self._d["charge"].sum()) * float(np.log(reference_radius))
+        return _out(k_e * phi, like_torch), _out(k_e * field, like_torch)
+
+    # ---------------------------------------------------------------- the FMM
+    def _m2l_coefficients(self) -> torch.Tensor:
+        """The real M2L matrix coeff[k, l] (flat_tree.py:234-245), on the device."""
+        return _device_table("m2l", _m2l_coefficients, self.order, self.symmetric)
+
+    def _lists(self, theta: float):
+        """The M2L and P2P lists of the traversal as rows by node, kept per theta.
+
+        flat_tree.py:1318-1358 (the lists depend on the cells only, so `set_charges`
+        keeps them).
+        """
+        key = ("lists", theta)
+        if key in self._traversals:
+            return self._traversals[key]
+        dev = self._d["pos"].device
+        c = self.node_count
+        caps = [48 * c + 64, 16 * c + 64, 10 * c + 64]
+        out = np.zeros(4, dtype=np.int64)
+        while True:
+            m2l = torch.empty((caps[0], 2), dtype=torch.int32, device=dev)
+            p2p = torch.empty((caps[1], 2), dtype=torch.int32, device=dev)
+            nbytes = _abi.lib().td3_traverse_workspace_bytes(caps[2])
+            ws = _abi.workspace(nbytes)
+            code = _abi.lib().td3_traverse(
+                _abi.ptr(self._d["node_center"]), _abi.ptr(self._d["node_half"]),
+                _abi.ptr(self._node_info), c, self._depth, float(theta), caps[0], caps[1],
+                caps[2], _abi.ptr(m2l), _abi.ptr(p2p), _abi.ptr(ws), nbytes, out.ctypes.data,
+                _abi.stream(),
+            )
+            if code == _abi.TD3_ERR_CAPACITY:
+                caps = [max(2 * caps[i], int(out[i]) + 64) for i in range(3)]
+                continue
+            _abi.check(code, "td3_traverse")
+            break
+        n_m2l, n_p2p = int(out[0]), int(out[1])
+        rows = []
+        for pairs, count, self_once in ((m2l, n_m2l, 0), (p2p, n_p2p, 1)):
+            row = torch.empty(c + 1, dtype=torch.int32, device=dev)
+            col = torch.empty(max(2 * count, 1), dtype=torch.int32, device=dev)
+            dst = torch.empty(max(2 * count, 1), dtype=torch.int32, device=dev)
+            nbytes = _abi.lib().td3_lists_workspace_bytes(c)
+            ws = _abi.workspace(nbytes)
+            _abi.call("td3_lists", _abi.ptr(pairs), count, self_once, c, _abi.ptr(row),
+                      _abi.ptr(col), _abi.ptr(dst), _abi.ptr(ws), nbytes, _abi.stream())
+            rows.append((row, col, dst))
+        lists = (rows[0], rows[1], (n_m2l, n_p2p))
+        self._traversals[key] = lists
+        return lists
+
+    def _scaled_moments(self, precision: int) -> tuple:
+        key = ("scaled", precision)
+        cached = self.__dict__.get("_scaled", {})
+        if key in cached:
+            return cached[key]
+        c, p1 = self.node_count, self.order + 1
+        dtype = torch.float64 if precision == _abi.FP64 else torch.float32
+        dev = self._d["pos"].device
+        re = torch.empty((c, p1), dtype=dtype, device=dev)
+        im = torch.empty((c, p1), dtype=dtype, device=dev)
+        _abi.call("td3_scale_moments", precision, _abi.ptr(self._d["moment_re"]),
+                  _abi.ptr(self._d["moment_im"]), _abi.ptr(self._d["node_half"]), c, self.order,
+                  _abi.ptr(re), _abi.ptr(im), _abi.stream())
+        cached[key] = (re, im)
+        self._scaled = cached
+        return re, im
+
+    #: in mixed precision, M2L into cells at depth <= this stays FP64 (their terms are the
+    #: largest: a cell's net charge grows with its size)
+    FP64_TOP_DEPTH = 5
+
+    def _local_expansions(self, theta: float, precision: int):
+        """Scaled local expansions B_l = b_l r^l of every cell: M2L then L2L."""
+        (mrow, mcol, mdst), _, _ = self._lists(theta)
+        c, p1 = self.node_count, self.order + 1
+        dev = self._d["pos"].device
+        lre = torch.zeros((c, p1), dtype=torch.float64, device=dev)
+        lim = torch.zeros((c, p1), dtype=torch.float64, device=dev)
+        coeff = self._m2l_coefficients()
+        levels = np.ascontiguousarray(self.levels, dtype=np.int64)
+        if precision == _abi.FP64:
+            ranges = [(_abi.FP64, 0, c)]
+        else:
+            split = int(levels[min(self.FP64_TOP_DEPTH + 1, len(levels) - 1)])
+            ranges = [(_abi.FP64, 0, split), (_abi.MIXED, split, c)]
+        for prec, lo, hi in ranges:
+            if hi <= lo:
+                continue
+            are, aim = self._scaled_moments(prec)
+            nbytes = _abi.lib().td3_m2l_workspace_bytes(int(mcol.shape[0]), self.order, prec)
+            ws = _abi.workspace(nbytes)
+            _abi.call("td3_m2l", prec, self.order, _abi.ptr(mcol), _abi.ptr(mdst),
+                      _abi.ptr(mrow), lo, hi, _abi.ptr(self._d["node_center"]),
+                      _abi.ptr(self._d["node_half"]), _abi.ptr(are), _abi.ptr(aim),
+                      _abi.ptr(coeff), _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(ws), nbytes,
+                      _abi.stream())
+        bsz = 2 * self.order + 1
+        binom = _device_table("binomial", _binomial_table, bsz)
+        _abi.call("td3_l2l", self.order, _abi.ptr(self._d["node_center"]),
+                  _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), levels.ctypes.data,
+                  len(levels) - 1, _abi.ptr(binom), bsz, _abi.ptr(lre), _abi.ptr(lim),
+                  _abi.stream())
+        return lre, lim
+
+    def local_expansions(self, theta: float = 0.5, precision=None):
+        """The reference's (unscaled) local expansions ``local_re``, ``local_im`` after L2L."""
+        lre, lim = self._local_expansions(float(theta), _precision(precision or self.precision))
+        _abi.call("td3_unscale_locals", _abi.ptr(self._d["node_half"]), self.node_count,
+                  self.order, _abi.ptr(lre), _abi.ptr(lim), _abi.stream())
+        return _out(lre, self._like_torch), _out(lim, self._like_torch)
+
+    def fmm(self, *, theta: float = 0.5, softening: float = 0.0, potential: bool = True,
+            precision=None):
+        """Potential ``(N,)`` and field ``(2, N)`` at every source by the FMM, without ``k_e``.
+
+        flat_tree.py:1461-1634: dual-tree traversal, M2L both ways, L2L, L2P and exact P2P
+        between near leaves (softened; the far field of the log kernel is unsoftened).
+        Each particle's own charge is excluded.  ``potential=False`` returns zeros for the
+        potential and skips the near-field logarithms.
+        """
+        from ._utils import _finite
+
+        theta = _finite("theta", theta, minimum=0)
+        if not 0 < theta < 1:
+            raise ValueError("FMM theta must lie in (0, 1)")
+        softening = _finite("softening", softening, minimum=0)
+        n = self._d["pos"].shape[1]
+        dev = self._d["pos"].device
+        phi = torch.zeros(n, dtype=torch.float64, device=dev)
+        field = torch.zeros((2, n), dtype=torch.float64, device=dev)
+        if not n:
+            return _out(phi, self._like_torch), _out(field, self._like_torch)
+        if self.kernel != "log":
+            return self._coulomb_fmm(theta, softening, potential, precision)
+        prec = _precision(precision or self.precision)
+        lre, lim = self._local_expansions(theta, prec)
+        _, (prow, pcol, _), counts = self._lists(theta)
+        status = torch.zeros(1, dtype=torch.int64, device=dev)
+        _abi.call(
+            "td3_fmm_eval", prec, self.order, _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
+            _abi.ptr(self._d["perm"]), n, _abi.ptr(self._d["node_center"]),
+            _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), self.node_count,
+            _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol), softening,
+            1 if potential else 0, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status),
+            _abi.stream(),
+        )
+        _check(status)
+        self.fmm_pairs = counts
+        return _out(phi, self._like_torch), _out(field, self._like_torch)
+
+    def _coulomb_fmm(self, theta, softening, potential, precision):
+        raise NotImplementedError("the Coulomb (Cartesian) FMM is not built yet on the GPU")
+
+
+def _m2l_coefficients(order: int, symmetric: bool) -> np.ndarray:
+    """coeff[k, l] of the M2L (flat_tree.py:234-245): coeff[0, l] = 1/l, coeff[k, 0] = 1/k,
+    coeff[k, l] = C(k + l - 1, l) / k, zero beyond the truncation (k <= p, or k <= p - l
+    when symmetric)."""
+    binomial = _binomial_table(2 * order + 1)
+    coeff = np.zeros((order + 1, order + 1))
+    for m in range(1, order + 1):
+        coeff[0, m] = 1.0 / m
+    for k in range(1, order + 1):
+        coeff[k, 0] = 1.0 / k
+        for m in range(1, order + 1):
+            if k <= (order - m if symmetric else order):
+                coeff[k, m] = binomial[k + m - 1, m] / k
+    return coeff
