@@ -1,0 +1,207 @@
+"""Benchmark of the field-evaluation hot path on one MI355X.
+
+Workload: the log (line-charge) FMM at N = 1,048,576 -- half the particles uniform on
+[-1, 1]^2, half in 16 Gaussian clusters (sigma 0.02, centres uniform on [-0.8, 0.8]^2),
+charges +-1, order 12, theta 0.5, softening 1e-4, leaf capacity 8 -- with inputs resident
+on the GPU.  One step = FlatQuadtree(...) (tree build + moments) + .fmm() (traversal, M2L,
+L2L, L2P + P2P): the whole job for one set of positions.  The mixed-precision step is the
+metric; the all-FP64 step is timed beside it, and the three errors against an FP64 direct
+sum are reported on 4096 sampled targets.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def make_inputs(n: int, seed: int = 7):
+    """The benchmark's synthetic particles (positions (2, n), charges (n,))."""
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    centres = rng.uniform(-0.8, 0.8, (2, 16))
+    per = (n - half) // 16
+    clusters = (centres[:, :, None] + rng.normal(0.0, 0.02, (2, 16, per))).reshape(2, -1)
+    rest = n - half - 16 * per
+    pos = np.concatenate((rng.uniform(-1, 1, (2, half + rest)), clusters), axis=1).clip(-1, 1)
+    q = rng.choice([-1.0, 1.0], n)
+    return np.ascontiguousarray(pos), q
+
+
+def relative(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def cpu_baseline(n_sample: int, seconds: float):
+    """The CPU oracle (a C/OpenMP restatement of the reference's Pyccel kernels) on a
+    bounded sample of the same workload: build + moments + FMM, all host cores."""
+    from oracle import oracle
+
+    pos, q = make_inputs(n_sample, seed=11)
+    runs, t_total = 0, 0.0
+    while t_total < seconds or runs < 1:
+        t0 = time.perf_counter()
+        tree = oracle.OracleTree(pos, q, (0.0, 0.0, 1.0), kernel="log", order=12)
+        tree.fmm(theta=0.5, softening=1e-4)
+        t_total += time.perf_counter() - t0
+        runs += 1
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": n_sample * runs / t_total,
+        "unit": "targets/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"log FMM N={n_sample} (same distribution, seed 11), order 12, theta 0.5, "
+        f"eps 1e-4, leaf 8: build + moments + traversal + M2L + L2L + L2P/P2P, {runs} runs, "
+        f"{t_total:.1f} s, OpenMP threads={cores}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import torch
+
+    td3 = __graft_entry__.load_package()
+    from tree_d_3d_amd import _abi
+
+    n = args.n
+    pos_h, q_h = make_inputs(n)
+    pos = torch.from_numpy(pos_h).cuda()
+    q = torch.from_numpy(q_h).cuda()
+    quad = td3.Quad(0.0, 0.0, 1.0)
+    params = dict(theta=0.5, softening=1e-4)
+
+    def step(precision):
+        tree = td3.FlatQuadtree(pos, q, quad, kernel="log", order=12, precision=precision)
+        phi, field = tree.fmm(**params)
+        return tree, phi, field
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup):
+            step(precision)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = step(precision)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, out
+
+    t_mixed, (tree, phi_m, field_m) = timed("mixed", args.steps, args.warmup)
+    result = {
+        "metric": "targets/s at N=1M FMM: mixed vs FP64 on GPU at <=1e-6 rel err vs direct",
+        "value": n / t_mixed,
+        "unit": "targets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (M2L below depth 5 and P2P; tree, moments, top M2L, L2L, L2P and sums f64)",
+        "data": "synthetic",
+        "config": {
+            "workload": "log-kernel FMM, build + moments + traversal + M2L + L2L + L2P/P2P",
+            "n": n,
+            "distribution": "half uniform [-1,1]^2, half 16 Gaussian clusters sigma 0.02",
+            "order": 12, "theta": 0.5, "softening": 1e-4, "leaf_capacity": 8,
+            "node_count": tree.node_count, "fmm_pairs": list(tree.fmm_pairs),
+            "levels": int(len(tree.levels) - 1),
+        },
+    }
+    if not args.no_fp64:
+        t_fp64, (_, phi_d, field_d) = timed("fp64", args.steps, args.warmup)
+        result["fp64"] = {"ms_per_step": 1e3 * t_fp64, "value": n / t_fp64,
+                          "mixed_speedup": t_fp64 / t_mixed}
+        # errors on 4096 sampled targets against the FP64 direct sum (GPU, FP64)
+        idx = np.sort(np.random.default_rng(0).choice(n, 4096, replace=False))
+        tidx = torch.from_numpy(idx).cuda()
+        dphi, dfield = td3.direct_field(pos[:, tidx], pos, q, exclude=tidx, kernel="log",
+                                        softening=1e-4)
+        dphi, dfield = dphi.cpu().numpy(), dfield.cpu().numpy()
+        pm, fm = phi_m[tidx].cpu().numpy(), field_m[:, tidx].cpu().numpy()
+        pd, fd = phi_d[tidx].cpu().numpy(), field_d[:, tidx].cpu().numpy()
+        result["accuracy"] = {
+            "metric": "max-norm relative error max|a-b|/max|b| on 4096 sampled targets",
+            "mixed_vs_fp64_fmm": {"phi": relative(pm, pd), "field": relative(fm, fd)},
+            "fp64_fmm_vs_direct": {"phi": relative(pd, dphi), "field": relative(fd, dfield)},
+            "mixed_vs_direct": {"phi": relative(pm, dphi), "field": relative(fm, dfield)},
+            "note": "the FMM softens only near pairs (reference semantics); far cells are "
+                    "unsoftened, so FMM-vs-softened-direct carries an O(eps^2/d^2) model error",
+        }
+
+    # per-kernel times of one more mixed step (HIP events on the launch stream)
+    _abi.timing_enable(True)
+    tree, _, _ = step("mixed")
+    torch.cuda.synchronize()
+    kernels = {}
+    for name in ("m2l", "m2l_fixup", "eval", "l2l", "traverse", "row_sort", "lists_fill",
+                 "lists_degree", "sort_scatter", "sort_hist", "scan", "tree_keys", "tree_count",
+                 "tree_nodes", "tree_leaf_keys", "tree_gather", "moments_p2m", "moments_m2m",
+                 "scale_moments"):
+        ms, count = _abi.timing_get(name)
+        if count:
+            kernels[name] = {"ms": ms, "launches": count}
+    _abi.timing_enable(False)
+    result["kernels_ms"] = kernels
+    # roofline of the dominant kernel: algorithmic flops per launch / average launch time
+    (mrow, mcol, _), (prow, pcol, _), counts = tree._lists(0.5)
+    node_n = (tree._d["node_end"] - tree._d["node_start"]).cpu().numpy()
+    prow_h = prow.cpu().numpy().astype(np.int64)
+    pcol_h = pcol.cpu().numpy()[: prow_h[-1]].astype(np.int64)
+    dst = np.repeat(np.arange(len(prow_h) - 1), np.diff(prow_h))
+    near_pairs = int(np.sum(node_n[dst] * node_n[pcol_h]) - np.sum(node_n[node_n > 0]))
+    m2l_directed = 2 * counts[0]
+    flops = {"m2l": 910.0 * m2l_directed, "eval": 17.0 * near_pairs + 16 * 12 * n}
+    dom = max(("m2l", "eval"), key=lambda k: kernels.get(k, {"ms": 0})["ms"])
+    ms = kernels[dom]["ms"] / kernels[dom]["launches"]
+    launches = kernels[dom]["launches"]
+    achieved = flops[dom] / launches / (ms * 1e-3) / 1e12
+    result["roofline"] = {
+        "kernel": dom,
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP32_PEAK_TFLOPS,
+        "traffic": None,
+        "algorithmic": {"m2l_directed_pairs": m2l_directed, "near_directed_pairs": near_pairs,
+                        "flops_per_m2l_pair": 910, "flops_per_near_pair": 17},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(200_000, args.cpu_seconds)
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -151,13 +151,13 @@ int td3_traverse(const double *node_center, const double *node_half, const void 
                  size_t workspace_bytes, int64_t *out_counts, void *stream);
 
 /*
- * Rows by node of a pair list (kernels/pair_csr/pair_csr_pyccel.py:12-45, mutual): each
- * pair (a, b) lists b under a and a under b (a self pair once when self_once).  row_start
- * (node_count + 1) uint32, col (2 count) int32 sorted ascending within each row, dst (may be
- * NULL) the row of each entry.
+ * Rows by node of a pair list (kernels/pair_csr/pair_csr_pyccel.py:12-45): mode 0 lists
+ * b under a and a under b; mode 1 the same with a self pair once (the P2P leaves); mode 2
+ * one way, b under a (fmm_at's target rows).  row_start (node_count + 1) uint32, col
+ * int32 sorted ascending within each row, dst (may be NULL) the row of each entry.
  */
 size_t td3_lists_workspace_bytes(int64_t node_count);
-int td3_lists(const int32_t *pairs, int64_t count, int self_once, int64_t node_count,
+int td3_lists(const int32_t *pairs, int64_t count, int mode, int64_t node_count,
               uint32_t *row_start, int32_t *col, int32_t *dst, void *workspace,
               size_t workspace_bytes, void *stream);
 
@@ -176,15 +176,17 @@ int td3_scale_moments(int precision, const double *moment_re, const double *mome
  * s_l = sum_k coeff[k, l] a_k w^k, w = 1/t, t = c_dst - c_src (ln t principal).  col/dst/
  * row_start are td3_lists rows of the M2L pairs; coeff (order+1)^2 device table
  * (flat_tree.py:234-245, either truncation); scaled_re/im from td3_scale_moments of the
- * same precision.  Rows of the range are WRITTEN (not accumulated); rows outside it are
+ * same precision; dst_* are the destination tree's cells (the rows), src_* the source
+ * tree's (the columns; the same arrays for fmm, the target tree for fmm_at).  Rows of the range are WRITTEN (not accumulated); rows outside it are
  * untouched.  Reads two row offsets back (synchronises the stream).
  */
 size_t td3_m2l_workspace_bytes(int64_t pair_count, int64_t order, int precision);
 int td3_m2l(int precision, int64_t order, const int32_t *col, const int32_t *dst,
             const uint32_t *row_start, int64_t node_lo, int64_t node_hi,
-            const double *node_center, const double *node_half, const void *scaled_re,
-            const void *scaled_im, const double *coeff, double *local_re, double *local_im,
-            void *workspace, size_t workspace_bytes, void *stream);
+            const double *dst_center, const double *dst_half, const double *src_center,
+            const double *src_half, const void *scaled_re, const void *scaled_im,
+            const double *coeff, double *local_re, double *local_im, void *workspace,
+            size_t workspace_bytes, void *stream);
 
 /*
  * L2L down the tree on scaled locals, level by level from the root (flat_tree.py:1540-1552,
@@ -212,6 +214,46 @@ int td3_fmm_eval(int precision, int64_t order, const double *sorted_pos,
                  int64_t node_count, const double *local_re, const double *local_im,
                  const uint32_t *p2p_row, const int32_t *p2p_col, double softening,
                  int with_potential, double *phi, double *field, int64_t *status, void *stream);
+
+/*
+ * Dual traversal of a target tree against a source tree (fmm_at's lists): replaces
+ * flat_tree.py:1086-1122 (_dual_lists) and kernels/dual_traverse/dual_traverse_pyccel.py:14-86
+ * (same pair sets; the reference walks depth first on the CPU).  m2l and p2p are (target
+ * cell, source cell) pairs, one way; out_counts as td3_traverse.
+ */
+int td3_dual_traverse(const double *t_center, const double *t_half, const void *t_info,
+                      int64_t t_count, int64_t t_depth, const double *s_center,
+                      const double *s_half, const void *s_info, int64_t s_count, int64_t s_depth,
+                      double theta, int64_t cap_m2l, int64_t cap_p2p, int64_t cap_front,
+                      int32_t *m2l, int32_t *p2p, void *workspace, size_t workspace_bytes,
+                      int64_t *out_counts, void *stream);
+
+/*
+ * fmm_at's evaluation (flat_tree.py:1267-1316, kernels/log_targets_host/log_targets_host_pyccel.py:14-94):
+ * per target leaf of the target tree, L2P of its scaled local expansion plus exact sums
+ * over its P2P source leaves (td3_lists mode 2), skipping uncharged sources and source
+ * exclude[i]; a coincident charge reports the target.  sorted_targets (2, M) and
+ * target_perm describe the target tree; sorted_pos / sorted_charge / perm / s_info the
+ * sources.  phi (M,), field (2, M) written in target order.
+ */
+int td3_fmm_at_eval(int precision, int64_t order, const double *sorted_targets,
+                    const int64_t *target_perm, const int64_t *exclude, int64_t m,
+                    const double *t_center, const double *t_half, const void *t_info,
+                    int64_t t_count, const double *local_re, const double *local_im,
+                    const uint32_t *p2p_row, const int32_t *p2p_col, const double *sorted_pos,
+                    const double *sorted_charge, const int64_t *perm, int64_t n,
+                    const void *s_info, double softening, int with_potential, double *phi,
+                    double *field, int64_t *status, void *stream);
+
+/*
+ * Kernel timing for measurement (no counterpart in the reference, whose profiling is
+ * host-side regions): td3_timing_enable(1) clears the record and brackets every later
+ * kernel launch with HIP events on its stream; td3_timing_get(name) synchronises and
+ * returns the summed elapsed milliseconds and launch count of the kernels of that name
+ * ("m2l", "eval", "traverse", "sort_scatter", ...).  Costs one branch per launch when off.
+ */
+int td3_timing_enable(int on);
+int td3_timing_get(const char *name, double *total_ms, int64_t *launches);
 
 #ifdef __cplusplus
 }

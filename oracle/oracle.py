@@ -69,6 +69,10 @@ def lib():
             "oracle_fmm_downward": [_I, _I, _P, _P, _P, _I, _I, _P, _P],
             "oracle_fmm_evaluate_host": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _I, _D, _C,
                                          _P, _P, _I, _P],
+            "oracle_dual_traverse": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _I, _P, _I, _P,
+                                     _I, _P],
+            "oracle_log_targets_host": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _I,
+                                        _P, _P, _P, _P, _D, _C, _P, _P, _I],
         }
         for name, args in sig.items():
             fn = getattr(_LIB, name)
@@ -412,3 +416,87 @@ class OracleTree:
         if not potential:
             phi[:] = 0.0
         return phi, field
+
+
+def _fmm_at_log(self, targets, *, exclude=None, theta=0.5, softening=0.0, potential=True,
+                leaf_capacity=None):
+    """flat_tree.py:1152-1316 for the log kernel: target tree, dual traversal, M2L into the
+    target cells, L2L down the target tree, log_targets_host."""
+    if not 0 < theta < 1:
+        raise ValueError("FMM theta must lie in (0, 1)")
+    targets = _points(targets, "targets")
+    if not np.all(np.isfinite(targets)):
+        raise ValueError("targets must be finite")
+    m, n = targets.shape[1], self.pos.shape[1]
+    phi, field = np.zeros(m), np.zeros((2, m))
+    if not m or not n:
+        return phi, field
+    exclude = np.full(m, -1, dtype=np.int64) if exclude is None else _i64(exclude).reshape(-1)
+    leaf = self.leaf_capacity if leaf_capacity is None else int(leaf_capacity)
+    lo, hi = targets.min(axis=1), targets.max(axis=1)
+    centre = 0.5 * (lo + hi)
+    half = 0.5 * float(np.max(hi - lo)) * (1 + 1e-12) + 1e-300
+    tree = OracleTree(targets, np.zeros(m), (float(centre[0]), float(centre[1]), half),
+                      leaf_capacity=leaf, max_depth=self.max_depth, kernel="coulomb")
+    depth = int(tree.node_depth.max()) + int(self.node_depth.max())
+    cap_stack = 3 * depth + 8
+    capacity = np.array([16, 8]) * (tree.node_count + self.node_count) + 64
+    counts = np.zeros(3, dtype=np.int64)
+    while True:
+        stack = np.empty((cap_stack, 2), dtype=np.int64)
+        m2l = np.empty((capacity[0], 2), dtype=np.int64)
+        p2p = np.empty((capacity[1], 2), dtype=np.int64)
+        lib().oracle_dual_traverse(
+            _ptr(tree.node_center), _ptr(tree.node_half), _ptr(tree.node_start),
+            _ptr(tree.node_end), _ptr(tree.node_child), _ptr(self.node_center),
+            _ptr(self.node_half), _ptr(self.node_start), _ptr(self.node_end),
+            _ptr(self.node_child), float(theta), _ptr(stack), cap_stack, _ptr(m2l), capacity[0],
+            _ptr(p2p), capacity[1], _ptr(counts))
+        if not counts[2]:
+            break
+        capacity = np.maximum(2 * capacity, counts[:2] + 64)
+    m2l, p2p = m2l[: counts[0]].copy(), p2p[: counts[1]].copy()
+    self.dual_lists = (m2l, p2p)
+    ct, order = tree.node_count, self.order
+    rows = []
+    for pairs in (m2l, p2p):
+        start = np.zeros(ct + 1, dtype=np.int64)
+        partners = np.empty(max(len(pairs), 1), dtype=np.int64)
+        lib().oracle_pair_csr(_ptr(_i64(pairs) if len(pairs) else np.zeros((1, 2), np.int64)),
+                              len(pairs), 0, ct, _ptr(start), _ptr(partners))
+        rows.append((start, partners))
+    p1 = order + 1
+    local_re = np.zeros((ct, p1))
+    local_im = np.zeros((ct, p1))
+    if len(m2l):
+        coeff = m2l_coefficients(order, self.symmetric)
+        lib().oracle_fmm_m2l_host(_ptr(rows[0][0]), _ptr(rows[0][1]), ct, _ptr(tree.node_center),
+                                  _ptr(self.node_center), _ptr(self.moment_re),
+                                  _ptr(self.moment_im), _ptr(coeff), order, _ptr(local_re),
+                                  _ptr(local_im))
+    bsz = 2 * order + 1
+    binom = binomial_table(bsz)
+    for a, b in zip(tree.levels[:-2], tree.levels[1:-1]):
+        lib().oracle_fmm_downward(int(a), int(b), _ptr(tree.node_center), _ptr(tree.node_child),
+                                  _ptr(binom), bsz, order, _ptr(local_re), _ptr(local_im))
+    tpos, _ = tree._sorted_sources()
+    texclude = _i64(exclude[tree.perm])
+    spos, scharge = self._sorted_sources()
+    phi_s = np.empty(m)
+    field_s = np.empty((2, m + 1))
+    lib().oracle_log_targets_host(
+        _ptr(tpos), m + 1, _ptr(texclude), ct, _ptr(tree.node_center), _ptr(tree.node_start),
+        _ptr(tree.node_end), _ptr(tree.node_child), _ptr(local_re), _ptr(local_im), order,
+        _ptr(rows[1][0]), _ptr(rows[1][1]), _ptr(spos), n + 1, _ptr(scharge), _ptr(self.perm),
+        _ptr(self.node_start), _ptr(self.node_end), float(softening), 1 if potential else 0,
+        _ptr(phi_s), _ptr(field_s), m + 1)
+    phi[tree.perm] = phi_s
+    field[:, tree.perm] = field_s[:, :m]
+    if np.any(phi == 1.0e300):
+        raise _coincident()
+    if not potential:
+        phi[:] = 0.0
+    return phi, field
+
+
+OracleTree.fmm_at = _fmm_at_log

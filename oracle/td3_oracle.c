@@ -835,3 +835,121 @@ void oracle_fmm_evaluate_host(const double *spos, i64 lds, const double *scharge
         }
     }
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* dual_traverse/dual_traverse_pyccel.py:14-86: depth-first target x source traversal,  */
+/* one way.  stack (cap_stack, 2); counts[3] = {m2l, p2p, overflow}.                     */
+/* ------------------------------------------------------------------------------------ */
+void oracle_dual_traverse(const double *t_center, const double *t_half, const i64 *t_start,
+                          const i64 *t_end, const i64 *t_child, const double *s_center,
+                          const double *s_half, const i64 *s_start, const i64 *s_end,
+                          const i64 *s_child, double theta, i64 *stack, i64 cap_stack, i64 *m2l,
+                          i64 cap_m2l, i64 *p2p, i64 cap_p2p, i64 *counts) {
+    const double root2 = sqrt(2.0);
+    counts[0] = counts[1] = counts[2] = 0;
+    stack[0] = 0;
+    stack[1] = 0;
+    i64 top = 1;
+    while (top > 0) {
+        top -= 1;
+        const i64 a = stack[2 * top], b = stack[2 * top + 1];
+        if (t_end[a] == t_start[a] || s_end[b] == s_start[b]) continue;
+        const double dx = t_center[2 * a] - s_center[2 * b];
+        const double dy = t_center[2 * a + 1] - s_center[2 * b + 1];
+        if (root2 * (t_half[a] + s_half[b]) < theta * sqrt(dx * dx + dy * dy)) {
+            const i64 i = counts[0]++;
+            if (i < cap_m2l) {
+                m2l[2 * i] = a;
+                m2l[2 * i + 1] = b;
+            } else {
+                counts[2] = 1;
+            }
+            continue;
+        }
+        const int leaf_a = t_child[a] < 0, leaf_b = s_child[b] < 0;
+        if (leaf_a && leaf_b) {
+            const i64 i = counts[1]++;
+            if (i < cap_p2p) {
+                p2p[2 * i] = a;
+                p2p[2 * i + 1] = b;
+            } else {
+                counts[2] = 1;
+            }
+            continue;
+        }
+        if (top + 4 > cap_stack) {
+            counts[2] = 1;
+            return;
+        }
+        if (leaf_b || (!leaf_a && t_half[a] >= s_half[b])) {
+            for (int c = 0; c < 4; ++c) {
+                stack[2 * (top + c)] = t_child[a] + c;
+                stack[2 * (top + c) + 1] = b;
+            }
+        } else {
+            for (int c = 0; c < 4; ++c) {
+                stack[2 * (top + c)] = a;
+                stack[2 * (top + c) + 1] = s_child[b] + c;
+            }
+        }
+        top += 4;
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* log_targets_host/log_targets_host_pyccel.py:14-94: per target leaf, L2P + exact sums  */
+/* over its source leaves.  tpos (2, ldt), spos (2, lds); field (2, ldf).                 */
+/* ------------------------------------------------------------------------------------ */
+void oracle_log_targets_host(const double *tpos, i64 ldt, const i64 *texclude, i64 t_count,
+                             const double *t_center, const i64 *t_start, const i64 *t_end,
+                             const i64 *t_child, const double *local_re, const double *local_im,
+                             i64 order, const i64 *row_start, const i64 *partners,
+                             const double *spos, i64 lds, const double *scharge, const i64 *sindex,
+                             const i64 *s_start, const i64 *s_end, double softening,
+                             int with_potential, double *phi, double *field, i64 ldf) {
+    const i64 p1 = order + 1;
+    const double eps2 = softening * softening;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (i64 node = 0; node < t_count; ++node) {
+        if (t_child[node] >= 0) continue;
+        const double *bre = local_re + node * p1, *bim = local_im + node * p1;
+        for (i64 k = t_start[node]; k < t_end[node]; ++k) {
+            const double u_re = tpos[k] - t_center[2 * node];
+            const double u_im = tpos[ldt + k] - t_center[2 * node + 1];
+            double w_re = bre[order], w_im = bim[order], d_re = 0.0, d_im = 0.0;
+            for (i64 l = order - 1; l >= 0; --l) {
+                const double nd_re = d_re * u_re - d_im * u_im + w_re;
+                d_im = d_re * u_im + d_im * u_re + w_im;
+                d_re = nd_re;
+                const double nw_re = w_re * u_re - w_im * u_im + bre[l];
+                w_im = w_re * u_im + w_im * u_re + bim[l];
+                w_re = nw_re;
+            }
+            double pot = -2.0 * w_re, ex = 2.0 * d_re, ey = -2.0 * d_im;
+            int bad = 0;
+            const i64 skip = texclude[k];
+            for (i64 e = row_start[node]; e < row_start[node + 1]; ++e) {
+                const i64 src = partners[e];
+                for (i64 j = s_start[src]; j < s_end[src]; ++j) {
+                    const double c = scharge[j];
+                    if (c == 0.0 || sindex[j] == skip) continue;
+                    const double dx = tpos[k] - spos[j], dy = tpos[ldt + k] - spos[lds + j];
+                    const double r2 = dx * dx + dy * dy + eps2;
+                    if (r2 == 0.0) {
+                        bad = 1;
+                        continue;
+                    }
+                    if (with_potential) pot -= c * log(r2);
+                    ex += 2.0 * c * dx / r2;
+                    ey += 2.0 * c * dy / r2;
+                }
+            }
+            if (bad)
+                phi[k] = TD3_BAD;
+            else
+                phi[k] = with_potential ? pot : 0.0;
+            field[k] = ex;
+            field[ldf + k] = ey;
+        }
+    }
+}

@@ -146,3 +146,52 @@ def test_fmm_is_deterministic(td3):
     b = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log").fmm(softening=1e-4)
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fmm_at_matches_oracle(td3, oracle, precision):
+    rng = np.random.default_rng(13)
+    pos, q = cloud(4000, seed=13)
+    targets = np.concatenate((rng.uniform(-0.9, 0.9, (2, 1500)), pos[:, :500]), axis=1)
+    exclude = np.concatenate((np.full(1500, -1), np.arange(500)))
+    print(f"fmm_at precision={precision}", flush=True)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=16, leaf_capacity=16)
+    ref = oracle.OracleTree(pos, q, (0, 0, 1), kernel="log", order=16, leaf_capacity=16)
+    phi, field = tree.fmm_at(targets, exclude=exclude, theta=0.5, precision=precision)
+    rphi, rfield = ref.fmm_at(targets, exclude=exclude, theta=0.5)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert relative(phi, rphi) < tol and relative(field, rfield) < tol
+    assert tree.fmm_at_pairs == (len(ref.dual_lists[0]), len(ref.dual_lists[1]))
+    nophi, same = tree.fmm_at(targets, exclude=exclude, theta=0.5, potential=False,
+                              precision=precision)
+    assert not np.any(nophi)
+    np.testing.assert_allclose(same, field, rtol=1e-12, atol=1e-12)
+
+
+def test_fmm_at_plan_kept_across_new_charges(td3):
+    # test_fmm.py:252-270
+    rng = np.random.default_rng(14)
+    pos = rng.uniform(-0.9, 0.9, (2, 800))
+    targets = rng.uniform(-0.9, 0.9, (2, 300))
+    q, other = rng.normal(size=800), rng.normal(size=800)
+    other[:400] = 0.0
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", leaf_capacity=8)
+    tree.fmm_at(targets)
+    tree.set_charges(other)
+    fresh = td3.FlatQuadtree(pos, other, td3.Quad(0, 0, 1), kernel="log", leaf_capacity=8)
+    np.testing.assert_allclose(tree.fmm_at(targets)[1], fresh.fmm_at(targets)[1], rtol=1e-12,
+                               atol=1e-12)
+
+
+def test_fmm_at_validation(td3):
+    # test_fmm.py:272-282
+    pos = np.array([[0.1, 0.5], [0.2, -0.3]])
+    tree = td3.FlatQuadtree(pos, np.ones(2), td3.Quad(0, 0, 1), kernel="log")
+    with pytest.raises(ValueError, match="Coincident"):
+        tree.fmm_at(pos[:, :1])
+    phi, field = tree.fmm_at(pos[:, :1], exclude=[0])
+    _, ref = td3.direct_field(pos[:, :1], pos, np.ones(2), exclude=[0], kernel="log")
+    np.testing.assert_allclose(field, ref, rtol=1e-12)
+    with pytest.raises(ValueError, match="theta"):
+        tree.fmm_at(pos, theta=1.0)
+    assert tree.fmm_at(np.zeros((2, 0)))[1].shape == (2, 0)

@@ -53,11 +53,18 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_lists": ([_P, _I, _C, _I, _P, _P, _P, _P, _Z, _P], _C),
     "td3_scale_moments": ([_C, _P, _P, _P, _I, _I, _P, _P, _P], _C),
     "td3_m2l_workspace_bytes": ([_I, _I, _C], _Z),
-    "td3_m2l": ([_C, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P], _C),
+    "td3_m2l": ([_C, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P], _C),
     "td3_l2l": ([_I, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P], _C),
     "td3_unscale_locals": ([_P, _I, _I, _P, _P, _P], _C),
     "td3_fmm_eval": ([_C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _D, _C, _P, _P, _P,
                       _P], _C),
+    "td3_dual_traverse": ([_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P, _Z,
+                           _P, _P], _C),
+    "td3_fmm_at_eval": ([_C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _I,
+                         _P, _D, _C, _P, _P, _P, _P], _C),
+    "td3_timing_enable": ([_C], _C),
+    "td3_timing_get": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                        ctypes.POINTER(ctypes.c_int64)], _C),
 }
 
 _LIB = None
@@ -132,3 +139,17 @@ def ptr(t: torch.Tensor | None) -> int | None:
 def workspace(nbytes: int) -> torch.Tensor:
     """A scratch buffer from torch's caching allocator (the library never allocates)."""
     return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=device())
+
+
+def timing_enable(on: bool) -> None:
+    """Start (clearing) or stop the per-kernel event timing of the library."""
+    check(lib().td3_timing_enable(1 if on else 0), "td3_timing_enable")
+
+
+def timing_get(name: str) -> tuple[float, int]:
+    """(summed milliseconds, launches) of the kernels called ``name`` since timing_enable."""
+    ms = ctypes.c_double()
+    count = ctypes.c_int64()
+    check(lib().td3_timing_get(name.encode(), ctypes.byref(ms), ctypes.byref(count)),
+          "td3_timing_get")
+    return ms.value, count.value

@@ -15,6 +15,17 @@ typedef uint32_t u32;
 // Error state of the library (host side, one string per thread).
 void td3_set_error(const char* fmt, ...);
 
+// Kernel timing (host side): TD3_TIMED(stream, "name", launch) brackets a launch with
+// HIP events when timing is enabled (td3_timing_enable), and costs one branch otherwise.
+void* td3_timing_begin(hipStream_t stream);
+void td3_timing_end(void* token, const char* name, hipStream_t stream);
+#define TD3_TIMED(stream, name, ...)                    \
+    do {                                                \
+        void* _tok = td3_timing_begin(stream);          \
+        __VA_ARGS__;                                    \
+        td3_timing_end(_tok, name, stream);             \
+    } while (0)
+
 #define TD3_HIP(call)                                                                   \
     do {                                                                                \
         hipError_t _e = (call);                                                         \

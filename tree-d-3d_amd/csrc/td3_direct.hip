@@ -127,8 +127,8 @@ template <int KERNEL, int PREC>
 void launch_direct(dim3 grid, hipStream_t s, const double* targets, i64 m, const i64* exclude,
                    const double* pos, i64 n, const double* charge, double eps2, i64 chunk,
                    double* out_phi, double* out_field, i64 ld, i64* status) {
-    hipLaunchKernelGGL((direct_tile_kernel<KERNEL, PREC>), grid, dim3(kThreads), 0, s, targets, m,
-                       exclude, pos, n, charge, eps2, chunk, out_phi, out_field, ld, status);
+    TD3_TIMED(s, "direct", hipLaunchKernelGGL((direct_tile_kernel<KERNEL, PREC>), grid, dim3(kThreads), 0, s, targets, m,
+                       exclude, pos, n, charge, eps2, chunk, out_phi, out_field, ld, status));
 }
 
 }  // namespace
@@ -186,8 +186,8 @@ extern "C" int td3_direct(int kernel, int precision, const double* targets, int6
     }
     TD3_LAUNCHED();
     if (chunks > 1) {
-        hipLaunchKernelGGL(direct_reduce_kernel, dim3(td3_blocks(m, 256)), dim3(256), 0, s, out_phi,
-                           out_field, m, chunks, phi, field);
+        TD3_TIMED(s, "direct_reduce", hipLaunchKernelGGL(direct_reduce_kernel, dim3(td3_blocks(m, 256)), dim3(256), 0, s, out_phi,
+                           out_field, m, chunks, phi, field));
         TD3_LAUNCHED();
     }
     return TD3_OK;

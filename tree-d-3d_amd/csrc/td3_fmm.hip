@@ -39,7 +39,8 @@ __device__ __forceinline__ float shfl_up_d(float v, int off) { return __shfl_up(
 template <int P, typename T>
 __global__ __launch_bounds__(256) void m2l_kernel(
     const int* __restrict__ col, const int* __restrict__ dst, const u32* __restrict__ row_start,
-    u32 e_lo, u32 e_hi, const double* __restrict__ center, const double* __restrict__ half,
+    u32 e_lo, u32 e_hi, const double* __restrict__ dcenter, const double* __restrict__ dhalf,
+    const double* __restrict__ scenter, const double* __restrict__ shalf,
     const T* __restrict__ mre, const T* __restrict__ mim, int p1, const double* __restrict__ coeff,
     double* __restrict__ lre, double* __restrict__ lim, T* __restrict__ side) {
     __shared__ T s_coeff[(P + 1) * (P + 1)];
@@ -60,12 +61,12 @@ __global__ __launch_bounds__(256) void m2l_kernel(
     for (int l = 0; l <= P; ++l) b[l] = {(T)0, (T)0};
     if (valid) {
         const int src = col[e];
-        const double tr = center[2 * d] - center[2 * src];
-        const double ti = center[2 * d + 1] - center[2 * src + 1];
+        const double tr = dcenter[2 * d] - scenter[2 * src];
+        const double ti = dcenter[2 * d + 1] - scenter[2 * src + 1];
         const double t2 = tr * tr + ti * ti;
         const double wr = tr / t2, wi = -ti / t2;
-        const double rs = 1.4142135623730951 * half[src];
-        const double rt = 1.4142135623730951 * half[d];
+        const double rs = 1.4142135623730951 * shalf[src];
+        const double rt = 1.4142135623730951 * dhalf[d];
         const Cx<T> u = {(T)(rs * wr), (T)(rs * wi)};
         const Cx<T> v = {(T)(rt * wr), (T)(rt * wi)};
         // A_k u^k
@@ -249,33 +250,42 @@ __global__ void l2l_kernel(const double* __restrict__ center, const double* __re
 // (kChunk at a time); a wave reduction finishes each target.
 constexpr int kChunk = 8;
 
-template <int PREC>
+// MODE 0 (fmm): targets are the sources themselves, in tree order; a particle's own slot is
+// skipped and a coincident pair is an error only when both are charged
+// (fmm_evaluate_host_pyccel.py:64-100).  MODE 1 (fmm_at): targets of their own tree;
+// uncharged sources and source exclude[i] are skipped; any remaining coincidence is an
+// error (log_targets_host_pyccel.py:14-94).
+template <int PREC, int MODE>
 __global__ __launch_bounds__(256) void eval_kernel(
-    const double* __restrict__ spos, const double* __restrict__ scharge, const i64* __restrict__ perm,
-    i64 n, const double* __restrict__ center, const double* __restrict__ half,
-    const int4* __restrict__ info, i64 node_count, const double* __restrict__ lre,
+    const double* __restrict__ tpos, const i64* __restrict__ tperm, const i64* __restrict__ exclude,
+    i64 m, const double* __restrict__ tcenter, const double* __restrict__ thalf,
+    const int4* __restrict__ tinfo, i64 t_count, const double* __restrict__ lre,
     const double* __restrict__ lim, int p1, const u32* __restrict__ prow,
-    const int* __restrict__ pcol, double eps2, int with_potential, double* __restrict__ phi,
+    const int* __restrict__ pcol, const double* __restrict__ spos,
+    const double* __restrict__ scharge, const i64* __restrict__ sperm, i64 n,
+    const int4* __restrict__ sinfo, double eps2, int with_potential, double* __restrict__ phi,
     double* __restrict__ field, i64* __restrict__ status) {
     const int lane = td3_lane();
     const i64 leaf = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    if (leaf >= node_count) return;
-    const int4 lf = info[leaf];
+    if (leaf >= t_count) return;
+    const int4 lf = tinfo[leaf];
     if (lf.z >= 0 || lf.y == lf.x) return;
     const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
-    const double cx = center[2 * leaf], cy = center[2 * leaf + 1];
-    const double inv_r = 1.0 / (1.4142135623730951 * half[leaf]);
+    const double cx = tcenter[2 * leaf], cy = tcenter[2 * leaf + 1];
+    const double inv_r = 1.0 / (1.4142135623730951 * thalf[leaf]);
     for (int t0 = lf.x; t0 < lf.y; t0 += kChunk) {
         const int nt = (lf.y - t0) < kChunk ? (lf.y - t0) : kChunk;
         double acc_p[kChunk], acc_x[kChunk], acc_y[kChunk];
         double tx[kChunk], ty[kChunk], tq[kChunk];
+        i64 tskip[kChunk];
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
             acc_p[j] = acc_x[j] = acc_y[j] = 0.0;
             const int k = t0 + (j < nt ? j : 0);
-            tx[j] = spos[k];
-            ty[j] = spos[n + k];
-            tq[j] = scharge[k];
+            tx[j] = tpos[k];
+            ty[j] = tpos[m + k];
+            tq[j] = MODE == 0 ? scharge[k] : 0.0;
+            tskip[j] = MODE == 1 ? exclude[tperm[k]] : -1;
         }
         int bad_slot = -1;
         // source leaves in batches of 64 lanes
@@ -283,7 +293,7 @@ __global__ __launch_bounds__(256) void eval_kernel(
             const u32 rr = rb + lane;
             int sstart = 0, scount = 0;
             if (rr < r1) {
-                const int4 sf = info[pcol[rr]];
+                const int4 sf = sinfo[pcol[rr]];
                 sstart = sf.x;
                 scount = sf.y - sf.x;
             }
@@ -311,14 +321,17 @@ __global__ __launch_bounds__(256) void eval_kernel(
                 if (idx >= total) continue;
                 const int ks = ostart + (idx - (oincl - ocount));
                 const double sx = spos[ks], sy = spos[n + ks], sq = scharge[ks];
+                if (MODE == 1 && sq == 0.0) continue;
+                const i64 sid = MODE == 1 ? sperm[ks] : -1;
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
                     if (j >= nt) break;
-                    if (ks == t0 + j) continue;
+                    if (MODE == 0 && ks == t0 + j) continue;
+                    if (MODE == 1 && sid == tskip[j]) continue;
                     const double dx = tx[j] - sx, dy = ty[j] - sy;
                     const double r2 = dx * dx + dy * dy + eps2;
                     if (r2 == 0.0) {
-                        if (tq[j] != 0.0 && sq != 0.0) bad_slot = t0 + j;
+                        if (MODE == 1 || (tq[j] != 0.0 && sq != 0.0)) bad_slot = t0 + j;
                         continue;
                     }
                     if (PREC == TD3_FP64) {
@@ -336,7 +349,7 @@ __global__ __launch_bounds__(256) void eval_kernel(
                 }
             }
         }
-        if (bad_slot >= 0) td3_report(status, perm[bad_slot] + 1);
+        if (bad_slot >= 0) td3_report(status, tperm[bad_slot] + 1);
         // reduce and L2P
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
@@ -356,7 +369,7 @@ __global__ __launch_bounds__(256) void eval_kernel(
                     fy = acc_y[j];
                 }
             // Horner on the scaled local expansion at u' = (z - c) / r
-            const double ur = (spos[k] - cx) * inv_r, ui = (spos[n + k] - cy) * inv_r;
+            const double ur = (tpos[k] - cx) * inv_r, ui = (tpos[m + k] - cy) * inv_r;
             double w_re = lre[leaf * p1 + p1 - 1], w_im = lim[leaf * p1 + p1 - 1];
             double d_re = 0.0, d_im = 0.0;
             for (int l = p1 - 2; l >= 0; --l) {
@@ -367,41 +380,64 @@ __global__ __launch_bounds__(256) void eval_kernel(
                 w_im = w_re * ui + w_im * ur + lim[leaf * p1 + l];
                 w_re = nw;
             }
-            const i64 i = perm[k];
+            const i64 i = tperm[k];
             phi[i] = with_potential ? p - 2.0 * w_re : 0.0;
             field[i] = fx + 2.0 * d_re * inv_r;
-            field[n + i] = fy - 2.0 * d_im * inv_r;
+            field[m + i] = fy - 2.0 * d_im * inv_r;
         }
     }
 }
 
+template <int MODE>
+int launch_eval(int precision, hipStream_t s, const double* tpos, const i64* tperm,
+                const i64* exclude, i64 m, const double* tcenter, const double* thalf,
+                const int4* tinfo, i64 t_count, const double* lre, const double* lim, int p1,
+                const u32* prow, const int* pcol, const double* spos, const double* scharge,
+                const i64* sperm, i64 n, const int4* sinfo, double eps2, int with_potential,
+                double* phi, double* field, i64* status) {
+    const unsigned blocks = td3_blocks(t_count * 64, 256);
+    if (precision == TD3_FP64)
+        TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<TD3_FP64, MODE>), dim3(blocks), dim3(256), 0, s,
+                           tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, prow,
+                           pcol, spos, scharge, sperm, n, sinfo, eps2, with_potential, phi, field,
+                           status));
+    else
+        TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<TD3_MIXED, MODE>), dim3(blocks), dim3(256), 0, s,
+                           tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, prow,
+                           pcol, spos, scharge, sperm, n, sinfo, eps2, with_potential, phi, field,
+                           status));
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
 template <int P, typename T>
 int launch_m2l(hipStream_t s, const int* col, const int* dst, const u32* row_start, u32 e_lo,
-               u32 e_hi, i64 node_lo, i64 node_hi, const double* center, const double* half,
-               const T* mre, const T* mim, int p1, const double* coeff, double* lre, double* lim,
-               T* side) {
+               u32 e_hi, i64 node_lo, i64 node_hi, const double* dcenter, const double* dhalf,
+               const double* scenter, const double* shalf, const T* mre, const T* mim, int p1,
+               const double* coeff, double* lre, double* lim, T* side) {
     if (e_hi <= e_lo) return TD3_OK;
     const i64 waves = ((i64)(e_hi - e_lo) + 63) / 64;
-    hipLaunchKernelGGL((m2l_kernel<P, T>), dim3(td3_blocks(waves * 64, 256)), dim3(256), 0, s, col, dst,
-                       row_start, e_lo, e_hi, center, half, mre, mim, p1, coeff, lre, lim, side);
+    TD3_TIMED(s, "m2l", hipLaunchKernelGGL((m2l_kernel<P, T>), dim3(td3_blocks(waves * 64, 256)), dim3(256), 0, s, col, dst,
+                       row_start, e_lo, e_hi, dcenter, dhalf, scenter, shalf, mre, mim, p1, coeff, lre,
+                       lim, side));
     TD3_LAUNCHED();
     const i64 items = (node_hi - node_lo) * 2 * p1;
-    hipLaunchKernelGGL((m2l_fixup_kernel<T>), dim3(td3_blocks(items, 256)), dim3(256), 0, s, row_start,
-                       node_lo, node_hi, e_lo, p1, side, lre, lim);
+    TD3_TIMED(s, "m2l_fixup", hipLaunchKernelGGL((m2l_fixup_kernel<T>), dim3(td3_blocks(items, 256)), dim3(256), 0, s, row_start,
+                       node_lo, node_hi, e_lo, p1, side, lre, lim));
     TD3_LAUNCHED();
     return TD3_OK;
 }
 
 template <typename T>
 int dispatch_m2l(hipStream_t s, int order, const int* col, const int* dst, const u32* row_start,
-                 u32 e_lo, u32 e_hi, i64 node_lo, i64 node_hi, const double* center,
-                 const double* half, const T* mre, const T* mim, const double* coeff, double* lre,
-                 double* lim, T* side) {
+                 u32 e_lo, u32 e_hi, i64 node_lo, i64 node_hi, const double* dcenter,
+                 const double* dhalf, const double* scenter, const double* shalf, const T* mre,
+                 const T* mim, const double* coeff, double* lre, double* lim, T* side) {
     const int p1 = order + 1;
 #define TD3_M2L_CASE(PP)                                                                          \
     if (order <= PP)                                                                             \
-        return launch_m2l<PP, T>(s, col, dst, row_start, e_lo, e_hi, node_lo, node_hi, center, half, \
-                                 mre, mim, p1, coeff, lre, lim, side);
+        return launch_m2l<PP, T>(s, col, dst, row_start, e_lo, e_hi, node_lo, node_hi, dcenter,   \
+                                 dhalf, scenter, shalf, mre, mim, p1, coeff, lre, lim, side);
     TD3_M2L_CASE(4)
     TD3_M2L_CASE(8)
     TD3_M2L_CASE(10)
@@ -431,22 +467,23 @@ extern "C" int td3_scale_moments(int precision, const double* moment_re, const d
     const i64 items = node_count * p1;
     hipStream_t s = (hipStream_t)stream;
     if (precision == TD3_FP64)
-        hipLaunchKernelGGL(scale_moments_kernel<double>, dim3(td3_blocks(items, 256)), dim3(256), 0, s,
+        TD3_TIMED(s, "scale_moments", hipLaunchKernelGGL(scale_moments_kernel<double>, dim3(td3_blocks(items, 256)), dim3(256), 0, s,
                            moment_re, moment_im, node_half, node_count, p1, (double*)out_re,
-                           (double*)out_im);
+                           (double*)out_im));
     else
-        hipLaunchKernelGGL(scale_moments_kernel<float>, dim3(td3_blocks(items, 256)), dim3(256), 0, s,
+        TD3_TIMED(s, "scale_moments", hipLaunchKernelGGL(scale_moments_kernel<float>, dim3(td3_blocks(items, 256)), dim3(256), 0, s,
                            moment_re, moment_im, node_half, node_count, p1, (float*)out_re,
-                           (float*)out_im);
+                           (float*)out_im));
     TD3_LAUNCHED();
     return TD3_OK;
 }
 
 extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const int32_t* dst,
                        const uint32_t* row_start, int64_t node_lo, int64_t node_hi,
-                       const double* node_center, const double* node_half, const void* scaled_re,
-                       const void* scaled_im, const double* coeff, double* local_re,
-                       double* local_im, void* workspace, size_t workspace_bytes, void* stream) {
+                       const double* dst_center, const double* dst_half, const double* src_center,
+                       const double* src_half, const void* scaled_re, const void* scaled_im,
+                       const double* coeff, double* local_re, double* local_im, void* workspace,
+                       size_t workspace_bytes, void* stream) {
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
     hipStream_t s = (hipStream_t)stream;
@@ -460,12 +497,13 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
     TD3_ARG(workspace_bytes >= td3_m2l_workspace_bytes(pairs, order, precision), "workspace too small");
     if (precision == TD3_FP64)
         return dispatch_m2l<double>(s, (int)order, col, dst, row_start, bounds[0], bounds[1], node_lo,
-                                    node_hi, node_center, node_half, (const double*)scaled_re,
-                                    (const double*)scaled_im, coeff, local_re, local_im,
-                                    (double*)workspace);
+                                    node_hi, dst_center, dst_half, src_center, src_half,
+                                    (const double*)scaled_re, (const double*)scaled_im, coeff,
+                                    local_re, local_im, (double*)workspace);
     return dispatch_m2l<float>(s, (int)order, col, dst, row_start, bounds[0], bounds[1], node_lo,
-                               node_hi, node_center, node_half, (const float*)scaled_re,
-                               (const float*)scaled_im, coeff, local_re, local_im, (float*)workspace);
+                               node_hi, dst_center, dst_half, src_center, src_half,
+                               (const float*)scaled_re, (const float*)scaled_im, coeff, local_re,
+                               local_im, (float*)workspace);
 }
 
 extern "C" int td3_l2l(int64_t order, const double* node_center, const double* node_half,
@@ -477,9 +515,9 @@ extern "C" int td3_l2l(int64_t order, const double* node_center, const double* n
     const int p1 = (int)order + 1;
     for (i64 d = 0; d + 1 < n_levels; ++d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
-        hipLaunchKernelGGL(l2l_kernel, dim3(td3_blocks((hi - lo) * 4 * p1, 256)), dim3(256), 0, s,
+        TD3_TIMED(s, "l2l", hipLaunchKernelGGL(l2l_kernel, dim3(td3_blocks((hi - lo) * 4 * p1, 256)), dim3(256), 0, s,
                            node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
-                           local_re, local_im);
+                           local_re, local_im));
         TD3_LAUNCHED();
     }
     return TD3_OK;
@@ -503,19 +541,27 @@ extern "C" int td3_fmm_eval(int precision, int64_t order, const double* sorted_p
                             int64_t* status, void* stream) {
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(n >= 1 && node_count >= 1, "sizes");
-    const int p1 = (int)order + 1;
-    hipStream_t s = (hipStream_t)stream;
-    const double eps2 = softening * softening;
-    const unsigned blocks = td3_blocks(node_count * 64, 256);
-    if (precision == TD3_FP64)
-        hipLaunchKernelGGL(eval_kernel<TD3_FP64>, dim3(blocks), dim3(256), 0, s, sorted_pos, sorted_charge,
-                           perm, n, node_center, node_half, (const int4*)node_info, node_count, local_re,
-                           local_im, p1, p2p_row, p2p_col, eps2, with_potential, phi, field, status);
-    else
-        hipLaunchKernelGGL(eval_kernel<TD3_MIXED>, dim3(blocks), dim3(256), 0, s, sorted_pos,
-                           sorted_charge, perm, n, node_center, node_half, (const int4*)node_info,
-                           node_count, local_re, local_im, p1, p2p_row, p2p_col, eps2, with_potential,
-                           phi, field, status);
-    TD3_LAUNCHED();
-    return TD3_OK;
+    return launch_eval<0>(precision, (hipStream_t)stream, sorted_pos, perm, nullptr, n, node_center,
+                          node_half, (const int4*)node_info, node_count, local_re, local_im,
+                          (int)order + 1, p2p_row, p2p_col, sorted_pos, sorted_charge, perm, n,
+                          (const int4*)node_info, softening * softening, with_potential, phi, field,
+                          status);
+}
+
+extern "C" int td3_fmm_at_eval(int precision, int64_t order, const double* sorted_targets,
+                               const int64_t* target_perm, const int64_t* exclude, int64_t m,
+                               const double* t_center, const double* t_half, const void* t_info,
+                               int64_t t_count, const double* local_re, const double* local_im,
+                               const uint32_t* p2p_row, const int32_t* p2p_col,
+                               const double* sorted_pos, const double* sorted_charge,
+                               const int64_t* perm, int64_t n, const void* s_info,
+                               double softening, int with_potential, double* phi, double* field,
+                               int64_t* status, void* stream) {
+    TD3_ARG(order >= 1 && order <= 64, "order");
+    TD3_ARG(m >= 1 && n >= 1 && t_count >= 1, "sizes");
+    return launch_eval<1>(precision, (hipStream_t)stream, sorted_targets, target_perm, exclude, m,
+                          t_center, t_half, (const int4*)t_info, t_count, local_re, local_im,
+                          (int)order + 1, p2p_row, p2p_col, sorted_pos, sorted_charge, perm, n,
+                          (const int4*)s_info, softening * softening, with_potential, phi, field,
+                          status);
 }

@@ -173,28 +173,28 @@ extern "C" int td3_moments(int kernel, int64_t order, const double* sorted_pos,
     if (kernel == TD3_LOG) {
         TD3_ARG(order >= 1 && order <= 64 && bsz >= order + 1, "order");
         const int p1 = (int)order + 1;
-        hipLaunchKernelGGL(log_p2m_kernel, dim3(td3_blocks(node_count * p1, 256)), dim3(256), 0, s,
+        TD3_TIMED(s, "moments_p2m", hipLaunchKernelGGL(log_p2m_kernel, dim3(td3_blocks(node_count * p1, 256)), dim3(256), 0, s,
                            sorted_pos, sorted_charge, n, node_center, info, node_count, p1, node_qabs,
-                           moment_re, moment_im);
+                           moment_re, moment_im));
         TD3_LAUNCHED();
         for (i64 d = n_levels - 2; d >= 0; --d) {
             const i64 lo = levels_host[d], hi = levels_host[d + 1];
-            hipLaunchKernelGGL(log_m2m_kernel, dim3(td3_blocks((hi - lo) * p1, 256)), dim3(256), 0, s,
+            TD3_TIMED(s, "moments_m2m", hipLaunchKernelGGL(log_m2m_kernel, dim3(td3_blocks((hi - lo) * p1, 256)), dim3(256), 0, s,
                                node_center, info, lo, hi, p1, binomial, (int)bsz, node_qabs, moment_re,
-                               moment_im);
+                               moment_im));
             TD3_LAUNCHED();
         }
         return TD3_OK;
     }
-    hipLaunchKernelGGL(coulomb_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0, s, sorted_pos,
+    TD3_TIMED(s, "moments_coulomb", hipLaunchKernelGGL(coulomb_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0, s, sorted_pos,
                        sorted_charge, n, node_center, info, (i64)0, (i64)node_count, 1, node_q,
-                       node_qabs, node_p, node_s);
+                       node_qabs, node_p, node_s));
     TD3_LAUNCHED();
     for (i64 d = n_levels - 2; d >= 0; --d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
-        hipLaunchKernelGGL(coulomb_kernel, dim3(td3_blocks(hi - lo, 256)), dim3(256), 0, s, sorted_pos,
+        TD3_TIMED(s, "moments_coulomb", hipLaunchKernelGGL(coulomb_kernel, dim3(td3_blocks(hi - lo, 256)), dim3(256), 0, s, sorted_pos,
                            sorted_charge, n, node_center, info, lo, hi, 0, node_q, node_qabs, node_p,
-                           node_s);
+                           node_s));
         TD3_LAUNCHED();
     }
     return TD3_OK;

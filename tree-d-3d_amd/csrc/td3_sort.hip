@@ -112,7 +112,7 @@ int exclusive_scan_u32(const u32* in, u32* out, i64 n, u32* total, u32* scratch,
         if (total) TD3_HIP(hipMemsetAsync(total, 0, sizeof(u32), stream));
         return TD3_OK;
     }
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, in, out, n, total);
+    TD3_TIMED(stream, "scan", hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, in, out, n, total));
     TD3_LAUNCHED();
     return TD3_OK;
 }
@@ -134,30 +134,30 @@ int sort_pairs(u64* keys, u32* vals, u64* aux, u64* keys_alt, u32* vals_alt, u64
     for (int bit = bit_lo; bit < bit_hi; bit += 8) {
         const int width = (bit_hi - bit) < 8 ? (bit_hi - bit) : 8;
         const u32 mask = (1u << width) - 1u;
-        hipLaunchKernelGGL(hist_kernel, dim3((unsigned)nt), dim3(kThreads), 0, stream, src_k, n, bit,
-                           mask, nt, hist);
+        TD3_TIMED(stream, "sort_hist", hipLaunchKernelGGL(hist_kernel, dim3((unsigned)nt), dim3(kThreads), 0, stream, src_k, n, bit,
+                           mask, nt, hist));
         TD3_LAUNCHED();
-        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, hist, hist, nt * kRadix,
-                           (u32*)nullptr);
+        TD3_TIMED(stream, "scan", hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, hist, hist, nt * kRadix,
+                           (u32*)nullptr));
         TD3_LAUNCHED();
         if (aux != nullptr) {
             if (ident)
-                hipLaunchKernelGGL((scatter_kernel<true, true>), dim3((unsigned)nt), dim3(kThreads), 0,
+                TD3_TIMED(stream, "sort_scatter", hipLaunchKernelGGL((scatter_kernel<true, true>), dim3((unsigned)nt), dim3(kThreads), 0,
                                    stream, src_k, src_v, src_a, dst_k, dst_v, dst_a, n, bit, mask,
-                                   hist, nt);
+                                   hist, nt));
             else
-                hipLaunchKernelGGL((scatter_kernel<true, false>), dim3((unsigned)nt), dim3(kThreads), 0,
+                TD3_TIMED(stream, "sort_scatter", hipLaunchKernelGGL((scatter_kernel<true, false>), dim3((unsigned)nt), dim3(kThreads), 0,
                                    stream, src_k, src_v, src_a, dst_k, dst_v, dst_a, n, bit, mask,
-                                   hist, nt);
+                                   hist, nt));
         } else {
             if (ident)
-                hipLaunchKernelGGL((scatter_kernel<false, true>), dim3((unsigned)nt), dim3(kThreads), 0,
+                TD3_TIMED(stream, "sort_scatter", hipLaunchKernelGGL((scatter_kernel<false, true>), dim3((unsigned)nt), dim3(kThreads), 0,
                                    stream, src_k, src_v, src_a, dst_k, dst_v, dst_a, n, bit, mask,
-                                   hist, nt);
+                                   hist, nt));
             else
-                hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)nt), dim3(kThreads),
+                TD3_TIMED(stream, "sort_scatter", hipLaunchKernelGGL((scatter_kernel<false, false>), dim3((unsigned)nt), dim3(kThreads),
                                    0, stream, src_k, src_v, src_a, dst_k, dst_v, dst_a, n, bit, mask,
-                                   hist, nt);
+                                   hist, nt));
         }
         TD3_LAUNCHED();
         ident = false;

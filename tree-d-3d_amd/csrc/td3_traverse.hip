@@ -38,9 +38,14 @@ __device__ __forceinline__ u32 wave_append(bool want, u32 count_each, u32* count
     return total;
 }
 
+// DUAL: a is a cell of the target tree (ca, ha, ia) and b of the source tree (cb, hb, ib),
+// with the rules of kernels/dual_traverse/dual_traverse_pyccel.py:14-86 (no self pairs,
+// one-way lists); otherwise both are cells of one tree.
+template <bool DUAL>
 __global__ __launch_bounds__(kTravThreads) void traverse_kernel(
-    const double* __restrict__ center, const double* __restrict__ half,
-    const int4* __restrict__ info, double theta, int g, const int2* __restrict__ frontier,
+    const double* __restrict__ ca, const double* __restrict__ ha, const int4* __restrict__ ia,
+    const double* __restrict__ cb, const double* __restrict__ hb, const int4* __restrict__ ib,
+    double theta, int g, const int2* __restrict__ frontier,
     u32 cap_front, int2* __restrict__ next, int2* __restrict__ m2l, u32 cap_m2l,
     int2* __restrict__ p2p, u32 cap_p2p, u32* __restrict__ ctr) {
     // a list or frontier overflowed earlier: the caller retries with larger buffers
@@ -59,22 +64,22 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(
             const int2 pr = frontier[k];
             a = pr.x;
             b = pr.y;
-            fa = info[a];
-            fb = info[b];
+            fa = ia[a];
+            fb = ib[b];
             if (fa.y != fa.x && fb.y != fb.x) {
                 const bool leaf_a = fa.z < 0, leaf_b = fb.z < 0;
-                if (a == b) {
+                if (!DUAL && a == b) {
                     kind = leaf_a ? 2 : 5;
                 } else {
-                    const double dx = __dadd_rn(center[2 * a], -center[2 * b]);
-                    const double dy = __dadd_rn(center[2 * a + 1], -center[2 * b + 1]);
+                    const double dx = __dadd_rn(ca[2 * a], -cb[2 * b]);
+                    const double dy = __dadd_rn(ca[2 * a + 1], -cb[2 * b + 1]);
                     const double distance = __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
-                    const double radii = __dmul_rn(1.4142135623730951, __dadd_rn(half[a], half[b]));
+                    const double radii = __dmul_rn(1.4142135623730951, __dadd_rn(ha[a], hb[b]));
                     if (radii < __dmul_rn(theta, distance))
                         kind = 1;
                     else if (leaf_a && leaf_b)
                         kind = 2;
-                    else if (leaf_b || (!leaf_a && half[a] >= half[b]))
+                    else if (leaf_b || (!leaf_a && ha[a] >= hb[b]))
                         kind = 3;
                     else
                         kind = 4;
@@ -120,17 +125,22 @@ __global__ void traverse_init_kernel(int2* frontier, u32* ctr) {
     ctr[3] = 1u;
 }
 
-// Row lengths of the directed lists: M2L both ways; P2P both ways, a self pair once.
-__global__ void degree_kernel(const int2* __restrict__ pairs, u32 count, int self_once,
+// Row lengths of the directed lists.  mode 0: both ways; 1: both ways, a self pair once;
+// 2: one way (b under a only).
+__device__ __forceinline__ bool both_ways(int mode, int2 p) {
+    return mode == 0 || (mode == 1 && p.x != p.y);
+}
+
+__global__ void degree_kernel(const int2* __restrict__ pairs, u32 count, int mode,
                               u32* __restrict__ deg) {
     const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     const int2 p = pairs[k];
     atomicAdd(&deg[p.x], 1u);
-    if (!(self_once && p.x == p.y)) atomicAdd(&deg[p.y], 1u);
+    if (both_ways(mode, p)) atomicAdd(&deg[p.y], 1u);
 }
 
-__global__ void fill_kernel(const int2* __restrict__ pairs, u32 count, int self_once,
+__global__ void fill_kernel(const int2* __restrict__ pairs, u32 count, int mode,
                             u32* __restrict__ cursor, int* __restrict__ col,
                             int* __restrict__ dst) {
     const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -139,7 +149,7 @@ __global__ void fill_kernel(const int2* __restrict__ pairs, u32 count, int self_
     u32 o = atomicAdd(&cursor[p.x], 1u);
     col[o] = p.y;
     if (dst) dst[o] = p.x;
-    if (!(self_once && p.x == p.y)) {
+    if (both_ways(mode, p)) {
         o = atomicAdd(&cursor[p.y], 1u);
         col[o] = p.x;
         if (dst) dst[o] = p.y;
@@ -223,10 +233,11 @@ extern "C" int td3_traverse(const double* node_center, const double* node_half,
     hipLaunchKernelGGL(traverse_init_kernel, dim3(1), dim3(1), 0, s, fa, ctr);
     TD3_LAUNCHED();
     for (int g = 0; g < gens; ++g) {
-        hipLaunchKernelGGL(traverse_kernel, dim3(kTravBlocks), dim3(kTravThreads), 0, s, node_center,
-                           node_half, (const int4*)node_info, theta, g, (g & 1) ? fb : fa,
+        TD3_TIMED(s, "traverse", hipLaunchKernelGGL(traverse_kernel<false>, dim3(kTravBlocks), dim3(kTravThreads), 0, s, node_center,
+                           node_half, (const int4*)node_info, node_center, node_half,
+                           (const int4*)node_info, theta, g, (g & 1) ? fb : fa,
                            (u32)cap_front, (g & 1) ? fa : fb, (int2*)m2l, (u32)cap_m2l, (int2*)p2p,
-                           (u32)cap_p2p, ctr);
+                           (u32)cap_p2p, ctr));
         TD3_LAUNCHED();
     }
     u32 host[4 + 130];
@@ -250,14 +261,63 @@ extern "C" int td3_traverse(const double* node_center, const double* node_half,
     return TD3_OK;
 }
 
+extern "C" int td3_dual_traverse(const double* t_center, const double* t_half, const void* t_info,
+                                 int64_t t_count, int64_t t_depth, const double* s_center,
+                                 const double* s_half, const void* s_info, int64_t s_count,
+                                 int64_t s_depth, double theta, int64_t cap_m2l, int64_t cap_p2p,
+                                 int64_t cap_front, int32_t* m2l, int32_t* p2p, void* workspace,
+                                 size_t workspace_bytes, int64_t* out_counts, void* stream) {
+    TD3_ARG(t_count >= 1 && t_count < 0x7fffffff && s_count >= 1 && s_count < 0x7fffffff, "node counts");
+    TD3_ARG(t_depth >= 0 && t_depth <= 64 && s_depth >= 0 && s_depth <= 64, "depths");
+    TD3_ARG(cap_front >= 16 && cap_front < 0x7fffffff && cap_m2l < 0xffffffffll &&
+                cap_p2p < 0xffffffffll, "capacities");
+    TD3_ARG(workspace_bytes >= td3_traverse_workspace_bytes(cap_front), "workspace too small");
+    TD3_ARG(out_counts != nullptr, "out_counts");
+    hipStream_t s = (hipStream_t)stream;
+    int2* fa = (int2*)workspace;
+    int2* fb = fa + cap_front;
+    u32* ctr = (u32*)(fb + cap_front);
+    const int gens = (int)(t_depth + s_depth + 1);
+    TD3_HIP(hipMemsetAsync(ctr, 0, sizeof(u32) * (size_t)(gens + 8), s));
+    hipLaunchKernelGGL(traverse_init_kernel, dim3(1), dim3(1), 0, s, fa, ctr);
+    TD3_LAUNCHED();
+    for (int g = 0; g < gens; ++g) {
+        TD3_TIMED(s, "traverse", hipLaunchKernelGGL(traverse_kernel<true>, dim3(kTravBlocks), dim3(kTravThreads), 0, s,
+                           t_center, t_half, (const int4*)t_info, s_center, s_half,
+                           (const int4*)s_info, theta, g, (g & 1) ? fb : fa, (u32)cap_front,
+                           (g & 1) ? fa : fb, (int2*)m2l, (u32)cap_m2l, (int2*)p2p, (u32)cap_p2p, ctr));
+        TD3_LAUNCHED();
+    }
+    u32 host[4 + 140];
+    TD3_HIP(hipMemcpyAsync(host, ctr, sizeof(u32) * (size_t)(gens + 4), hipMemcpyDeviceToHost, s));
+    TD3_HIP(hipStreamSynchronize(s));
+    u32 max_front = 0;
+    for (int g = 0; g <= gens; ++g) max_front = host[3 + g] > max_front ? host[3 + g] : max_front;
+    out_counts[0] = host[0];
+    out_counts[1] = host[1];
+    out_counts[2] = max_front;
+    out_counts[3] = host[2] ? 1 : 0;
+    if (host[2]) {
+        td3_set_error("dual traversal lists overflowed (m2l %u, p2p %u, frontier %u)", host[0],
+                      host[1], max_front);
+        return TD3_ERR_CAPACITY;
+    }
+    if (host[3 + gens] != 0) {
+        td3_set_error("dual traversal did not finish in %d generations", gens);
+        return TD3_ERR_ARG;
+    }
+    return TD3_OK;
+}
+
 extern "C" size_t td3_lists_workspace_bytes(int64_t node_count) {
     return (size_t)(node_count + 1) * sizeof(u32) * 2 + td3sort::scratch_bytes(node_count + 1) + 1024;
 }
 
-extern "C" int td3_lists(const int32_t* pairs, int64_t count, int self_once, int64_t node_count,
+extern "C" int td3_lists(const int32_t* pairs, int64_t count, int mode, int64_t node_count,
                          uint32_t* row_start, int32_t* col, int32_t* dst, void* workspace,
                          size_t workspace_bytes, void* stream) {
     TD3_ARG(count >= 0 && node_count >= 1, "sizes");
+    TD3_ARG(mode >= 0 && mode <= 2, "mode");
     TD3_ARG(workspace_bytes >= td3_lists_workspace_bytes(node_count), "workspace too small");
     hipStream_t s = (hipStream_t)stream;
     u32* deg = (u32*)workspace;
@@ -265,21 +325,21 @@ extern "C" int td3_lists(const int32_t* pairs, int64_t count, int self_once, int
     u32* scratch = (u32*)(((uintptr_t)(cursor + node_count + 1) + 255) & ~(uintptr_t)255);
     TD3_HIP(hipMemsetAsync(deg, 0, sizeof(u32) * (size_t)(node_count + 1), s));
     if (count > 0) {
-        hipLaunchKernelGGL(degree_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, (const int2*)pairs,
-                           (u32)count, self_once, deg);
+        TD3_TIMED(s, "lists_degree", hipLaunchKernelGGL(degree_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, (const int2*)pairs,
+                           (u32)count, mode, deg));
         TD3_LAUNCHED();
     }
     if (int rc = td3sort::exclusive_scan_u32(deg, row_start, node_count + 1, nullptr, scratch, s)) return rc;
     if (count > 0) {
         TD3_HIP(hipMemcpyAsync(cursor, row_start, sizeof(u32) * (size_t)node_count, hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(fill_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, (const int2*)pairs,
-                           (u32)count, self_once, cursor, col, (int*)nullptr);
+        TD3_TIMED(s, "lists_fill", hipLaunchKernelGGL(fill_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, (const int2*)pairs,
+                           (u32)count, mode, cursor, col, (int*)nullptr));
         TD3_LAUNCHED();
         u32* err = deg;  // reuse: zero it first
         TD3_HIP(hipMemsetAsync(err, 0, sizeof(u32), s));
         unsigned blocks = td3_blocks(node_count, 4);
         if (blocks > 8192) blocks = 8192;
-        hipLaunchKernelGGL(row_sort_kernel, dim3(blocks), dim3(256), 0, s, row_start, node_count, col, err);
+        TD3_TIMED(s, "row_sort", hipLaunchKernelGGL(row_sort_kernel, dim3(blocks), dim3(256), 0, s, row_start, node_count, col, err));
         TD3_LAUNCHED();
         if (dst) {
             hipLaunchKernelGGL(row_dst_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0, s, row_start,

@@ -324,15 +324,15 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     u32 host_small[81];
     for (int attempt = 0; attempt < 2; ++attempt) {
         if (!deep) {
-            hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
-                               w.khi, (u64*)nullptr);
+            TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
+                               w.khi, (u64*)nullptr));
             TD3_LAUNCHED();
             if (int rc = td3sort::sort_pairs(w.khi, w.idx, nullptr, w.khi_alt, w.idx_alt, nullptr, n,
                                              64 - 2 * nkey, 64, true, w.scratch, s))
                 return rc;
         } else {
-            hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
-                               w.khi, w.klo);
+            TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
+                               w.khi, w.klo));
             TD3_LAUNCHED();
             if (int rc = td3sort::sort_pairs(w.klo, w.idx, w.khi, w.klo_alt, w.idx_alt, w.khi_alt, n,
                                              64 - 2 * (nkey - 32), 64, true, w.scratch, s))
@@ -343,8 +343,8 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
         }
         // 3. splitting-node starts
         TD3_HIP(hipMemsetAsync(w.small, 0, sizeof(u32) * 96, s));
-        hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
-                           (i64)leaf_capacity, max_split, nkey, w.cnt, depth_count, flag);
+        TD3_TIMED(s, "tree_count", hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
+                           (i64)leaf_capacity, max_split, nkey, w.cnt, depth_count, flag));
         TD3_LAUNCHED();
         TD3_HIP(hipMemcpyAsync(host_small, w.small, sizeof(u32) * 81, hipMemcpyDeviceToHost, s));
         TD3_HIP(hipStreamSynchronize(s));
@@ -390,20 +390,20 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
                        node_start, node_end, node_child, node_depth, (int4*)node_info);
     TD3_LAUNCHED();
     if (s_sum > 0) {
-        hipLaunchKernelGGL(nodes_kernel, dim3(td3_blocks(s_sum, 256)), dim3(256), 0, s, w.khi, klo, n,
+        TD3_TIMED(s, "tree_nodes", hipLaunchKernelGGL(nodes_kernel, dim3(td3_blocks(s_sum, 256)), dim3(256), 0, s, w.khi, klo, n,
                            (i64)leaf_capacity, (int)max_depth, nkey, xmid, ymid, half, w.ekey, w.eval,
                            s_sum, bstart, depth_count, levels, node_center, node_half, node_start,
-                           node_end, node_child, node_depth, (int4*)node_info);
+                           node_end, node_child, node_depth, (int4*)node_info));
         TD3_LAUNCHED();
     }
     // 5. input order inside each leaf: sort the input indices by their leaf's start
-    hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_start,
-                       node_end, node_child, count, w.idx, w.khi_alt);
+    TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_start,
+                       node_end, node_child, count, w.idx, w.khi_alt));
     TD3_LAUNCHED();
     if (int rc = td3sort::sort_pairs(w.khi_alt, w.idx_alt, nullptr, w.khi, w.idx, nullptr, n, 0,
                                      bits_for(n), true, w.scratch, s))
         return rc;
-    hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, w.idx_alt, pos, n, perm, sorted_pos);
+    TD3_TIMED(s, "tree_gather", hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, w.idx_alt, pos, n, perm, sorted_pos));
     TD3_LAUNCHED();
     return TD3_OK;
 }

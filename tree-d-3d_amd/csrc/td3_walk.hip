@@ -246,8 +246,8 @@ template <int KERNEL, int PREC>
 void launch_walk(unsigned blocks, hipStream_t s, const WalkTree& tr, const double* targets, i64 m,
                  const i64* order, const i64* exclude, double theta, double eps2, int with_potential,
                  int stack_size, double* phi, double* field, i64* status) {
-    hipLaunchKernelGGL((walk_kernel<KERNEL, PREC>), dim3(blocks), dim3(64 * kWaves), 0, s, tr, targets,
-                       m, order, exclude, theta, eps2, with_potential, stack_size, phi, field, status);
+    TD3_TIMED(s, "walk", hipLaunchKernelGGL((walk_kernel<KERNEL, PREC>), dim3(blocks), dim3(64 * kWaves), 0, s, tr, targets,
+                       m, order, exclude, theta, eps2, with_potential, stack_size, phi, field, status));
 }
 
 }  // namespace

@@ -572,20 +572,25 @@ class FlatQuadtree:
     #: largest: a cell's net charge grows with its size)
     FP64_TOP_DEPTH = 5
 
-    def _local_expansions(self, theta: float, precision: int):
-        """Scaled local expansions B_l = b_l r^l of every cell: M2L then L2L."""
-        (mrow, mcol, mdst), _, _ = self._lists(theta)
-        c, p1 = self.node_count, self.order + 1
+    def _local_expansions(self, theta: float, precision: int, target=None, m2l_rows=None):
+        """Scaled local expansions B_l = b_l r^l of every cell of ``target`` (this tree by
+        default): M2L from this tree's cells along ``m2l_rows``, then L2L down ``target``."""
+        target = self if target is None else target
+        if m2l_rows is None:
+            m2l_rows = self._lists(theta)[0]
+        mrow, mcol, mdst = m2l_rows
+        c, p1 = target.node_count, self.order + 1
         dev = self._d["pos"].device
         lre = torch.zeros((c, p1), dtype=torch.float64, device=dev)
         lim = torch.zeros((c, p1), dtype=torch.float64, device=dev)
         coeff = self._m2l_coefficients()
-        levels = np.ascontiguousarray(self.levels, dtype=np.int64)
+        levels = np.ascontiguousarray(target.levels, dtype=np.int64)
         if precision == _abi.FP64:
             ranges = [(_abi.FP64, 0, c)]
         else:
             split = int(levels[min(self.FP64_TOP_DEPTH + 1, len(levels) - 1)])
             ranges = [(_abi.FP64, 0, split), (_abi.MIXED, split, c)]
+        td = target._d
         for prec, lo, hi in ranges:
             if hi <= lo:
                 continue
@@ -593,16 +598,16 @@ class FlatQuadtree:
             nbytes = _abi.lib().td3_m2l_workspace_bytes(int(mcol.shape[0]), self.order, prec)
             ws = _abi.workspace(nbytes)
             _abi.call("td3_m2l", prec, self.order, _abi.ptr(mcol), _abi.ptr(mdst),
-                      _abi.ptr(mrow), lo, hi, _abi.ptr(self._d["node_center"]),
+                      _abi.ptr(mrow), lo, hi, _abi.ptr(td["node_center"]),
+                      _abi.ptr(td["node_half"]), _abi.ptr(self._d["node_center"]),
                       _abi.ptr(self._d["node_half"]), _abi.ptr(are), _abi.ptr(aim),
                       _abi.ptr(coeff), _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(ws), nbytes,
                       _abi.stream())
         bsz = 2 * self.order + 1
         binom = _device_table("binomial", _binomial_table, bsz)
-        _abi.call("td3_l2l", self.order, _abi.ptr(self._d["node_center"]),
-                  _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), levels.ctypes.data,
-                  len(levels) - 1, _abi.ptr(binom), bsz, _abi.ptr(lre), _abi.ptr(lim),
-                  _abi.stream())
+        _abi.call("td3_l2l", self.order, _abi.ptr(td["node_center"]), _abi.ptr(td["node_half"]),
+                  _abi.ptr(target._node_info), levels.ctypes.data, len(levels) - 1,
+                  _abi.ptr(binom), bsz, _abi.ptr(lre), _abi.ptr(lim), _abi.stream())
         return lre, lim
 
     def local_expansions(self, theta: float = 0.5, precision=None):
@@ -653,6 +658,113 @@ class FlatQuadtree:
 
     def _coulomb_fmm(self, theta, softening, potential, precision):
         raise NotImplementedError("the Coulomb (Cartesian) FMM is not built yet on the GPU")
+
+    # ---------------------------------------------------------------- fmm_at
+    def _target_plan(self, targets: torch.Tensor, theta: float, leaf: int):
+        """flat_tree.py:1124-1149: the target tree and the dual lists, kept while the
+        targets are the same."""
+        key = ("targets", theta, leaf)
+        cached = self._traversals.get(key)
+        if (cached is not None and cached[0].shape == targets.shape
+                and bool(torch.equal(cached[0], targets))):
+            return cached[1], cached[2]
+        from .quad import Quad
+
+        m = targets.shape[1]
+        lo = targets.min(dim=1).values.cpu().numpy()
+        hi = targets.max(dim=1).values.cpu().numpy()
+        centre = 0.5 * (lo + hi)
+        half = 0.5 * float(np.max(hi - lo)) * (1 + 1e-12) + 1e-300
+        tree = FlatQuadtree(targets, torch.zeros(m, dtype=torch.float64, device=targets.device),
+                            Quad(float(centre[0]), float(centre[1]), half), leaf_capacity=leaf,
+                            max_depth=self.max_depth, kernel="coulomb")
+        dev = targets.device
+        ct, cs = tree.node_count, self.node_count
+        caps = [16 * (ct + cs) + 64, 8 * (ct + cs) + 64, 10 * (ct + cs) + 64]
+        out = np.zeros(4, dtype=np.int64)
+        while True:
+            m2l = torch.empty((caps[0], 2), dtype=torch.int32, device=dev)
+            p2p = torch.empty((caps[1], 2), dtype=torch.int32, device=dev)
+            nbytes = _abi.lib().td3_traverse_workspace_bytes(caps[2])
+            ws = _abi.workspace(nbytes)
+            code = _abi.lib().td3_dual_traverse(
+                _abi.ptr(tree._d["node_center"]), _abi.ptr(tree._d["node_half"]),
+                _abi.ptr(tree._node_info), ct, tree._depth, _abi.ptr(self._d["node_center"]),
+                _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), cs, self._depth,
+                float(theta), caps[0], caps[1], caps[2], _abi.ptr(m2l), _abi.ptr(p2p),
+                _abi.ptr(ws), nbytes, out.ctypes.data, _abi.stream(),
+            )
+            if code == _abi.TD3_ERR_CAPACITY:
+                caps = [max(2 * caps[i], int(out[i]) + 64) for i in range(3)]
+                continue
+            _abi.check(code, "td3_dual_traverse")
+            break
+        rows = []
+        for pairs, count in ((m2l, int(out[0])), (p2p, int(out[1]))):
+            row = torch.empty(ct + 1, dtype=torch.int32, device=dev)
+            col = torch.empty(max(count, 1), dtype=torch.int32, device=dev)
+            dst = torch.empty(max(count, 1), dtype=torch.int32, device=dev)
+            nbytes = _abi.lib().td3_lists_workspace_bytes(ct)
+            ws = _abi.workspace(nbytes)
+            _abi.call("td3_lists", _abi.ptr(pairs), count, 2, ct, _abi.ptr(row), _abi.ptr(col),
+                      _abi.ptr(dst), _abi.ptr(ws), nbytes, _abi.stream())
+            rows.append((row, col, dst))
+        lists = (rows[0], rows[1], (int(out[0]), int(out[1])))
+        self._traversals[key] = (targets.clone(), tree, lists)
+        return tree, lists
+
+    def fmm_at(self, targets, *, exclude=None, theta: float = 0.5, softening: float = 0.0,
+               potential: bool = True, leaf_capacity=None, precision=None):
+        """Potential ``(M,)`` and field ``(2, M)`` at ``targets`` ``(2, M)`` by the FMM.
+
+        flat_tree.py:1152-1316: the targets get a tree of their own (``leaf_capacity``
+        per leaf, by default the sources'), a dual traversal pairs its cells with the
+        source cells, and local expansions run down the target tree to the targets.
+        ``exclude[i]`` is a source index skipped for target ``i`` in the exact sums.
+        """
+        from ._utils import _finite
+
+        theta = _finite("theta", theta, minimum=0)
+        if not 0 < theta < 1:
+            raise ValueError("FMM theta must lie in (0, 1)")
+        softening = _finite("softening", softening, minimum=0)
+        like_torch = _is_torch(targets) or self._like_torch
+        t = _points(targets, "targets")
+        if not bool(torch.isfinite(t).all()):
+            raise ValueError("targets must be finite")
+        m, n = t.shape[1], self._d["pos"].shape[1]
+        phi = torch.zeros(m, dtype=torch.float64, device=t.device)
+        field = torch.zeros((2, m), dtype=torch.float64, device=t.device)
+        if not m or not n:
+            return _out(phi, like_torch), _out(field, like_torch)
+        if exclude is None:
+            ex = torch.full((m,), -1, dtype=torch.int64, device=t.device)
+        else:
+            ex = _device(exclude, torch.int64).reshape(-1)
+        leaf = self.leaf_capacity if leaf_capacity is None else int(leaf_capacity)
+        tree, (m2l_rows, (prow, pcol, _), counts) = self._target_plan(t, theta, leaf)
+        if self.kernel != "log":
+            return self._coulomb_fmm_at(t, ex, tree, m2l_rows, prow, pcol, softening, potential,
+                                        precision, like_torch)
+        prec = _precision(precision or self.precision)
+        lre, lim = self._local_expansions(theta, prec, target=tree, m2l_rows=m2l_rows)
+        status = torch.zeros(1, dtype=torch.int64, device=t.device)
+        td = tree._d
+        _abi.call(
+            "td3_fmm_at_eval", prec, self.order, _abi.ptr(tree._sorted_pos), _abi.ptr(td["perm"]),
+            _abi.ptr(ex), m, _abi.ptr(td["node_center"]), _abi.ptr(td["node_half"]),
+            _abi.ptr(tree._node_info), tree.node_count, _abi.ptr(lre), _abi.ptr(lim),
+            _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
+            _abi.ptr(self._d["perm"]), n, _abi.ptr(self._node_info), softening,
+            1 if potential else 0, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status),
+            _abi.stream(),
+        )
+        _check(status)
+        self.fmm_at_pairs = counts
+        return _out(phi, like_torch), _out(field, like_torch)
+
+    def _coulomb_fmm_at(self, *args, **kwargs):
+        raise NotImplementedError("the Coulomb (Cartesian) fmm_at is not built yet on the GPU")
 
 
 def _m2l_coefficients(order: int, symmetric: bool) -> np.ndarray:
