@@ -205,10 +205,12 @@ int td3_unscale_locals(const double *node_half, int64_t node_count, int64_t orde
  * L2P + P2P at every source (flat_tree.py:1580-1600, kernels/fmm_evaluate
  * fmm_evaluate_cuda.cu:11-87; kernels/fmm_evaluate_host/fmm_evaluate_host_pyccel.py:14-100):
  * one wavefront per leaf, owner-computes over the leaf's P2P rows (td3_lists of the P2P
- * pairs with self_once).  Own charge excluded; softening on the near pairs only.  phi (N,),
- * field (2, N) written in input order (phi zero when with_potential is 0).
+ * pairs, mode 1).  Own charge excluded; softening on the near pairs only.  kernel TD3_LOG:
+ * scaled complex locals in local_re/local_im; TD3_COULOMB: Cartesian locals (C, nc) in
+ * local_re (cart_evaluate_host_pyccel.py:16-110).  phi (N,), field (2, N) written in input
+ * order (phi zero when with_potential is 0).
  */
-int td3_fmm_eval(int precision, int64_t order, const double *sorted_pos,
+int td3_fmm_eval(int kernel, int precision, int64_t order, const double *sorted_pos,
                  const double *sorted_charge, const int64_t *perm, int64_t n,
                  const double *node_center, const double *node_half, const void *node_info,
                  int64_t node_count, const double *local_re, const double *local_im,
@@ -236,7 +238,7 @@ int td3_dual_traverse(const double *t_center, const double *t_half, const void *
  * target_perm describe the target tree; sorted_pos / sorted_charge / perm / s_info the
  * sources.  phi (M,), field (2, M) written in target order.
  */
-int td3_fmm_at_eval(int precision, int64_t order, const double *sorted_targets,
+int td3_fmm_at_eval(int kernel, int precision, int64_t order, const double *sorted_targets,
                     const int64_t *target_perm, const int64_t *exclude, int64_t m,
                     const double *t_center, const double *t_half, const void *t_info,
                     int64_t t_count, const double *local_re, const double *local_im,
@@ -244,6 +246,28 @@ int td3_fmm_at_eval(int precision, int64_t order, const double *sorted_targets,
                     const double *sorted_charge, const int64_t *perm, int64_t n,
                     const void *s_info, double softening, int with_potential, double *phi,
                     double *field, int64_t *status, void *stream);
+
+/*
+ * The Coulomb (Cartesian) FMM of the softened kernel (flat_tree.py:1360-1428, _cartesian.py,
+ * kernels/cart_*_pyccel.py; CPU-only in the reference).  td3_cart_moments: moments (C, nc)
+ * with nc = (p+1)(p+2)/2, m_g = sum q (r - c)^g by P2M + M2M, and the primed moments
+ * m'_a = (-1)^|a| m_a / a! the M2L reads.  td3_cart_m2l: T_b = sum_a (-1)^|a| C(a+b, a)
+ * m_a A_{a+b}(c_dst - c_src) (A the Taylor coefficients of the softened kernel) into the
+ * rows [node_lo, node_hi) of local (written).  td3_cart_l2l: shift down the tree.
+ */
+int td3_cart_moments(int64_t order, const double *sorted_pos, const double *sorted_charge,
+                     int64_t n, const double *node_center, const void *node_info,
+                     int64_t node_count, const int64_t *levels_host, int64_t n_levels,
+                     const double *binomial, int64_t bsz, double *moments, double *mprime,
+                     void *stream);
+size_t td3_cart_m2l_workspace_bytes(int64_t pair_count, int64_t order, int64_t node_count);
+int td3_cart_m2l(int64_t order, const int32_t *col, const int32_t *dst, const uint32_t *row_start,
+                 int64_t node_lo, int64_t node_hi, const double *dst_center,
+                 const double *src_center, const double *mprime, double softening, double *local,
+                 void *workspace, size_t workspace_bytes, void *stream);
+int td3_cart_l2l(int64_t order, const double *node_center, const void *node_info,
+                 const int64_t *levels_host, int64_t n_levels, const double *binomial,
+                 int64_t bsz, double *local, void *stream);
 
 /*
  * Kernel timing for measurement (no counterpart in the reference, whose profiling is

@@ -73,6 +73,14 @@ def lib():
                                      _I, _P],
             "oracle_log_targets_host": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _I,
                                         _P, _P, _P, _P, _D, _C, _P, _P, _I],
+            "oracle_cart_moments_upward": [_P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
+                                           _I, _I, _P],
+            "oracle_cart_m2l_host": [_P, _P, _I, _P, _P, _P, _I, _D, _P, _P, _P, _P, _P, _I, _P],
+            "oracle_cart_downward": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P],
+            "oracle_cart_evaluate_host": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
+                                          _P, _P, _I, _D, _P, _P, _I, _P],
+            "oracle_cart_targets_host": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P,
+                                         _P, _P, _P, _I, _P, _P, _P, _P, _D, _P, _P, _I],
         }
         for name, args in sig.items():
             fn = getattr(_LIB, name)
@@ -458,6 +466,8 @@ def _fmm_at_log(self, targets, *, exclude=None, theta=0.5, softening=0.0, potent
     m2l, p2p = m2l[: counts[0]].copy(), p2p[: counts[1]].copy()
     self.dual_lists = (m2l, p2p)
     ct, order = tree.node_count, self.order
+    if self.kernel != "log":
+        return _fmm_at_coulomb(self, tree, m2l, p2p, targets, exclude, softening, potential)
     rows = []
     for pairs in (m2l, p2p):
         start = np.zeros(ct + 1, dtype=np.int64)
@@ -500,3 +510,185 @@ def _fmm_at_log(self, targets, *, exclude=None, theta=0.5, softening=0.0, potent
 
 
 OracleTree.fmm_at = _fmm_at_log
+
+
+# ------------------------------------------------------------------ the Coulomb FMM
+_CART: dict = {}
+
+
+def cart_size(order: int) -> int:
+    """_cartesian.py:29-31."""
+    return (order + 1) * (order + 2) // 2
+
+
+def cart_index(a: int, b: int) -> int:
+    """_cartesian.py:34-37: coefficient (a, b) at n(n+1)/2 + b, n = a + b."""
+    n = a + b
+    return n * (n + 1) // 2 + b
+
+
+def cart_tables(order: int) -> dict:
+    """_cartesian.py:40-107: the index tables of the Cartesian expansions."""
+    from math import comb
+
+    if order in _CART:
+        return _CART[order]
+    nc = cart_size(order)
+    ex = np.empty(nc, dtype=np.int64)
+    ey = np.empty(nc, dtype=np.int64)
+    for n in range(order + 1):
+        for b in range(n + 1):
+            ex[cart_index(n - b, b)] = n - b
+            ey[cart_index(n - b, b)] = b
+
+    def at(a, b):
+        return cart_index(a, b) if a >= 0 and b >= 0 else -1
+
+    mono_parent = np.zeros(nc, dtype=np.int64)
+    mono_dir = np.zeros(nc, dtype=np.int64)
+    rec = np.full((nc, 4), -1, dtype=np.int64)
+    grad = np.full((nc, 2), -1, dtype=np.int64)
+    for i in range(nc):
+        a, b = int(ex[i]), int(ey[i])
+        if a > 0:
+            mono_parent[i], mono_dir[i] = at(a - 1, b), 0
+        elif b > 0:
+            mono_parent[i], mono_dir[i] = at(a, b - 1), 1
+        rec[i] = (at(a - 1, b), at(a, b - 1), at(a - 2, b), at(a, b - 2))
+        grad[i] = (at(a - 1, b), at(a, b - 1))
+    m2l, m2l_coef, shift, shift_coef = [], [], [], []
+    for ib in range(nc):
+        bx, by = int(ex[ib]), int(ey[ib])
+        for ia in range(cart_size(order - bx - by)):
+            ax, ay = int(ex[ia]), int(ey[ia])
+            m2l.append((ib, ia, cart_index(ax + bx, ay + by)))
+            m2l_coef.append((-1) ** (ax + ay) * comb(ax + bx, ax) * comb(ay + by, ay))
+    for ig in range(nc):
+        gx, gy = int(ex[ig]), int(ey[ig])
+        for bx in range(gx + 1):
+            for by in range(gy + 1):
+                shift.append((ig, cart_index(bx, by), cart_index(gx - bx, gy - by)))
+                shift_coef.append(comb(gx, bx) * comb(gy, by))
+    out = dict(ex=ex, ey=ey, mono_parent=mono_parent, mono_dir=mono_dir, rec=rec, grad=grad,
+               m2l=np.array(m2l, dtype=np.int64), m2l_coef=np.array(m2l_coef, dtype=float),
+               shift=np.array(shift, dtype=np.int64), shift_coef=np.array(shift_coef, dtype=float))
+    _CART[order] = out
+    return out
+
+
+def _cart_moments(self):
+    """flat_tree.py:1063-1084 (_cartesian_moments)."""
+    tb = cart_tables(self.order)
+    nc = cart_size(self.order)
+    moments = np.empty((self.node_count, nc))
+    lev = _i64(self.levels)
+    lib().oracle_cart_moments_upward(
+        _ptr(self.pos), self.pos.shape[1], _ptr(self.charge), _ptr(self.perm), _ptr(lev),
+        len(lev) - 1, _ptr(self.node_center), _ptr(self.node_start), _ptr(self.node_end),
+        _ptr(self.node_child), _ptr(tb["mono_parent"]), _ptr(tb["mono_dir"]), _ptr(tb["shift"]),
+        _ptr(tb["shift_coef"]), len(tb["shift"]), nc, _ptr(moments))
+    return moments
+
+
+def _cart_local(self, dst_tree, rows, partners, softening):
+    tb = cart_tables(self.order)
+    nc = cart_size(self.order)
+    local = np.zeros((dst_tree.node_count, nc))
+    moments = _cart_moments(self)
+    lib().oracle_cart_m2l_host(
+        _ptr(rows), _ptr(partners), dst_tree.node_count, _ptr(dst_tree.node_center),
+        _ptr(self.node_center), _ptr(moments), nc, float(softening), _ptr(tb["ex"]),
+        _ptr(tb["ey"]), _ptr(tb["rec"]), _ptr(tb["m2l"]), _ptr(tb["m2l_coef"]), len(tb["m2l"]),
+        _ptr(local))
+    lev = _i64(dst_tree.levels)
+    lib().oracle_cart_downward(
+        _ptr(lev), len(lev) - 1, _ptr(dst_tree.node_center), _ptr(dst_tree.node_child),
+        _ptr(tb["mono_parent"]), _ptr(tb["mono_dir"]), _ptr(tb["shift"]), _ptr(tb["shift_coef"]),
+        len(tb["shift"]), nc, _ptr(local))
+    return local
+
+
+def _coulomb_fmm(self, *, theta=0.5, softening=0.0, potential=True):
+    """flat_tree.py:1360-1428 (_coulomb_fmm, CPU)."""
+    if not 0 < theta < 1:
+        raise ValueError("FMM theta must lie in (0, 1)")
+    n = self.pos.shape[1]
+    phi, field = np.zeros(n), np.zeros((2, n))
+    if not n:
+        return phi, field
+    m2l, p2p = self.traverse(theta)
+    c = self.node_count
+    rows = np.zeros(c + 1, dtype=np.int64)
+    partners = np.empty(max(2 * len(m2l), 1), dtype=np.int64)
+    if len(m2l):
+        lib().oracle_pair_csr(_ptr(_i64(m2l)), len(m2l), 1, c, _ptr(rows), _ptr(partners))
+    local = _cart_local(self, self, rows, partners, softening)
+    self.cart_local = local
+    tb = cart_tables(self.order)
+    nc = cart_size(self.order)
+    spos, scharge = self._sorted_sources()
+    phi_s = np.zeros(n)
+    field_s = np.zeros((2, n + 1))
+    status = np.zeros(1, dtype=np.int64)
+    pairs = _i64(p2p) if len(p2p) else np.zeros((1, 2), dtype=np.int64)
+    lib().oracle_cart_evaluate_host(
+        _ptr(spos), n + 1, _ptr(scharge), c, _ptr(self.node_center), _ptr(self.node_start),
+        _ptr(self.node_end), _ptr(self.node_child), _ptr(local), nc, _ptr(tb["mono_parent"]),
+        _ptr(tb["mono_dir"]), _ptr(tb["ex"]), _ptr(tb["ey"]), _ptr(tb["grad"]), _ptr(pairs),
+        len(p2p), float(softening), _ptr(phi_s), _ptr(field_s), n + 1, _ptr(status))
+    if int(status[0]):
+        status[0] = self.perm[int(status[0]) - 1] + 1
+    _check(status)
+    phi[self.perm] = phi_s
+    field[:, self.perm] = field_s[:, :n]
+    self.fmm_pairs = (len(m2l), len(p2p))
+    return phi, field
+
+
+_log_fmm = OracleTree.fmm
+
+
+def _fmm(self, *, theta=0.5, softening=0.0, potential=True):
+    if self.kernel == "log":
+        return _log_fmm(self, theta=theta, softening=softening, potential=potential)
+    return _coulomb_fmm(self, theta=theta, softening=softening, potential=potential)
+
+
+OracleTree.fmm = _fmm
+
+
+def _fmm_at_coulomb(self, tree, m2l, p2p, targets, exclude, softening, potential):
+    """fmm_at's Coulomb branch (flat_tree.py:1247-1295): cart_m2l_host, cart_downward,
+    cart_targets_host."""
+    m, n = targets.shape[1], self.pos.shape[1]
+    ct = tree.node_count
+    rows = []
+    for pairs in (m2l, p2p):
+        start = np.zeros(ct + 1, dtype=np.int64)
+        partners = np.empty(max(len(pairs), 1), dtype=np.int64)
+        lib().oracle_pair_csr(_ptr(_i64(pairs) if len(pairs) else np.zeros((1, 2), np.int64)),
+                              len(pairs), 0, ct, _ptr(start), _ptr(partners))
+        rows.append((start, partners))
+    local = _cart_local(self, tree, rows[0][0], rows[0][1], softening)
+    tb = cart_tables(self.order)
+    nc = cart_size(self.order)
+    tpos, _ = tree._sorted_sources()
+    texclude = _i64(exclude[tree.perm])
+    spos, scharge = self._sorted_sources()
+    phi_s = np.empty(m)
+    field_s = np.empty((2, m + 1))
+    lib().oracle_cart_targets_host(
+        _ptr(tpos), m + 1, _ptr(texclude), ct, _ptr(tree.node_center), _ptr(tree.node_start),
+        _ptr(tree.node_end), _ptr(tree.node_child), _ptr(local), nc, _ptr(tb["mono_parent"]),
+        _ptr(tb["mono_dir"]), _ptr(tb["ex"]), _ptr(tb["ey"]), _ptr(tb["grad"]),
+        _ptr(rows[1][0]), _ptr(rows[1][1]), _ptr(spos), n + 1, _ptr(scharge), _ptr(self.perm),
+        _ptr(self.node_start), _ptr(self.node_end), float(softening), _ptr(phi_s),
+        _ptr(field_s), m + 1)
+    phi, field = np.zeros(m), np.zeros((2, m))
+    phi[tree.perm] = phi_s
+    field[:, tree.perm] = field_s[:, :m]
+    if np.any(phi == 1.0e300):
+        raise _coincident()
+    if not potential:
+        phi[:] = 0.0
+    return phi, field

@@ -953,3 +953,229 @@ void oracle_log_targets_host(const double *tpos, i64 ldt, const i64 *texclude, i
         }
     }
 }
+
+/* ==================================================================================== */
+/* The Coulomb (Cartesian) FMM: kernels/cart_*_pyccel.py with the tables of _cartesian.py */
+/* (built by oracle.py).  nc = (p+1)(p+2)/2 coefficients per cell.                       */
+/* ==================================================================================== */
+
+/* cart_moments_upward/cart_moments_upward_pyccel.py:14-82 */
+void oracle_cart_moments_upward(const double *pos, i64 ldp, const double *charge, const i64 *perm,
+                                const i64 *levels, i64 n_levels, const double *node_center,
+                                const i64 *node_start, const i64 *node_end, const i64 *node_child,
+                                const i64 *mono_parent, const i64 *mono_dir, const i64 *shift,
+                                const double *shift_coef, i64 n_shift, i64 nc, double *moments) {
+    for (i64 level = n_levels - 1; level >= 0; --level) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (i64 node = levels[level]; node < levels[level + 1]; ++node) {
+            double mono[2200];
+            const double xmid = node_center[2 * node], ymid = node_center[2 * node + 1];
+            double *m = moments + node * nc;
+            for (i64 i = 0; i < nc; ++i) m[i] = 0.0;
+            const i64 first = node_child[node];
+            if (first < 0) {
+                for (i64 k = node_start[node]; k < node_end[node]; ++k) {
+                    const i64 j = perm[k];
+                    mono[0] = charge[j];
+                    for (i64 i = 1; i < nc; ++i)
+                        mono[i] = mono[mono_parent[i]] *
+                                  (mono_dir[i] == 0 ? pos[j] - xmid : pos[ldp + j] - ymid);
+                    for (i64 i = 0; i < nc; ++i) m[i] += mono[i];
+                }
+            } else {
+                for (i64 child = first; child < first + 4; ++child) {
+                    if (node_end[child] == node_start[child]) continue;
+                    mono[0] = 1.0;
+                    for (i64 i = 1; i < nc; ++i)
+                        mono[i] = mono[mono_parent[i]] * (mono_dir[i] == 0
+                                                              ? node_center[2 * child] - xmid
+                                                              : node_center[2 * child + 1] - ymid);
+                    const double *cm = moments + child * nc;
+                    for (i64 t = 0; t < n_shift; ++t)
+                        m[shift[3 * t]] += shift_coef[t] * cm[shift[3 * t + 1]] * mono[shift[3 * t + 2]];
+                }
+            }
+        }
+    }
+}
+
+/* cart_m2l_host/cart_m2l_host_pyccel.py:16-80 */
+void oracle_cart_m2l_host(const i64 *row_start, const i64 *sources, i64 n_nodes,
+                          const double *dst_center, const double *src_center,
+                          const double *moments, i64 nc, double softening, const i64 *ex,
+                          const i64 *ey, const i64 *rec, const i64 *m2l, const double *m2l_coef,
+                          i64 n_terms, double *local) {
+    const double eps2 = softening * softening;
+    const i64 n_chunks = (n_nodes + 63) / 64;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (i64 chunk = 0; chunk < n_chunks; ++chunk) {
+        double acc[2200], coef[2200];
+        const i64 last = (64 * chunk + 64 < n_nodes) ? 64 * chunk + 64 : n_nodes;
+        for (i64 dst = 64 * chunk; dst < last; ++dst) {
+            if (row_start[dst + 1] == row_start[dst]) continue;
+            for (i64 i = 0; i < nc; ++i) acc[i] = 0.0;
+            const double cx = dst_center[2 * dst], cy = dst_center[2 * dst + 1];
+            for (i64 e = row_start[dst]; e < row_start[dst + 1]; ++e) {
+                const i64 src = sources[e];
+                const double tx = cx - src_center[2 * src], ty = cy - src_center[2 * src + 1];
+                const double r2 = tx * tx + ty * ty + eps2;
+                coef[0] = 1.0 / sqrt(r2);
+                for (i64 i = 1; i < nc; ++i) {
+                    const i64 n = ex[i] + ey[i];
+                    double s = 0.0;
+                    if (rec[4 * i] >= 0) s += (2 * n - 1) * tx * coef[rec[4 * i]];
+                    if (rec[4 * i + 1] >= 0) s += (2 * n - 1) * ty * coef[rec[4 * i + 1]];
+                    if (rec[4 * i + 2] >= 0) s += (n - 1) * coef[rec[4 * i + 2]];
+                    if (rec[4 * i + 3] >= 0) s += (n - 1) * coef[rec[4 * i + 3]];
+                    coef[i] = -s / (n * r2);
+                }
+                const double *ms = moments + src * nc;
+                for (i64 t = 0; t < n_terms; ++t)
+                    acc[m2l[3 * t]] += m2l_coef[t] * ms[m2l[3 * t + 1]] * coef[m2l[3 * t + 2]];
+            }
+            for (i64 i = 0; i < nc; ++i) local[dst * nc + i] += acc[i];
+        }
+    }
+}
+
+/* cart_downward/cart_downward_pyccel.py:14-55 */
+void oracle_cart_downward(const i64 *levels, i64 n_levels, const double *node_center,
+                          const i64 *node_child, const i64 *mono_parent, const i64 *mono_dir,
+                          const i64 *shift, const double *shift_coef, i64 n_shift, i64 nc,
+                          double *local) {
+    for (i64 level = 0; level < n_levels - 1; ++level) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (i64 node = levels[level]; node < levels[level + 1]; ++node) {
+            double mono[2200];
+            const i64 first = node_child[node];
+            if (first < 0) continue;
+            for (i64 child = first; child < first + 4; ++child) {
+                mono[0] = 1.0;
+                for (i64 i = 1; i < nc; ++i)
+                    mono[i] = mono[mono_parent[i]] *
+                              (mono_dir[i] == 0 ? node_center[2 * child] - node_center[2 * node]
+                                                : node_center[2 * child + 1] - node_center[2 * node + 1]);
+                for (i64 t = 0; t < n_shift; ++t)
+                    local[child * nc + shift[3 * t + 1]] +=
+                        shift_coef[t] * local[node * nc + shift[3 * t]] * mono[shift[3 * t + 2]];
+            }
+        }
+    }
+}
+
+/* cart_evaluate_host/cart_evaluate_host_pyccel.py:16-110 (far part: local expansions at
+ * the leaves' particles; near part: the mutual softened pairs, serial). */
+void oracle_cart_evaluate_host(const double *spos, i64 lds, const double *scharge, i64 node_count,
+                               const double *node_center, const i64 *node_start,
+                               const i64 *node_end, const i64 *node_child, const double *local,
+                               i64 nc, const i64 *mono_parent, const i64 *mono_dir, const i64 *ex,
+                               const i64 *ey, const i64 *grad, const i64 *pairs, i64 count,
+                               double softening, double *phi, double *field, i64 ldf, i64 *status) {
+    const double eps2 = softening * softening;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (i64 node = 0; node < node_count; ++node) {
+        double mono[2200];
+        if (node_child[node] >= 0) continue;
+        const double *L = local + node * nc;
+        for (i64 k = node_start[node]; k < node_end[node]; ++k) {
+            mono[0] = 1.0;
+            for (i64 i = 1; i < nc; ++i)
+                mono[i] = mono[mono_parent[i]] * (mono_dir[i] == 0 ? spos[k] - node_center[2 * node]
+                                                                   : spos[lds + k] - node_center[2 * node + 1]);
+            double w = 0.0, gx = 0.0, gy = 0.0;
+            for (i64 i = 0; i < nc; ++i) {
+                w += L[i] * mono[i];
+                if (grad[2 * i] >= 0) gx += ex[i] * L[i] * mono[grad[2 * i]];
+                if (grad[2 * i + 1] >= 0) gy += ey[i] * L[i] * mono[grad[2 * i + 1]];
+            }
+            phi[k] += w;
+            field[k] -= gx;
+            field[ldf + k] -= gy;
+        }
+    }
+    for (i64 pair = 0; pair < count; ++pair) {
+        const i64 a = pairs[2 * pair], b = pairs[2 * pair + 1];
+        for (i64 ka = node_start[a]; ka < node_end[a]; ++ka) {
+            const double xi = spos[ka], yi = spos[lds + ka], qi = scharge[ka];
+            double pi = 0.0, fxi = 0.0, fyi = 0.0;
+            const i64 start_b = (a == b) ? ka + 1 : node_start[b];
+            for (i64 kb = start_b; kb < node_end[b]; ++kb) {
+                const double qj = scharge[kb];
+                const double dx = xi - spos[kb], dy = yi - spos[lds + kb];
+                const double r2 = dx * dx + dy * dy + eps2;
+                if (r2 == 0.0) {
+                    if (qi != 0.0 && qj != 0.0) {
+                        status[0] = ka + 1;
+                        return;
+                    }
+                    continue;
+                }
+                const double inv_r = 1.0 / sqrt(r2);
+                const double inv_r3 = inv_r * inv_r * inv_r;
+                pi += qj * inv_r;
+                phi[kb] += qi * inv_r;
+                fxi += qj * inv_r3 * dx;
+                fyi += qj * inv_r3 * dy;
+                field[kb] -= qi * inv_r3 * dx;
+                field[ldf + kb] -= qi * inv_r3 * dy;
+            }
+            phi[ka] += pi;
+            field[ka] += fxi;
+            field[ldf + ka] += fyi;
+        }
+    }
+}
+
+/* cart_targets_host/cart_targets_host_pyccel.py:16-98 */
+void oracle_cart_targets_host(const double *tpos, i64 ldt, const i64 *texclude, i64 t_count,
+                              const double *t_center, const i64 *t_start, const i64 *t_end,
+                              const i64 *t_child, const double *local, i64 nc,
+                              const i64 *mono_parent, const i64 *mono_dir, const i64 *ex_pow,
+                              const i64 *ey_pow, const i64 *grad, const i64 *row_start,
+                              const i64 *partners, const double *spos, i64 lds,
+                              const double *scharge, const i64 *sindex, const i64 *s_start,
+                              const i64 *s_end, double softening, double *phi, double *field,
+                              i64 ldf) {
+    const double eps2 = softening * softening;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (i64 node = 0; node < t_count; ++node) {
+        double mono[2200];
+        if (t_child[node] >= 0) continue;
+        const double *L = local + node * nc;
+        for (i64 k = t_start[node]; k < t_end[node]; ++k) {
+            mono[0] = 1.0;
+            for (i64 i = 1; i < nc; ++i)
+                mono[i] = mono[mono_parent[i]] * (mono_dir[i] == 0 ? tpos[k] - t_center[2 * node]
+                                                                   : tpos[ldt + k] - t_center[2 * node + 1]);
+            double pot = 0.0, ex = 0.0, ey = 0.0;
+            for (i64 i = 0; i < nc; ++i) {
+                pot += L[i] * mono[i];
+                if (grad[2 * i] >= 0) ex -= ex_pow[i] * L[i] * mono[grad[2 * i]];
+                if (grad[2 * i + 1] >= 0) ey -= ey_pow[i] * L[i] * mono[grad[2 * i + 1]];
+            }
+            int bad = 0;
+            const i64 skip = texclude[k];
+            for (i64 e = row_start[node]; e < row_start[node + 1]; ++e) {
+                const i64 src = partners[e];
+                for (i64 j = s_start[src]; j < s_end[src]; ++j) {
+                    const double c = scharge[j];
+                    if (c == 0.0 || sindex[j] == skip) continue;
+                    const double dx = tpos[k] - spos[j], dy = tpos[ldt + k] - spos[lds + j];
+                    const double r2 = dx * dx + dy * dy + eps2;
+                    if (r2 == 0.0) {
+                        bad = 1;
+                        continue;
+                    }
+                    const double inv_r = 1.0 / sqrt(r2);
+                    const double inv_r3 = inv_r * inv_r * inv_r;
+                    pot += c * inv_r;
+                    ex += c * inv_r3 * dx;
+                    ey += c * inv_r3 * dy;
+                }
+            }
+            phi[k] = bad ? TD3_BAD : pot;
+            field[k] = ex;
+            field[ldf + k] = ey;
+        }
+    }
+}

@@ -1,0 +1,188 @@
+"""Test infrastructure: runs the package's real orchestration on the CPU build of the
+kernels (libtd3_emu.so, see build.sh), under AddressSanitizer, on small inputs, and checks
+the results against the oracle.  An out-of-bounds access aborts the process.
+
+Run by tests/test_cpu_emulation.py in a subprocess with LD_PRELOAD=libasan.  Usage:
+    python run_emulated.py [case ...]
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+
+td3 = __graft_entry__.load_package()
+from tree_d_3d_amd import _abi  # noqa: E402
+
+from oracle import oracle  # noqa: E402
+
+# the emulated library on the CPU, instead of the GPU build
+_abi.LIB_PATH = os.path.join(HERE, "libtd3_emu.so")
+_abi._LIB = None
+_abi.device = lambda: torch.device("cpu")
+_abi.stream = lambda: 0
+_abi.ptr = lambda t: None if t is None else t.data_ptr()
+
+
+def relative(a, b):
+    scale = np.max(np.abs(b)) if np.size(b) else 0.0
+    diff = np.max(np.abs(a - b)) if np.size(b) else 0.0
+    return diff / scale if scale > 0 else diff
+
+
+def cloud(n, seed):
+    rng = np.random.default_rng(seed)
+    pos = np.concatenate(
+        (rng.normal([[-0.3], [0.1]], 0.1, (2, n // 2)),
+         rng.normal([[0.4], [-0.2]], 0.25, (2, n - n // 2))), axis=1).clip(-0.99, 0.99)
+    return pos, rng.normal(0.3, 1, n)
+
+
+TREE = ("perm", "node_start", "node_end", "node_child", "node_depth", "node_center",
+        "node_half", "levels")
+
+
+def same_tree(a, b):
+    assert a.node_count == b.node_count, (a.node_count, b.node_count)
+    for name in TREE:
+        x, y = np.asarray(getattr(a, name)), np.asarray(getattr(b, name))
+        assert x.shape == y.shape and np.array_equal(x, y), name
+
+
+def case_direct():
+    pos, q = cloud(300, 1)
+    q[::5] = 0
+    rng = np.random.default_rng(2)
+    targets = rng.uniform(-1, 1, (2, 70))
+    exclude = rng.integers(-1, 300, 70)
+    for kernel in ("coulomb", "log"):
+        for prec, tol in (("fp64", 1e-12), ("mixed", 1e-5)):
+            got = td3.direct_field(targets, pos, q, softening=1e-3, exclude=exclude,
+                                   kernel=kernel, precision=prec)
+            ref = oracle.direct_field(targets, pos, q, softening=1e-3, exclude=exclude,
+                                      kernel=kernel)
+            assert relative(got[0], ref[0]) < tol and relative(got[1], ref[1]) < tol
+
+
+def case_tree():
+    rng = np.random.default_rng(5)
+    angle = rng.uniform(0, 2 * np.pi, 300)
+    clouds = {
+        "uniform": rng.uniform(-0.99, 0.99, (2, 400)),
+        "circle": 0.9 * np.stack((np.cos(angle), np.sin(angle))),
+        "clustered": np.clip(rng.normal(0.3, 0.01, (2, 200)), -1, 1),
+        "lattice": np.stack(np.meshgrid(np.linspace(-0.5, 0.5, 9),
+                                        np.linspace(-0.5, 0.5, 9))).reshape(2, -1),
+        "coincident": np.full((2, 20), 0.25),
+        "edges": np.array([[-1.0, 1.0, 0.0, 0.0, 0.5, -0.5, 1.0, -1.0],
+                           [1.0, -1.0, 0.0, 0.5, 0.0, -0.5, 1.0, -1.0]]),
+    }
+    for name, pos in clouds.items():
+        for leaf in (1, 6):
+            q = np.ones(pos.shape[1])
+            same_tree(td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), leaf_capacity=leaf),
+                      oracle.OracleTree(pos, q, (0, 0, 1), leaf_capacity=leaf))
+    pos = np.array([[0.3, 0.3, -0.5, 0.3000000001], [0.3, 0.3, 0.2, 0.3]])
+    q = np.array([1.0, 1.0, -1.0, 2.0])
+    for depth in (1, 5, 31, 32, 33, 48, 64):
+        same_tree(td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), leaf_capacity=1, max_depth=depth),
+                  oracle.OracleTree(pos, q, (0, 0, 1), leaf_capacity=1, max_depth=depth))
+
+
+def case_moments():
+    pos, q = cloud(400, 3)
+    q[::9] = 0.0
+    for kernel in ("coulomb", "log"):
+        t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel=kernel, order=8, leaf_capacity=4)
+        r = oracle.OracleTree(pos, q, (0, 0, 1), kernel=kernel, order=8, leaf_capacity=4)
+        same_tree(t, r)
+        assert np.allclose(t.node_qabs, r.node_qabs, rtol=1e-12, atol=1e-12)
+        if kernel == "log":
+            assert np.allclose(t.moment_re, r.moment_re, rtol=1e-10, atol=1e-12)
+        else:
+            assert np.allclose(t.node_s, r.node_s, rtol=1e-10, atol=1e-12)
+
+
+def case_walk():
+    pos, q = cloud(300, 4)
+    own = np.arange(300)
+    for kernel in ("coulomb", "log"):
+        for prec, tol in (("fp64", 1e-12), ("mixed", 1e-6)):
+            t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel=kernel, order=8,
+                                 leaf_capacity=4, precision=prec)
+            r = oracle.OracleTree(pos, q, (0, 0, 1), kernel=kernel, order=8, leaf_capacity=4)
+            got = t.evaluate(pos, exclude=own, theta=0.5, softening=0.01)
+            want = r.evaluate(pos, exclude=own, theta=0.5, softening=0.01)
+            assert relative(got[0], want[0]) < tol and relative(got[1], want[1]) < tol
+        probes = np.random.default_rng(1).uniform(-1, 1, (2, 90))
+        got = t.evaluate(probes, theta=0.6)
+        want = r.evaluate(probes, theta=0.6)
+        assert relative(got[1], want[1]) < 1e-6
+
+
+def case_fmm():
+    pos, q = cloud(260, 0)
+    for leaf, order in ((1, 4), (8, 12)):
+        t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=order,
+                             leaf_capacity=leaf)
+        r = oracle.OracleTree(pos, q, (0, 0, 1), kernel="log", order=order, leaf_capacity=leaf)
+        got = t.fmm(theta=0.5, softening=1e-3)
+        want = r.fmm(theta=0.5, softening=1e-3)
+        assert t.fmm_pairs == r.fmm_pairs, (t.fmm_pairs, r.fmm_pairs)
+        assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
+        mixed = t.fmm(theta=0.5, softening=1e-3, precision="mixed")
+        assert relative(mixed[1], want[1]) < 1e-5
+        t.set_charges(q[::-1].copy())
+        r.set_charges(q[::-1].copy())
+        got, want = t.fmm(theta=0.5, softening=1e-3), r.fmm(theta=0.5, softening=1e-3)
+        assert relative(got[1], want[1]) < 1e-12
+
+
+def case_fmm_at():
+    rng = np.random.default_rng(13)
+    pos, q = cloud(250, 13)
+    targets = np.concatenate((rng.uniform(-0.9, 0.9, (2, 100)), pos[:, :30]), axis=1)
+    exclude = np.concatenate((np.full(100, -1), np.arange(30)))
+    t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10, leaf_capacity=8)
+    r = oracle.OracleTree(pos, q, (0, 0, 1), kernel="log", order=10, leaf_capacity=8)
+    got = t.fmm_at(targets, exclude=exclude, theta=0.5)
+    want = r.fmm_at(targets, exclude=exclude, theta=0.5)
+    assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
+
+
+def case_coulomb():
+    pos, q = cloud(200, 11)
+    own = np.arange(300)
+    for order in (4, 13):
+        t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), order=order, leaf_capacity=8)
+        r = oracle.OracleTree(pos, q, (0, 0, 1), order=order, leaf_capacity=8)
+        got = t.fmm(theta=0.5, softening=0.01)
+        want = r.fmm(theta=0.5, softening=0.01)
+        assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-11
+    rng = np.random.default_rng(3)
+    targets = np.concatenate((rng.uniform(-0.9, 0.9, (2, 60)), pos[:, :20]), axis=1)
+    exclude = np.concatenate((np.full(60, -1), np.arange(20)))
+    t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), order=6, leaf_capacity=8)
+    r = oracle.OracleTree(pos, q, (0, 0, 1), order=6, leaf_capacity=8)
+    got = t.fmm_at(targets, exclude=exclude, theta=0.5, softening=0.01)
+    want = r.fmm_at(targets, exclude=exclude, theta=0.5, softening=0.01)
+    assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
+
+
+CASES = {name[5:]: fn for name, fn in globals().items() if name.startswith("case_")}
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(CASES)
+    for name in names:
+        print(f"emulated case {name}", flush=True)
+        CASES[name]()
+    print("EMULATION OK", flush=True)

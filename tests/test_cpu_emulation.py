@@ -1,0 +1,34 @@
+"""The library's kernels compiled for the CPU (tests/cpu_emulation) under AddressSanitizer:
+the real orchestration runs on small inputs against the oracle, so an out-of-bounds index
+or a wrong buffer size fails here, on the CPU, before any GPU run.
+
+TD3_EMULATION=quick runs only the fast cases; the default runs all (about two minutes).
+"""
+
+import os
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_emulation")
+QUICK = ["direct", "moments"]
+FULL = ["direct", "tree", "moments", "walk", "fmm", "fmm_at", "coulomb"]
+
+
+@pytest.fixture(scope="module")
+def emulated_library():
+    r = subprocess.run(["bash", os.path.join(HERE, "build.sh")], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.fail(f"CPU build of the kernels failed:\n{r.stderr[-4000:]}")
+    return os.path.join(HERE, "libtd3_emu.so")
+
+
+def test_kernels_under_address_sanitizer(emulated_library):
+    asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True,
+                          text=True).stdout.strip()
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
+    cases = QUICK if os.environ.get("TD3_EMULATION") == "quick" else FULL
+    r = subprocess.run([sys.executable, os.path.join(HERE, "run_emulated.py"), *cases],
+                       capture_output=True, text=True, env=env, timeout=3000)
+    assert r.returncode == 0 and "EMULATION OK" in r.stdout, (r.stdout[-3000:] + r.stderr[-6000:])

@@ -195,3 +195,45 @@ def test_fmm_at_validation(td3):
     with pytest.raises(ValueError, match="theta"):
         tree.fmm_at(pos, theta=1.0)
     assert tree.fmm_at(np.zeros((2, 0)))[1].shape == (2, 0)
+
+
+@pytest.mark.parametrize("softening", [0.0, 0.02])
+def test_coulomb_fmm_matches_oracle(td3, oracle, softening):
+    rng = np.random.default_rng(11)
+    pos = np.concatenate((rng.uniform(-0.9, 0.9, (2, 1500)), rng.normal(0.3, 0.05, (2, 1500))),
+                         axis=1).clip(-0.99, 0.99)
+    q = rng.normal(size=pos.shape[1])
+    own = np.arange(pos.shape[1])
+    _, exact = oracle.direct_field(pos, pos, q, exclude=own, softening=softening)
+    errors = []
+    for order in (2, 4, 6, 8, 10, 13):
+        print(f"coulomb fmm order={order} eps={softening}", flush=True)
+        tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), order=order, leaf_capacity=16)
+        phi, field = tree.fmm(theta=0.5, softening=softening)
+        ref = oracle.OracleTree(pos, q, (0, 0, 1), order=order, leaf_capacity=16)
+        rphi, rfield = ref.fmm(theta=0.5, softening=softening)
+        assert tree.fmm_pairs == ref.fmm_pairs
+        assert relative(phi, rphi) < 1e-12 and relative(field, rfield) < 1e-11
+        errors.append(relative(field, exact))
+        net = np.abs((q * field).sum(axis=1)).max()
+        assert net < 1e-12 * np.abs(q * field).sum()
+    assert np.all(np.diff(errors[:4]) < 0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_coulomb_fmm_at_matches_oracle(td3, oracle, precision):
+    rng = np.random.default_rng(13)
+    pos, q = cloud(4000, seed=13)
+    targets = np.concatenate((rng.uniform(-0.9, 0.9, (2, 1500)), pos[:, :500]), axis=1)
+    exclude = np.concatenate((np.full(1500, -1), np.arange(500)))
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), order=8, leaf_capacity=16)
+    ref = oracle.OracleTree(pos, q, (0, 0, 1), order=8, leaf_capacity=16)
+    phi, field = tree.fmm_at(targets, exclude=exclude, theta=0.5, softening=0.01,
+                             precision=precision)
+    rphi, rfield = ref.fmm_at(targets, exclude=exclude, theta=0.5, softening=0.01)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert relative(phi, rphi) < tol and relative(field, rfield) < tol
+    # test_fmm.py:244-246: fmm_at against the walk
+    _, walk = tree.evaluate(targets, exclude=exclude, theta=0.5, softening=0.01)
+    _, direct = oracle.direct_field(targets, pos, q, exclude=exclude, softening=0.01)
+    assert np.max(np.abs(field - walk)) < 20 * 2e-4 * np.max(np.abs(direct))

@@ -255,3 +255,61 @@ def test_fmm_at_matches_direct_and_covers_pairs(oracle):
     for a, b in np.concatenate((m2l, p2p)):
         cover[np.ix_(tm[a], sm[b])] += 1
     assert np.all(cover == 1)
+
+
+def test_cartesian_taylor_coefficients_match_sympy(oracle):
+    # the M2L's A_g(t) = D^g f(t) / g! of f = (|t|^2 + eps^2)^(-1/2) (_cartesian.py:11-14):
+    # one unit moment m_0 = 1 at the source gives local T_b = A_b(t) exactly
+    x, y, e = sympy.symbols("x y e", real=True)
+    f = 1 / sympy.sqrt(x**2 + y**2 + e**2)
+    order = 5
+    tb = oracle.cart_tables(order)
+    nc = oracle.cart_size(order)
+    tx, ty, eps = 0.7, -0.45, 0.2
+    centers = np.array([[tx, ty], [0.0, 0.0]])
+    moments = np.zeros((2, nc))
+    moments[1, 0] = 1.0
+    local = np.zeros((2, nc))
+    rows = np.array([0, 1, 1], dtype=np.int64)
+    src = np.array([1], dtype=np.int64)
+    oracle.lib().oracle_cart_m2l_host(
+        rows.ctypes.data, src.ctypes.data, 2, centers.ctypes.data, centers.ctypes.data,
+        moments.ctypes.data, nc, eps, tb["ex"].ctypes.data, tb["ey"].ctypes.data,
+        tb["rec"].ctypes.data, tb["m2l"].ctypes.data, tb["m2l_coef"].ctypes.data, len(tb["m2l"]),
+        local.ctypes.data)
+    for i in range(nc):
+        a, b = int(tb["ex"][i]), int(tb["ey"][i])
+        d = sympy.diff(f, x, a, y, b) if a or b else f
+        want = float(d.subs({x: tx, y: ty, e: eps})) / (sympy.factorial(a) * sympy.factorial(b))
+        assert local[0, i] == pytest.approx(want, rel=1e-12, abs=1e-14)
+
+
+@pytest.mark.parametrize("softening", [0.0, 0.02])
+def test_coulomb_fmm_converges_and_conserves_momentum(oracle, softening):
+    # test_fmm.py:189-207
+    rng = np.random.default_rng(11)
+    pos = np.concatenate((rng.uniform(-0.9, 0.9, (2, 1500)), rng.normal(0.3, 0.05, (2, 1500))),
+                         axis=1).clip(-0.99, 0.99)
+    q = rng.normal(size=pos.shape[1])
+    own = np.arange(pos.shape[1])
+    pe, exact = oracle.direct_field(pos, pos, q, exclude=own, softening=softening)
+    errors = []
+    for order in (2, 4, 6, 8):
+        t = oracle.OracleTree(pos, q, (0, 0, 1), order=order, leaf_capacity=16)
+        phi, field = t.fmm(theta=0.5, softening=softening)
+        errors.append(relative(field, exact))
+        assert np.max(np.abs(phi - pe)) < 0.05 * np.max(np.abs(pe))
+        net = np.abs((q * field).sum(axis=1)).max()
+        assert net < 1e-12 * np.abs(q * field).sum()
+    assert errors[-1] < 1e-2 * errors[0] and np.all(np.diff(errors) < 0)
+
+
+def test_coulomb_fmm_at(oracle):
+    rng = np.random.default_rng(13)
+    pos, q = cloud(4000, seed=13)
+    targets = np.concatenate((rng.uniform(-0.9, 0.9, (2, 1500)), pos[:, :500]), axis=1)
+    exclude = np.concatenate((np.full(1500, -1), np.arange(500)))
+    t = oracle.OracleTree(pos, q, (0, 0, 1), order=8, leaf_capacity=16)
+    phi, field = t.fmm_at(targets, exclude=exclude, theta=0.5, softening=0.01)
+    rp, rf = oracle.direct_field(targets, pos, q, exclude=exclude, softening=0.01)
+    assert relative(field, rf) < 2e-4 and relative(phi, rp) < 2e-4

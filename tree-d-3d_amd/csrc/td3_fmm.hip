@@ -255,12 +255,16 @@ constexpr int kChunk = 8;
 // (fmm_evaluate_host_pyccel.py:64-100).  MODE 1 (fmm_at): targets of their own tree;
 // uncharged sources and source exclude[i] are skipped; any remaining coincidence is an
 // error (log_targets_host_pyccel.py:14-94).
-template <int PREC, int MODE>
+// KERNEL TD3_LOG: scaled complex locals (lre, lim; p1 = order + 1 coefficients);
+// TD3_COULOMB: the Cartesian locals T_g of degree <= order in lre (reference layout),
+// softened near field q / sqrt(r^2 + eps^2) (cart_evaluate_host_pyccel.py:16-110,
+// cart_targets_host_pyccel.py:16-98).
+template <int KERNEL, int PREC, int MODE>
 __global__ __launch_bounds__(256) void eval_kernel(
     const double* __restrict__ tpos, const i64* __restrict__ tperm, const i64* __restrict__ exclude,
     i64 m, const double* __restrict__ tcenter, const double* __restrict__ thalf,
     const int4* __restrict__ tinfo, i64 t_count, const double* __restrict__ lre,
-    const double* __restrict__ lim, int p1, const u32* __restrict__ prow,
+    const double* __restrict__ lim, int p1, int order, const u32* __restrict__ prow,
     const int* __restrict__ pcol, const double* __restrict__ spos,
     const double* __restrict__ scharge, const i64* __restrict__ sperm, i64 n,
     const int4* __restrict__ sinfo, double eps2, int with_potential, double* __restrict__ phi,
@@ -307,14 +311,15 @@ __global__ __launch_bounds__(256) void eval_kernel(
             const int total = __shfl(incl, 63, 64);
             for (int base = 0; base < total; base += 64) {
                 const int idx = base + lane;
-                // owning lane: first lane whose inclusive prefix exceeds idx
-                int lo_l = 0, hi_l = 63;
-                while (lo_l < hi_l) {
-                    const int mid = (lo_l + hi_l) >> 1;
-                    const int pv = __shfl(incl, mid, 64);
-                    if (pv > idx) hi_l = mid; else lo_l = mid + 1;
+                // owning lane: first lane whose inclusive prefix exceeds idx (a fixed
+                // six-step search, so every lane takes part in every shuffle)
+                int lo_l = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int pv = __shfl(incl, lo_l + step - 1, 64);
+                    if (pv <= idx) lo_l += step;
                 }
-                const int owner = lo_l;
+                const int owner = lo_l < 63 ? lo_l : 63;
                 const int ostart = __shfl(sstart, owner, 64);
                 const int oincl = __shfl(incl, owner, 64);
                 const int ocount = __shfl(scount, owner, 64);
@@ -334,7 +339,21 @@ __global__ __launch_bounds__(256) void eval_kernel(
                         if (MODE == 1 || (tq[j] != 0.0 && sq != 0.0)) bad_slot = t0 + j;
                         continue;
                     }
-                    if (PREC == TD3_FP64) {
+                    if (KERNEL == TD3_COULOMB) {
+                        if (PREC == TD3_FP64) {
+                            const double inv_r = 1.0 / sqrt(r2);
+                            const double cr3 = sq * inv_r * inv_r * inv_r;
+                            acc_p[j] += sq * inv_r;
+                            acc_x[j] += cr3 * dx;
+                            acc_y[j] += cr3 * dy;
+                        } else {
+                            const float inv_r = __frsqrt_rn((float)r2), cf = (float)sq;
+                            const float cr3 = cf * inv_r * inv_r * inv_r;
+                            acc_p[j] += (double)(cf * inv_r);
+                            acc_x[j] += (double)(cr3 * (float)dx);
+                            acc_y[j] += (double)(cr3 * (float)dy);
+                        }
+                    } else if (PREC == TD3_FP64) {
                         if (with_potential) acc_p[j] -= sq * log(r2);
                         const double inv = 2.0 * sq / r2;
                         acc_x[j] += inv * dx;
@@ -368,43 +387,67 @@ __global__ __launch_bounds__(256) void eval_kernel(
                     fx = acc_x[j];
                     fy = acc_y[j];
                 }
-            // Horner on the scaled local expansion at u' = (z - c) / r
-            const double ur = (tpos[k] - cx) * inv_r, ui = (tpos[m + k] - cy) * inv_r;
-            double w_re = lre[leaf * p1 + p1 - 1], w_im = lim[leaf * p1 + p1 - 1];
-            double d_re = 0.0, d_im = 0.0;
-            for (int l = p1 - 2; l >= 0; --l) {
-                const double nd = d_re * ur - d_im * ui + w_re;
-                d_im = d_re * ui + d_im * ur + w_im;
-                d_re = nd;
-                const double nw = w_re * ur - w_im * ui + lre[leaf * p1 + l];
-                w_im = w_re * ui + w_im * ur + lim[leaf * p1 + l];
-                w_re = nw;
-            }
             const i64 i = tperm[k];
-            phi[i] = with_potential ? p - 2.0 * w_re : 0.0;
-            field[i] = fx + 2.0 * d_re * inv_r;
-            field[m + i] = fy - 2.0 * d_im * inv_r;
+            if (KERNEL == TD3_COULOMB) {
+                // phi += sum_g T_g u^g, E -= grad (u = z - c, unscaled Cartesian locals)
+                const double ux = tpos[k] - cx, uy = tpos[m + k] - cy;
+                const double* T = lre + leaf * (i64)p1;
+                double w = 0.0, gx = 0.0, gy = 0.0, pyb = 1.0, pyb1 = 0.0;
+                for (int by = 0; by <= order; ++by) {
+                    double px = 1.0, px1 = 0.0;
+                    for (int bx = 0; bx + by <= order; ++bx) {
+                        const double t = T[(bx + by) * (bx + by + 1) / 2 + by];
+                        w += t * px * pyb;
+                        gx += bx * t * px1 * pyb;
+                        gy += by * t * px * pyb1;
+                        px1 = px;
+                        px *= ux;
+                    }
+                    pyb1 = pyb;
+                    pyb *= uy;
+                }
+                phi[i] = with_potential ? p + w : 0.0;
+                field[i] = fx - gx;
+                field[m + i] = fy - gy;
+            } else {
+                // Horner on the scaled local expansion at u' = (z - c) / r
+                const double ur = (tpos[k] - cx) * inv_r, ui = (tpos[m + k] - cy) * inv_r;
+                double w_re = lre[leaf * p1 + p1 - 1], w_im = lim[leaf * p1 + p1 - 1];
+                double d_re = 0.0, d_im = 0.0;
+                for (int l = p1 - 2; l >= 0; --l) {
+                    const double nd = d_re * ur - d_im * ui + w_re;
+                    d_im = d_re * ui + d_im * ur + w_im;
+                    d_re = nd;
+                    const double nw = w_re * ur - w_im * ui + lre[leaf * p1 + l];
+                    w_im = w_re * ui + w_im * ur + lim[leaf * p1 + l];
+                    w_re = nw;
+                }
+                phi[i] = with_potential ? p - 2.0 * w_re : 0.0;
+                field[i] = fx + 2.0 * d_re * inv_r;
+                field[m + i] = fy - 2.0 * d_im * inv_r;
+            }
         }
     }
 }
 
-template <int MODE>
+template <int KERNEL, int MODE>
 int launch_eval(int precision, hipStream_t s, const double* tpos, const i64* tperm,
                 const i64* exclude, i64 m, const double* tcenter, const double* thalf,
-                const int4* tinfo, i64 t_count, const double* lre, const double* lim, int p1,
+                const int4* tinfo, i64 t_count, const double* lre, const double* lim, int order,
                 const u32* prow, const int* pcol, const double* spos, const double* scharge,
                 const i64* sperm, i64 n, const int4* sinfo, double eps2, int with_potential,
                 double* phi, double* field, i64* status) {
     const unsigned blocks = td3_blocks(t_count * 64, 256);
+    const int p1 = KERNEL == TD3_LOG ? order + 1 : (order + 1) * (order + 2) / 2;
     if (precision == TD3_FP64)
-        TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<TD3_FP64, MODE>), dim3(blocks), dim3(256), 0, s,
-                           tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, prow,
-                           pcol, spos, scharge, sperm, n, sinfo, eps2, with_potential, phi, field,
+        TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<KERNEL, TD3_FP64, MODE>), dim3(blocks), dim3(256), 0, s,
+                           tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, order,
+                           prow, pcol, spos, scharge, sperm, n, sinfo, eps2, with_potential, phi, field,
                            status));
     else
-        TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<TD3_MIXED, MODE>), dim3(blocks), dim3(256), 0, s,
-                           tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, prow,
-                           pcol, spos, scharge, sperm, n, sinfo, eps2, with_potential, phi, field,
+        TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<KERNEL, TD3_MIXED, MODE>), dim3(blocks), dim3(256), 0, s,
+                           tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, order,
+                           prow, pcol, spos, scharge, sperm, n, sinfo, eps2, with_potential, phi, field,
                            status));
     TD3_LAUNCHED();
     return TD3_OK;
@@ -532,36 +575,39 @@ extern "C" int td3_unscale_locals(const double* node_half, int64_t node_count, i
     return TD3_OK;
 }
 
-extern "C" int td3_fmm_eval(int precision, int64_t order, const double* sorted_pos,
+extern "C" int td3_fmm_eval(int kernel, int precision, int64_t order, const double* sorted_pos,
                             const double* sorted_charge, const int64_t* perm, int64_t n,
                             const double* node_center, const double* node_half,
                             const void* node_info, int64_t node_count, const double* local_re,
                             const double* local_im, const uint32_t* p2p_row, const int32_t* p2p_col,
                             double softening, int with_potential, double* phi, double* field,
                             int64_t* status, void* stream) {
+    TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(n >= 1 && node_count >= 1, "sizes");
-    return launch_eval<0>(precision, (hipStream_t)stream, sorted_pos, perm, nullptr, n, node_center,
-                          node_half, (const int4*)node_info, node_count, local_re, local_im,
-                          (int)order + 1, p2p_row, p2p_col, sorted_pos, sorted_charge, perm, n,
-                          (const int4*)node_info, softening * softening, with_potential, phi, field,
-                          status);
+    auto go = kernel == TD3_LOG ? launch_eval<TD3_LOG, 0> : launch_eval<TD3_COULOMB, 0>;
+    return go(precision, (hipStream_t)stream, sorted_pos, perm, nullptr, n, node_center, node_half,
+              (const int4*)node_info, node_count, local_re, local_im, (int)order, p2p_row, p2p_col,
+              sorted_pos, sorted_charge, perm, n, (const int4*)node_info, softening * softening,
+              with_potential, phi, field, status);
 }
 
-extern "C" int td3_fmm_at_eval(int precision, int64_t order, const double* sorted_targets,
-                               const int64_t* target_perm, const int64_t* exclude, int64_t m,
-                               const double* t_center, const double* t_half, const void* t_info,
-                               int64_t t_count, const double* local_re, const double* local_im,
+extern "C" int td3_fmm_at_eval(int kernel, int precision, int64_t order,
+                               const double* sorted_targets, const int64_t* target_perm,
+                               const int64_t* exclude, int64_t m, const double* t_center,
+                               const double* t_half, const void* t_info, int64_t t_count,
+                               const double* local_re, const double* local_im,
                                const uint32_t* p2p_row, const int32_t* p2p_col,
                                const double* sorted_pos, const double* sorted_charge,
                                const int64_t* perm, int64_t n, const void* s_info,
                                double softening, int with_potential, double* phi, double* field,
                                int64_t* status, void* stream) {
+    TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(m >= 1 && n >= 1 && t_count >= 1, "sizes");
-    return launch_eval<1>(precision, (hipStream_t)stream, sorted_targets, target_perm, exclude, m,
-                          t_center, t_half, (const int4*)t_info, t_count, local_re, local_im,
-                          (int)order + 1, p2p_row, p2p_col, sorted_pos, sorted_charge, perm, n,
-                          (const int4*)s_info, softening * softening, with_potential, phi, field,
-                          status);
+    auto go = kernel == TD3_LOG ? launch_eval<TD3_LOG, 1> : launch_eval<TD3_COULOMB, 1>;
+    return go(precision, (hipStream_t)stream, sorted_targets, target_perm, exclude, m, t_center,
+              t_half, (const int4*)t_info, t_count, local_re, local_im, (int)order, p2p_row,
+              p2p_col, sorted_pos, sorted_charge, perm, n, (const int4*)s_info,
+              softening * softening, with_potential, phi, field, status);
 }

@@ -17,7 +17,11 @@
 namespace {
 
 constexpr int kTravThreads = 256;
-constexpr unsigned kTravBlocks = 2048;
+// grid of every generation (grid-stride over the frontier; any size is correct)
+#ifndef TD3_TRAVERSE_BLOCKS
+#define TD3_TRAVERSE_BLOCKS 2048
+#endif
+constexpr unsigned kTravBlocks = TD3_TRAVERSE_BLOCKS;
 
 // ctr: [0] m2l count, [1] p2p count, [2] overflow, [3 + g] frontier size of generation g.
 __device__ __forceinline__ u32 wave_append(bool want, u32 count_each, u32* counter, u32* base_out) {

@@ -250,7 +250,7 @@ __global__ void gather_kernel(const u32* __restrict__ order, const double* __res
 
 struct TreeWs {
     u64 *khi, *khi_alt, *klo, *klo_alt, *ekey, *ekey_alt;
-    u32 *idx, *idx_alt, *cnt, *eval, *eval_alt, *scratch, *small;
+    u32 *idx, *idx_alt, *cnt, *off, *eval, *eval_alt, *scratch, *small;
 };
 
 i64 entries_cap(i64 capacity) { return (capacity - 1) / 4 + 1; }
@@ -271,6 +271,7 @@ size_t carve(TreeWs* w, void* base, i64 n, i64 capacity) {
     w->idx = (u32*)take(sizeof(u32) * (size_t)big);
     w->idx_alt = (u32*)take(sizeof(u32) * (size_t)big);
     w->cnt = (u32*)take(sizeof(u32) * (size_t)(n + 1));
+    w->off = (u32*)take(sizeof(u32) * (size_t)(n + 1));
     w->ekey = (u64*)take(sizeof(u64) * (size_t)ne);
     w->ekey_alt = (u64*)take(sizeof(u64) * (size_t)ne);
     w->eval = (u32*)take(sizeof(u32) * (size_t)ne);
@@ -377,9 +378,10 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     const u64* klo = deep ? w.klo : nullptr;
     // 4. entries (depth, particle) in particle order, then bucketed by depth (stable)
     if (s_sum > 0) {
-        if (int rc = td3sort::exclusive_scan_u32(w.cnt, w.cnt, n, s_total, w.scratch, s)) return rc;
+        // offsets into a separate array: entries_kernel needs the counts and the offsets
+        if (int rc = td3sort::exclusive_scan_u32(w.cnt, w.off, n, s_total, w.scratch, s)) return rc;
         hipLaunchKernelGGL(entries_kernel, dim3(nb), dim3(256), 0, s, w.khi, klo, n, nkey, w.cnt,
-                           w.cnt, w.ekey, w.eval);
+                           w.off, w.ekey, w.eval);
         TD3_LAUNCHED();
         if (int rc = td3sort::sort_pairs(w.ekey, w.eval, nullptr, w.ekey_alt, w.eval_alt, nullptr,
                                          s_sum, 0, 8, false, w.scratch, s))

@@ -406,6 +406,8 @@ class FlatQuadtree:
             raise ValueError("charges must have one entry per particle")
         self._d["charge"].copy_(charges)
         self._invalidate_host("charge")
+        self._cart = None
+        self._scaled = {}
         if self._d["pos"].shape[1]:
             self._sorted_charge()
             self._moments()
@@ -645,7 +647,8 @@ class FlatQuadtree:
         _, (prow, pcol, _), counts = self._lists(theta)
         status = torch.zeros(1, dtype=torch.int64, device=dev)
         _abi.call(
-            "td3_fmm_eval", prec, self.order, _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
+            "td3_fmm_eval", _abi.LOG, prec, self.order, _abi.ptr(self._sorted_pos),
+            _abi.ptr(self._scharge),
             _abi.ptr(self._d["perm"]), n, _abi.ptr(self._d["node_center"]),
             _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), self.node_count,
             _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol), softening,
@@ -656,8 +659,77 @@ class FlatQuadtree:
         self.fmm_pairs = counts
         return _out(phi, self._like_torch), _out(field, self._like_torch)
 
+    # ---------------------------------------------------------------- Coulomb FMM
+    def _cart_moments(self):
+        """Cartesian moments m_g and primed moments m'_g of every cell (kept until
+        `set_charges`; flat_tree.py:1063-1084)."""
+        cached = self.__dict__.get("_cart")
+        if cached is not None:
+            return cached
+        dev = self._d["pos"].device
+        nc = (self.order + 1) * (self.order + 2) // 2
+        mom = torch.empty((self.node_count, nc), dtype=torch.float64, device=dev)
+        prime = torch.empty_like(mom)
+        bsz = 2 * self.order + 1
+        binom = _device_table("binomial", _binomial_table, bsz)
+        levels = np.ascontiguousarray(self.levels, dtype=np.int64)
+        _abi.call("td3_cart_moments", self.order, _abi.ptr(self._sorted_pos),
+                  _abi.ptr(self._scharge), self._d["pos"].shape[1],
+                  _abi.ptr(self._d["node_center"]), _abi.ptr(self._node_info), self.node_count,
+                  levels.ctypes.data, len(levels) - 1, _abi.ptr(binom), bsz, _abi.ptr(mom),
+                  _abi.ptr(prime), _abi.stream())
+        self._cart = (mom, prime)
+        return self._cart
+
+    def _cart_locals(self, target, m2l_rows, softening):
+        """Cartesian local expansions of every cell of ``target``: M2L then L2L."""
+        mrow, mcol, mdst = m2l_rows
+        nc = (self.order + 1) * (self.order + 2) // 2
+        dev = self._d["pos"].device
+        local = torch.zeros((target.node_count, nc), dtype=torch.float64, device=dev)
+        _, prime = self._cart_moments()
+        nbytes = _abi.lib().td3_cart_m2l_workspace_bytes(int(mcol.shape[0]), self.order,
+                                                         target.node_count)
+        ws = _abi.workspace(nbytes)
+        _abi.call("td3_cart_m2l", self.order, _abi.ptr(mcol), _abi.ptr(mdst), _abi.ptr(mrow), 0,
+                  target.node_count, _abi.ptr(target._d["node_center"]),
+                  _abi.ptr(self._d["node_center"]), _abi.ptr(prime), float(softening),
+                  _abi.ptr(local), _abi.ptr(ws), nbytes, _abi.stream())
+        bsz = 2 * self.order + 1
+        binom = _device_table("binomial", _binomial_table, bsz)
+        levels = np.ascontiguousarray(target.levels, dtype=np.int64)
+        _abi.call("td3_cart_l2l", self.order, _abi.ptr(target._d["node_center"]),
+                  _abi.ptr(target._node_info), levels.ctypes.data, len(levels) - 1,
+                  _abi.ptr(binom), bsz, _abi.ptr(local), _abi.stream())
+        return local
+
+    def cartesian_locals(self, theta: float = 0.5, softening: float = 0.0):
+        """The Coulomb FMM's Cartesian local expansions (C, nc) after L2L."""
+        local = self._cart_locals(self, self._lists(float(theta))[0], softening)
+        return _out(local, self._like_torch)
+
     def _coulomb_fmm(self, theta, softening, potential, precision):
-        raise NotImplementedError("the Coulomb (Cartesian) FMM is not built yet on the GPU")
+        """flat_tree.py:1360-1428: the Coulomb FMM with Cartesian expansions of the softened
+        kernel (softened in the far field too)."""
+        prec = _precision(precision or self.precision)
+        m2l_rows, (prow, pcol, _), counts = self._lists(theta)
+        local = self._cart_locals(self, m2l_rows, softening)
+        n = self._d["pos"].shape[1]
+        dev = self._d["pos"].device
+        phi = torch.zeros(n, dtype=torch.float64, device=dev)
+        field = torch.zeros((2, n), dtype=torch.float64, device=dev)
+        status = torch.zeros(1, dtype=torch.int64, device=dev)
+        _abi.call(
+            "td3_fmm_eval", _abi.COULOMB, prec, self.order, _abi.ptr(self._sorted_pos),
+            _abi.ptr(self._scharge), _abi.ptr(self._d["perm"]), n,
+            _abi.ptr(self._d["node_center"]), _abi.ptr(self._d["node_half"]),
+            _abi.ptr(self._node_info), self.node_count, _abi.ptr(local), _abi.ptr(local),
+            _abi.ptr(prow), _abi.ptr(pcol), float(softening), 1 if potential else 0,
+            _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status), _abi.stream(),
+        )
+        _check(status)
+        self.fmm_pairs = counts
+        return _out(phi, self._like_torch), _out(field, self._like_torch)
 
     # ---------------------------------------------------------------- fmm_at
     def _target_plan(self, targets: torch.Tensor, theta: float, leaf: int):
@@ -751,7 +823,8 @@ class FlatQuadtree:
         status = torch.zeros(1, dtype=torch.int64, device=t.device)
         td = tree._d
         _abi.call(
-            "td3_fmm_at_eval", prec, self.order, _abi.ptr(tree._sorted_pos), _abi.ptr(td["perm"]),
+            "td3_fmm_at_eval", _abi.LOG, prec, self.order, _abi.ptr(tree._sorted_pos),
+            _abi.ptr(td["perm"]),
             _abi.ptr(ex), m, _abi.ptr(td["node_center"]), _abi.ptr(td["node_half"]),
             _abi.ptr(tree._node_info), tree.node_count, _abi.ptr(lre), _abi.ptr(lim),
             _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
@@ -763,8 +836,27 @@ class FlatQuadtree:
         self.fmm_at_pairs = counts
         return _out(phi, like_torch), _out(field, like_torch)
 
-    def _coulomb_fmm_at(self, *args, **kwargs):
-        raise NotImplementedError("the Coulomb (Cartesian) fmm_at is not built yet on the GPU")
+    def _coulomb_fmm_at(self, t, ex, tree, m2l_rows, prow, pcol, softening, potential,
+                        precision, like_torch):
+        """fmm_at's Coulomb branch (flat_tree.py:1247-1295)."""
+        prec = _precision(precision or self.precision)
+        local = self._cart_locals(tree, m2l_rows, softening)
+        m, n = t.shape[1], self._d["pos"].shape[1]
+        phi = torch.zeros(m, dtype=torch.float64, device=t.device)
+        field = torch.zeros((2, m), dtype=torch.float64, device=t.device)
+        status = torch.zeros(1, dtype=torch.int64, device=t.device)
+        td = tree._d
+        _abi.call(
+            "td3_fmm_at_eval", _abi.COULOMB, prec, self.order, _abi.ptr(tree._sorted_pos),
+            _abi.ptr(td["perm"]), _abi.ptr(ex), m, _abi.ptr(td["node_center"]),
+            _abi.ptr(td["node_half"]), _abi.ptr(tree._node_info), tree.node_count,
+            _abi.ptr(local), _abi.ptr(local), _abi.ptr(prow), _abi.ptr(pcol),
+            _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge), _abi.ptr(self._d["perm"]), n,
+            _abi.ptr(self._node_info), float(softening), 1 if potential else 0, _abi.ptr(phi),
+            _abi.ptr(field), _abi.ptr(status), _abi.stream(),
+        )
+        _check(status)
+        return _out(phi, like_torch), _out(field, like_torch)
 
 
 def _m2l_coefficients(order: int, symmetric: bool) -> np.ndarray:
