@@ -270,6 +270,17 @@ int td3_cart_l2l(int64_t order, const double *node_center, const void *node_info
                  int64_t bsz, double *local, void *stream);
 
 /*
+ * Particle push of the simulation step (simulation.py:800-835).  first_half: the electric
+ * half-kick v += dt/(2m) qE, then for mobile particles the exact drift
+ * r += dt (sinc(a) v + (a/2) sinc^2(a/2) (v_y, -v_x)) and rotation by a = (q/m) B_z dt;
+ * otherwise the second half-kick and forces = qE + q B_z (v_y, -v_x).  electric holds qE
+ * (k_e included); positions, velocities, forces (2, N); fixed (N,) uint8 (not moved).
+ */
+int td3_push(int first_half, int64_t n, double *positions, double *velocities, double *forces,
+             const double *electric, const double *masses, const double *charges,
+             const uint8_t *fixed, double dt, double b_z, void *stream);
+
+/*
  * Kernel timing for measurement (no counterpart in the reference, whose profiling is
  * host-side regions): td3_timing_enable(1) clears the record and brackets every later
  * kernel launch with HIP events on its stream; td3_timing_get(name) synchronises and

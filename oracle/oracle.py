@@ -692,3 +692,23 @@ def _fmm_at_coulomb(self, tree, m2l, p2p, targets, exclude, softening, potential
     if not potential:
         phi[:] = 0.0
     return phi, field
+
+
+def push(positions, velocities, forces, electric, masses, charges, fixed, dt, b_z, first_half):
+    """The kick-drift (``first_half``) or closing kick of a step, in place on NumPy arrays
+    (simulation.py:812-835): half-kick of mobile particles by ``electric`` (qE), then the
+    exact cyclotron drift and rotation by alpha = (q/m) B_z dt; the closing half sets
+    ``forces = electric + q B_z (v_y, -v_x)`` for every particle."""
+    mobile = ~np.asarray(fixed, dtype=bool)
+    inv_mass = 1 / masses[mobile]
+    velocities[:, mobile] += 0.5 * dt * electric[:, mobile] * inv_mass
+    if first_half:
+        angle = charges[mobile] / masses[mobile] * b_z * dt
+        c, s = np.cos(angle), np.sin(angle)
+        a = np.sinc(angle / np.pi)
+        b = 0.5 * angle * np.sinc(angle / (2 * np.pi)) ** 2
+        vx, vy = velocities[0, mobile], velocities[1, mobile]
+        positions[:, mobile] += dt * np.stack((a * vx + b * vy, a * vy - b * vx))
+        velocities[:, mobile] = np.stack((c * vx + s * vy, c * vy - s * vx))
+    else:
+        forces[:] = electric + charges * b_z * np.stack((velocities[1], -velocities[0]))

@@ -178,6 +178,37 @@ def case_coulomb():
     assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
 
 
+def case_push():
+    from tree_d_3d_amd import simulation
+
+    rng = np.random.default_rng(3)
+    n = 1001
+    for first_half in (True, False):
+        host = dict(positions=rng.uniform(-1, 1, (2, n)), velocities=rng.normal(0, 1, (2, n)),
+                    forces=np.zeros((2, n)), electric=rng.normal(0, 1, (2, n)),
+                    masses=rng.uniform(0.5, 2, n), charges=rng.normal(0, 1, n),
+                    fixed=rng.random(n) < 0.1)
+        sim = td3.ElectrostaticSimulation([td3.ChargedBody(0.0, 0.0) for _ in range(n)],
+                                          td3.Quad(0, 0, 1), dt=0.03, B_z=1.7)
+        for name in ("positions", "velocities", "masses", "charges", "fixed"):
+            setattr(sim, name, torch.tensor(host[name]))
+        sim._electric = torch.tensor(host["electric"])
+        sim._fixed_u8 = sim.fixed.to(torch.uint8)
+        simulation._push(sim, first_half)
+        oracle.push(**host, dt=0.03, b_z=1.7, first_half=first_half)
+        for name in ("positions", "velocities", "forces"):
+            assert np.allclose(getattr(sim, name).numpy(), host[name], rtol=1e-14, atol=1e-15)
+    # a short run through every field method
+    pos, q = cloud(400, 5)
+    for method in ("direct", "tree", "fmm"):
+        bodies = [td3.ChargedBody(*pos[:, i], charge=q[i], fixed=(i % 40 == 0)) for i in range(400)]
+        sim = td3.ElectrostaticSimulation(bodies, td3.Quad(0, 0, 1), k_e=1.0, softening=0.05,
+                                          dt=1e-4, method=method)
+        for _ in range(3):
+            sim.step()
+        assert np.all(np.isfinite(sim.positions.numpy()))
+
+
 CASES = {name[5:]: fn for name, fn in globals().items() if name.startswith("case_")}
 
 if __name__ == "__main__":

@@ -54,7 +54,7 @@ def test_direct_coincident_and_empty(td3):
         td3.direct_field(pos, pos, q, exclude=[0, -1, 2])
     # an excluded or uncharged coincident source is fine
     q0 = np.array([1.0, 0.0, -1.0])
-    phi, field = td3.direct_field(pos, pos, q0, exclude=[0, 1, 2], kernel="log")
+    phi, field = td3.direct_field(pos, pos, q0, exclude=[0, 0, 2], kernel="log")
     assert np.all(np.isfinite(phi)) and np.all(np.isfinite(field))
     phi, field = td3.direct_field(np.zeros((2, 0)), pos, q)
     assert phi.shape == (0,) and field.shape == (2, 0)

@@ -9,5 +9,7 @@ Import it as ``tree_d_3d_amd`` (see ``__graft_entry__.load_package``).
 from .constants import K_E
 from .flat_tree import MAX_ORDER, FlatNode, FlatQuadtree, direct_field
 from .quad import Quad
+from .simulation import ChargedBody, ElectrostaticSimulation
 
-__all__ = ["K_E", "MAX_ORDER", "FlatNode", "FlatQuadtree", "Quad", "direct_field"]
+__all__ = ["K_E", "MAX_ORDER", "ChargedBody", "ElectrostaticSimulation", "FlatNode",
+           "FlatQuadtree", "Quad", "direct_field"]
