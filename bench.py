@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = ("m2l", "m2l_fixup", "eval", "l2l", "traverse", "row_sort", "lists_fill",
+                "lists_degree", "sort_scatter", "sort_hist", "scan", "tree_keys", "tree_count",
+                "tree_nodes", "tree_leaf_keys", "tree_gather", "moments_p2m", "moments_m2m",
+                "scale_moments")
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
@@ -159,19 +163,22 @@ def main() -> int:
                     "unsoftened, so FMM-vs-softened-direct carries an O(eps^2/d^2) model error",
         }
 
-    # per-kernel times of one more mixed step (HIP events on the launch stream)
-    _abi.timing_enable(True)
-    tree, _, _ = step("mixed")
-    torch.cuda.synchronize()
-    kernels = {}
-    for name in ("m2l", "m2l_fixup", "eval", "l2l", "traverse", "row_sort", "lists_fill",
-                 "lists_degree", "sort_scatter", "sort_hist", "scan", "tree_keys", "tree_count",
-                 "tree_nodes", "tree_leaf_keys", "tree_gather", "moments_p2m", "moments_m2m",
-                 "scale_moments"):
-        ms, count = _abi.timing_get(name)
-        if count:
-            kernels[name] = {"ms": ms, "launches": count}
-    _abi.timing_enable(False)
+    # per-kernel times of one more step of each precision (HIP events on the launch stream)
+    def kernel_times(precision):
+        _abi.timing_enable(True)
+        tree, _, _ = step(precision)
+        torch.cuda.synchronize()
+        out = {}
+        for name in KERNEL_NAMES:
+            ms, count = _abi.timing_get(name)
+            if count:
+                out[name] = {"ms": ms, "launches": count}
+        _abi.timing_enable(False)
+        return tree, out
+
+    if not args.no_fp64:
+        result["fp64"]["kernels_ms"] = kernel_times("fp64")[1]
+    tree, kernels = kernel_times("mixed")
     result["kernels_ms"] = kernels
     # roofline of the dominant kernel: algorithmic flops per launch / average launch time
     (mrow, mcol, _), (prow, pcol, _), counts = tree._lists(0.5)

@@ -98,6 +98,18 @@ def case_tree():
                   oracle.OracleTree(pos, q, (0, 0, 1), leaf_capacity=1, max_depth=depth))
 
 
+def case_big_tree():
+    # several scan and sort tiles (4096 items each), and a tree deeper than 16 levels
+    pos, q = cloud(70000, 21)
+    pos[:, :40] = 0.31 + np.arange(40) * 1e-7
+    for leaf in (8, 3):
+        same_tree(td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), leaf_capacity=leaf),
+                  oracle.OracleTree(pos, q, (0, 0, 1), leaf_capacity=leaf))
+    pos2, _ = cloud(70000, 22)
+    same_tree(td3.FlatQuadtree(pos2, q, td3.Quad(0, 0, 1), leaf_capacity=8),
+              oracle.OracleTree(pos2, q, (0, 0, 1), leaf_capacity=8))
+
+
 def case_moments():
     pos, q = cloud(400, 3)
     q[::9] = 0.0

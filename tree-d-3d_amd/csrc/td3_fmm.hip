@@ -242,13 +242,16 @@ __global__ void l2l_kernel(const double* __restrict__ center, const double* __re
 }
 
 // ------------------------------------------------------------------------------ L2P + P2P
-// One wavefront per target leaf.  kernels/fmm_evaluate_host/fmm_evaluate_host_pyccel.py:14-100:
-// L2P W = sum_l b_l u^l, phi += -2 Re W, E_x - i E_y += 2 W'; near field
-// phi -= q ln(r^2 + eps^2), E += 2 q d / (r^2 + eps^2) over the P2P partner leaves, each
-// particle's own charge excluded; a charged coincident pair sets status.  Lanes run over
-// the partners' particles (64 at a time), each accumulating every target of the leaf
-// (kChunk at a time); a wave reduction finishes each target.
-constexpr int kChunk = 8;
+// kernels/fmm_evaluate_host/fmm_evaluate_host_pyccel.py:14-100: L2P W = sum_l b_l u^l,
+// phi += -2 Re W, E_x - i E_y += 2 W'; near field phi -= q ln(r^2 + eps^2),
+// E += 2 q d / (r^2 + eps^2) over the P2P partner leaves, each particle's own charge
+// excluded; a charged coincident pair sets status.
+// One lane per target.  A wavefront takes kEvalNodes consecutive cells (breadth-first
+// order, so neighbouring leaves of one depth) and spreads the particles of the leaves among
+// them over its lanes; each lane then walks its leaf's partner row on its own.  Lanes of one
+// leaf read the same sources (one broadcast load), and a row's order fixes the order of
+// every sum, so results are deterministic.
+constexpr int kEvalNodes = 16;
 
 // MODE 0 (fmm): targets are the sources themselves, in tree order; a particle's own slot is
 // skipped and a coincident pair is an error only when both are charged
@@ -270,162 +273,129 @@ __global__ __launch_bounds__(256) void eval_kernel(
     const int4* __restrict__ sinfo, double eps2, int with_potential, double* __restrict__ phi,
     double* __restrict__ field, i64* __restrict__ status) {
     const int lane = td3_lane();
-    const i64 leaf = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    if (leaf >= t_count) return;
-    const int4 lf = tinfo[leaf];
-    if (lf.z >= 0 || lf.y == lf.x) return;
-    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
-    const double cx = tcenter[2 * leaf], cy = tcenter[2 * leaf + 1];
-    const double inv_r = 1.0 / (1.4142135623730951 * thalf[leaf]);
-    for (int t0 = lf.x; t0 < lf.y; t0 += kChunk) {
-        const int nt = (lf.y - t0) < kChunk ? (lf.y - t0) : kChunk;
-        double acc_p[kChunk], acc_x[kChunk], acc_y[kChunk];
-        double tx[kChunk], ty[kChunk], tq[kChunk];
-        i64 tskip[kChunk];
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-            acc_p[j] = acc_x[j] = acc_y[j] = 0.0;
-            const int k = t0 + (j < nt ? j : 0);
-            tx[j] = tpos[k];
-            ty[j] = tpos[m + k];
-            tq[j] = MODE == 0 ? scharge[k] : 0.0;
-            tskip[j] = MODE == 1 ? exclude[tperm[k]] : -1;
+    const i64 node0 = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64 * kEvalNodes;
+    if (node0 >= t_count) return;  // uniform over the wavefront
+    // lanes below kEvalNodes: the particle count of their cell when it is a leaf
+    int cnt = 0, first = 0;
+    if (lane < kEvalNodes && node0 + lane < t_count) {
+        const int4 lf = tinfo[node0 + lane];
+        if (lf.z < 0) {
+            cnt = lf.y - lf.x;
+            first = lf.x;
         }
-        int bad_slot = -1;
-        // source leaves in batches of 64 lanes
-        for (u32 rb = r0; rb < r1; rb += 64) {
-            const u32 rr = rb + lane;
-            int sstart = 0, scount = 0;
-            if (rr < r1) {
-                const int4 sf = sinfo[pcol[rr]];
-                sstart = sf.x;
-                scount = sf.y - sf.x;
-            }
-            // inclusive prefix of counts over lanes
-            int incl = scount;
+    }
+    int incl = cnt;
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int v = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += v;
-            }
-            const int total = __shfl(incl, 63, 64);
-            for (int base = 0; base < total; base += 64) {
-                const int idx = base + lane;
-                // owning lane: first lane whose inclusive prefix exceeds idx (a fixed
-                // six-step search, so every lane takes part in every shuffle)
-                int lo_l = 0;
+    for (int off = 1; off < kEvalNodes; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, kEvalNodes - 1, 64);
+    for (int base = 0; base < total; base += 64) {
+        const int idx = base + lane;
+        // owning cell: the first of the kEvalNodes lanes whose inclusive count exceeds idx
+        // (a fixed search, so every lane takes part in every shuffle)
+        int lo_l = 0;
 #pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    const int pv = __shfl(incl, lo_l + step - 1, 64);
-                    if (pv <= idx) lo_l += step;
+        for (int step = kEvalNodes / 2; step > 0; step >>= 1) {
+            const int pv = __shfl(incl, lo_l + step - 1, 64);
+            if (pv <= idx) lo_l += step;
+        }
+        const int ofirst = __shfl(first, lo_l, 64);
+        const int oincl = __shfl(incl, lo_l, 64);
+        const int ocnt = __shfl(cnt, lo_l, 64);
+        if (idx >= total) continue;  // no cross-lane operation below
+        const i64 leaf = node0 + lo_l;
+        const int k = ofirst + idx - (oincl - ocnt);
+        const double tx = tpos[k], ty = tpos[m + k];
+        const double tq = MODE == 0 ? scharge[k] : 0.0;
+        const i64 tskip = MODE == 1 ? exclude[tperm[k]] : -1;
+        double acc_p = 0.0, acc_x = 0.0, acc_y = 0.0;
+        bool bad = false;
+        const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
+        for (u32 r = r0; r < r1; ++r) {
+            const int4 sf = sinfo[pcol[r]];
+            for (int ks = sf.x; ks < sf.y; ++ks) {
+                const double sq = scharge[ks];
+                if (MODE == 0 && ks == k) continue;
+                if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
+                const double dx = tx - spos[ks], dy = ty - spos[n + ks];
+                const double r2 = dx * dx + dy * dy + eps2;
+                if (r2 == 0.0) {
+                    if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
+                    continue;
                 }
-                const int owner = lo_l < 63 ? lo_l : 63;
-                const int ostart = __shfl(sstart, owner, 64);
-                const int oincl = __shfl(incl, owner, 64);
-                const int ocount = __shfl(scount, owner, 64);
-                if (idx >= total) continue;
-                const int ks = ostart + (idx - (oincl - ocount));
-                const double sx = spos[ks], sy = spos[n + ks], sq = scharge[ks];
-                if (MODE == 1 && sq == 0.0) continue;
-                const i64 sid = MODE == 1 ? sperm[ks] : -1;
-#pragma unroll
-                for (int j = 0; j < kChunk; ++j) {
-                    if (j >= nt) break;
-                    if (MODE == 0 && ks == t0 + j) continue;
-                    if (MODE == 1 && sid == tskip[j]) continue;
-                    const double dx = tx[j] - sx, dy = ty[j] - sy;
-                    const double r2 = dx * dx + dy * dy + eps2;
-                    if (r2 == 0.0) {
-                        if (MODE == 1 || (tq[j] != 0.0 && sq != 0.0)) bad_slot = t0 + j;
-                        continue;
-                    }
-                    if (KERNEL == TD3_COULOMB) {
-                        if (PREC == TD3_FP64) {
-                            const double inv_r = 1.0 / sqrt(r2);
-                            const double cr3 = sq * inv_r * inv_r * inv_r;
-                            acc_p[j] += sq * inv_r;
-                            acc_x[j] += cr3 * dx;
-                            acc_y[j] += cr3 * dy;
-                        } else {
-                            const float inv_r = __frsqrt_rn((float)r2), cf = (float)sq;
-                            const float cr3 = cf * inv_r * inv_r * inv_r;
-                            acc_p[j] += (double)(cf * inv_r);
-                            acc_x[j] += (double)(cr3 * (float)dx);
-                            acc_y[j] += (double)(cr3 * (float)dy);
-                        }
-                    } else if (PREC == TD3_FP64) {
-                        if (with_potential) acc_p[j] -= sq * log(r2);
-                        const double inv = 2.0 * sq / r2;
-                        acc_x[j] += inv * dx;
-                        acc_y[j] += inv * dy;
+                if (KERNEL == TD3_COULOMB) {
+                    if (PREC == TD3_FP64) {
+                        const double inv_r = 1.0 / sqrt(r2);
+                        const double cr3 = sq * inv_r * inv_r * inv_r;
+                        acc_p += sq * inv_r;
+                        acc_x += cr3 * dx;
+                        acc_y += cr3 * dy;
                     } else {
-                        const float r2f = (float)r2;
-                        if (with_potential) acc_p[j] -= (double)((float)sq * __logf(r2f));
-                        const float inv = 2.0f * (float)sq * __frcp_rn(r2f);
-                        acc_x[j] += (double)(inv * (float)dx);
-                        acc_y[j] += (double)(inv * (float)dy);
+                        const float inv_r = __frsqrt_rn((float)r2), cf = (float)sq;
+                        const float cr3 = cf * inv_r * inv_r * inv_r;
+                        acc_p += (double)(cf * inv_r);
+                        acc_x += (double)(cr3 * (float)dx);
+                        acc_y += (double)(cr3 * (float)dy);
                     }
+                } else if (PREC == TD3_FP64) {
+                    if (with_potential) acc_p -= sq * log(r2);
+                    const double inv = 2.0 * sq / r2;
+                    acc_x += inv * dx;
+                    acc_y += inv * dy;
+                } else {
+                    const float r2f = (float)r2;
+                    if (with_potential) acc_p -= (double)((float)sq * __logf(r2f));
+                    const float inv = 2.0f * (float)sq * __frcp_rn(r2f);
+                    acc_x += (double)(inv * (float)dx);
+                    acc_y += (double)(inv * (float)dy);
                 }
             }
         }
-        if (bad_slot >= 0) td3_report(status, tperm[bad_slot] + 1);
-        // reduce and L2P
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-            if (j >= nt) break;
-            acc_p[j] = td3_wave_sum(acc_p[j]);
-            acc_x[j] = td3_wave_sum(acc_x[j]);
-            acc_y[j] = td3_wave_sum(acc_y[j]);
-        }
-        if (lane < nt) {
-            const int k = t0 + lane;
-            double p = 0, fx = 0, fy = 0;
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j)
-                if (j == lane) {
-                    p = acc_p[j];
-                    fx = acc_x[j];
-                    fy = acc_y[j];
+        const i64 i = tperm[k];
+        if (bad) td3_report(status, i + 1);
+        const double cx = tcenter[2 * leaf], cy = tcenter[2 * leaf + 1];
+        if (KERNEL == TD3_COULOMB) {
+            // phi += sum_g T_g u^g, E -= grad (u = z - c, unscaled Cartesian locals)
+            const double ux = tx - cx, uy = ty - cy;
+            const double* T = lre + leaf * (i64)p1;
+            double w = 0.0, gx = 0.0, gy = 0.0, pyb = 1.0, pyb1 = 0.0;
+            for (int by = 0; by <= order; ++by) {
+                double px = 1.0, px1 = 0.0;
+                for (int bx = 0; bx + by <= order; ++bx) {
+                    const double t = T[(bx + by) * (bx + by + 1) / 2 + by];
+                    w += t * px * pyb;
+                    gx += bx * t * px1 * pyb;
+                    gy += by * t * px * pyb1;
+                    px1 = px;
+                    px *= ux;
                 }
-            const i64 i = tperm[k];
-            if (KERNEL == TD3_COULOMB) {
-                // phi += sum_g T_g u^g, E -= grad (u = z - c, unscaled Cartesian locals)
-                const double ux = tpos[k] - cx, uy = tpos[m + k] - cy;
-                const double* T = lre + leaf * (i64)p1;
-                double w = 0.0, gx = 0.0, gy = 0.0, pyb = 1.0, pyb1 = 0.0;
-                for (int by = 0; by <= order; ++by) {
-                    double px = 1.0, px1 = 0.0;
-                    for (int bx = 0; bx + by <= order; ++bx) {
-                        const double t = T[(bx + by) * (bx + by + 1) / 2 + by];
-                        w += t * px * pyb;
-                        gx += bx * t * px1 * pyb;
-                        gy += by * t * px * pyb1;
-                        px1 = px;
-                        px *= ux;
-                    }
-                    pyb1 = pyb;
-                    pyb *= uy;
-                }
-                phi[i] = with_potential ? p + w : 0.0;
-                field[i] = fx - gx;
-                field[m + i] = fy - gy;
-            } else {
-                // Horner on the scaled local expansion at u' = (z - c) / r
-                const double ur = (tpos[k] - cx) * inv_r, ui = (tpos[m + k] - cy) * inv_r;
-                double w_re = lre[leaf * p1 + p1 - 1], w_im = lim[leaf * p1 + p1 - 1];
-                double d_re = 0.0, d_im = 0.0;
-                for (int l = p1 - 2; l >= 0; --l) {
-                    const double nd = d_re * ur - d_im * ui + w_re;
-                    d_im = d_re * ui + d_im * ur + w_im;
-                    d_re = nd;
-                    const double nw = w_re * ur - w_im * ui + lre[leaf * p1 + l];
-                    w_im = w_re * ui + w_im * ur + lim[leaf * p1 + l];
-                    w_re = nw;
-                }
-                phi[i] = with_potential ? p - 2.0 * w_re : 0.0;
-                field[i] = fx + 2.0 * d_re * inv_r;
-                field[m + i] = fy - 2.0 * d_im * inv_r;
+                pyb1 = pyb;
+                pyb *= uy;
             }
+            phi[i] = with_potential ? acc_p + w : 0.0;
+            field[i] = acc_x - gx;
+            field[m + i] = acc_y - gy;
+        } else {
+            // Horner on the scaled local expansion at u' = (z - c) / r
+            const double inv_r = 1.0 / (1.4142135623730951 * thalf[leaf]);
+            const double ur = (tx - cx) * inv_r, ui = (ty - cy) * inv_r;
+            const double* bre = lre + leaf * (i64)p1;
+            const double* bim = lim + leaf * (i64)p1;
+            double w_re = bre[p1 - 1], w_im = bim[p1 - 1];
+            double d_re = 0.0, d_im = 0.0;
+            for (int l = p1 - 2; l >= 0; --l) {
+                const double nd = d_re * ur - d_im * ui + w_re;
+                d_im = d_re * ui + d_im * ur + w_im;
+                d_re = nd;
+                const double nw = w_re * ur - w_im * ui + bre[l];
+                w_im = w_re * ui + w_im * ur + bim[l];
+                w_re = nw;
+            }
+            phi[i] = with_potential ? acc_p - 2.0 * w_re : 0.0;
+            field[i] = acc_x + 2.0 * d_re * inv_r;
+            field[m + i] = acc_y - 2.0 * d_im * inv_r;
         }
     }
 }
@@ -437,7 +407,7 @@ int launch_eval(int precision, hipStream_t s, const double* tpos, const i64* tpe
                 const u32* prow, const int* pcol, const double* spos, const double* scharge,
                 const i64* sperm, i64 n, const int4* sinfo, double eps2, int with_potential,
                 double* phi, double* field, i64* status) {
-    const unsigned blocks = td3_blocks(t_count * 64, 256);
+    const unsigned blocks = td3_blocks((t_count + kEvalNodes - 1) / kEvalNodes * 64, 256);
     const int p1 = KERNEL == TD3_LOG ? order + 1 : (order + 1) * (order + 2) / 2;
     if (precision == TD3_FP64)
         TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<KERNEL, TD3_FP64, MODE>), dim3(blocks), dim3(256), 0, s,

@@ -15,9 +15,10 @@ constexpr int kRadix = 256;
 
 inline i64 tiles(i64 n) { return (n + kTile - 1) / kTile; }
 
-// Bytes of scratch one sort of n items needs (histograms and scan partials).
+// Bytes of scratch one sort or scan of n items needs: the digit histograms (tiles x radix)
+// and the per-tile sums of the scan over them.
 inline size_t scratch_bytes(i64 n) {
-    return (size_t)(tiles(n) * kRadix + 1024) * sizeof(u32);
+    return (size_t)(tiles(n) * kRadix + tiles(tiles(n) * kRadix) + 64) * sizeof(u32);
 }
 
 // Sorts keys[0:n) (with values, and aux when non-null) by bits [bit_lo, bit_hi) of the key.

@@ -24,18 +24,18 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(const u64* __restrict__ 
     hist[(i64)threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
 }
 
-// Single-workgroup exclusive scan: each of 1024 threads owns a contiguous segment.
-__global__ __launch_bounds__(1024) void scan_kernel(const u32* in, u32* out, i64 n,
-                                                    u32* __restrict__ total) {
+// Single-workgroup exclusive scan of a short array (the per-tile sums): each of 1024
+// threads owns a contiguous segment.  Writes the grand total to *total when non-null.
+__global__ __launch_bounds__(1024) void scan_small_kernel(u32* data, i64 n,
+                                                          u32* __restrict__ total) {
     __shared__ u32 s_sum[1024];
     const i64 per = (n + 1023) / 1024;
     const i64 lo = (i64)threadIdx.x * per;
     const i64 hi = lo + per < n ? lo + per : n;
     u32 sum = 0;
-    for (i64 k = lo; k < hi; ++k) sum += in[k];
+    for (i64 k = lo; k < hi; ++k) sum += data[k];
     s_sum[threadIdx.x] = sum;
     __syncthreads();
-    // Hillis-Steele inclusive scan over the 1024 segment sums.
     for (int off = 1; off < 1024; off <<= 1) {
         const u32 v = threadIdx.x >= (unsigned)off ? s_sum[threadIdx.x - off] : 0u;
         __syncthreads();
@@ -44,11 +44,65 @@ __global__ __launch_bounds__(1024) void scan_kernel(const u32* in, u32* out, i64
     }
     u32 run = threadIdx.x > 0 ? s_sum[threadIdx.x - 1] : 0u;
     for (i64 k = lo; k < hi; ++k) {
-        const u32 v = in[k];
-        out[k] = run;
+        const u32 v = data[k];
+        data[k] = run;
         run += v;
     }
     if (total != nullptr && threadIdx.x == 1023) *total = s_sum[1023];
+}
+
+// Sum of each tile of kTile counts (coalesced loads).
+__global__ __launch_bounds__(kThreads) void tile_sum_kernel(const u32* __restrict__ in, i64 n,
+                                                            u32* __restrict__ part) {
+    __shared__ u32 s_wave[kThreads / TD3_WAVE];
+    const i64 base = (i64)blockIdx.x * kTile;
+    u32 sum = 0;
+#pragma unroll 4
+    for (int r = 0; r < kItems; ++r) {
+        const i64 idx = base + r * kThreads + threadIdx.x;
+        if (idx < n) sum += in[idx];
+    }
+    for (int off = TD3_WAVE / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, TD3_WAVE);
+    if (threadIdx.x % TD3_WAVE == 0) s_wave[threadIdx.x / TD3_WAVE] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u32 t = 0;
+        for (int w = 0; w < kThreads / TD3_WAVE; ++w) t += s_wave[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of each tile, offset by the scanned tile sums.  Each element is read and
+// written by the same thread, so `out` may alias `in`.
+__global__ __launch_bounds__(kThreads) void tile_scan_kernel(const u32* in, u32* out, i64 n,
+                                                             const u32* __restrict__ part) {
+    __shared__ u32 s_wave[kThreads / TD3_WAVE];
+    const int lane = threadIdx.x % TD3_WAVE;
+    const int wave = threadIdx.x / TD3_WAVE;
+    const i64 base = (i64)blockIdx.x * kTile;
+    u32 carry = part[blockIdx.x];
+    for (int r = 0; r < kItems; ++r) {
+        const i64 idx = base + r * kThreads + threadIdx.x;
+        const u32 v = idx < n ? in[idx] : 0u;
+        u32 incl = v;
+#pragma unroll
+        for (int off = 1; off < TD3_WAVE; off <<= 1) {
+            const u32 t = __shfl_up(incl, off, TD3_WAVE);
+            if (lane >= off) incl += t;
+        }
+        if (lane == TD3_WAVE - 1) s_wave[wave] = incl;
+        __syncthreads();
+        u32 before = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / TD3_WAVE; ++w) {
+            const u32 t = s_wave[w];
+            before += w < wave ? t : 0u;
+            all += t;
+        }
+        if (idx < n) out[idx] = carry + before + incl - v;
+        carry += all;
+        __syncthreads();
+    }
 }
 
 template <bool AUX, bool IDENT>
@@ -107,12 +161,17 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(
 
 int exclusive_scan_u32(const u32* in, u32* out, i64 n, u32* total, u32* scratch,
                        hipStream_t stream) {
-    (void)scratch;
     if (n <= 0) {
         if (total) TD3_HIP(hipMemsetAsync(total, 0, sizeof(u32), stream));
         return TD3_OK;
     }
-    TD3_TIMED(stream, "scan", hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, in, out, n, total));
+    const i64 nt = tiles(n);
+    u32* part = scratch;
+    TD3_TIMED(stream, "scan", hipLaunchKernelGGL(tile_sum_kernel, dim3((unsigned)nt), dim3(kThreads), 0, stream, in, n, part));
+    TD3_LAUNCHED();
+    TD3_TIMED(stream, "scan", hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(1024), 0, stream, part, nt, total));
+    TD3_LAUNCHED();
+    TD3_TIMED(stream, "scan", hipLaunchKernelGGL(tile_scan_kernel, dim3((unsigned)nt), dim3(kThreads), 0, stream, in, out, n, part));
     TD3_LAUNCHED();
     return TD3_OK;
 }
@@ -137,9 +196,9 @@ int sort_pairs(u64* keys, u32* vals, u64* aux, u64* keys_alt, u32* vals_alt, u64
         TD3_TIMED(stream, "sort_hist", hipLaunchKernelGGL(hist_kernel, dim3((unsigned)nt), dim3(kThreads), 0, stream, src_k, n, bit,
                            mask, nt, hist));
         TD3_LAUNCHED();
-        TD3_TIMED(stream, "scan", hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, hist, hist, nt * kRadix,
-                           (u32*)nullptr));
-        TD3_LAUNCHED();
+        if (int rc = exclusive_scan_u32(hist, hist, nt * kRadix, nullptr, hist + nt * kRadix,
+                                        stream))
+            return rc;
         if (aux != nullptr) {
             if (ident)
                 TD3_TIMED(stream, "sort_scatter", hipLaunchKernelGGL((scatter_kernel<true, true>), dim3((unsigned)nt), dim3(kThreads), 0,

@@ -23,28 +23,26 @@ constexpr int kTravThreads = 256;
 #endif
 constexpr unsigned kTravBlocks = TD3_TRAVERSE_BLOCKS;
 
-// ctr: [0] m2l count, [1] p2p count, [2] overflow, [3 + g] frontier size of generation g.
-__device__ __forceinline__ u32 wave_append(bool want, u32 count_each, u32* counter, u32* base_out) {
-    // exclusive prefix of count_each over the lanes that want, one atomic per wave
-    const int lane = td3_lane();
-    u32 v = want ? count_each : 0u;
-    u32 incl = v;
+// Each block takes kTravChunk consecutive frontier pairs per iteration and reserves the
+// room for its M2L, P2P and child pairs with one atomic per counter: appends from all
+// eight XCDs meet on three counters, so fewer, larger reservations matter.
+constexpr int kTravItems = 4;
+constexpr u32 kTravChunk = kTravThreads * kTravItems;
+constexpr int kTravWaves = kTravThreads / 64;
+
+__device__ __forceinline__ u32 wave_inclusive(u32 v, int lane) {
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const u32 t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
+        const u32 t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
     }
-    const u32 total = __shfl(incl, 63, 64);
-    u32 base = 0;
-    if (lane == 0 && total) base = atomicAdd(counter, total);
-    base = __shfl(base, 0, 64);
-    *base_out = base + incl - v;
-    return total;
+    return v;
 }
 
 // DUAL: a is a cell of the target tree (ca, ha, ia) and b of the source tree (cb, hb, ib),
 // with the rules of kernels/dual_traverse/dual_traverse_pyccel.py:14-86 (no self pairs,
 // one-way lists); otherwise both are cells of one tree.
+// ctr: [0] m2l count, [1] p2p count, [2] overflow, [3 + g] frontier size of generation g.
 template <bool DUAL>
 __global__ __launch_bounds__(kTravThreads) void traverse_kernel(
     const double* __restrict__ ca, const double* __restrict__ ha, const int4* __restrict__ ia,
@@ -52,73 +50,120 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(
     double theta, int g, const int2* __restrict__ frontier,
     u32 cap_front, int2* __restrict__ next, int2* __restrict__ m2l, u32 cap_m2l,
     int2* __restrict__ p2p, u32 cap_p2p, u32* __restrict__ ctr) {
-    // a list or frontier overflowed earlier: the caller retries with larger buffers
-    if (__hip_atomic_load(&ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    const u32 nf = __hip_atomic_load(&ctr[3 + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const u32 n_front = nf < cap_front ? nf : cap_front;
-    const u32 stride = gridDim.x * kTravThreads;
+    __shared__ u32 s_tot[2][kTravWaves];
+    __shared__ u32 s_base[3];
+    __shared__ u32 s_front;
+    // one read for the whole block, so every thread takes the same path to the barriers:
+    // the frontier size, or zero when a list or frontier overflowed earlier (the caller
+    // then retries with larger buffers)
+    if (threadIdx.x == 0) {
+        const u32 stop = __hip_atomic_load(&ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u32 nf = __hip_atomic_load(&ctr[3 + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_front = stop ? 0u : (nf < cap_front ? nf : cap_front);
+    }
+    __syncthreads();
+    const u32 n_front = s_front;
     const int lane = td3_lane();
-    for (u32 base = blockIdx.x * kTravThreads + (threadIdx.x & ~63u); base < n_front; base += stride) {
-        const u32 k = base + lane;
-        const bool valid = k < n_front;
-        int a = 0, b = 0;
-        int kind = 0;  // 0 none, 1 m2l, 2 p2p, 3 split a (4), 4 split b (4), 5 self split (10)
-        int4 fa = make_int4(0, 0, -1, 0), fb = fa;
-        if (valid) {
-            const int2 pr = frontier[k];
-            a = pr.x;
-            b = pr.y;
-            fa = ia[a];
-            fb = ib[b];
-            if (fa.y != fa.x && fb.y != fb.x) {
-                const bool leaf_a = fa.z < 0, leaf_b = fb.z < 0;
-                if (!DUAL && a == b) {
-                    kind = leaf_a ? 2 : 5;
-                } else {
-                    const double dx = __dadd_rn(ca[2 * a], -cb[2 * b]);
-                    const double dy = __dadd_rn(ca[2 * a + 1], -cb[2 * b + 1]);
-                    const double distance = __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
-                    const double radii = __dmul_rn(1.4142135623730951, __dadd_rn(ha[a], hb[b]));
-                    if (radii < __dmul_rn(theta, distance))
-                        kind = 1;
-                    else if (leaf_a && leaf_b)
-                        kind = 2;
-                    else if (leaf_b || (!leaf_a && ha[a] >= hb[b]))
-                        kind = 3;
-                    else
-                        kind = 4;
+    const int wave = threadIdx.x / 64;
+    for (u32 base = blockIdx.x * kTravChunk; base < n_front; base += gridDim.x * kTravChunk) {
+        int a[kTravItems], b[kTravItems], kind[kTravItems], child[kTravItems];
+        u32 n_pair = 0, n_next = 0;  // m2l in the low, p2p in the high 16 bits
+#pragma unroll
+        for (int r = 0; r < kTravItems; ++r) {
+            const u32 k = base + r * kTravThreads + threadIdx.x;
+            a[r] = b[r] = child[r] = 0;
+            kind[r] = 0;  // 0 none, 1 m2l, 2 p2p, 3 split a (4), 4 split b (4), 5 self split (10)
+            if (k < n_front) {
+                const int2 pr = frontier[k];
+                a[r] = pr.x;
+                b[r] = pr.y;
+                const int4 fa = ia[pr.x];
+                const int4 fb = ib[pr.y];
+                if (fa.y != fa.x && fb.y != fb.x) {
+                    const bool leaf_a = fa.z < 0, leaf_b = fb.z < 0;
+                    if (!DUAL && pr.x == pr.y) {
+                        kind[r] = leaf_a ? 2 : 5;
+                    } else {
+                        const int i = pr.x, j = pr.y;
+                        const double dx = __dadd_rn(ca[2 * i], -cb[2 * j]);
+                        const double dy = __dadd_rn(ca[2 * i + 1], -cb[2 * j + 1]);
+                        const double distance =
+                            __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
+                        const double radii = __dmul_rn(1.4142135623730951, __dadd_rn(ha[i], hb[j]));
+                        if (radii < __dmul_rn(theta, distance))
+                            kind[r] = 1;
+                        else if (leaf_a && leaf_b)
+                            kind[r] = 2;
+                        else if (leaf_b || (!leaf_a && ha[i] >= hb[j]))
+                            kind[r] = 3;
+                        else
+                            kind[r] = 4;
+                    }
+                    child[r] = kind[r] == 4 ? fb.z : fa.z;
                 }
             }
+            n_pair += kind[r] == 1 ? 1u : kind[r] == 2 ? 0x10000u : 0u;
+            n_next += kind[r] == 5 ? 10u : kind[r] >= 3 ? 4u : 0u;
         }
-        u32 pos;
-        wave_append(kind == 1, 1u, &ctr[0], &pos);
-        if (kind == 1) {
-            if (pos < cap_m2l)
-                m2l[pos] = make_int2(a, b);
-            else
-                ctr[2] = 1u;
+        const u32 inc_pair = wave_inclusive(n_pair, lane);
+        const u32 inc_next = wave_inclusive(n_next, lane);
+        if (lane == 63) {
+            s_tot[0][wave] = inc_pair;
+            s_tot[1][wave] = inc_next;
         }
-        wave_append(kind == 2, 1u, &ctr[1], &pos);
-        if (kind == 2) {
-            if (pos < cap_p2p)
-                p2p[pos] = make_int2(a, b);
-            else
-                ctr[2] = 1u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            u32 tp = 0, tn = 0;
+#pragma unroll
+            for (int w = 0; w < kTravWaves; ++w) {
+                tp += s_tot[0][w];
+                tn += s_tot[1][w];
+            }
+            s_base[0] = (tp & 0xffffu) ? atomicAdd(&ctr[0], tp & 0xffffu) : 0u;
+            s_base[1] = (tp >> 16) ? atomicAdd(&ctr[1], tp >> 16) : 0u;
+            s_base[2] = tn ? atomicAdd(&ctr[3 + g + 1], tn) : 0u;
         }
-        const u32 cnt = kind == 5 ? 10u : (kind == 3 || kind == 4) ? 4u : 0u;
-        wave_append(cnt != 0, cnt, &ctr[3 + g + 1], &pos);
-        if (cnt) {
-            if (pos + cnt > cap_front) {
-                ctr[2] = 1u;
-            } else if (kind == 5) {
-                const int first = fa.z;
-                u32 o = pos;
-                for (int c = 0; c < 4; ++c)
-                    for (int d = c; d < 4; ++d) next[o++] = make_int2(first + c, first + d);
-            } else if (kind == 3) {
-                for (int c = 0; c < 4; ++c) next[pos + c] = make_int2(fa.z + c, b);
-            } else {
-                for (int c = 0; c < 4; ++c) next[pos + c] = make_int2(a, fb.z + c);
+        __syncthreads();
+        u32 before_pair = 0, before_next = 0;
+#pragma unroll
+        for (int w = 0; w < kTravWaves; ++w) {
+            before_pair += w < wave ? s_tot[0][w] : 0u;
+            before_next += w < wave ? s_tot[1][w] : 0u;
+        }
+        const u32 excl_pair = before_pair + inc_pair - n_pair;
+        u32 pos_m2l = s_base[0] + (excl_pair & 0xffffu);
+        u32 pos_p2p = s_base[1] + (excl_pair >> 16);
+        u32 pos_next = s_base[2] + before_next + inc_next - n_next;
+        __syncthreads();  // s_tot and s_base are rewritten by the next iteration
+#pragma unroll
+        for (int r = 0; r < kTravItems; ++r) {
+            if (kind[r] == 1) {
+                if (pos_m2l < cap_m2l)
+                    m2l[pos_m2l] = make_int2(a[r], b[r]);
+                else
+                    ctr[2] = 1u;
+                ++pos_m2l;
+            } else if (kind[r] == 2) {
+                if (pos_p2p < cap_p2p)
+                    p2p[pos_p2p] = make_int2(a[r], b[r]);
+                else
+                    ctr[2] = 1u;
+                ++pos_p2p;
+            } else if (kind[r] >= 3) {
+                const u32 cnt = kind[r] == 5 ? 10u : 4u;
+                if (pos_next + cnt > cap_front) {
+                    ctr[2] = 1u;
+                } else if (kind[r] == 5) {
+                    const int first = child[r];
+                    u32 o = pos_next;
+                    for (int c = 0; c < 4; ++c)
+                        for (int d = c; d < 4; ++d) next[o++] = make_int2(first + c, first + d);
+                } else if (kind[r] == 3) {
+                    for (int c = 0; c < 4; ++c) next[pos_next + c] = make_int2(child[r] + c, b[r]);
+                } else {
+                    for (int c = 0; c < 4; ++c) next[pos_next + c] = make_int2(a[r], child[r] + c);
+                }
+                pos_next += cnt;
             }
         }
     }
@@ -171,6 +216,22 @@ __global__ __launch_bounds__(256) void row_sort_kernel(const u32* __restrict__ r
         const u32 s = row_start[r], e = row_start[r + 1];
         const int len = (int)(e - s);
         if (len <= 1) continue;
+        if (len <= 64) {
+            // bitonic network across the lanes, one entry per lane
+            int v = lane < len ? col[s + lane] : 0x7fffffff;
+#pragma unroll
+            for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    const int o = __shfl_xor(v, stride, 64);
+                    const bool up = (lane & size) == 0;
+                    const bool lower = (lane & stride) == 0;
+                    v = (lower == up) ? (v < o ? v : o) : (v > o ? v : o);
+                }
+            }
+            if (lane < len) col[s + lane] = v;
+            continue;
+        }
         if (len > kSortMax) {
             if (lane == 0) atomicOr(err, 1u);
             continue;

@@ -319,11 +319,20 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     u32* s_total = w.small + 176;        // [1]
     const unsigned nb = td3_blocks(n, 256);
 
-    // 1-2. keys and sort by the first min(32, nlev) levels
-    int nkey = nlev < 32 ? nlev : 32;
+    // 1-2. keys and sort by the first 16 levels; when a cell that deep may still split, by
+    // the first 32, then by all of them.  Ties keep input order (the sort is stable), and
+    // the topology depends only on prefixes no longer than the deepest split.
+    int stages[3], nstage = 0;
+    for (int k : {16, 32, nlev}) {
+        const int key = k < nlev ? k : nlev;
+        if (nstage == 0 || key > stages[nstage - 1]) stages[nstage++] = key;
+    }
+    int nkey = 0;
     bool deep = false;
     u32 host_small[81];
-    for (int attempt = 0; attempt < 2; ++attempt) {
+    for (int stage = 0; stage < nstage; ++stage) {
+        nkey = stages[stage];
+        deep = nkey > 32;
         if (!deep) {
             TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
                                w.khi, (u64*)nullptr));
@@ -350,13 +359,11 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
         TD3_HIP(hipMemcpyAsync(host_small, w.small, sizeof(u32) * 81, hipMemcpyDeviceToHost, s));
         TD3_HIP(hipStreamSynchronize(s));
         if (!host_small[80]) break;
-        if (deep || nlev <= 32) {
+        if (stage == nstage - 1) {
             td3_set_error("the tree needs more than %d levels (max_depth %lld)", kMaxLevels,
                           (long long)max_depth);
             return TD3_ERR_ARG;
         }
-        deep = true;
-        nkey = nlev;
     }
     // node count and level count
     i64 count = 1, s_sum = 0;
