@@ -77,6 +77,27 @@ int td3_direct(int kernel, int precision, const double *targets, int64_t m,
  * read-back of the split counts).  Supports N < 2^31 and trees of depth <= 64.
  */
 size_t td3_tree_workspace_bytes(int64_t n, int64_t capacity);
+
+/*
+ * The trees of S independent systems at once (an ensemble; one tree per
+ * ElectrostaticSimulation in the reference, simulation.py:641-667).  pos (2, N) holds the
+ * systems one after another; particle_system (N,) int32 (non-decreasing) and system_start
+ * (S + 1,) int64 (device) say which; roots (S, 3) f64 (device) are the systems' root
+ * squares (xmid, ymid, half).  Each system's tree is the one td3_tree_build makes from
+ * its own particles and root (same cells, same input order in the leaves); node s is the
+ * root of system s, and below the roots the numbering is breadth first over all systems
+ * (levels[d]:levels[d+1] are the depth-d nodes of every system).  perm and node ranges
+ * index the concatenated particles.  Otherwise as td3_tree_build (same workspace size).
+ * No counterpart in the reference (its systems are built one by one).
+ */
+int td3_forest_build(const double *pos, int64_t n, int64_t systems,
+                     const int32_t *particle_system, const int64_t *system_start,
+                     const double *roots, int64_t leaf_capacity, int64_t max_depth,
+                     int64_t capacity, double *node_center, double *node_half,
+                     int64_t *node_start, int64_t *node_end, int64_t *node_child,
+                     int64_t *node_depth, int64_t *levels, int64_t *perm, double *sorted_pos,
+                     void *node_info, void *workspace, size_t workspace_bytes,
+                     int64_t *out_sizes, void *stream);
 int td3_tree_build(const double *pos, int64_t n, double xmid, double ymid, double half,
                    int64_t leaf_capacity, int64_t max_depth, int64_t capacity, double *node_center,
                    double *node_half, int64_t *node_start, int64_t *node_end, int64_t *node_child,
@@ -124,16 +145,19 @@ int td3_target_order(const double *targets, int64_t m, double lox, double loy, d
  * 4h^2 < theta^2 |r - c|^2 (log, order-p series, unsoftened); leaves sum their particles
  * directly (softened), skipping exclude[i] and zero charges.  order (M,) is the target
  * order of the wavefronts (td3_target_order); sorted_pos / sorted_charge / perm are the
- * sources in tree order.  phi (M,), field (2, M) written in target order.
+ * sources in tree order.  target_root (M,) int32, by target index, is the node the walk
+ * of each target starts from: NULL for node 0 (one tree); in a forest (td3_forest_build)
+ * the root of the target's system, so that systems never see each other.  phi (M,),
+ * field (2, M) written in target order.
  */
 int td3_walk(int kernel, int precision, const double *targets, int64_t m, const int64_t *order,
              const int64_t *exclude, const double *sorted_pos, const double *sorted_charge,
              const int64_t *perm, int64_t n, const double *node_center, const double *node_half,
              const void *node_info, const double *node_qabs, const double *node_q,
              const double *node_p, const double *node_s, const double *moment_re,
-             const double *moment_im, int64_t order_p, int64_t tree_depth, double theta,
-             double softening, int with_potential, double *phi, double *field, int64_t *status,
-             void *stream);
+             const double *moment_im, int64_t order_p, int64_t tree_depth,
+             const int32_t *target_root, double theta, double softening, int with_potential,
+             double *phi, double *field, int64_t *status, void *stream);
 
 /*
  * Dual-tree traversal of one tree with itself (the FMM's interaction lists).  Replaces
@@ -204,8 +228,8 @@ int td3_unscale_locals(const double *node_half, int64_t node_count, int64_t orde
 /*
  * L2P + P2P at every source (flat_tree.py:1580-1600, kernels/fmm_evaluate
  * fmm_evaluate_cuda.cu:11-87; kernels/fmm_evaluate_host/fmm_evaluate_host_pyccel.py:14-100):
- * one wavefront per leaf, owner-computes over the leaf's P2P rows (td3_lists of the P2P
- * pairs, mode 1).  Own charge excluded; softening on the near pairs only.  kernel TD3_LOG:
+ * one lane per particle, owner-computes over its leaf's P2P row (td3_lists of the P2P
+ * pairs, mode 1); leaf_of from td3_leaf_index.  Own charge excluded; softening on the near pairs only.  kernel TD3_LOG:
  * scaled complex locals in local_re/local_im; TD3_COULOMB: Cartesian locals (C, nc) in
  * local_re (cart_evaluate_host_pyccel.py:16-110).  phi (N,), field (2, N) written in input
  * order (phi zero when with_potential is 0).
@@ -214,8 +238,16 @@ int td3_fmm_eval(int kernel, int precision, int64_t order, const double *sorted_
                  const double *sorted_charge, const int64_t *perm, int64_t n,
                  const double *node_center, const double *node_half, const void *node_info,
                  int64_t node_count, const double *local_re, const double *local_im,
-                 const uint32_t *p2p_row, const int32_t *p2p_col, double softening,
-                 int with_potential, double *phi, double *field, int64_t *status, void *stream);
+                 const uint32_t *p2p_row, const int32_t *p2p_col, const int32_t *leaf_of,
+                 double softening, int with_potential, double *phi, double *field,
+                 int64_t *status, void *stream);
+
+/*
+ * leaf_of[k] (int32, N) = the leaf holding sorted particle k: the per-particle map the
+ * evaluation kernels read (one lane per particle).  No counterpart in the reference (its
+ * CPU loops run leaf by leaf).
+ */
+int td3_leaf_index(const void *node_info, int64_t node_count, int32_t *leaf_of, void *stream);
 
 /*
  * Dual traversal of a target tree against a source tree (fmm_at's lists): replaces
@@ -241,7 +273,8 @@ int td3_dual_traverse(const double *t_center, const double *t_half, const void *
 int td3_fmm_at_eval(int kernel, int precision, int64_t order, const double *sorted_targets,
                     const int64_t *target_perm, const int64_t *exclude, int64_t m,
                     const double *t_center, const double *t_half, const void *t_info,
-                    int64_t t_count, const double *local_re, const double *local_im,
+                    int64_t t_count, const int32_t *t_leaf_of, const double *local_re,
+                    const double *local_im,
                     const uint32_t *p2p_row, const int32_t *p2p_col, const double *sorted_pos,
                     const double *sorted_charge, const int64_t *perm, int64_t n,
                     const void *s_info, double softening, int with_potential, double *phi,

@@ -36,6 +36,10 @@ struct int2 {
 struct int4 {
     int x, y, z, w;
 };
+struct alignas(16) float4 {
+    float x, y, z, w;
+};
+inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }
 inline int2 make_int2(int x, int y) { return {x, y}; }
 inline int4 make_int4(int x, int y, int z, int w) { return {x, y, z, w}; }
 struct dim3 {
@@ -337,7 +341,28 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     return td3emu_from<T>(all[td3emu::lane_id() ^ (unsigned)mask]);
 }
 
+// Matrix cores: v_mfma_f32_16x16x4_f32 on one wavefront.  Lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]; the result D[4 (l / 16) + r][l % 16] lands in element r of lane l.
+#define ext_vector_type(n) vector_size(sizeof(float) * (n))
+typedef float td3emu_f32x4 __attribute__((vector_size(16)));
+inline td3emu_f32x4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, td3emu_f32x4 c, int,
+                                                         int, int) {
+    auto A = td3emu::exchange(td3emu_bits(a));
+    auto B = td3emu::exchange(td3emu_bits(b));
+    const unsigned lane = td3emu::lane_id();
+    const unsigned col = lane % 16, row0 = 4 * (lane / 16);
+    td3emu_f32x4 d;
+    for (unsigned r = 0; r < 4; ++r) {
+        float sum = c[r];
+        for (unsigned kk = 0; kk < 4; ++kk)
+            sum += td3emu_from<float>(A[kk * 16 + row0 + r]) * td3emu_from<float>(B[kk * 16 + col]);
+        d[r] = sum;
+    }
+    return d;
+}
+
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+inline int __ffsll(unsigned long long v) { return __builtin_ffsll((long long)v); }
 inline int __clzll(long long v) { return v ? __builtin_clzll((unsigned long long)v) : 64; }
 
 // atomics (host atomics on plain memory)
@@ -368,6 +393,10 @@ inline double __dsqrt_rn(double a) { return std::sqrt(a); }
 inline float __logf(float a) { return std::log(a); }
 inline float __frcp_rn(float a) { return 1.0f / a; }
 inline float __frsqrt_rn(float a) { return 1.0f / std::sqrt(a); }
+// the hardware's approximate transcendentals (v_log_f32 is log2)
+inline float __builtin_amdgcn_logf(float a) { return std::log2(a); }
+inline float __builtin_amdgcn_rcpf(float a) { return 1.0f / a; }
+inline float __builtin_amdgcn_rsqf(float a) { return 1.0f / std::sqrt(a); }
 using std::atan2;
 using std::fabs;
 using std::log;

@@ -169,6 +169,8 @@ def case_fmm_at():
     got = t.fmm_at(targets, exclude=exclude, theta=0.5)
     want = r.fmm_at(targets, exclude=exclude, theta=0.5)
     assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
+    mixed = t.fmm_at(targets, exclude=exclude, theta=0.5, precision="mixed")
+    assert relative(mixed[0], want[0]) < 1e-5 and relative(mixed[1], want[1]) < 1e-5
 
 
 def case_coulomb():
@@ -180,6 +182,8 @@ def case_coulomb():
         got = t.fmm(theta=0.5, softening=0.01)
         want = r.fmm(theta=0.5, softening=0.01)
         assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-11
+        mixed = t.fmm(theta=0.5, softening=0.01, precision="mixed")
+        assert relative(mixed[0], want[0]) < 1e-5 and relative(mixed[1], want[1]) < 1e-5
     rng = np.random.default_rng(3)
     targets = np.concatenate((rng.uniform(-0.9, 0.9, (2, 60)), pos[:, :20]), axis=1)
     exclude = np.concatenate((np.full(60, -1), np.arange(20)))
@@ -188,6 +192,8 @@ def case_coulomb():
     got = t.fmm_at(targets, exclude=exclude, theta=0.5, softening=0.01)
     want = r.fmm_at(targets, exclude=exclude, theta=0.5, softening=0.01)
     assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
+    mixed = t.fmm_at(targets, exclude=exclude, theta=0.5, softening=0.01, precision="mixed")
+    assert relative(mixed[0], want[0]) < 1e-5 and relative(mixed[1], want[1]) < 1e-5
 
 
 def case_push():
@@ -219,6 +225,33 @@ def case_push():
         for _ in range(3):
             sim.step()
         assert np.all(np.isfinite(sim.positions.numpy()))
+
+
+def case_forest():
+    from tree_d_3d_amd import ensemble
+
+    rng = np.random.default_rng(31)
+    sizes = [300, 17, 1, 520, 64]
+    parts = [rng.normal(0, 0.1 + 0.05 * i, (2, k)).clip(-0.9, 0.9) for i, k in enumerate(sizes)]
+    parts[2][:] = 0.0
+    roots = [(0.0, 0.0, 1.0), (0.1, -0.2, 1.2), (0.0, 0.0, 1.0), (0.05, 0.0, 0.97), (0.0, 0.0, 1.0)]
+    pos = torch.tensor(np.concatenate(parts, axis=1))
+    q = torch.tensor(rng.choice([-1.0, 1.0], pos.shape[1]))
+    start = torch.tensor(np.concatenate(([0], np.cumsum(sizes))), dtype=torch.int64)
+    system = torch.repeat_interleave(torch.arange(len(sizes), dtype=torch.int32),
+                                     torch.tensor(sizes))
+    forest = ensemble._Forest(pos, q, system, start, torch.tensor(roots, dtype=torch.float64),
+                              leaf_capacity=8, max_depth=48)
+    phi, field = forest.walk(0.5, 1e-3)
+    total = len(sizes)
+    for s_i, (k, root) in enumerate(zip(sizes, roots)):
+        sl = slice(int(start[s_i]), int(start[s_i + 1]))
+        single = td3.FlatQuadtree(pos[:, sl], q[sl], td3.Quad(*root), leaf_capacity=8)
+        total += single.node_count - 1
+        want = single.evaluate(pos[:, sl], exclude=np.arange(k), theta=0.5, softening=1e-3)
+        assert np.array_equal(phi[sl].numpy(), want[0].numpy())
+        assert np.array_equal(field[:, sl].numpy(), want[1].numpy())
+    assert forest.node_count == total, (forest.node_count, total)
 
 
 CASES = {name[5:]: fn for name, fn in globals().items() if name.startswith("case_")}

@@ -13,7 +13,7 @@ import pytest
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_emulation")
 QUICK = ["direct", "moments"]
-FULL = ["direct", "tree", "big_tree", "moments", "walk", "fmm", "fmm_at", "coulomb", "push"]
+FULL = ["direct", "tree", "big_tree", "forest", "moments", "walk", "fmm", "fmm_at", "coulomb", "push"]
 
 
 @pytest.fixture(scope="module")

@@ -1,0 +1,103 @@
+"""Adjoint gradients of losses on FMM potentials (log kernel) against FP64 torch autograd
+through a direct sum and against central finite differences of the FMM loss itself
+(the pattern of tests/test_smooth.py:27-37 in the reference)."""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def uniform(n, m, seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1, 1, (2, n))
+    q = rng.choice([-1.0, 1.0], n)
+    probes = rng.uniform(-1, 1, (2, m))
+    return pos, q, probes
+
+
+def torch_direct(pos, q, probes, eps):
+    """phi at the sources (own charge excluded) and at the probes, FP64, differentiable."""
+    d2 = ((pos[:, :, None] - pos[:, None, :]) ** 2).sum(0) + eps * eps
+    n = pos.shape[1]
+    eye = torch.eye(n, dtype=torch.float64, device=pos.device)
+    kern = -torch.log(d2 + eye) * (1 - eye)
+    phi = kern @ q
+    dp2 = ((probes[:, :, None] - pos[:, None, :]) ** 2).sum(0) + eps * eps
+    phi_p = -torch.log(dp2) @ q
+    return phi, phi_p
+
+
+def loss_of(phi, phi_p, q):
+    # config 5: mean squared potential at the probes plus the energy 1/2 sum q phi
+    return (phi_p ** 2).mean() + 0.5 * torch.dot(q, phi)
+
+
+def relative(a, b):
+    return float((a - b).abs().max() / b.abs().max())
+
+
+def test_adjoint_matches_torch_autograd_of_direct_sum(td3):
+    # unsoftened: the FMM softens only its near pairs, so only eps = 0 has a direct-sum
+    # equivalent; the softened case is checked by finite differences below
+    eps = 0.0
+    pos_h, q_h, probes_h = uniform(8192, 512, 3)
+    dev = torch.device("cuda")
+    pos = torch.tensor(pos_h, device=dev, requires_grad=True)
+    q = torch.tensor(q_h, device=dev, requires_grad=True)
+    probes = torch.tensor(probes_h, device=dev, requires_grad=True)
+    print(f"adjoint vs torch eps={eps}", flush=True)
+    phi, phi_p = td3.log_potentials(pos, q, probes, root=td3.Quad(0, 0, 1), softening=eps)
+    got = torch.autograd.grad(loss_of(phi, phi_p, q), (pos, q, probes))
+    rphi, rphi_p = torch_direct(pos, q, probes, eps)
+    want = torch.autograd.grad(loss_of(rphi, rphi_p, q), (pos, q, probes))
+    # the FMM's truncation error at order 12, and the 1e-5 of the gradient target
+    assert relative(phi.detach(), rphi.detach()) < 1e-7
+    for g, w in zip(got, want):
+        assert relative(g, w) < 1e-5
+
+
+def test_adjoint_matches_central_differences(td3):
+    # h = 1e-6, random particles, the FMM loss itself
+    pos_h, q_h, probes_h = uniform(20000, 1024, 4)
+    dev = torch.device("cuda")
+    root = td3.Quad(0, 0, 1)
+    kw = dict(root=root, softening=1e-3)
+
+    def loss(pos, q):
+        phi, phi_p = td3.log_potentials(pos, q, torch.tensor(probes_h, device=dev), **kw)
+        return loss_of(phi, phi_p, q)
+
+    pos = torch.tensor(pos_h, device=dev, requires_grad=True)
+    q = torch.tensor(q_h, device=dev, requires_grad=True)
+    g_pos, g_q = torch.autograd.grad(loss(pos, q), (pos, q))
+    scale_pos, scale_q = float(g_pos.abs().max()), float(g_q.abs().max())
+    h = 1e-6
+    for i in np.random.default_rng(5).choice(20000, 8, replace=False):
+        for axis in range(2):
+            pp, pm = pos_h.copy(), pos_h.copy()
+            pp[axis, i] += h
+            pm[axis, i] -= h
+            fd = (float(loss(torch.tensor(pp, device=dev), torch.tensor(q_h, device=dev)))
+                  - float(loss(torch.tensor(pm, device=dev), torch.tensor(q_h, device=dev)))) / (2 * h)
+            assert abs(fd - float(g_pos[axis, i])) < 1e-5 * scale_pos
+        qp, qm = q_h.copy(), q_h.copy()
+        qp[i] += h
+        qm[i] -= h
+        fd = (float(loss(torch.tensor(pos_h, device=dev), torch.tensor(qp, device=dev)))
+              - float(loss(torch.tensor(pos_h, device=dev), torch.tensor(qm, device=dev)))) / (2 * h)
+        assert abs(fd - float(g_q[i])) < 1e-5 * scale_q
+
+
+def test_energy_gradient_is_minus_the_force(td3):
+    pos_h, q_h, _ = uniform(3000, 0, 6)
+    dev = torch.device("cuda")
+    pos = torch.tensor(pos_h, device=dev, requires_grad=True)
+    q = torch.tensor(q_h, device=dev)
+    energy = td3.log_energy(pos, q, root=td3.Quad(0, 0, 1), softening=1e-3)
+    (g,) = torch.autograd.grad(energy, (pos,))
+    tree = td3.FlatQuadtree(pos_h, q_h, td3.Quad(0, 0, 1), kernel="log")
+    _, field = tree.fmm(softening=1e-3)
+    force = q_h * field
+    np.testing.assert_allclose(-g.cpu().numpy(), force, rtol=0, atol=1e-10 * np.abs(force).max())

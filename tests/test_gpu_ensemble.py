@@ -1,0 +1,82 @@
+"""The batched ensemble: every system's forest tree and walk equal its own tree's, and
+ensemble steps equal the systems' own ElectrostaticSimulation steps (method "tree")."""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def gaussian_systems(td3, sizes, seed, **kwargs):
+    rng = np.random.default_rng(seed)
+    sims = []
+    for k in sizes:
+        pos = rng.normal(0.0, 0.1, (2, k)).clip(-0.95, 0.95)
+        bodies = [td3.ChargedBody(pos[0, i], pos[1, i], charge=1.0, mass=1.0) for i in range(k)]
+        kw = dict(theta=0.5, k_e=1.0, softening=1e-3, dt=1e-4, B_z=1.0, method="tree")
+        kw.update(kwargs)
+        sims.append(td3.ElectrostaticSimulation(bodies, td3.Quad(0.0, 0.0, 1.0), **kw))
+    return sims
+
+
+def test_forest_walk_equals_each_systems_own_tree(td3):
+    ens_mod = sys.modules["tree_d_3d_amd.ensemble"]
+    rng = np.random.default_rng(2)
+    sizes = [1, 9, 300, 4097, 12000, 64, 2500]
+    roots = [(0.0, 0.0, 1.0), (0.2, -0.1, 1.5), (0.0, 0.0, 1.0), (0.01, 0.02, 0.99),
+             (0.0, 0.0, 2.0), (-0.3, 0.3, 1.7), (0.0, 0.0, 1.0)]
+    parts = [rng.normal(0, 0.12, (2, k)).clip(-0.95, 0.95) for k in sizes]
+    pos = torch.tensor(np.concatenate(parts, axis=1), device="cuda")
+    q = torch.tensor(rng.choice([-1.0, 1.0], pos.shape[1]), device="cuda")
+    start = torch.tensor(np.concatenate(([0], np.cumsum(sizes))), device="cuda")
+    system = torch.repeat_interleave(torch.arange(len(sizes), dtype=torch.int32, device="cuda"),
+                                     torch.tensor(sizes, device="cuda"))
+    print("forest walk", flush=True)
+    forest = ens_mod._Forest(pos, q, system, start, torch.tensor(roots, device="cuda"), 8, 48)
+    phi, field = forest.walk(0.5, 1e-3)
+    total = len(sizes)
+    for s, (k, root) in enumerate(zip(sizes, roots)):
+        sl = slice(int(start[s]), int(start[s + 1]))
+        single = td3.FlatQuadtree(pos[:, sl], q[sl], td3.Quad(*root), leaf_capacity=8)
+        total += single.node_count - 1
+        want_phi, want_field = single.evaluate(pos[:, sl], exclude=np.arange(k), theta=0.5,
+                                               softening=1e-3)
+        assert torch.equal(phi[sl], want_phi) and torch.equal(field[:, sl], want_field)
+    assert forest.node_count == total
+
+
+def test_ensemble_steps_equal_each_simulation(td3):
+    sizes = [1500, 3000, 700, 5000]
+    alone = gaussian_systems(td3, sizes, seed=4)
+    together = td3.Ensemble(gaussian_systems(td3, sizes, seed=4))
+    print("ensemble steps", flush=True)
+    for _ in range(5):
+        for sim in alone:
+            sim.step()
+        together.step()
+    for s, sim in enumerate(alone):
+        sl = together.system_slice(s)
+        np.testing.assert_allclose(together.positions[:, sl].cpu().numpy(),
+                                   sim.positions.cpu().numpy(), rtol=0, atol=1e-14)
+        np.testing.assert_allclose(together.velocities[:, sl].cpu().numpy(),
+                                   sim.velocities.cpu().numpy(), rtol=1e-12, atol=1e-12)
+    energies = together.total_energies()
+    want = np.array([sim.get_total_energy() for sim in alone])
+    np.testing.assert_allclose(energies, want, rtol=1e-12)
+
+
+def test_ensemble_roots_grow_and_energy_drift_is_small(td3):
+    # particles pushed out of their root square: each system's root grows on its own
+    sims = gaussian_systems(td3, [800, 2000], seed=6, dt=1e-5)
+    sims[1].velocities[0] += 3000.0
+    ens = td3.Ensemble(sims)
+    e0 = ens.total_energies()
+    for _ in range(40):
+        ens.step()
+    roots = ens.roots.cpu().numpy()
+    assert roots[0, 2] == 1.0 and roots[1, 2] > 1.0
+    drift = np.abs(ens.total_energies() - e0) / np.abs(e0)
+    assert drift[0] < 1e-3 and drift[1] < 1e-3, drift

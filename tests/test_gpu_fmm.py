@@ -237,3 +237,26 @@ def test_coulomb_fmm_at_matches_oracle(td3, oracle, precision):
     _, walk = tree.evaluate(targets, exclude=exclude, theta=0.5, softening=0.01)
     _, direct = oracle.direct_field(targets, pos, q, exclude=exclude, softening=0.01)
     assert np.max(np.abs(field - walk)) < 20 * 2e-4 * np.max(np.abs(direct))
+
+
+@pytest.mark.parametrize("kernel", ["log", "coulomb"])
+def test_mixed_near_field_of_close_pairs_across_cells(td3, oracle, kernel):
+    # pairs 1e-7 apart straddling the dyadic lines x = 0.5 and y = -0.25 (a cell boundary
+    # at every depth): the mixed near field must keep their differences exact
+    pos, q = cloud(3000, seed=21)
+    rng = np.random.default_rng(22)
+    ys = rng.uniform(-0.9, 0.9, 40)
+    xs = rng.uniform(-0.9, 0.9, 40)
+    close = np.concatenate((
+        np.stack((np.full(40, 0.5 - 5e-8), ys)), np.stack((np.full(40, 0.5 + 5e-8), ys)),
+        np.stack((xs, np.full(40, -0.25 - 5e-8))), np.stack((xs, np.full(40, -0.25 + 5e-8)))),
+        axis=1)
+    pos = np.concatenate((pos, close), axis=1)
+    q = np.concatenate((q, rng.choice([-1.0, 1.0], close.shape[1])))
+    print(f"close pairs kernel={kernel}", flush=True)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel=kernel, order=12, leaf_capacity=8)
+    ref = oracle.OracleTree(pos, q, (0, 0, 1), kernel=kernel, order=12, leaf_capacity=8)
+    eps = 1e-9 if kernel == "coulomb" else 0.0
+    want = ref.fmm(theta=0.5, softening=eps)
+    got = tree.fmm(theta=0.5, softening=eps, precision="mixed")
+    assert relative(got[1], want[1]) < 1e-6 and relative(got[0], want[0]) < 1e-6

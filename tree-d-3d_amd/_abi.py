@@ -40,13 +40,15 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_tree_workspace_bytes": ([_I, _I], _Z),
     "td3_tree_build": ([_P, _I, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                         _P, _Z, _P, _P], _C),
+    "td3_forest_build": ([_P, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                          _P, _P, _Z, _P, _P], _C),
     "td3_permute": ([_P, _I, _I, _P, _I, _P, _P], _C),
     "td3_moments": ([_C, _I, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P],
                     _C),
     "td3_target_order_workspace_bytes": ([_I], _Z),
     "td3_target_order": ([_P, _I, _D, _D, _D, _P, _P, _Z, _P], _C),
     "td3_walk": ([_C, _C, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I,
-                  _I, _D, _D, _C, _P, _P, _P, _P], _C),
+                  _I, _P, _D, _D, _C, _P, _P, _P, _P], _C),
     "td3_traverse_workspace_bytes": ([_I], _Z),
     "td3_traverse": ([_P, _P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P, _Z, _P, _P], _C),
     "td3_lists_workspace_bytes": ([_I], _Z),
@@ -56,12 +58,13 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_m2l": ([_C, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P], _C),
     "td3_l2l": ([_I, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P], _C),
     "td3_unscale_locals": ([_P, _I, _I, _P, _P, _P], _C),
-    "td3_fmm_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _D, _C, _P, _P,
-                      _P, _P], _C),
+    "td3_fmm_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _C, _P,
+                      _P, _P, _P], _C),
+    "td3_leaf_index": ([_P, _I, _P, _P], _C),
     "td3_dual_traverse": ([_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P, _Z,
                            _P, _P], _C),
     "td3_fmm_at_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
-                         _I, _P, _D, _C, _P, _P, _P, _P], _C),
+                         _P, _I, _P, _D, _C, _P, _P, _P, _P], _C),
     "td3_cart_moments": ([_I, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P], _C),
     "td3_cart_m2l_workspace_bytes": ([_I, _I, _I], _Z),
     "td3_cart_m2l": ([_I, _P, _P, _P, _I, _I, _P, _P, _P, _D, _P, _P, _Z, _P], _C),

@@ -87,3 +87,4 @@ __device__ __forceinline__ float td3_wave_sumf(float v) {
 }
 
 __device__ __forceinline__ int td3_lane() { return (int)(threadIdx.x & (TD3_WAVE - 1)); }
+

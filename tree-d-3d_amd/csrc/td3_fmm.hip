@@ -150,6 +150,142 @@ __global__ __launch_bounds__(256) void m2l_kernel(
     }
 }
 
+// ------------------------------------------------------------------------------ M2L on MFMA
+// The same translation with S = coeff^T X as a matrix product on the matrix cores
+// (v_mfma_f32_16x16x4_f32), for orders up to 15 (p + 1 <= 16 coefficients, zero padded).
+// One wavefront per destination row; the row's pairs go 16 at a time.  Lane (j, q) =
+// (lane % 16, lane / 16) takes pair j of the tile: it forms X_k = A_k u^k for
+// k = q, q + 4, q + 8, q + 12 (the B operand of the four K = 4 products) and holds
+// coeff[4 kb + q, j] (the A operand); the result D[l][j] = S_l of pair j comes back in lane
+// (j, l / 4), element l % 4.  The lane scales it by (-1)^(l+1) v^l (l >= 1) or forms
+// A_0 ln t - S_0 (l = 0) and adds it to its running sum; the 16 lanes of each q are summed
+// at the end in a fixed butterfly, so the result is deterministic.
+typedef float td3_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ Cx<float> cpow_small(Cx<float> u, Cx<float> u2, Cx<float> u3, int q) {
+    // u^q for q in 0..3, selected without divergence
+    Cx<float> r = {1.0f, 0.0f};
+    r = q == 1 ? u : r;
+    r = q == 2 ? u2 : r;
+    r = q == 3 ? u3 : r;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void m2l_mfma_kernel(
+    const int* __restrict__ col, const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
+    const double* __restrict__ dcenter, const double* __restrict__ dhalf,
+    const double* __restrict__ scenter, const double* __restrict__ shalf,
+    const float* __restrict__ mre, const float* __restrict__ mim, int p1,
+    const double* __restrict__ coeff, double* __restrict__ lre, double* __restrict__ lim) {
+    const int lane = td3_lane();
+    const i64 d = node_lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    if (d >= node_hi) return;  // uniform over the wavefront
+    const u32 r0 = row_start[d], r1 = row_start[d + 1];
+    if (r0 == r1) return;
+    const int j = lane & 15, q = lane >> 4;
+    float ca[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        const int k = 4 * kb + q;
+        ca[kb] = (k < p1 && j < p1) ? (float)coeff[k * p1 + j] : 0.0f;
+    }
+    const double cx = dcenter[2 * d], cy = dcenter[2 * d + 1];
+    const float rt = (float)(1.4142135623730951 * dhalf[d]);
+    float acc_re[4] = {0.0f, 0.0f, 0.0f, 0.0f}, acc_im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (u32 e0 = r0; e0 < r1; e0 += 16) {
+        const u32 e = e0 + (u32)j;
+        const bool valid = e < r1;
+        float x_re[4] = {0.0f, 0.0f, 0.0f, 0.0f}, x_im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        Cx<float> v = {0.0f, 0.0f};
+        float lnr = 0.0f;
+        if (valid) {
+            const u32 src = (u32)col[e];
+            // t = c_dst - c_src in FP64 (exact for nearby centres), then FP32
+            const float tr = (float)(cx - scenter[2 * src]);
+            const float ti = (float)(cy - scenter[2 * src + 1]);
+            const float t2 = fmaf(tr, tr, ti * ti);
+            const float inv = __builtin_amdgcn_rcpf(t2);
+            const float wr = tr * inv, wi = -ti * inv;
+            const float rs = (float)(1.4142135623730951 * shalf[src]);
+            const Cx<float> u = {rs * wr, rs * wi};
+            v = {rt * wr, rt * wi};
+            const Cx<float> u2 = cmul(u, u);
+            const Cx<float> u3 = cmul(u2, u);
+            const Cx<float> u4 = cmul(u2, u2);
+            Cx<float> up = cpow_small(u, u2, u3, q);
+            // rows of this lane's k = q + 4 kb: one address, immediate offsets
+            const float* ar = mre + (src * (u32)p1 + (u32)q);
+            const float* ai = mim + (src * (u32)p1 + (u32)q);
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                const int k = 4 * kb + q;
+                if (k < p1) {
+                    const Cx<float> x = cmul(Cx<float>{ar[4 * kb], ai[4 * kb]}, up);
+                    x_re[kb] = x.re;
+                    x_im[kb] = x.im;
+                }
+                up = cmul(up, u4);
+            }
+            // ln |t| (v_log_f32 is log2)
+            lnr = 0.34657359027997264f * __builtin_amdgcn_logf(t2);
+        }
+        td3_f32x4 dre = {0.0f, 0.0f, 0.0f, 0.0f}, dim = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            dre = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[kb], x_re[kb], dre, 0, 0, 0);
+            dim = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[kb], x_im[kb], dim, 0, 0, 0);
+        }
+        if (valid) {
+            // v^(4q), then one more factor of v per element
+            const Cx<float> v2 = cmul(v, v);
+            const Cx<float> v4 = cmul(v2, v2);
+            const Cx<float> v8 = cmul(v4, v4);
+            Cx<float> vp = {1.0f, 0.0f};
+            vp = q == 1 ? v4 : vp;
+            vp = q == 2 ? v8 : vp;
+            vp = q == 3 ? cmul(v8, v4) : vp;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int l = 4 * q + r;
+                const Cx<float> sl = {dre[r], dim[r]};
+                if (l == 0) {
+                    // A_0 = X_0 (u^0 = 1 in lane q = 0) is real: the cell's charge.  Of
+                    // A_0 ln t only the real part A_0 ln|t| is kept: the imaginary part of
+                    // the constant term reaches neither the potential (-2 Re W), nor the
+                    // field (W'), nor, through L2L, any other coefficient.
+                    acc_re[0] += x_re[0] * lnr - sl.re;
+                    acc_im[0] -= sl.im;
+                } else if (l < p1) {
+                    const Cx<float> b = cmul(sl, vp);
+                    const float sign = (l & 1) ? 1.0f : -1.0f;
+                    acc_re[r] += sign * b.re;
+                    acc_im[r] += sign * b.im;
+                }
+                vp = cmul(vp, v);
+            }
+        }
+    }
+    // sum the 16 pair lanes of each q
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) {
+            acc_re[r] += __shfl_xor(acc_re[r], off, 64);
+            acc_im[r] += __shfl_xor(acc_im[r], off, 64);
+        }
+    }
+    if (j == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int l = 4 * q + r;
+            if (l < p1) {
+                lre[d * p1 + l] = (double)acc_re[r];
+                lim[d * p1 + l] = (double)acc_im[r];
+            }
+        }
+    }
+}
+
 // Destinations whose row crosses a wave boundary: last-segment partial of the first wave
 // plus the first-segment partials of the following waves, in order.
 template <typename T>
@@ -246,12 +382,9 @@ __global__ void l2l_kernel(const double* __restrict__ center, const double* __re
 // phi += -2 Re W, E_x - i E_y += 2 W'; near field phi -= q ln(r^2 + eps^2),
 // E += 2 q d / (r^2 + eps^2) over the P2P partner leaves, each particle's own charge
 // excluded; a charged coincident pair sets status.
-// One lane per target.  A wavefront takes kEvalNodes consecutive cells (breadth-first
-// order, so neighbouring leaves of one depth) and spreads the particles of the leaves among
-// them over its lanes; each lane then walks its leaf's partner row on its own.  Lanes of one
-// leaf read the same sources (one broadcast load), and a row's order fixes the order of
-// every sum, so results are deterministic.
-constexpr int kEvalNodes = 16;
+// One lane per target, in tree order (64 consecutive particles per wavefront), each walking its leaf's partner row on its own.  Lanes of one leaf read
+// the same sources (one broadcast load), and a row's order fixes the order of every sum,
+// so results are deterministic.
 
 // MODE 0 (fmm): targets are the sources themselves, in tree order; a particle's own slot is
 // skipped and a coincident pair is an error only when both are charged
@@ -270,86 +403,102 @@ __global__ __launch_bounds__(256) void eval_kernel(
     const double* __restrict__ lim, int p1, int order, const u32* __restrict__ prow,
     const int* __restrict__ pcol, const double* __restrict__ spos,
     const double* __restrict__ scharge, const i64* __restrict__ sperm, i64 n,
-    const int4* __restrict__ sinfo, double eps2, int with_potential, double* __restrict__ phi,
-    double* __restrict__ field, i64* __restrict__ status) {
-    const int lane = td3_lane();
-    const i64 node0 = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64 * kEvalNodes;
-    if (node0 >= t_count) return;  // uniform over the wavefront
-    // lanes below kEvalNodes: the particle count of their cell when it is a leaf
-    int cnt = 0, first = 0;
-    if (lane < kEvalNodes && node0 + lane < t_count) {
-        const int4 lf = tinfo[node0 + lane];
-        if (lf.z < 0) {
-            cnt = lf.y - lf.x;
-            first = lf.x;
-        }
-    }
-    int incl = cnt;
-#pragma unroll
-    for (int off = 1; off < kEvalNodes; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
-    }
-    const int total = __shfl(incl, kEvalNodes - 1, 64);
-    for (int base = 0; base < total; base += 64) {
-        const int idx = base + lane;
-        // owning cell: the first of the kEvalNodes lanes whose inclusive count exceeds idx
-        // (a fixed search, so every lane takes part in every shuffle)
-        int lo_l = 0;
-#pragma unroll
-        for (int step = kEvalNodes / 2; step > 0; step >>= 1) {
-            const int pv = __shfl(incl, lo_l + step - 1, 64);
-            if (pv <= idx) lo_l += step;
-        }
-        const int ofirst = __shfl(first, lo_l, 64);
-        const int oincl = __shfl(incl, lo_l, 64);
-        const int ocnt = __shfl(cnt, lo_l, 64);
-        if (idx >= total) continue;  // no cross-lane operation below
-        const i64 leaf = node0 + lo_l;
-        const int k = ofirst + idx - (oincl - ocnt);
+    const int4* __restrict__ sinfo, const int* __restrict__ leaf_of, double eps2,
+    int with_potential, double* __restrict__ phi, double* __restrict__ field,
+    i64* __restrict__ status) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    {
+        const i64 leaf = leaf_of[k];
         const double tx = tpos[k], ty = tpos[m + k];
         const double tq = MODE == 0 ? scharge[k] : 0.0;
         const i64 tskip = MODE == 1 ? exclude[tperm[k]] : -1;
         double acc_p = 0.0, acc_x = 0.0, acc_y = 0.0;
         bool bad = false;
         const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
-        for (u32 r = r0; r < r1; ++r) {
-            const int4 sf = sinfo[pcol[r]];
-            for (int ks = sf.x; ks < sf.y; ++ks) {
-                const double sq = scharge[ks];
-                if (MODE == 0 && ks == k) continue;
-                if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
-                const double dx = tx - spos[ks], dy = ty - spos[n + ks];
-                const double r2 = dx * dx + dy * dy + eps2;
-                if (r2 == 0.0) {
-                    if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
-                    continue;
-                }
-                if (KERNEL == TD3_COULOMB) {
-                    if (PREC == TD3_FP64) {
+        if (PREC == TD3_FP64) {
+            for (u32 r = r0; r < r1; ++r) {
+                const int4 sf = sinfo[pcol[r]];
+                for (int ks = sf.x; ks < sf.y; ++ks) {
+                    const double sq = scharge[ks];
+                    if (MODE == 0 && ks == k) continue;
+                    if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
+                    const double dx = tx - spos[ks], dy = ty - spos[n + ks];
+                    const double r2 = dx * dx + dy * dy + eps2;
+                    if (r2 == 0.0) {
+                        if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
+                        continue;
+                    }
+                    if (KERNEL == TD3_COULOMB) {
                         const double inv_r = 1.0 / sqrt(r2);
                         const double cr3 = sq * inv_r * inv_r * inv_r;
                         acc_p += sq * inv_r;
                         acc_x += cr3 * dx;
                         acc_y += cr3 * dy;
                     } else {
-                        const float inv_r = __frsqrt_rn((float)r2), cf = (float)sq;
-                        const float cr3 = cf * inv_r * inv_r * inv_r;
-                        acc_p += (double)(cf * inv_r);
-                        acc_x += (double)(cr3 * (float)dx);
-                        acc_y += (double)(cr3 * (float)dy);
+                        if (with_potential) acc_p -= sq * log(r2);
+                        const double inv = 2.0 * sq / r2;
+                        acc_x += inv * dx;
+                        acc_y += inv * dy;
                     }
-                } else if (PREC == TD3_FP64) {
-                    if (with_potential) acc_p -= sq * log(r2);
-                    const double inv = 2.0 * sq / r2;
-                    acc_x += inv * dx;
-                    acc_y += inv * dy;
+                }
+            }
+        } else {
+            // Differences in FP64 (exact for nearby particles, whatever cells hold them),
+            // kernel in FP32; FP32 sums over one partner leaf, FP64 sums over the leaves.
+            const float eps2f = (float)eps2;
+            for (u32 r = r0; r < r1; ++r) {
+                const int4 sf = sinfo[pcol[r]];
+                float fp = 0.0f, fx = 0.0f, fy = 0.0f;
+                for (int ks = sf.x; ks < sf.y; ++ks) {
+                    const double sq = scharge[ks];
+                    if (MODE == 0 && ks == k) continue;
+                    if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
+                    const double ddx = tx - spos[ks], ddy = ty - spos[n + ks];
+                    const float dx = (float)ddx, dy = (float)ddy, qf = (float)sq;
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
+                    if (r2 == 0.0f) {
+                        // zero or below FP32 range: decide on the FP64 pair
+                        const double dr2 = ddx * ddx + ddy * ddy + eps2;
+                        if (dr2 == 0.0) {
+                            if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
+                            continue;
+                        }
+                        if (KERNEL == TD3_COULOMB) {
+                            const double inv_r = 1.0 / sqrt(dr2);
+                            acc_p += sq * inv_r;
+                            acc_x += sq * inv_r * inv_r * inv_r * ddx;
+                            acc_y += sq * inv_r * inv_r * inv_r * ddy;
+                        } else {
+                            if (with_potential) acc_p -= sq * log(dr2);
+                            acc_x += 2.0 * sq / dr2 * ddx;
+                            acc_y += 2.0 * sq / dr2 * ddy;
+                        }
+                        continue;
+                    }
+                    if (KERNEL == TD3_COULOMB) {
+                        const float inv_r = __builtin_amdgcn_rsqf(r2);
+                        const float c = qf * inv_r;
+                        const float cr3 = c * inv_r * inv_r;
+                        fp += c;
+                        fx = fmaf(cr3, dx, fx);
+                        fy = fmaf(cr3, dy, fy);
+                    } else {
+                        // -q ln r2 = -ln 2 q log2 r2 and 2 q d / r2; the constants at the end
+                        if (with_potential) fp = fmaf(qf, __builtin_amdgcn_logf(r2), fp);
+                        const float inv = qf * __builtin_amdgcn_rcpf(r2);
+                        fx = fmaf(inv, dx, fx);
+                        fy = fmaf(inv, dy, fy);
+                    }
+                }
+                if (KERNEL == TD3_COULOMB) {
+                    acc_p += (double)fp;
+                    acc_x += (double)fx;
+                    acc_y += (double)fy;
                 } else {
-                    const float r2f = (float)r2;
-                    if (with_potential) acc_p -= (double)((float)sq * __logf(r2f));
-                    const float inv = 2.0f * (float)sq * __frcp_rn(r2f);
-                    acc_x += (double)(inv * (float)dx);
-                    acc_y += (double)(inv * (float)dy);
+                    acc_p -= 0.6931471805599453 * (double)fp;
+                    acc_x += 2.0 * (double)fx;
+                    acc_y += 2.0 * (double)fy;
                 }
             }
         }
@@ -400,25 +549,35 @@ __global__ __launch_bounds__(256) void eval_kernel(
     }
 }
 
+// leaf_of[k] = the leaf holding sorted particle k.
+__global__ void leaf_index_kernel(const int4* __restrict__ info, i64 count, int* __restrict__ leaf_of) {
+    const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= count) return;
+    const int4 f = info[node];
+    if (f.z >= 0) return;
+    for (int k = f.x; k < f.y; ++k) leaf_of[k] = (int)node;
+}
+
 template <int KERNEL, int MODE>
 int launch_eval(int precision, hipStream_t s, const double* tpos, const i64* tperm,
                 const i64* exclude, i64 m, const double* tcenter, const double* thalf,
                 const int4* tinfo, i64 t_count, const double* lre, const double* lim, int order,
                 const u32* prow, const int* pcol, const double* spos, const double* scharge,
-                const i64* sperm, i64 n, const int4* sinfo, double eps2, int with_potential,
-                double* phi, double* field, i64* status) {
-    const unsigned blocks = td3_blocks((t_count + kEvalNodes - 1) / kEvalNodes * 64, 256);
+                const i64* sperm, i64 n, const int4* sinfo, const int* leaf_of, double eps2,
+                int with_potential, double* phi, double* field, i64* status) {
+    TD3_ARG(leaf_of != nullptr, "leaf_of (td3_leaf_index of the target tree)");
+    const unsigned blocks = td3_blocks(m, 256);
     const int p1 = KERNEL == TD3_LOG ? order + 1 : (order + 1) * (order + 2) / 2;
     if (precision == TD3_FP64)
         TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<KERNEL, TD3_FP64, MODE>), dim3(blocks), dim3(256), 0, s,
                            tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, order,
-                           prow, pcol, spos, scharge, sperm, n, sinfo, eps2, with_potential, phi, field,
-                           status));
+                           prow, pcol, spos, scharge, sperm, n, sinfo, leaf_of, eps2,
+                           with_potential, phi, field, status));
     else
         TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<KERNEL, TD3_MIXED, MODE>), dim3(blocks), dim3(256), 0, s,
                            tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, order,
-                           prow, pcol, spos, scharge, sperm, n, sinfo, eps2, with_potential, phi, field,
-                           status));
+                           prow, pcol, spos, scharge, sperm, n, sinfo, leaf_of, eps2,
+                           with_potential, phi, field, status));
     TD3_LAUNCHED();
     return TD3_OK;
 }
@@ -447,6 +606,15 @@ int dispatch_m2l(hipStream_t s, int order, const int* col, const int* dst, const
                  const double* dhalf, const double* scenter, const double* shalf, const T* mre,
                  const T* mim, const double* coeff, double* lre, double* lim, T* side) {
     const int p1 = order + 1;
+    if constexpr (sizeof(T) == 4) {
+        if (p1 <= 16) {
+            TD3_TIMED(s, "m2l", hipLaunchKernelGGL(m2l_mfma_kernel, dim3(td3_blocks((node_hi - node_lo) * 64, 256)),
+                               dim3(256), 0, s, col, row_start, node_lo, node_hi, dcenter, dhalf,
+                               scenter, shalf, mre, mim, p1, coeff, lre, lim));
+            TD3_LAUNCHED();
+            return TD3_OK;
+        }
+    }
 #define TD3_M2L_CASE(PP)                                                                          \
     if (order <= PP)                                                                             \
         return launch_m2l<PP, T>(s, col, dst, row_start, e_lo, e_hi, node_lo, node_hi, dcenter,   \
@@ -550,34 +718,43 @@ extern "C" int td3_fmm_eval(int kernel, int precision, int64_t order, const doub
                             const double* node_center, const double* node_half,
                             const void* node_info, int64_t node_count, const double* local_re,
                             const double* local_im, const uint32_t* p2p_row, const int32_t* p2p_col,
-                            double softening, int with_potential, double* phi, double* field,
-                            int64_t* status, void* stream) {
+                            const int32_t* leaf_of, double softening, int with_potential,
+                            double* phi, double* field, int64_t* status, void* stream) {
     TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(n >= 1 && node_count >= 1, "sizes");
     auto go = kernel == TD3_LOG ? launch_eval<TD3_LOG, 0> : launch_eval<TD3_COULOMB, 0>;
     return go(precision, (hipStream_t)stream, sorted_pos, perm, nullptr, n, node_center, node_half,
               (const int4*)node_info, node_count, local_re, local_im, (int)order, p2p_row, p2p_col,
-              sorted_pos, sorted_charge, perm, n, (const int4*)node_info, softening * softening,
-              with_potential, phi, field, status);
+              sorted_pos, sorted_charge, perm, n, (const int4*)node_info, leaf_of,
+              softening * softening, with_potential, phi, field, status);
 }
 
 extern "C" int td3_fmm_at_eval(int kernel, int precision, int64_t order,
                                const double* sorted_targets, const int64_t* target_perm,
                                const int64_t* exclude, int64_t m, const double* t_center,
                                const double* t_half, const void* t_info, int64_t t_count,
-                               const double* local_re, const double* local_im,
-                               const uint32_t* p2p_row, const int32_t* p2p_col,
-                               const double* sorted_pos, const double* sorted_charge,
-                               const int64_t* perm, int64_t n, const void* s_info,
-                               double softening, int with_potential, double* phi, double* field,
-                               int64_t* status, void* stream) {
+                               const int32_t* t_leaf_of, const double* local_re,
+                               const double* local_im, const uint32_t* p2p_row,
+                               const int32_t* p2p_col, const double* sorted_pos,
+                               const double* sorted_charge, const int64_t* perm, int64_t n,
+                               const void* s_info, double softening, int with_potential,
+                               double* phi, double* field, int64_t* status, void* stream) {
     TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(m >= 1 && n >= 1 && t_count >= 1, "sizes");
     auto go = kernel == TD3_LOG ? launch_eval<TD3_LOG, 1> : launch_eval<TD3_COULOMB, 1>;
     return go(precision, (hipStream_t)stream, sorted_targets, target_perm, exclude, m, t_center,
               t_half, (const int4*)t_info, t_count, local_re, local_im, (int)order, p2p_row,
-              p2p_col, sorted_pos, sorted_charge, perm, n, (const int4*)s_info,
+              p2p_col, sorted_pos, sorted_charge, perm, n, (const int4*)s_info, t_leaf_of,
               softening * softening, with_potential, phi, field, status);
+}
+
+extern "C" int td3_leaf_index(const void* node_info, int64_t node_count, int32_t* leaf_of,
+                              void* stream) {
+    TD3_ARG(node_count >= 1 && node_count < 0x7fffffff && leaf_of != nullptr, "sizes");
+    hipLaunchKernelGGL(leaf_index_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const int4*)node_info, node_count, leaf_of);
+    TD3_LAUNCHED();
+    return TD3_OK;
 }

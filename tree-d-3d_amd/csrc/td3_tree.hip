@@ -22,6 +22,12 @@
 // 4. particles keep input order inside each leaf: a stable radix sort of the input
 //    indices by the start of their leaf gives `perm`.
 // One host read-back (the splitting-node count per depth) sizes the node arrays.
+//
+// A forest of S independent systems (an ensemble) is built by the same pass: a system's
+// index heads its particles' keys (sbits bits, so levels 1..l1 = (64 - sbits) / 2 sit in
+// `hi` after it), particles of different systems share no prefix, and the S roots are
+// nodes 0..S-1; below them the numbering is breadth first over all systems at once.
+// With S = 1 and sbits = 0 this is exactly the single tree.
 
 #include "td3_sort.h"
 
@@ -29,41 +35,73 @@ namespace {
 
 constexpr int kMaxLevels = 64;
 
-__device__ __forceinline__ int lcp_levels(u64 ahi, u64 alo, u64 bhi, u64 blo, int nkey) {
+// Key layout: the system index in the top sbits bits of hi, then levels 1..l1 (2 bits
+// each) in hi, levels l1+1.. in lo from its top.
+struct Keys {
+    int sbits, l1;
+};
+
+__device__ __forceinline__ int lcp_levels(u64 ahi, u64 alo, u64 bhi, u64 blo, int nkey, Keys kl) {
     u64 x = ahi ^ bhi;
+    if (kl.sbits && (x >> (64 - kl.sbits))) return -1;  // different systems: no common cell
+    x <<= kl.sbits;
     int l;
     if (x)
         l = __clzll(x) >> 1;
     else if ((x = alo ^ blo))
-        l = 32 + (__clzll(x) >> 1);
+        l = kl.l1 + (__clzll(x) >> 1);
     else
         l = 64;
     return l < nkey ? l : nkey;
 }
 
-__device__ __forceinline__ int digit_at(u64 hi, u64 lo, int level) {
-    return level <= 32 ? (int)((hi >> (64 - 2 * level)) & 3ull)
-                       : (int)((lo >> (64 - 2 * (level - 32))) & 3ull);
+__device__ __forceinline__ int digit_at(u64 hi, u64 lo, int level, Keys kl) {
+    return level <= kl.l1 ? (int)((hi >> (64 - kl.sbits - 2 * level)) & 3ull)
+                          : (int)((lo >> (64 - 2 * (level - kl.l1))) & 3ull);
 }
 
-__global__ void keys_kernel(const double* __restrict__ pos, i64 n, double cx0, double cy0,
-                            double h0, int nlev, u64* __restrict__ key_hi,
-                            u64* __restrict__ key_lo) {
+__device__ __forceinline__ int system_of(u64 hi, Keys kl) {
+    return kl.sbits ? (int)(hi >> (64 - kl.sbits)) : 0;
+}
+
+// Root (centre, half) of a system: roots (S, 3) when given, else the single root.
+struct Root {
+    const double* roots;
+    double cx, cy, h;
+};
+
+__device__ __forceinline__ void root_of(Root rt, int sys, double* cx, double* cy, double* h) {
+    if (rt.roots) {
+        *cx = rt.roots[3 * sys];
+        *cy = rt.roots[3 * sys + 1];
+        *h = rt.roots[3 * sys + 2];
+    } else {
+        *cx = rt.cx;
+        *cy = rt.cy;
+        *h = rt.h;
+    }
+}
+
+__global__ void keys_kernel(const double* __restrict__ pos, i64 n, Root rt,
+                            const int* __restrict__ psys, Keys kl, int nlev,
+                            u64* __restrict__ key_hi, u64* __restrict__ key_lo) {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double x = pos[i];
     const double y = pos[n + i];
-    double cx = cx0, cy = cy0, h = h0;
-    u64 hi = 0, lo = 0;
+    const int sys = psys ? psys[i] : 0;
+    double cx, cy, h;
+    root_of(rt, sys, &cx, &cy, &h);
+    u64 hi = kl.sbits ? (u64)sys << (64 - kl.sbits) : 0ull, lo = 0;
     for (int level = 1; level <= nlev; ++level) {
         const int q = (x > cx ? 1 : 0) + (y < cy ? 2 : 0);
         h = 0.5 * h;
         cx = (q & 1) ? cx + h : cx - h;
         cy = (q & 2) ? cy - h : cy + h;
-        if (level <= 32)
-            hi |= (u64)q << (64 - 2 * level);
+        if (level <= kl.l1)
+            hi |= (u64)q << (64 - kl.sbits - 2 * level);
         else
-            lo |= (u64)q << (64 - 2 * (level - 32));
+            lo |= (u64)q << (64 - 2 * (level - kl.l1));
     }
     key_hi[i] = hi;
     if (key_lo != nullptr) key_lo[i] = lo;
@@ -72,7 +110,7 @@ __global__ void keys_kernel(const double* __restrict__ pos, i64 n, double cx0, d
 // Splitting-node starts per sorted particle, and their count per depth.
 __global__ __launch_bounds__(256) void count_kernel(const u64* __restrict__ khi,
                                                     const u64* __restrict__ klo, i64 n, i64 cap,
-                                                    int max_split_depth, int nkey,
+                                                    int max_split_depth, int nkey, Keys kl,
                                                     u32* __restrict__ cnt,
                                                     u32* __restrict__ depth_count,
                                                     u32* __restrict__ flag) {
@@ -83,9 +121,9 @@ __global__ __launch_bounds__(256) void count_kernel(const u64* __restrict__ khi,
     if (i < n) {
         const u64 ahi = khi[i], alo = klo ? klo[i] : 0ull;
         int prev = -1;
-        if (i > 0) prev = lcp_levels(khi[i - 1], klo ? klo[i - 1] : 0ull, ahi, alo, nkey);
+        if (i > 0) prev = lcp_levels(khi[i - 1], klo ? klo[i - 1] : 0ull, ahi, alo, nkey, kl);
         int m = -1;
-        if (i + cap < n) m = lcp_levels(ahi, alo, khi[i + cap], klo ? klo[i + cap] : 0ull, nkey);
+        if (i + cap < n) m = lcp_levels(ahi, alo, khi[i + cap], klo ? klo[i + cap] : 0ull, nkey, kl);
         const int top = m < max_split_depth ? m : max_split_depth;
         const int c = top > prev ? top - prev : 0;
         cnt[i] = (u32)c;
@@ -101,14 +139,14 @@ __global__ __launch_bounds__(256) void count_kernel(const u64* __restrict__ khi,
 
 // One (depth, particle) entry per splitting node, in particle order.
 __global__ void entries_kernel(const u64* __restrict__ khi, const u64* __restrict__ klo, i64 n,
-                               int nkey, const u32* __restrict__ cnt, const u32* __restrict__ off,
+                               int nkey, Keys kl, const u32* __restrict__ cnt, const u32* __restrict__ off,
                                u64* __restrict__ ekey, u32* __restrict__ eval) {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const u32 c = cnt[i];
     if (!c) return;
     int prev = -1;
-    if (i > 0) prev = lcp_levels(khi[i - 1], klo ? klo[i - 1] : 0ull, khi[i], klo ? klo[i] : 0ull, nkey);
+    if (i > 0) prev = lcp_levels(khi[i - 1], klo ? klo[i - 1] : 0ull, khi[i], klo ? klo[i] : 0ull, nkey, kl);
     const u32 o = off[i];
     for (u32 j = 0; j < c; ++j) {
         ekey[o + j] = (u64)(prev + 1 + (int)j);
@@ -117,17 +155,13 @@ __global__ void entries_kernel(const u64* __restrict__ khi, const u64* __restric
 }
 
 // Level offsets (int64, reference layout) and per-depth bucket starts from the counts of
-// splitting nodes per depth; also writes the root node.
-__global__ void levels_kernel(const u32* __restrict__ depth_count, i64 n, i64 cap, int max_depth,
-                              double cx0, double cy0, double h0, i64* __restrict__ levels,
-                              u32* __restrict__ bstart, double* __restrict__ node_center,
-                              double* __restrict__ node_half, i64* __restrict__ node_start,
-                              i64* __restrict__ node_end, i64* __restrict__ node_child,
-                              i64* __restrict__ node_depth, int4* __restrict__ node_info) {
+// splitting nodes per depth; the S roots are nodes 0..S-1 (depth 0).
+__global__ void levels_kernel(const u32* __restrict__ depth_count, i64 roots_n,
+                              i64* __restrict__ levels, u32* __restrict__ bstart) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     levels[0] = 0;
-    levels[1] = 1;
-    i64 lv = 1;
+    levels[1] = roots_n;
+    i64 lv = roots_n;
     u32 b = 0;
     for (int d = 0; d <= kMaxLevels; ++d) {
         bstart[d] = b;
@@ -139,20 +173,51 @@ __global__ void levels_kernel(const u32* __restrict__ depth_count, i64 n, i64 ca
         }
     }
     bstart[kMaxLevels + 1] = b;
-    const bool split = n > cap && max_depth > 0;
-    node_center[0] = cx0;
-    node_center[1] = cy0;
-    node_half[0] = h0;
-    node_start[0] = 0;
-    node_end[0] = n;
-    node_child[0] = split ? 1 : -1;
-    node_depth[0] = 0;
-    if (node_info) node_info[0] = make_int4(0, (int)n, split ? 1 : -1, 0);
+}
+
+// The root of every system: its particle range [sys_start[s], sys_start[s + 1]) (the whole
+// set when sys_start is null) and, when it splits, its children from the rank of its
+// depth-0 entry.
+__global__ void roots_kernel(i64 roots_n, const i64* __restrict__ sys_start, i64 n, i64 cap,
+                             int max_depth, Root rt, const u32* __restrict__ epos,
+                             const u32* __restrict__ depth_count, const i64* __restrict__ levels,
+                             double* __restrict__ node_center, double* __restrict__ node_half,
+                             i64* __restrict__ node_start, i64* __restrict__ node_end,
+                             i64* __restrict__ node_child, i64* __restrict__ node_depth,
+                             int4* __restrict__ node_info) {
+    const i64 sys = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sys >= roots_n) return;
+    const i64 start = sys_start ? sys_start[sys] : 0;
+    const i64 end = sys_start ? sys_start[sys + 1] : n;
+    const bool split = end - start > cap && max_depth > 0;
+    i64 child = -1;
+    if (split) {
+        // depth-0 entries are the first bucket, in particle order
+        i64 a = 0, z = depth_count[0];
+        while (a < z) {
+            const i64 mid = (a + z) >> 1;
+            if ((i64)epos[mid] < start)
+                a = mid + 1;
+            else
+                z = mid;
+        }
+        child = levels[1] + 4 * a;
+    }
+    double cx, cy, h;
+    root_of(rt, (int)sys, &cx, &cy, &h);
+    node_center[2 * sys] = cx;
+    node_center[2 * sys + 1] = cy;
+    node_half[sys] = h;
+    node_start[sys] = start;
+    node_end[sys] = end;
+    node_child[sys] = child;
+    node_depth[sys] = 0;
+    if (node_info) node_info[sys] = make_int4((int)start, (int)end, (int)child, 0);
 }
 
 // One thread per splitting node: writes its four children.
 __global__ void nodes_kernel(const u64* __restrict__ khi, const u64* __restrict__ klo, i64 n,
-                             i64 cap, int max_depth, int nkey, double cx0, double cy0, double h0,
+                             i64 cap, int max_depth, int nkey, Keys kl, Root rt,
                              const u64* __restrict__ edepth, const u32* __restrict__ epos, i64 s_total,
                              const u32* __restrict__ bstart, const u32* __restrict__ depth_count,
                              const i64* __restrict__ levels, double* __restrict__ node_center,
@@ -166,9 +231,10 @@ __global__ void nodes_kernel(const u64* __restrict__ khi, const u64* __restrict_
     const i64 r = g - (i64)bstart[d];
     const u64 ihi = khi[i], ilo = klo ? klo[i] : 0ull;
     // Centre and half-width of the depth-d cell of particle i: the exact descent.
-    double cx = cx0, cy = cy0, h = h0;
+    double cx, cy, h;
+    root_of(rt, system_of(ihi, kl), &cx, &cy, &h);
     for (int level = 1; level <= d; ++level) {
-        const int q = digit_at(ihi, ilo, level);
+        const int q = digit_at(ihi, ilo, level, kl);
         h = 0.5 * h;
         cx = (q & 1) ? cx + h : cx - h;
         cy = (q & 2) ? cy - h : cy + h;
@@ -177,7 +243,7 @@ __global__ void nodes_kernel(const u64* __restrict__ khi, const u64* __restrict_
     i64 lo_j = i + 1, hi_j = n;
     while (lo_j < hi_j) {
         const i64 mid = (lo_j + hi_j) >> 1;
-        if (lcp_levels(ihi, ilo, khi[mid], klo ? klo[mid] : 0ull, nkey) >= d)
+        if (lcp_levels(ihi, ilo, khi[mid], klo ? klo[mid] : 0ull, nkey, kl) >= d)
             lo_j = mid + 1;
         else
             hi_j = mid;
@@ -190,7 +256,7 @@ __global__ void nodes_kernel(const u64* __restrict__ khi, const u64* __restrict_
         i64 a = bound[q - 1], z = e;
         while (a < z) {
             const i64 mid = (a + z) >> 1;
-            if (digit_at(khi[mid], klo ? klo[mid] : 0ull, d + 1) < q)
+            if (digit_at(khi[mid], klo ? klo[mid] : 0ull, d + 1, kl) < q)
                 a = mid + 1;
             else
                 z = mid;
@@ -294,25 +360,33 @@ extern "C" size_t td3_tree_workspace_bytes(int64_t n, int64_t capacity) {
     return carve(&w, nullptr, n < 1 ? 1 : n, capacity < 1 ? 1 : capacity);
 }
 
-extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double ymid, double half,
-                              int64_t leaf_capacity, int64_t max_depth, int64_t capacity,
-                              double* node_center, double* node_half, int64_t* node_start,
-                              int64_t* node_end, int64_t* node_child, int64_t* node_depth,
-                              int64_t* levels, int64_t* perm, double* sorted_pos, void* node_info,
-                              void* workspace, size_t workspace_bytes, int64_t* out_sizes,
-                              void* stream) {
+namespace {
+
+// The build of S >= 1 systems: roots (S, 3) on the device (or the single root), psys the
+// system of every input particle (non-decreasing; null for one system), sys_start (S + 1)
+// the systems' particle offsets on the device (null for one system).
+int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* psys,
+               const i64* sys_start, int64_t leaf_capacity, int64_t max_depth, int64_t capacity,
+               double* node_center, double* node_half, int64_t* node_start, int64_t* node_end,
+               int64_t* node_child, int64_t* node_depth, int64_t* levels, int64_t* perm,
+               double* sorted_pos, void* node_info, void* workspace, size_t workspace_bytes,
+               int64_t* out_sizes, hipStream_t s) {
     TD3_ARG(n > 0 && n < (int64_t)0x7fffffff, "n must be in [1, 2^31)");
     TD3_ARG(leaf_capacity >= 1 && max_depth >= 1, "leaf_capacity and max_depth must be positive");
-    TD3_ARG(capacity >= 1, "capacity");
+    TD3_ARG(capacity >= roots_n, "capacity");
     TD3_ARG(out_sizes != nullptr, "out_sizes");
     TD3_ARG(levels != nullptr, "levels (int64[67])");
     TreeWs w;
     const size_t need = carve(&w, nullptr, n, capacity);
     TD3_ARG(workspace != nullptr && workspace_bytes >= need, "workspace too small");
     carve(&w, workspace, n, capacity);
-    hipStream_t s = (hipStream_t)stream;
-    const int nlev = (int)(max_depth < kMaxLevels ? max_depth : kMaxLevels);
-    const int max_split = (int)(max_depth - 1 < kMaxLevels ? max_depth - 1 : kMaxLevels);
+    Keys kl;
+    kl.sbits = roots_n > 1 ? bits_for(roots_n - 1) : 0;
+    kl.l1 = (64 - kl.sbits) / 2;
+    const int max_levels = kl.l1 + 32 < kMaxLevels ? kl.l1 + 32 : kMaxLevels;
+    TD3_ARG(kl.sbits <= 24, "at most 2^24 systems");
+    const int nlev = (int)(max_depth < max_levels ? max_depth : max_levels);
+    const int max_split = (int)(max_depth - 1 < max_levels ? max_depth - 1 : max_levels);
     u32* depth_count = w.small;          // [65]
     u32* flag = w.small + 80;            // [1]
     u32* bstart = w.small + 96;          // [66]
@@ -320,10 +394,10 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     const unsigned nb = td3_blocks(n, 256);
 
     // 1-2. keys and sort by the first 16 levels; when a cell that deep may still split, by
-    // the first 32, then by all of them.  Ties keep input order (the sort is stable), and
-    // the topology depends only on prefixes no longer than the deepest split.
+    // all levels held in hi, then by all of them.  Ties keep input order (the sort is
+    // stable), and the topology depends only on prefixes no longer than the deepest split.
     int stages[3], nstage = 0;
-    for (int k : {16, 32, nlev}) {
+    for (int k : {16, kl.l1, nlev}) {
         const int key = k < nlev ? k : nlev;
         if (nstage == 0 || key > stages[nstage - 1]) stages[nstage++] = key;
     }
@@ -332,20 +406,20 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     u32 host_small[81];
     for (int stage = 0; stage < nstage; ++stage) {
         nkey = stages[stage];
-        deep = nkey > 32;
+        deep = nkey > kl.l1;
         if (!deep) {
-            TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
+            TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, rt, psys, kl, nkey,
                                w.khi, (u64*)nullptr));
             TD3_LAUNCHED();
             if (int rc = td3sort::sort_pairs(w.khi, w.idx, nullptr, w.khi_alt, w.idx_alt, nullptr, n,
-                                             64 - 2 * nkey, 64, true, w.scratch, s))
+                                             64 - kl.sbits - 2 * nkey, 64, true, w.scratch, s))
                 return rc;
         } else {
-            TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, xmid, ymid, half, nkey,
+            TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, rt, psys, kl, nkey,
                                w.khi, w.klo));
             TD3_LAUNCHED();
             if (int rc = td3sort::sort_pairs(w.klo, w.idx, w.khi, w.klo_alt, w.idx_alt, w.khi_alt, n,
-                                             64 - 2 * (nkey - 32), 64, true, w.scratch, s))
+                                             64 - 2 * (nkey - kl.l1), 64, true, w.scratch, s))
                 return rc;
             if (int rc = td3sort::sort_pairs(w.khi, w.idx, w.klo, w.khi_alt, w.idx_alt, w.klo_alt, n,
                                              0, 64, false, w.scratch, s))
@@ -354,19 +428,19 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
         // 3. splitting-node starts
         TD3_HIP(hipMemsetAsync(w.small, 0, sizeof(u32) * 96, s));
         TD3_TIMED(s, "tree_count", hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
-                           (i64)leaf_capacity, max_split, nkey, w.cnt, depth_count, flag));
+                           (i64)leaf_capacity, max_split, nkey, kl, w.cnt, depth_count, flag));
         TD3_LAUNCHED();
         TD3_HIP(hipMemcpyAsync(host_small, w.small, sizeof(u32) * 81, hipMemcpyDeviceToHost, s));
         TD3_HIP(hipStreamSynchronize(s));
         if (!host_small[80]) break;
         if (stage == nstage - 1) {
-            td3_set_error("the tree needs more than %d levels (max_depth %lld)", kMaxLevels,
+            td3_set_error("the tree needs more than %d levels (max_depth %lld)", max_levels,
                           (long long)max_depth);
             return TD3_ERR_ARG;
         }
     }
     // node count and level count
-    i64 count = 1, s_sum = 0;
+    i64 count = roots_n, s_sum = 0;
     int n_levels = 1;
     for (int d = 0; d <= kMaxLevels; ++d) {
         if (host_small[d]) {
@@ -387,20 +461,23 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     if (s_sum > 0) {
         // offsets into a separate array: entries_kernel needs the counts and the offsets
         if (int rc = td3sort::exclusive_scan_u32(w.cnt, w.off, n, s_total, w.scratch, s)) return rc;
-        hipLaunchKernelGGL(entries_kernel, dim3(nb), dim3(256), 0, s, w.khi, klo, n, nkey, w.cnt,
+        hipLaunchKernelGGL(entries_kernel, dim3(nb), dim3(256), 0, s, w.khi, klo, n, nkey, kl, w.cnt,
                            w.off, w.ekey, w.eval);
         TD3_LAUNCHED();
         if (int rc = td3sort::sort_pairs(w.ekey, w.eval, nullptr, w.ekey_alt, w.eval_alt, nullptr,
                                          s_sum, 0, 8, false, w.scratch, s))
             return rc;
     }
-    hipLaunchKernelGGL(levels_kernel, dim3(1), dim3(64), 0, s, depth_count, n, (i64)leaf_capacity,
-                       (int)max_depth, xmid, ymid, half, levels, bstart, node_center, node_half,
-                       node_start, node_end, node_child, node_depth, (int4*)node_info);
+    hipLaunchKernelGGL(levels_kernel, dim3(1), dim3(64), 0, s, depth_count, roots_n, levels, bstart);
+    TD3_LAUNCHED();
+    hipLaunchKernelGGL(roots_kernel, dim3(td3_blocks(roots_n, 256)), dim3(256), 0, s, roots_n,
+                       sys_start, (i64)n, (i64)leaf_capacity, (int)max_depth, rt, w.eval,
+                       depth_count, levels, node_center, node_half, node_start, node_end,
+                       node_child, node_depth, (int4*)node_info);
     TD3_LAUNCHED();
     if (s_sum > 0) {
         TD3_TIMED(s, "tree_nodes", hipLaunchKernelGGL(nodes_kernel, dim3(td3_blocks(s_sum, 256)), dim3(256), 0, s, w.khi, klo, n,
-                           (i64)leaf_capacity, (int)max_depth, nkey, xmid, ymid, half, w.ekey, w.eval,
+                           (i64)leaf_capacity, (int)max_depth, nkey, kl, rt, w.ekey, w.eval,
                            s_sum, bstart, depth_count, levels, node_center, node_half, node_start,
                            node_end, node_child, node_depth, (int4*)node_info));
         TD3_LAUNCHED();
@@ -415,4 +492,44 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     TD3_TIMED(s, "tree_gather", hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, w.idx_alt, pos, n, perm, sorted_pos));
     TD3_LAUNCHED();
     return TD3_OK;
+}
+
+}  // namespace
+
+extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double ymid, double half,
+                              int64_t leaf_capacity, int64_t max_depth, int64_t capacity,
+                              double* node_center, double* node_half, int64_t* node_start,
+                              int64_t* node_end, int64_t* node_child, int64_t* node_depth,
+                              int64_t* levels, int64_t* perm, double* sorted_pos, void* node_info,
+                              void* workspace, size_t workspace_bytes, int64_t* out_sizes,
+                              void* stream) {
+    Root rt;
+    rt.roots = nullptr;
+    rt.cx = xmid;
+    rt.cy = ymid;
+    rt.h = half;
+    return build_impl(pos, n, 1, rt, nullptr, nullptr, leaf_capacity, max_depth, capacity,
+                      node_center, node_half, node_start, node_end, node_child, node_depth, levels,
+                      perm, sorted_pos, node_info, workspace, workspace_bytes, out_sizes,
+                      (hipStream_t)stream);
+}
+
+extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
+                                const int32_t* particle_system, const int64_t* system_start,
+                                const double* roots, int64_t leaf_capacity, int64_t max_depth,
+                                int64_t capacity, double* node_center, double* node_half,
+                                int64_t* node_start, int64_t* node_end, int64_t* node_child,
+                                int64_t* node_depth, int64_t* levels, int64_t* perm,
+                                double* sorted_pos, void* node_info, void* workspace,
+                                size_t workspace_bytes, int64_t* out_sizes, void* stream) {
+    TD3_ARG(systems >= 1 && systems <= (1 << 24), "systems");
+    TD3_ARG(particle_system != nullptr && system_start != nullptr && roots != nullptr,
+            "particle_system, system_start and roots");
+    Root rt;
+    rt.roots = roots;
+    rt.cx = rt.cy = rt.h = 0.0;
+    return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
+                      capacity, node_center, node_half, node_start, node_end, node_child,
+                      node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
+                      out_sizes, (hipStream_t)stream);
 }

@@ -39,9 +39,9 @@ struct WalkTree {
 template <int KERNEL, int PREC>
 __global__ __launch_bounds__(64 * kWaves) void walk_kernel(
     WalkTree tr, const double* __restrict__ targets, i64 m, const i64* __restrict__ order,
-    const i64* __restrict__ exclude, double theta, double eps2, int with_potential,
-    int stack_size, double* __restrict__ phi, double* __restrict__ field,
-    i64* __restrict__ status) {
+    const i64* __restrict__ exclude, const int* __restrict__ target_root, double theta,
+    double eps2, int with_potential, int stack_size, double* __restrict__ phi,
+    double* __restrict__ field, i64* __restrict__ status) {
     __shared__ int s_node[kWaves][kMaxStack];
     __shared__ u64 s_mask[kWaves][kMaxStack];
     const int wave = threadIdx.x / TD3_WAVE;
@@ -59,12 +59,31 @@ __global__ __launch_bounds__(64 * kWaves) void walk_kernel(
     bool bad = false;
     int* stn = s_node[wave];
     u64* stm = s_mask[wave];
-    if (lane == 0) {
-        stn[0] = 0;
-        stm[0] = full;
-    }
-    int top = 1;
+    // one stack entry per root among the wave's targets (node 0 for a single tree; the
+    // target's system root in a forest), with the mask of the lanes it serves
+    int top = 0;
     bool overflow = false;
+    {
+        const int my_root = active ? (target_root ? target_root[ti] : 0) : -1;
+        u64 left = full;
+        while (left) {
+            const int lead = __ffsll((unsigned long long)left) - 1;
+            const int r = __shfl(my_root, lead, 64);
+            const u64 mk = __ballot(active && my_root == r);
+            if (top + 4 > stack_size) {
+                overflow = true;
+                break;
+            }
+            if (lane == 0) {
+                stn[top] = r;
+                stm[top] = mk;
+            }
+            ++top;
+            left &= ~mk;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (overflow) top = 0;
     while (top > 0) {
         --top;
         __builtin_amdgcn_wave_barrier();
@@ -244,10 +263,12 @@ __global__ void widen_kernel(const u32* __restrict__ v, i64 n, i64* __restrict__
 
 template <int KERNEL, int PREC>
 void launch_walk(unsigned blocks, hipStream_t s, const WalkTree& tr, const double* targets, i64 m,
-                 const i64* order, const i64* exclude, double theta, double eps2, int with_potential,
-                 int stack_size, double* phi, double* field, i64* status) {
+                 const i64* order, const i64* exclude, const int* target_root, double theta,
+                 double eps2, int with_potential, int stack_size, double* phi, double* field,
+                 i64* status) {
     TD3_TIMED(s, "walk", hipLaunchKernelGGL((walk_kernel<KERNEL, PREC>), dim3(blocks), dim3(64 * kWaves), 0, s, tr, targets,
-                       m, order, exclude, theta, eps2, with_potential, stack_size, phi, field, status));
+                       m, order, exclude, target_root, theta, eps2, with_potential, stack_size, phi,
+                       field, status));
 }
 
 }  // namespace
@@ -290,9 +311,9 @@ extern "C" int td3_walk(int kernel, int precision, const double* targets, int64_
                         const double* node_center, const double* node_half, const void* node_info,
                         const double* node_qabs, const double* node_q, const double* node_p,
                         const double* node_s, const double* moment_re, const double* moment_im,
-                        int64_t order_p, int64_t tree_depth, double theta, double softening,
-                        int with_potential, double* phi, double* field, int64_t* status,
-                        void* stream) {
+                        int64_t order_p, int64_t tree_depth, const int32_t* target_root,
+                        double theta, double softening, int with_potential, double* phi,
+                        double* field, int64_t* status, void* stream) {
     TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(precision == TD3_FP64 || precision == TD3_MIXED, "precision");
     TD3_ARG(m >= 0 && n >= 1, "sizes");
@@ -313,23 +334,26 @@ extern "C" int td3_walk(int kernel, int precision, const double* targets, int64_
     tr.mre = moment_re;
     tr.mim = moment_im;
     tr.p1 = (int)order_p + 1;
-    const int stack_size = (int)(3 * tree_depth + 8) < kMaxStack ? (int)(3 * tree_depth + 8) : kMaxStack;
+    // a forest's waves may start from several roots: give them the whole stack
+    const int stack_size = target_root != nullptr ? kMaxStack
+                           : (int)(3 * tree_depth + 8) < kMaxStack ? (int)(3 * tree_depth + 8)
+                                                                   : kMaxStack;
     const double eps2 = softening * softening;
     const unsigned blocks = td3_blocks(m, 64 * kWaves);
     hipStream_t s = (hipStream_t)stream;
     if (kernel == TD3_LOG) {
         if (precision == TD3_FP64)
-            launch_walk<TD3_LOG, TD3_FP64>(blocks, s, tr, targets, m, order, exclude, theta, eps2,
+            launch_walk<TD3_LOG, TD3_FP64>(blocks, s, tr, targets, m, order, exclude, target_root, theta, eps2,
                                            with_potential, stack_size, phi, field, status);
         else
-            launch_walk<TD3_LOG, TD3_MIXED>(blocks, s, tr, targets, m, order, exclude, theta, eps2,
+            launch_walk<TD3_LOG, TD3_MIXED>(blocks, s, tr, targets, m, order, exclude, target_root, theta, eps2,
                                             with_potential, stack_size, phi, field, status);
     } else {
         if (precision == TD3_FP64)
-            launch_walk<TD3_COULOMB, TD3_FP64>(blocks, s, tr, targets, m, order, exclude, theta, eps2,
+            launch_walk<TD3_COULOMB, TD3_FP64>(blocks, s, tr, targets, m, order, exclude, target_root, theta, eps2,
                                                with_potential, stack_size, phi, field, status);
         else
-            launch_walk<TD3_COULOMB, TD3_MIXED>(blocks, s, tr, targets, m, order, exclude, theta,
+            launch_walk<TD3_COULOMB, TD3_MIXED>(blocks, s, tr, targets, m, order, exclude, target_root, theta,
                                                 eps2, with_potential, stack_size, phi, field, status);
     }
     TD3_LAUNCHED();

@@ -1,0 +1,253 @@
+"""Many independent electrostatic systems stepped together on one GPU.
+
+The reference steps one `ElectrostaticSimulation` at a time (simulation.py:800-835), each
+with its own tree (simulation.py:641-667).  An `Ensemble` holds S systems of uneven size
+in one set of device arrays (the systems one after another) and takes every step for all
+of them at once:
+
+* one forest build (`td3_forest_build`): every system's tree, identical to the tree the
+  system would get alone, in one sort and one topology pass;
+* one upward pass for the moments of all trees;
+* one Barnes-Hut walk launch over every particle of every system, each wavefront
+  starting from the roots of its targets' systems (`td3_walk` with ``target_root``);
+* one push launch (`td3_push`) for every particle.
+
+Each system keeps the reference's semantics: its root grows to hold its particles
+(simulation.py:625-639), its fields are cached between the end of one step and the
+start of the next (one evaluation per step), fixed particles act as sources only, and
+its energy is ``K + 1/2 sum q phi`` (simulation.py:837-916).  The Coulomb kernel and the
+walk (``method="tree"``) are what an ensemble runs; the systems share ``theta``,
+``softening``, ``k_e``, ``dt``, ``B_z``, ``leaf_capacity`` and ``max_depth``.
+"""
+
+from __future__ import annotations
+
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._utils import _check, _finite, _host
+from .flat_tree import COULOMB_ORDER, FlatQuadtree
+from .simulation import ElectrostaticSimulation
+
+
+class _Forest(FlatQuadtree):
+    """The Coulomb trees of all systems as one node set (node s is the root of system s)."""
+
+    def __init__(self, positions: torch.Tensor, charges: torch.Tensor, system: torch.Tensor,
+                 system_start: torch.Tensor, roots: torch.Tensor, leaf_capacity: int,
+                 max_depth: int):
+        self.kernel, self.order, self.symmetric, self.comm = "coulomb", COULOMB_ORDER, False, None
+        self.precision = "fp64"
+        self._like_torch = True
+        self._host_cache: dict = {}
+        self._d = {"pos": positions, "charge": charges}
+        self.leaf_capacity, self.max_depth = int(leaf_capacity), int(max_depth)
+        self._traversals: dict = {}
+        self.fmm_pairs = (0, 0)
+        self._system = system.to(torch.int32).contiguous()
+        self._system_start = system_start.to(torch.int64).contiguous()
+        self._roots = roots.to(torch.float64).contiguous()
+        self._root = (0.0, 0.0, 1.0)
+        self._build()
+        self._sorted_charge()
+        self._moments()
+
+    def _build(self) -> None:
+        pos = self._d["pos"]
+        dev, n = pos.device, pos.shape[1]
+        systems = self._roots.shape[0]
+        capacity = 4 * (2 * n // self.leaf_capacity + 16 * systems)
+        out = np.zeros(3, dtype=np.int64)
+        while True:
+            f64 = dict(dtype=torch.float64, device=dev)
+            i64 = dict(dtype=torch.int64, device=dev)
+            center = torch.empty((capacity, 2), **f64)
+            half = torch.empty(capacity, **f64)
+            start = torch.empty(capacity, **i64)
+            end = torch.empty(capacity, **i64)
+            child = torch.empty(capacity, **i64)
+            depth = torch.empty(capacity, **i64)
+            levels = torch.zeros(67, **i64)
+            perm = torch.empty(n, **i64)
+            sorted_pos = torch.empty((2, n), **f64)
+            info = torch.empty((capacity, 4), dtype=torch.int32, device=dev)
+            nbytes = _abi.lib().td3_tree_workspace_bytes(n, capacity)
+            ws = _abi.workspace(nbytes)
+            code = _abi.lib().td3_forest_build(
+                _abi.ptr(pos), n, systems, _abi.ptr(self._system), _abi.ptr(self._system_start),
+                _abi.ptr(self._roots), self.leaf_capacity, self.max_depth, capacity,
+                _abi.ptr(center), _abi.ptr(half), _abi.ptr(start), _abi.ptr(end),
+                _abi.ptr(child), _abi.ptr(depth), _abi.ptr(levels), _abi.ptr(perm),
+                _abi.ptr(sorted_pos), _abi.ptr(info), _abi.ptr(ws), nbytes, out.ctypes.data,
+                _abi.stream(),
+            )
+            if code == _abi.TD3_ERR_CAPACITY:
+                capacity = max(2 * capacity, int(out[2]))
+                continue
+            _abi.check(code, "td3_forest_build")
+            break
+        count, n_levels = int(out[0]), int(out[1])
+        self.node_count = count
+        self.levels = levels[: n_levels + 1].cpu().numpy()
+        self._d.update(perm=perm, node_center=center[:count], node_half=half[:count],
+                       node_start=start[:count], node_end=end[:count], node_child=child[:count],
+                       node_depth=depth[:count])
+        self._sorted_pos = sorted_pos
+        self._node_info = info[:count]
+        self._depth = n_levels - 1
+
+    def walk(self, theta: float, softening: float, potential: bool = True):
+        """Potential and field (raw, without k_e) at every particle of every system from
+        its own system's particles, its own charge excluded: one walk launch."""
+        pos, d = self._d["pos"], self._d
+        n = pos.shape[1]
+        phi = torch.zeros(n, dtype=torch.float64, device=pos.device)
+        field = torch.zeros((2, n), dtype=torch.float64, device=pos.device)
+        status = torch.zeros(1, dtype=torch.int64, device=pos.device)
+        everyone = torch.arange(n, device=pos.device)
+        _abi.call(
+            "td3_walk", _abi.COULOMB, _abi.FP64, _abi.ptr(pos), n, _abi.ptr(d["perm"]),
+            _abi.ptr(everyone), _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
+            _abi.ptr(d["perm"]), n, _abi.ptr(d["node_center"]), _abi.ptr(d["node_half"]),
+            _abi.ptr(self._node_info), _abi.ptr(d["node_qabs"]), _abi.ptr(d["node_q"]),
+            _abi.ptr(d["node_p"]), _abi.ptr(d["node_s"]), _abi.ptr(d["moment_re"]),
+            _abi.ptr(d["moment_im"]), self.order, self._depth, _abi.ptr(self._system),
+            float(theta), float(softening), 1 if potential else 0, _abi.ptr(phi),
+            _abi.ptr(field), _abi.ptr(status), _abi.stream(),
+        )
+        _check(status)
+        return phi, field
+
+
+class Ensemble:
+    """S independent Coulomb systems (``ElectrostaticSimulation`` semantics with
+    ``method="tree"``) advanced together; see the module docstring."""
+
+    def __init__(self, simulations: Sequence[ElectrostaticSimulation]):
+        sims = list(simulations)
+        if not sims:
+            raise ValueError("an ensemble needs at least one system")
+        first = sims[0]
+        shared = ("theta", "softening", "k_e", "dt", "B_z", "max_depth", "kernel")
+        for sim in sims:
+            for name in shared:
+                if getattr(sim, name) != getattr(first, name):
+                    raise ValueError(f"every system of an ensemble must share {name}")
+            if sim.kernel != "coulomb":
+                raise ValueError("an ensemble runs the Coulomb kernel")
+            if sim.method not in ("auto", "tree"):
+                raise ValueError("an ensemble runs the Barnes-Hut walk (method 'tree')")
+        leaf = first.leaf_capacity if first.leaf_capacity is not None else 8
+        if any((s.leaf_capacity if s.leaf_capacity is not None else 8) != leaf for s in sims):
+            raise ValueError("every system of an ensemble must share leaf_capacity")
+        self.theta, self.softening = float(first.theta), float(first.softening)
+        self.k_e, self.dt, self.B_z = float(first.k_e), float(first.dt), float(first.B_z)
+        self.leaf_capacity, self.max_depth = int(leaf), int(first.max_depth)
+        self.current_time, self.iteration = 0.0, 0
+        sizes = [s.num_particles for s in sims]
+        if min(sizes) < 1:
+            raise ValueError("every system needs at least one particle")
+        self.sizes = np.array(sizes, dtype=np.int64)
+        self.systems = len(sims)
+        dev = first.positions.device
+        self.positions = torch.cat([s.positions for s in sims], dim=1).contiguous()
+        self.velocities = torch.cat([s.velocities for s in sims], dim=1).contiguous()
+        self.forces = torch.zeros_like(self.positions)
+        self.masses = torch.cat([s.masses for s in sims]).contiguous()
+        self.charges = torch.cat([s.charges for s in sims]).contiguous()
+        self.fixed = torch.cat([s.fixed for s in sims]).contiguous()
+        self._fixed_u8 = self.fixed.to(torch.uint8)
+        offsets = np.concatenate(([0], np.cumsum(self.sizes)))
+        self.system_start = torch.tensor(offsets, dtype=torch.int64, device=dev)
+        self.system = torch.repeat_interleave(
+            torch.arange(self.systems, dtype=torch.int32, device=dev),
+            torch.tensor(self.sizes, device=dev))
+        self._index = self.system.to(torch.int64)
+        self.roots = torch.tensor([[s.bounds.xmid, s.bounds.ymid, s.bounds.half] for s in sims],
+                                  dtype=torch.float64, device=dev)
+        self._electric = torch.zeros_like(self.positions)
+        self._potential = torch.zeros_like(self.charges)
+        self._current = False
+        self.forest = None
+
+    @property
+    def num_particles(self) -> int:
+        return int(self.charges.shape[0])
+
+    # ------------------------------------------------------------------ fields
+    def _expand_roots(self) -> None:
+        """simulation.py:625-639 for every system at once: a root that no longer holds its
+        particles becomes the square around them and the old root."""
+        p, s = self.positions, self._index
+        if not bool(torch.isfinite(p).all()):
+            raise ValueError("Particle positions must be finite; reduce dt")
+        big = torch.full((2, self.systems), np.inf, dtype=torch.float64, device=p.device)
+        lo = big.scatter_reduce(1, s.expand(2, -1), p, "amin", include_self=True)
+        hi = (-big).scatter_reduce(1, s.expand(2, -1), p, "amax", include_self=True)
+        r = self.roots
+        rlo = torch.stack((r[:, 0] - r[:, 2], r[:, 1] - r[:, 2]))
+        rhi = torch.stack((r[:, 0] + r[:, 2], r[:, 1] + r[:, 2]))
+        grow = ((lo < rlo) | (hi > rhi)).any(dim=0)
+        lo, hi = torch.minimum(lo, rlo), torch.maximum(hi, rhi)
+        center = lo / 2 + hi / 2
+        half = (hi / 2 - lo / 2).max(dim=0).values * (1 + 1e-12)
+        new = torch.stack((center[0], center[1], half), dim=1)
+        self.roots = torch.where(grow[:, None], new, r).contiguous()
+
+    def compute_fields(self):
+        """Force and potential on every particle of every system (one forest build, one
+        walk), cached until the positions change.  Returns the forest."""
+        if self._current:
+            return self.forest
+        self._expand_roots()
+        self.forest = _Forest(self.positions, self.charges, self.system, self.system_start,
+                              self.roots, self.leaf_capacity, self.max_depth)
+        phi, field = self.forest.walk(self.theta, self.softening)
+        mobile = ~self.fixed
+        self._potential = self.k_e * phi
+        self._electric = torch.where(mobile[None, :], self.k_e * self.charges[None, :] * field,
+                                     torch.zeros_like(field))
+        self._current = True
+        return self.forest
+
+    # ------------------------------------------------------------------ dynamics
+    def _push(self, first_half: bool) -> None:
+        _abi.call("td3_push", 1 if first_half else 0, self.num_particles,
+                  _abi.ptr(self.positions), _abi.ptr(self.velocities), _abi.ptr(self.forces),
+                  _abi.ptr(self._electric), _abi.ptr(self.masses), _abi.ptr(self.charges),
+                  _abi.ptr(self._fixed_u8), float(self.dt), float(self.B_z), _abi.stream())
+
+    def step(self, dt=None):
+        """simulation.py:800-835 for every system: half-kick, exact cyclotron drift, fields
+        at the new positions, half-kick."""
+        self.dt = _finite("dt", self.dt if dt is None else dt, positive=True)
+        self.compute_fields()
+        self._push(first_half=True)
+        self._current = False
+        forest = self.compute_fields()
+        self._push(first_half=False)
+        self.current_time += self.dt
+        self.iteration += 1
+        return forest
+
+    # ------------------------------------------------------------------ diagnostics
+    def _per_system(self, values: torch.Tensor) -> np.ndarray:
+        out = torch.zeros(self.systems, dtype=torch.float64, device=values.device)
+        return _host(out.index_add_(0, self._index, values))
+
+    def kinetic_energies(self) -> np.ndarray:
+        return self._per_system(0.5 * self.masses * (self.velocities ** 2).sum(dim=0))
+
+    def potential_energies(self) -> np.ndarray:
+        """1/2 sum q phi per system (simulation.py:853-916, from the walk's potential)."""
+        self.compute_fields()
+        return self._per_system(0.5 * self.charges * self._potential)
+
+    def total_energies(self) -> np.ndarray:
+        return self.kinetic_energies() + self.potential_energies()
+
+    def system_slice(self, s: int) -> slice:
+        return slice(int(self.system_start[s]), int(self.system_start[s + 1]))

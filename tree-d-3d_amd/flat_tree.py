@@ -468,7 +468,7 @@ class FlatQuadtree:
                 _abi.ptr(d["perm"]), n, _abi.ptr(d["node_center"]), _abi.ptr(d["node_half"]),
                 _abi.ptr(self._node_info), _abi.ptr(d["node_qabs"]), _abi.ptr(d["node_q"]),
                 _abi.ptr(d["node_p"]), _abi.ptr(d["node_s"]), _abi.ptr(d["moment_re"]),
-                _abi.ptr(d["moment_im"]), self.order, self._depth, float(theta),
+                _abi.ptr(d["moment_im"]), self.order, self._depth, None, float(theta),
                 float(softening), 1 if potential else 0, _abi.ptr(phi), _abi.ptr(field),
                 _abi.ptr(status), _abi.stream(),
             )
@@ -552,6 +552,18 @@ class FlatQuadtree:
         lists = (rows[0], rows[1], (n_m2l, n_p2p))
         self._traversals[key] = lists
         return lists
+
+    def _leaf_index(self) -> torch.Tensor:
+        """(N,) int32: the leaf of every sorted particle (kept with the tree)."""
+        cached = self.__dict__.get("_leaf_of")
+        if cached is not None:
+            return cached
+        n = self._d["pos"].shape[1]
+        leaf_of = torch.zeros(max(n, 1), dtype=torch.int32, device=self._d["pos"].device)
+        _abi.call("td3_leaf_index", _abi.ptr(self._node_info), self.node_count,
+                  _abi.ptr(leaf_of), _abi.stream())
+        self._leaf_of = leaf_of
+        return leaf_of
 
     def _scaled_moments(self, precision: int) -> tuple:
         key = ("scaled", precision)
@@ -651,7 +663,8 @@ class FlatQuadtree:
             _abi.ptr(self._scharge),
             _abi.ptr(self._d["perm"]), n, _abi.ptr(self._d["node_center"]),
             _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), self.node_count,
-            _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol), softening,
+            _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol),
+            _abi.ptr(self._leaf_index()), softening,
             1 if potential else 0, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status),
             _abi.stream(),
         )
@@ -724,7 +737,8 @@ class FlatQuadtree:
             _abi.ptr(self._scharge), _abi.ptr(self._d["perm"]), n,
             _abi.ptr(self._d["node_center"]), _abi.ptr(self._d["node_half"]),
             _abi.ptr(self._node_info), self.node_count, _abi.ptr(local), _abi.ptr(local),
-            _abi.ptr(prow), _abi.ptr(pcol), float(softening), 1 if potential else 0,
+            _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._leaf_index()), float(softening),
+            1 if potential else 0,
             _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status), _abi.stream(),
         )
         _check(status)
@@ -826,7 +840,8 @@ class FlatQuadtree:
             "td3_fmm_at_eval", _abi.LOG, prec, self.order, _abi.ptr(tree._sorted_pos),
             _abi.ptr(td["perm"]),
             _abi.ptr(ex), m, _abi.ptr(td["node_center"]), _abi.ptr(td["node_half"]),
-            _abi.ptr(tree._node_info), tree.node_count, _abi.ptr(lre), _abi.ptr(lim),
+            _abi.ptr(tree._node_info), tree.node_count, _abi.ptr(tree._leaf_index()),
+            _abi.ptr(lre), _abi.ptr(lim),
             _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
             _abi.ptr(self._d["perm"]), n, _abi.ptr(self._node_info), softening,
             1 if potential else 0, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status),
@@ -850,7 +865,8 @@ class FlatQuadtree:
             "td3_fmm_at_eval", _abi.COULOMB, prec, self.order, _abi.ptr(tree._sorted_pos),
             _abi.ptr(td["perm"]), _abi.ptr(ex), m, _abi.ptr(td["node_center"]),
             _abi.ptr(td["node_half"]), _abi.ptr(tree._node_info), tree.node_count,
-            _abi.ptr(local), _abi.ptr(local), _abi.ptr(prow), _abi.ptr(pcol),
+            _abi.ptr(tree._leaf_index()), _abi.ptr(local), _abi.ptr(local), _abi.ptr(prow),
+            _abi.ptr(pcol),
             _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge), _abi.ptr(self._d["perm"]), n,
             _abi.ptr(self._node_info), float(softening), 1 if potential else 0, _abi.ptr(phi),
             _abi.ptr(field), _abi.ptr(status), _abi.stream(),
