@@ -52,6 +52,146 @@ def relative(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
+def _timed(fn, repeats=3):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    out = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best, out
+
+
+def extra_config1(td3):
+    """Plain Barnes-Hut, Coulomb, FP64: N = 5000 uniform, charges +-1, theta 0.5,
+    eps 1e-3, own charge excluded, against the FP64 direct sum."""
+    import torch
+
+    rng = np.random.default_rng(1)
+    pos = torch.tensor(rng.uniform(-1, 1, (2, 5000)), device="cuda")
+    q = torch.tensor(rng.choice([-1.0, 1.0], 5000), device="cuda")
+    own = torch.arange(5000, device="cuda")
+
+    def run():
+        tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1))
+        return tree.evaluate(pos, exclude=own, theta=0.5, softening=1e-3)
+
+    t, (phi, field) = _timed(run)
+    dphi, dfield = td3.direct_field(pos, pos, q, softening=1e-3, exclude=own)
+    return {"workload": "BH walk, Coulomb, N=5000, theta 0.5, eps 1e-3 (build + walk)",
+            "ms": 1e3 * t, "targets_per_s": 5000 / t,
+            "err_vs_direct": {"phi": relative(phi.cpu().numpy(), dphi.cpu().numpy()),
+                              "field": relative(field.cpu().numpy(), dfield.cpu().numpy())}}
+
+
+def extra_config3(td3, n_src=1 << 22, grid=1024):
+    """Coulomb fmm_at, order 10: 4,194,304 sources from a 2D Plummer-like profile (scale
+    0.1, inside [-1, 1]^2), charges U[-1, 1], at a 1024 x 1024 grid over [-1, 1]^2;
+    mixed vs FP64, and both against the FP64 direct sum at 2048 sampled grid points."""
+    import torch
+
+    rng = np.random.default_rng(3)
+    pts = np.empty((2, 0))
+    while pts.shape[1] < n_src:
+        u = rng.uniform(0, 1, n_src)
+        r = 0.1 * np.sqrt(u / (1 - u))
+        a = rng.uniform(0, 2 * np.pi, n_src)
+        cand = np.stack((r * np.cos(a), r * np.sin(a)))
+        cand = cand[:, np.all(np.abs(cand) <= 1, axis=0)]
+        pts = np.concatenate((pts, cand), axis=1)
+    pos = torch.tensor(np.ascontiguousarray(pts[:, :n_src]), device="cuda")
+    q = torch.tensor(rng.uniform(-1, 1, n_src), device="cuda")
+    g = np.linspace(-1, 1, grid)
+    gx, gy = np.meshgrid(g, g)
+    targets = torch.tensor(np.stack((gx.ravel(), gy.ravel())), device="cuda")
+    out = {"workload": f"Coulomb fmm_at, order 10, theta 0.5, eps 0: {n_src} Plummer sources, "
+                       f"{grid}x{grid} grid targets (build + plan + FMM)"}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        def run():
+            tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), order=10, precision=prec)
+            return tree.fmm_at(targets, theta=0.5)
+
+        t, (phi, field) = _timed(run, repeats=2)
+        res[prec] = (phi.cpu().numpy(), field.cpu().numpy())
+        out[prec] = {"ms": 1e3 * t, "targets_per_s": grid * grid / t}
+    idx = np.sort(rng.choice(grid * grid, 2048, replace=False))
+    dphi, dfield = td3.direct_field(targets[:, idx], pos, q)
+    dphi, dfield = dphi.cpu().numpy(), dfield.cpu().numpy()
+    out["err"] = {
+        "mixed_vs_fp64": {"phi": relative(res["mixed"][0], res["fp64"][0]),
+                          "field": relative(res["mixed"][1], res["fp64"][1])},
+        "fp64_vs_direct": {"phi": relative(res["fp64"][0][idx], dphi),
+                           "field": relative(res["fp64"][1][:, idx], dfield)},
+        "mixed_vs_direct": {"phi": relative(res["mixed"][0][idx], dphi),
+                            "field": relative(res["mixed"][1][:, idx], dfield)},
+    }
+    return out
+
+
+def extra_config4(td3, systems=512, steps=100):
+    """Batched ensemble: 512 Coulomb Barnes-Hut systems, N log-uniform in [1024, 65536]
+    (seed 4), Gaussian clouds (sigma 0.1) of unit charges, B_z 1, k_e 1, dt 1e-4,
+    theta 0.5, eps 1e-3; every step one forest build, one walk launch, two push launches."""
+    rng = np.random.default_rng(4)
+    sizes = np.round(np.exp(rng.uniform(np.log(1024), np.log(65536), systems))).astype(int)
+    n = int(sizes.sum())
+    pos = rng.normal(0.0, 0.1, (2, n)).clip(-0.999, 0.999)
+    roots = np.tile([0.0, 0.0, 1.0], (systems, 1))
+    ens = td3.Ensemble.from_arrays(pos, np.zeros((2, n)), np.ones(n), np.ones(n), sizes, roots,
+                                   theta=0.5, softening=1e-3, k_e=1.0, dt=1e-4, B_z=1.0)
+    import torch
+
+    e0 = ens.total_energies()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens.step()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    drift = np.abs(ens.total_energies() - e0) / np.abs(e0)
+    return {"workload": f"{systems} systems, sum N = {n}, {steps} steps, method tree",
+            "ms_per_step": 1e3 * t / steps, "particle_steps_per_s": n * steps / t,
+            "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))}}
+
+
+def extra_config5(td3, n=262144, probes=8192):
+    """Differentiable log FMM: N = 262,144 uniform, charges +-1, order 12; loss = mean
+    squared potential at 8192 probes + 1/2 sum q phi; forward + adjoint backward."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    pos = torch.tensor(rng.uniform(-1, 1, (2, n)), device="cuda", requires_grad=True)
+    q = torch.tensor(rng.choice([-1.0, 1.0], n), device="cuda", requires_grad=True)
+    pr = torch.tensor(rng.uniform(-1, 1, (2, probes)), device="cuda")
+
+    def run():
+        phi, phi_p = td3.log_potentials(pos, q, pr, root=td3.Quad(0, 0, 1))
+        loss = (phi_p ** 2).mean() + 0.5 * torch.dot(q, phi)
+        return torch.autograd.grad(loss, (pos, q))
+
+    t, _ = _timed(run, repeats=2)
+    return {"workload": f"log FMM N={n}, {probes} probes, order 12: loss + dL/dq + dL/dpos",
+            "ms": 1e3 * t}
+
+
+def extras(td3):
+    out = {}
+    for name, fn in (("config1", extra_config1), ("config3", extra_config3),
+                     ("config4", extra_config4), ("config5", extra_config5)):
+        try:
+            out[name] = fn(td3)
+        except Exception as exc:  # reported, not raised: the metric above stands alone
+            out[name] = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"extra {name} done", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(n_sample: int, seconds: float):
     """The CPU oracle (a C/OpenMP restatement of the reference's Pyccel kernels) on a
     bounded sample of the same workload: build + moments + FMM, all host cores."""
@@ -86,6 +226,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other configurations (ensemble, fmm_at, adjoint, walk)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -204,6 +346,8 @@ def main() -> int:
         "algorithmic": {"m2l_directed_pairs": m2l_directed, "near_directed_pairs": near_pairs,
                         "flops_per_m2l_pair": 910, "flops_per_near_pair": 17},
     }
+    if not args.no_extras:
+        result["extras"] = extras(td3)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(200_000, args.cpu_seconds)
     print(json.dumps(result))

@@ -35,7 +35,8 @@ def test_forest_walk_equals_each_systems_own_tree(td3):
     system = torch.repeat_interleave(torch.arange(len(sizes), dtype=torch.int32, device="cuda"),
                                      torch.tensor(sizes, device="cuda"))
     print("forest walk", flush=True)
-    forest = ens_mod._Forest(pos, q, system, start, torch.tensor(roots, device="cuda"), 8, 48)
+    forest = ens_mod._Forest(pos, q, system, start,
+                             torch.tensor(roots, dtype=torch.float64, device="cuda"), 8, 48)
     phi, field = forest.walk(0.5, 1e-3)
     total = len(sizes)
     for s, (k, root) in enumerate(zip(sizes, roots)):

@@ -143,22 +143,62 @@ class Ensemble:
         leaf = first.leaf_capacity if first.leaf_capacity is not None else 8
         if any((s.leaf_capacity if s.leaf_capacity is not None else 8) != leaf for s in sims):
             raise ValueError("every system of an ensemble must share leaf_capacity")
-        self.theta, self.softening = float(first.theta), float(first.softening)
-        self.k_e, self.dt, self.B_z = float(first.k_e), float(first.dt), float(first.B_z)
-        self.leaf_capacity, self.max_depth = int(leaf), int(first.max_depth)
+        dev = first.positions.device
+        self._setup(
+            torch.cat([s.positions for s in sims], dim=1),
+            torch.cat([s.velocities for s in sims], dim=1),
+            torch.cat([s.masses for s in sims]), torch.cat([s.charges for s in sims]),
+            torch.cat([s.fixed for s in sims]), [s.num_particles for s in sims],
+            torch.tensor([[s.bounds.xmid, s.bounds.ymid, s.bounds.half] for s in sims],
+                         dtype=torch.float64, device=dev),
+            theta=first.theta, softening=first.softening, k_e=first.k_e, dt=first.dt,
+            B_z=first.B_z, leaf_capacity=leaf, max_depth=first.max_depth)
+
+    @classmethod
+    def from_arrays(cls, positions, velocities, masses, charges, sizes, roots, *, fixed=None,
+                    theta: float = 0.5, softening: float = 0.0, k_e: float = 1.0,
+                    dt: float = 1e-12, B_z: float = 0.0, leaf_capacity: int = 8,
+                    max_depth: int = 48) -> "Ensemble":
+        """An ensemble from the systems' arrays laid one after another: positions and
+        velocities (2, N), masses and charges (N,), ``sizes`` the particle count of every
+        system, ``roots`` (S, 3) their root squares (xmid, ymid, half)."""
+        dev = _abi.device()
+        f64 = dict(dtype=torch.float64, device=dev)
+        positions = torch.as_tensor(positions, **f64)
+        n = positions.shape[1]
+        fixed = torch.zeros(n, dtype=torch.bool, device=dev) if fixed is None else \
+            torch.as_tensor(fixed, dtype=torch.bool, device=dev)
+        ens = cls.__new__(cls)
+        ens._setup(positions, torch.as_tensor(velocities, **f64), torch.as_tensor(masses, **f64),
+                   torch.as_tensor(charges, **f64), fixed, [int(k) for k in sizes],
+                   torch.as_tensor(roots, **f64), theta=theta, softening=softening, k_e=k_e,
+                   dt=dt, B_z=B_z, leaf_capacity=leaf_capacity, max_depth=max_depth)
+        return ens
+
+    def _setup(self, positions, velocities, masses, charges, fixed, sizes, roots, *, theta,
+               softening, k_e, dt, B_z, leaf_capacity, max_depth) -> None:
+        self.theta = _finite("theta", theta, minimum=0)
+        self.softening = _finite("softening", softening, minimum=0)
+        self.k_e = _finite("k_e", k_e, minimum=0)
+        self.dt = _finite("dt", dt, positive=True)
+        self.B_z = _finite("B_z", B_z)
+        self.leaf_capacity, self.max_depth = int(leaf_capacity), int(max_depth)
         self.current_time, self.iteration = 0.0, 0
-        sizes = [s.num_particles for s in sims]
-        if min(sizes) < 1:
+        if not sizes or min(sizes) < 1:
             raise ValueError("every system needs at least one particle")
         self.sizes = np.array(sizes, dtype=np.int64)
-        self.systems = len(sims)
-        dev = first.positions.device
-        self.positions = torch.cat([s.positions for s in sims], dim=1).contiguous()
-        self.velocities = torch.cat([s.velocities for s in sims], dim=1).contiguous()
+        self.systems = len(sizes)
+        n = int(self.sizes.sum())
+        if positions.shape != (2, n) or velocities.shape != (2, n) or masses.shape != (n,) \
+                or charges.shape != (n,) or fixed.shape != (n,) or roots.shape != (self.systems, 3):
+            raise ValueError("array shapes do not match the system sizes")
+        dev = positions.device
+        self.positions = positions.contiguous().clone()
+        self.velocities = velocities.contiguous().clone()
         self.forces = torch.zeros_like(self.positions)
-        self.masses = torch.cat([s.masses for s in sims]).contiguous()
-        self.charges = torch.cat([s.charges for s in sims]).contiguous()
-        self.fixed = torch.cat([s.fixed for s in sims]).contiguous()
+        self.masses = masses.contiguous().clone()
+        self.charges = charges.contiguous().clone()
+        self.fixed = fixed.contiguous().clone()
         self._fixed_u8 = self.fixed.to(torch.uint8)
         offsets = np.concatenate(([0], np.cumsum(self.sizes)))
         self.system_start = torch.tensor(offsets, dtype=torch.int64, device=dev)
@@ -166,8 +206,7 @@ class Ensemble:
             torch.arange(self.systems, dtype=torch.int32, device=dev),
             torch.tensor(self.sizes, device=dev))
         self._index = self.system.to(torch.int64)
-        self.roots = torch.tensor([[s.bounds.xmid, s.bounds.ymid, s.bounds.half] for s in sims],
-                                  dtype=torch.float64, device=dev)
+        self.roots = roots.contiguous().clone()
         self._electric = torch.zeros_like(self.positions)
         self._potential = torch.zeros_like(self.charges)
         self._current = False
