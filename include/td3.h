@@ -175,6 +175,27 @@ int td3_traverse(const double *node_center, const double *node_half, const void 
                  size_t workspace_bytes, int64_t *out_counts, void *stream);
 
 /*
+ * The FMM's interaction rows by node, straight from the traversal rules of
+ * kernels/fmm_traverse/fmm_traverse_pyccel.py:15-106 (flat_tree.py:1318-1358) without
+ * the pair lists: per level, every node walks the pairs it inherits from its parent
+ * (one lane per pair, depth first over the other cell) and writes its own rows.  The
+ * rows hold the same pairs as td3_traverse + td3_lists (M2L both ways, mode 0; P2P both
+ * ways with a self pair once, mode 1), each row in a fixed walk order (not sorted);
+ * m2l_dst (may be NULL) gets the row's node of every M2L entry.  levels_host as
+ * td3_moments.  out_counts (HOST int64[4]): M2L entries, P2P entries, stack overflow,
+ * capacity overflow.  TD3_ERR_CAPACITY asks for larger cap_m2l / cap_p2p / cap_inherit
+ * (cap_inherit bounds the pairs handed to one level).  Any depth (the walks follow child
+ * and parent links, no stack).  Synchronises the stream once.
+ */
+size_t td3_rows_workspace_bytes(const int64_t *levels_host, int64_t n_levels,
+                                int64_t cap_inherit);
+int td3_rows(const double *node_center, const double *node_half, const void *node_info,
+             const int64_t *levels_host, int64_t n_levels, double theta, int64_t cap_m2l,
+             int64_t cap_p2p, int64_t cap_inherit, uint32_t *m2l_row, int32_t *m2l_col,
+             int32_t *m2l_dst, uint32_t *p2p_row, int32_t *p2p_col, void *workspace,
+             size_t workspace_bytes, int64_t *out_counts, void *stream);
+
+/*
  * Rows by node of a pair list (kernels/pair_csr/pair_csr_pyccel.py:12-45): mode 0 lists
  * b under a and a under b; mode 1 the same with a self pair once (the P2P leaves); mode 2
  * one way, b under a (fmm_at's target rows).  row_start (node_count + 1) uint32, col
@@ -312,6 +333,17 @@ int td3_cart_l2l(int64_t order, const double *node_center, const void *node_info
 int td3_push(int first_half, int64_t n, double *positions, double *velocities, double *forces,
              const double *electric, const double *masses, const double *charges,
              const uint8_t *fixed, double dt, double b_z, void *stream);
+
+/*
+ * The roots of an ensemble's systems (simulation.py:625-639 per system): a root square
+ * that no longer holds its system's particles becomes the square around them and the old
+ * root (centre lo/2 + hi/2, half max(hi/2 - lo/2) (1 + 1e-12)).  positions (2, N), the
+ * systems one after another (system_start (S + 1) int64); roots (S, 3) updated in place;
+ * *grown (device u32) counts the roots that grew.  Non-finite positions leave NaN in the
+ * bounds: the caller checks them.  No counterpart in the reference.
+ */
+int td3_system_bounds(const double *positions, int64_t n, int64_t systems,
+                      const int64_t *system_start, double *roots, uint32_t *grown, void *stream);
 
 /*
  * Kernel timing for measurement (no counterpart in the reference, whose profiling is

@@ -254,6 +254,35 @@ def case_forest():
     assert forest.node_count == total, (forest.node_count, total)
 
 
+def sorted_rows(row, col):
+    row = row.numpy().astype(np.int64)
+    col = col.numpy()[: row[-1]].astype(np.int64)
+    dst = np.repeat(np.arange(len(row) - 1), np.diff(row))
+    pairs = np.stack((dst, col), axis=1)
+    return pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+
+
+def case_rows():
+    rng = np.random.default_rng(41)
+    clouds = [cloud(2500, 41)[0], rng.uniform(-0.99, 0.99, (2, 1500)),
+              np.concatenate((np.full((2, 30), 0.3), rng.uniform(-1, 1, (2, 500))), axis=1)]
+    for pos in clouds:
+        q = np.ones(pos.shape[1])
+        for leaf in (1, 8, 64):
+            t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", leaf_capacity=leaf)
+            for theta in (0.3, 0.5, 0.9):
+                new = t._rows(theta)
+                old = t._traverse_lists(theta)
+                assert new is not None
+                assert new[2] == old[2], (new[2], old[2])
+                for a, b in zip(new[:2], old[:2]):
+                    assert np.array_equal(sorted_rows(a[0], a[1]), sorted_rows(b[0], b[1]))
+                m_row, m_col, m_dst = new[0]
+                n = int(m_row[-1])
+                assert np.array_equal(m_dst[:n].numpy(),
+                                      np.repeat(np.arange(t.node_count), np.diff(m_row.numpy())))
+
+
 CASES = {name[5:]: fn for name, fn in globals().items() if name.startswith("case_")}
 
 if __name__ == "__main__":

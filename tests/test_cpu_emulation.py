@@ -2,7 +2,8 @@
 the real orchestration runs on small inputs against the oracle, so an out-of-bounds index
 or a wrong buffer size fails here, on the CPU, before any GPU run.
 
-TD3_EMULATION=quick runs only the fast cases; the default runs all (about two minutes).
+The default runs every case but the two large ones (a few minutes); TD3_EMULATION=full
+adds them (a 70k-particle tree and the row construction on many trees), quick runs two.
 """
 
 import os
@@ -13,7 +14,8 @@ import pytest
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_emulation")
 QUICK = ["direct", "moments"]
-FULL = ["direct", "tree", "big_tree", "forest", "moments", "walk", "fmm", "fmm_at", "coulomb", "push"]
+DEFAULT = ["direct", "tree", "forest", "moments", "walk", "fmm", "fmm_at", "coulomb", "push"]
+FULL = DEFAULT + ["big_tree", "rows"]
 
 
 @pytest.fixture(scope="module")
@@ -28,7 +30,8 @@ def test_kernels_under_address_sanitizer(emulated_library):
     asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True,
                           text=True).stdout.strip()
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
-    cases = QUICK if os.environ.get("TD3_EMULATION") == "quick" else FULL
+    mode = os.environ.get("TD3_EMULATION")
+    cases = QUICK if mode == "quick" else FULL if mode == "full" else DEFAULT
     r = subprocess.run([sys.executable, os.path.join(HERE, "run_emulated.py"), *cases],
                        capture_output=True, text=True, env=env, timeout=3000)
     assert r.returncode == 0 and "EMULATION OK" in r.stdout, (r.stdout[-3000:] + r.stderr[-6000:])

@@ -30,10 +30,12 @@ def config2(n, seed=7):
 
 
 def directed(rows, cols):
+    """(dst, src) of every row entry, sorted (a row's order is a walk order)."""
     row = np.asarray(rows.cpu()).astype(np.int64)
     col = np.asarray(cols.cpu()).astype(np.int64)[: row[-1]]
     dst = np.repeat(np.arange(len(row) - 1), np.diff(row))
-    return np.stack((dst, col), axis=1)
+    pairs = np.stack((dst, col), axis=1)
+    return pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
 
 
 def as_directed(pairs, self_once):

@@ -45,7 +45,71 @@ __global__ void push_kernel(int first_half, i64 n, double* __restrict__ pos,
     vel[n + i] = vy;
 }
 
+// One block per system: the bounding box of its particles and, when they leave its root
+// square, the new root (simulation.py:625-639): the square around the box and the old
+// root, centre lo/2 + hi/2, half max(hi/2 - lo/2) (1 + 1e-12).
+__global__ __launch_bounds__(256) void system_bounds_kernel(const double* __restrict__ pos, i64 n,
+                                                            const i64* __restrict__ start,
+                                                            double* __restrict__ roots,
+                                                            u32* __restrict__ grown) {
+    __shared__ double s_v[4][256 / TD3_WAVE];
+    const i64 sys = blockIdx.x;
+    const i64 a = start[sys], b = start[sys + 1];
+    double lo_x = INFINITY, lo_y = INFINITY, hi_x = -INFINITY, hi_y = -INFINITY;
+    for (i64 i = a + threadIdx.x; i < b; i += blockDim.x) {
+        const double x = pos[i], y = pos[n + i];
+        lo_x = fmin(lo_x, x);
+        hi_x = fmax(hi_x, x);
+        lo_y = fmin(lo_y, y);
+        hi_y = fmax(hi_y, y);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo_x = fmin(lo_x, __shfl_xor(lo_x, off, 64));
+        hi_x = fmax(hi_x, __shfl_xor(hi_x, off, 64));
+        lo_y = fmin(lo_y, __shfl_xor(lo_y, off, 64));
+        hi_y = fmax(hi_y, __shfl_xor(hi_y, off, 64));
+    }
+    const int wave = threadIdx.x / TD3_WAVE;
+    if (td3_lane() == 0) {
+        s_v[0][wave] = lo_x;
+        s_v[1][wave] = hi_x;
+        s_v[2][wave] = lo_y;
+        s_v[3][wave] = hi_y;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < (int)(blockDim.x / TD3_WAVE); ++w) {
+        lo_x = fmin(lo_x, s_v[0][w]);
+        hi_x = fmax(hi_x, s_v[1][w]);
+        lo_y = fmin(lo_y, s_v[2][w]);
+        hi_y = fmax(hi_y, s_v[3][w]);
+    }
+    const double cx = roots[3 * sys], cy = roots[3 * sys + 1], h = roots[3 * sys + 2];
+    const double rlo_x = cx - h, rlo_y = cy - h, rhi_x = cx + h, rhi_y = cy + h;
+    if (!(lo_x < rlo_x || lo_y < rlo_y || hi_x > rhi_x || hi_y > rhi_y)) return;
+    lo_x = fmin(lo_x, rlo_x);
+    lo_y = fmin(lo_y, rlo_y);
+    hi_x = fmax(hi_x, rhi_x);
+    hi_y = fmax(hi_y, rhi_y);
+    const double wx = hi_x / 2 - lo_x / 2, wy = hi_y / 2 - lo_y / 2;
+    roots[3 * sys] = lo_x / 2 + hi_x / 2;
+    roots[3 * sys + 1] = lo_y / 2 + hi_y / 2;
+    roots[3 * sys + 2] = (wx > wy ? wx : wy) * (1 + 1e-12);
+    atomicAdd(grown, 1u);
+}
+
 }  // namespace
+
+extern "C" int td3_system_bounds(const double* positions, int64_t n, int64_t systems,
+                                 const int64_t* system_start, double* roots, uint32_t* grown,
+                                 void* stream) {
+    TD3_ARG(n >= 0 && systems >= 1 && system_start != nullptr && roots != nullptr &&
+                grown != nullptr, "arguments");
+    hipLaunchKernelGGL(system_bounds_kernel, dim3((unsigned)systems), dim3(256), 0,
+                       (hipStream_t)stream, positions, n, system_start, roots, grown);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
 
 extern "C" int td3_push(int first_half, int64_t n, double* positions, double* velocities,
                         double* forces, const double* electric, const double* masses,

@@ -74,13 +74,15 @@ __global__ __launch_bounds__(kThreads) void tile_sum_kernel(const u32* __restric
 
 // Exclusive scan of each tile, offset by the scanned tile sums.  Each element is read and
 // written by the same thread, so `out` may alias `in`.
+// With part null (one tile) the carry starts at zero and the total goes to *total.
 __global__ __launch_bounds__(kThreads) void tile_scan_kernel(const u32* in, u32* out, i64 n,
-                                                             const u32* __restrict__ part) {
+                                                             const u32* __restrict__ part,
+                                                             u32* __restrict__ total) {
     __shared__ u32 s_wave[kThreads / TD3_WAVE];
     const int lane = threadIdx.x % TD3_WAVE;
     const int wave = threadIdx.x / TD3_WAVE;
     const i64 base = (i64)blockIdx.x * kTile;
-    u32 carry = part[blockIdx.x];
+    u32 carry = part ? part[blockIdx.x] : 0u;
     for (int r = 0; r < kItems; ++r) {
         const i64 idx = base + r * kThreads + threadIdx.x;
         const u32 v = idx < n ? in[idx] : 0u;
@@ -103,6 +105,7 @@ __global__ __launch_bounds__(kThreads) void tile_scan_kernel(const u32* in, u32*
         carry += all;
         __syncthreads();
     }
+    if (total != nullptr && threadIdx.x == 0) *total = carry;
 }
 
 template <bool AUX, bool IDENT>
@@ -166,12 +169,20 @@ int exclusive_scan_u32(const u32* in, u32* out, i64 n, u32* total, u32* scratch,
         return TD3_OK;
     }
     const i64 nt = tiles(n);
+    if (nt == 1) {
+        // one tile: a single launch
+        TD3_TIMED(stream, "scan", hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(kThreads), 0, stream, in, out, n,
+                                                     (const u32*)nullptr, total));
+        TD3_LAUNCHED();
+        return TD3_OK;
+    }
     u32* part = scratch;
     TD3_TIMED(stream, "scan", hipLaunchKernelGGL(tile_sum_kernel, dim3((unsigned)nt), dim3(kThreads), 0, stream, in, n, part));
     TD3_LAUNCHED();
     TD3_TIMED(stream, "scan", hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(1024), 0, stream, part, nt, total));
     TD3_LAUNCHED();
-    TD3_TIMED(stream, "scan", hipLaunchKernelGGL(tile_scan_kernel, dim3((unsigned)nt), dim3(kThreads), 0, stream, in, out, n, part));
+    TD3_TIMED(stream, "scan", hipLaunchKernelGGL(tile_scan_kernel, dim3((unsigned)nt), dim3(kThreads), 0, stream, in, out, n, part,
+                                                 (u32*)nullptr));
     TD3_LAUNCHED();
     return TD3_OK;
 }

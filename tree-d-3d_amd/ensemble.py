@@ -218,23 +218,15 @@ class Ensemble:
 
     # ------------------------------------------------------------------ fields
     def _expand_roots(self) -> None:
-        """simulation.py:625-639 for every system at once: a root that no longer holds its
-        particles becomes the square around them and the old root."""
-        p, s = self.positions, self._index
-        if not bool(torch.isfinite(p).all()):
+        """simulation.py:625-639 for every system at once (one block per system): a root
+        that no longer holds its particles becomes the square around them and the old
+        root."""
+        if not bool(torch.isfinite(self.positions).all()):
             raise ValueError("Particle positions must be finite; reduce dt")
-        big = torch.full((2, self.systems), np.inf, dtype=torch.float64, device=p.device)
-        lo = big.scatter_reduce(1, s.expand(2, -1), p, "amin", include_self=True)
-        hi = (-big).scatter_reduce(1, s.expand(2, -1), p, "amax", include_self=True)
-        r = self.roots
-        rlo = torch.stack((r[:, 0] - r[:, 2], r[:, 1] - r[:, 2]))
-        rhi = torch.stack((r[:, 0] + r[:, 2], r[:, 1] + r[:, 2]))
-        grow = ((lo < rlo) | (hi > rhi)).any(dim=0)
-        lo, hi = torch.minimum(lo, rlo), torch.maximum(hi, rhi)
-        center = lo / 2 + hi / 2
-        half = (hi / 2 - lo / 2).max(dim=0).values * (1 + 1e-12)
-        new = torch.stack((center[0], center[1], half), dim=1)
-        self.roots = torch.where(grow[:, None], new, r).contiguous()
+        grown = torch.zeros(1, dtype=torch.int32, device=self.positions.device)
+        _abi.call("td3_system_bounds", _abi.ptr(self.positions), self.num_particles,
+                  self.systems, _abi.ptr(self.system_start), _abi.ptr(self.roots),
+                  _abi.ptr(grown), _abi.stream())
 
     def compute_fields(self):
         """Force and potential on every particle of every system (one forest build, one

@@ -518,6 +518,47 @@ class FlatQuadtree:
         key = ("lists", theta)
         if key in self._traversals:
             return self._traversals[key]
+        lists = self._rows(theta)
+        self._traversals[key] = lists
+        return lists
+
+    def _rows(self, theta: float):
+        """td3_rows: every node's rows from the pairs it inherits, level by level."""
+        dev = self._d["pos"].device
+        c = self.node_count
+        levels = np.ascontiguousarray(self.levels, dtype=np.int64)
+        n_levels = len(levels) - 1
+        caps = [96 * c + 64, 40 * c + 64, 24 * c + 64]
+        out = np.zeros(4, dtype=np.int64)
+        while True:
+            i32 = dict(dtype=torch.int32, device=dev)
+            mrow, prow = torch.empty(c + 1, **i32), torch.empty(c + 1, **i32)
+            mcol, mdst = torch.empty(caps[0], **i32), torch.empty(caps[0], **i32)
+            pcol = torch.empty(caps[1], **i32)
+            nbytes = _abi.lib().td3_rows_workspace_bytes(levels.ctypes.data, n_levels, caps[2])
+            ws = _abi.workspace(nbytes)
+            code = _abi.lib().td3_rows(
+                _abi.ptr(self._d["node_center"]), _abi.ptr(self._d["node_half"]),
+                _abi.ptr(self._node_info), levels.ctypes.data, n_levels, float(theta), caps[0],
+                caps[1], caps[2], _abi.ptr(mrow), _abi.ptr(mcol), _abi.ptr(mdst),
+                _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(ws), nbytes, out.ctypes.data,
+                _abi.stream(),
+            )
+            if code == _abi.TD3_ERR_CAPACITY:
+                caps = [max(2 * caps[i], int(out[i]) + 64) if i < 2 else 2 * caps[i]
+                        for i in range(3)]
+                continue
+            _abi.check(code, "td3_rows")
+            break
+        n_m2l, n_p2p = int(out[0]), int(out[1])
+        leaves = int(((self._d["node_child"] < 0)
+                      & (self._d["node_end"] > self._d["node_start"])).sum())
+        counts = (n_m2l // 2, (n_p2p + leaves) // 2)
+        return (mrow, mcol[: max(n_m2l, 1)], mdst[: max(n_m2l, 1)]), \
+            (prow, pcol[: max(n_p2p, 1)], None), counts
+
+    def _traverse_lists(self, theta: float):
+        """td3_traverse (pair lists by generation) + td3_lists (rows, each sorted)."""
         dev = self._d["pos"].device
         c = self.node_count
         caps = [48 * c + 64, 16 * c + 64, 10 * c + 64]
@@ -549,9 +590,7 @@ class FlatQuadtree:
             _abi.call("td3_lists", _abi.ptr(pairs), count, self_once, c, _abi.ptr(row),
                       _abi.ptr(col), _abi.ptr(dst), _abi.ptr(ws), nbytes, _abi.stream())
             rows.append((row, col, dst))
-        lists = (rows[0], rows[1], (n_m2l, n_p2p))
-        self._traversals[key] = lists
-        return lists
+        return (rows[0], rows[1], (n_m2l, n_p2p))
 
     def _leaf_index(self) -> torch.Tensor:
         """(N,) int32: the leaf of every sorted particle (kept with the tree)."""
