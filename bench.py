@@ -227,6 +227,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--pmc", action="store_true",
+                    help="one warm mixed step and a 256 MiB device copy only (for rocprofv3 "
+                         "--pmc runs: the copy calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other configurations (ensemble, fmm_at, adjoint, walk)")
     args = ap.parse_args()
@@ -262,6 +265,17 @@ def main() -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, out
 
+    if args.pmc:
+        step("mixed")
+        torch.cuda.synchronize()
+        step("mixed")
+        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc": "one mixed step, then two 256 MiB copies"}))
+        return 0
     t_mixed, (tree, phi_m, field_m) = timed("mixed", args.steps, args.warmup)
     result = {
         "metric": "targets/s at N=1M FMM: mixed vs FP64 on GPU at <=1e-6 rel err vs direct",

@@ -260,8 +260,16 @@ int td3_fmm_eval(int kernel, int precision, int64_t order, const double *sorted_
                  const double *node_center, const double *node_half, const void *node_info,
                  int64_t node_count, const double *local_re, const double *local_im,
                  const uint32_t *p2p_row, const int32_t *p2p_col, const int32_t *leaf_of,
-                 double softening, int with_potential, double *phi, double *field,
-                 int64_t *status, void *stream);
+                 const void *sources4, double softening, int with_potential, double *phi,
+                 double *field, int64_t *status, void *stream);
+
+/*
+ * (x, y, q, 0) float64 per sorted source (N x 32 bytes): the near field's source records
+ * (td3_fmm_eval's sources4, td3_fmm_at_eval's s_sources4).  No counterpart in the
+ * reference (it reads the (2, N) positions and the charges apart).
+ */
+int td3_pack_sources(const double *sorted_pos, const double *sorted_charge, int64_t n,
+                     void *sources4, void *stream);
 
 /*
  * leaf_of[k] (int32, N) = the leaf holding sorted particle k: the per-particle map the
@@ -295,11 +303,11 @@ int td3_fmm_at_eval(int kernel, int precision, int64_t order, const double *sort
                     const int64_t *target_perm, const int64_t *exclude, int64_t m,
                     const double *t_center, const double *t_half, const void *t_info,
                     int64_t t_count, const int32_t *t_leaf_of, const double *local_re,
-                    const double *local_im,
-                    const uint32_t *p2p_row, const int32_t *p2p_col, const double *sorted_pos,
-                    const double *sorted_charge, const int64_t *perm, int64_t n,
-                    const void *s_info, double softening, int with_potential, double *phi,
-                    double *field, int64_t *status, void *stream);
+                    const double *local_im, const uint32_t *p2p_row, const int32_t *p2p_col,
+                    const double *sorted_pos, const double *sorted_charge, const int64_t *perm,
+                    int64_t n, const void *s_info, const void *s_sources4, double softening,
+                    int with_potential, double *phi, double *field, int64_t *status,
+                    void *stream);
 
 /*
  * The Coulomb (Cartesian) FMM of the softened kernel (flat_tree.py:1360-1428, _cartesian.py,

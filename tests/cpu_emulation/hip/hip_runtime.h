@@ -40,6 +40,10 @@ struct alignas(16) float4 {
     float x, y, z, w;
 };
 inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }
+struct alignas(32) double4 {
+    double x, y, z, w;
+};
+inline double4 make_double4(double x, double y, double z, double w) { return {x, y, z, w}; }
 inline int2 make_int2(int x, int y) { return {x, y}; }
 inline int4 make_int4(int x, int y, int z, int w) { return {x, y, z, w}; }
 struct dim3 {

@@ -58,13 +58,14 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_m2l": ([_C, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P], _C),
     "td3_l2l": ([_I, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P], _C),
     "td3_unscale_locals": ([_P, _I, _I, _P, _P, _P], _C),
-    "td3_fmm_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _C, _P,
-                      _P, _P, _P], _C),
+    "td3_fmm_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _D, _C,
+                      _P, _P, _P, _P], _C),
+    "td3_pack_sources": ([_P, _P, _I, _P, _P], _C),
     "td3_leaf_index": ([_P, _I, _P, _P], _C),
     "td3_dual_traverse": ([_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P, _Z,
                            _P, _P], _C),
     "td3_fmm_at_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
-                         _P, _I, _P, _D, _C, _P, _P, _P, _P], _C),
+                         _P, _I, _P, _P, _D, _C, _P, _P, _P, _P], _C),
     "td3_cart_moments": ([_I, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P], _C),
     "td3_cart_m2l_workspace_bytes": ([_I, _I, _I], _Z),
     "td3_cart_m2l": ([_I, _P, _P, _P, _I, _I, _P, _P, _P, _D, _P, _P, _Z, _P], _C),

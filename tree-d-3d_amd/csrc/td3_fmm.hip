@@ -403,9 +403,9 @@ __global__ __launch_bounds__(256) void eval_kernel(
     const double* __restrict__ lim, int p1, int order, const u32* __restrict__ prow,
     const int* __restrict__ pcol, const double* __restrict__ spos,
     const double* __restrict__ scharge, const i64* __restrict__ sperm, i64 n,
-    const int4* __restrict__ sinfo, const int* __restrict__ leaf_of, double eps2,
-    int with_potential, double* __restrict__ phi, double* __restrict__ field,
-    i64* __restrict__ status) {
+    const int4* __restrict__ sinfo, const int* __restrict__ leaf_of,
+    const double4* __restrict__ src4, double eps2, int with_potential,
+    double* __restrict__ phi, double* __restrict__ field, i64* __restrict__ status) {
     const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
     {
@@ -420,10 +420,11 @@ __global__ __launch_bounds__(256) void eval_kernel(
             for (u32 r = r0; r < r1; ++r) {
                 const int4 sf = sinfo[pcol[r]];
                 for (int ks = sf.x; ks < sf.y; ++ks) {
-                    const double sq = scharge[ks];
+                    const double4 sv = src4[ks];
+                    const double sq = sv.z;
                     if (MODE == 0 && ks == k) continue;
                     if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
-                    const double dx = tx - spos[ks], dy = ty - spos[n + ks];
+                    const double dx = tx - sv.x, dy = ty - sv.y;
                     const double r2 = dx * dx + dy * dy + eps2;
                     if (r2 == 0.0) {
                         if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
@@ -451,10 +452,11 @@ __global__ __launch_bounds__(256) void eval_kernel(
                 const int4 sf = sinfo[pcol[r]];
                 float fp = 0.0f, fx = 0.0f, fy = 0.0f;
                 for (int ks = sf.x; ks < sf.y; ++ks) {
-                    const double sq = scharge[ks];
+                    const double4 sv = src4[ks];
+                    const double sq = sv.z;
                     if (MODE == 0 && ks == k) continue;
                     if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
-                    const double ddx = tx - spos[ks], ddy = ty - spos[n + ks];
+                    const double ddx = tx - sv.x, ddy = ty - sv.y;
                     const float dx = (float)ddx, dy = (float)ddy, qf = (float)sq;
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
                     if (r2 == 0.0f) {
@@ -549,6 +551,14 @@ __global__ __launch_bounds__(256) void eval_kernel(
     }
 }
 
+// (x, y, q, 0) of every sorted source: one 32-byte load per source in the near field.
+__global__ void pack_sources_kernel(const double* __restrict__ pos, const double* __restrict__ charge,
+                                    i64 n, double4* __restrict__ out) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out[k] = make_double4(pos[k], pos[n + k], charge[k], 0.0);
+}
+
 // leaf_of[k] = the leaf holding sorted particle k.
 __global__ void leaf_index_kernel(const int4* __restrict__ info, i64 count, int* __restrict__ leaf_of) {
     const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -563,20 +573,22 @@ int launch_eval(int precision, hipStream_t s, const double* tpos, const i64* tpe
                 const i64* exclude, i64 m, const double* tcenter, const double* thalf,
                 const int4* tinfo, i64 t_count, const double* lre, const double* lim, int order,
                 const u32* prow, const int* pcol, const double* spos, const double* scharge,
-                const i64* sperm, i64 n, const int4* sinfo, const int* leaf_of, double eps2,
-                int with_potential, double* phi, double* field, i64* status) {
+                const i64* sperm, i64 n, const int4* sinfo, const int* leaf_of,
+                const double4* src4, double eps2, int with_potential, double* phi,
+                double* field, i64* status) {
     TD3_ARG(leaf_of != nullptr, "leaf_of (td3_leaf_index of the target tree)");
+    TD3_ARG(src4 != nullptr, "sources4 (td3_pack_sources of the source tree)");
     const unsigned blocks = td3_blocks(m, 256);
     const int p1 = KERNEL == TD3_LOG ? order + 1 : (order + 1) * (order + 2) / 2;
     if (precision == TD3_FP64)
         TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<KERNEL, TD3_FP64, MODE>), dim3(blocks), dim3(256), 0, s,
                            tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, order,
-                           prow, pcol, spos, scharge, sperm, n, sinfo, leaf_of, eps2,
+                           prow, pcol, spos, scharge, sperm, n, sinfo, leaf_of, src4, eps2,
                            with_potential, phi, field, status));
     else
         TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_kernel<KERNEL, TD3_MIXED, MODE>), dim3(blocks), dim3(256), 0, s,
                            tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, order,
-                           prow, pcol, spos, scharge, sperm, n, sinfo, leaf_of, eps2,
+                           prow, pcol, spos, scharge, sperm, n, sinfo, leaf_of, src4, eps2,
                            with_potential, phi, field, status));
     TD3_LAUNCHED();
     return TD3_OK;
@@ -718,8 +730,9 @@ extern "C" int td3_fmm_eval(int kernel, int precision, int64_t order, const doub
                             const double* node_center, const double* node_half,
                             const void* node_info, int64_t node_count, const double* local_re,
                             const double* local_im, const uint32_t* p2p_row, const int32_t* p2p_col,
-                            const int32_t* leaf_of, double softening, int with_potential,
-                            double* phi, double* field, int64_t* status, void* stream) {
+                            const int32_t* leaf_of, const void* sources4, double softening,
+                            int with_potential, double* phi, double* field, int64_t* status,
+                            void* stream) {
     TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(n >= 1 && node_count >= 1, "sizes");
@@ -727,7 +740,8 @@ extern "C" int td3_fmm_eval(int kernel, int precision, int64_t order, const doub
     return go(precision, (hipStream_t)stream, sorted_pos, perm, nullptr, n, node_center, node_half,
               (const int4*)node_info, node_count, local_re, local_im, (int)order, p2p_row, p2p_col,
               sorted_pos, sorted_charge, perm, n, (const int4*)node_info, leaf_of,
-              softening * softening, with_potential, phi, field, status);
+              (const double4*)sources4, softening * softening, with_potential, phi, field,
+              status);
 }
 
 extern "C" int td3_fmm_at_eval(int kernel, int precision, int64_t order,
@@ -738,8 +752,9 @@ extern "C" int td3_fmm_at_eval(int kernel, int precision, int64_t order,
                                const double* local_im, const uint32_t* p2p_row,
                                const int32_t* p2p_col, const double* sorted_pos,
                                const double* sorted_charge, const int64_t* perm, int64_t n,
-                               const void* s_info, double softening, int with_potential,
-                               double* phi, double* field, int64_t* status, void* stream) {
+                               const void* s_info, const void* s_sources4, double softening,
+                               int with_potential, double* phi, double* field, int64_t* status,
+                               void* stream) {
     TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(m >= 1 && n >= 1 && t_count >= 1, "sizes");
@@ -747,7 +762,8 @@ extern "C" int td3_fmm_at_eval(int kernel, int precision, int64_t order,
     return go(precision, (hipStream_t)stream, sorted_targets, target_perm, exclude, m, t_center,
               t_half, (const int4*)t_info, t_count, local_re, local_im, (int)order, p2p_row,
               p2p_col, sorted_pos, sorted_charge, perm, n, (const int4*)s_info, t_leaf_of,
-              softening * softening, with_potential, phi, field, status);
+              (const double4*)s_sources4, softening * softening, with_potential, phi, field,
+              status);
 }
 
 extern "C" int td3_leaf_index(const void* node_info, int64_t node_count, int32_t* leaf_of,
@@ -755,6 +771,16 @@ extern "C" int td3_leaf_index(const void* node_info, int64_t node_count, int32_t
     TD3_ARG(node_count >= 1 && node_count < 0x7fffffff && leaf_of != nullptr, "sizes");
     hipLaunchKernelGGL(leaf_index_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0,
                        (hipStream_t)stream, (const int4*)node_info, node_count, leaf_of);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+extern "C" int td3_pack_sources(const double* sorted_pos, const double* sorted_charge, int64_t n,
+                                void* sources4, void* stream) {
+    TD3_ARG(n >= 0 && sources4 != nullptr, "arguments");
+    if (n == 0) return TD3_OK;
+    hipLaunchKernelGGL(pack_sources_kernel, dim3(td3_blocks(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, sorted_pos, sorted_charge, n, (double4*)sources4);
     TD3_LAUNCHED();
     return TD3_OK;
 }

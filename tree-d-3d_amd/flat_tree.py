@@ -408,6 +408,7 @@ class FlatQuadtree:
         self._invalidate_host("charge")
         self._cart = None
         self._scaled = {}
+        self._src4 = None
         if self._d["pos"].shape[1]:
             self._sorted_charge()
             self._moments()
@@ -592,6 +593,18 @@ class FlatQuadtree:
             rows.append((row, col, dst))
         return (rows[0], rows[1], (n_m2l, n_p2p))
 
+    def _sources4(self) -> torch.Tensor:
+        """(N, 4) float64 (x, y, q, 0) of the sorted sources (kept until `set_charges`)."""
+        cached = self.__dict__.get("_src4")
+        if cached is not None:
+            return cached
+        n = self._d["pos"].shape[1]
+        src4 = torch.zeros((max(n, 1), 4), dtype=torch.float64, device=self._d["pos"].device)
+        _abi.call("td3_pack_sources", _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge), n,
+                  _abi.ptr(src4), _abi.stream())
+        self._src4 = src4
+        return src4
+
     def _leaf_index(self) -> torch.Tensor:
         """(N,) int32: the leaf of every sorted particle (kept with the tree)."""
         cached = self.__dict__.get("_leaf_of")
@@ -703,7 +716,7 @@ class FlatQuadtree:
             _abi.ptr(self._d["perm"]), n, _abi.ptr(self._d["node_center"]),
             _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), self.node_count,
             _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol),
-            _abi.ptr(self._leaf_index()), softening,
+            _abi.ptr(self._leaf_index()), _abi.ptr(self._sources4()), softening,
             1 if potential else 0, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status),
             _abi.stream(),
         )
@@ -776,8 +789,8 @@ class FlatQuadtree:
             _abi.ptr(self._scharge), _abi.ptr(self._d["perm"]), n,
             _abi.ptr(self._d["node_center"]), _abi.ptr(self._d["node_half"]),
             _abi.ptr(self._node_info), self.node_count, _abi.ptr(local), _abi.ptr(local),
-            _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._leaf_index()), float(softening),
-            1 if potential else 0,
+            _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._leaf_index()),
+            _abi.ptr(self._sources4()), float(softening), 1 if potential else 0,
             _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status), _abi.stream(),
         )
         _check(status)
@@ -882,7 +895,8 @@ class FlatQuadtree:
             _abi.ptr(tree._node_info), tree.node_count, _abi.ptr(tree._leaf_index()),
             _abi.ptr(lre), _abi.ptr(lim),
             _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
-            _abi.ptr(self._d["perm"]), n, _abi.ptr(self._node_info), softening,
+            _abi.ptr(self._d["perm"]), n, _abi.ptr(self._node_info),
+            _abi.ptr(self._sources4()), softening,
             1 if potential else 0, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status),
             _abi.stream(),
         )
@@ -907,7 +921,8 @@ class FlatQuadtree:
             _abi.ptr(tree._leaf_index()), _abi.ptr(local), _abi.ptr(local), _abi.ptr(prow),
             _abi.ptr(pcol),
             _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge), _abi.ptr(self._d["perm"]), n,
-            _abi.ptr(self._node_info), float(softening), 1 if potential else 0, _abi.ptr(phi),
+            _abi.ptr(self._node_info), _abi.ptr(self._sources4()), float(softening),
+            1 if potential else 0, _abi.ptr(phi),
             _abi.ptr(field), _abi.ptr(status), _abi.stream(),
         )
         _check(status)
