@@ -27,7 +27,8 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAMES = ("m2l", "m2l_fixup", "eval", "l2l", "rows_count", "rows_write", "traverse",
+KERNEL_NAMES = ("m2l", "m2l_fixup", "eval", "l2l", "rows_count", "rows_scan", "rows_write",
+                "traverse",
                 "row_sort", "lists_fill",
                 "lists_degree", "sort_scatter", "sort_hist", "scan", "tree_keys", "tree_count",
                 "tree_nodes", "tree_leaf_keys", "tree_gather", "moments_p2m", "moments_m2m",

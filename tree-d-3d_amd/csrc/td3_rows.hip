@@ -19,18 +19,17 @@
 // without atomics.  The pair sets are those of td3_traverse (both rows of a mutual pair
 // come from the same decision, evaluated with the same FP64 operations).
 //
-// Per level three steps: count (each lane walks and counts M2L, P2P and hand-downs per
-// child), exclusive scans (row starts of this level, inherited-list starts of the next),
-// write (each lane walks again and writes at offsets from the stored counts).
-// A walk needs no stack: it moves through the other cell's subtree by child and parent
-// links, so any depth works.
+// Per level, one thread per inherited pair (the pairs of all the level's nodes, in node
+// order): count (each walk counted per column: M2L, P2P, hand-downs to each child), an
+// exclusive scan of the counts (which gives every write position, the rows of a node being
+// its pairs' outputs in order), write (each walk again).  A walk needs no stack: it moves
+// through the other cell's subtree by child and parent links, so any depth works.
 
 #include "td3_common.h"
 #include "td3_sort.h"
 
 namespace {
 
-constexpr int kRowsWaves = 4;
 constexpr u32 kSecond = 0x80000000u;
 
 struct RowsTree {
@@ -68,16 +67,26 @@ __device__ __forceinline__ u32 entry_with(int x, bool second) {
     return (u32)x | (second ? kSecond : 0u);
 }
 
-// Walks one inherited entry of node a.  WRITE: emit at the given offsets (advanced).
+// Where a writing walk puts its outputs: the rows, the next level's pairs and their nodes,
+// and the offsets of a's children's lists.
+struct WriteTargets {
+    int* m2l_col;
+    int* p2p_col;
+    u32* inh_next;
+    int* enode_next;
+    const u32* child_at;
+};
+
+// Walks one inherited entry of node a, counting into c (and, WRITE, writing at the offsets
+// c holds, which it advances).
 template <bool WRITE>
-__device__ void walk_entry(const RowsTree& t, int a, int4 fa, u32 entry, double theta,
-                           Counts* c, int* m2l_col, int* p2p_col, u32* inh_out,
-                           const u32* child_base) {
+__device__ void walk_entry(const RowsTree& t, int a, int4 fa, u32 entry, double theta, Counts* c,
+                           const WriteTargets* wt) {
     if ((int)(entry & ~kSecond) == a) {
         // self pair {a, a}
         if (fa.y == fa.x) return;
         if (fa.z < 0) {
-            if (WRITE) p2p_col[c->p2p] = a;
+            if (WRITE) wt->p2p_col[c->p2p] = a;
             ++c->p2p;
             return;
         }
@@ -85,9 +94,11 @@ __device__ void walk_entry(const RowsTree& t, int a, int4 fa, u32 entry, double 
         // self pair, c_j (j > k) with k first, c_i (i < k) with k second
         for (int k = 0; k < 4; ++k) {
             if (WRITE) {
-                u32* out = inh_out + child_base[k] + c->inh[k];
-                for (int j = 0; j < 4; ++j)
-                    out[j] = (u32)(fa.z + j) | (j < k ? kSecond : 0u);
+                const u32 at = wt->child_at[k] + c->inh[k];
+                for (int j = 0; j < 4; ++j) {
+                    wt->inh_next[at + j] = (u32)(fa.z + j) | (j < k ? kSecond : 0u);
+                    wt->enode_next[at + j] = fa.z + k;
+                }
             }
             c->inh[k] += 4;
         }
@@ -105,14 +116,18 @@ __device__ void walk_entry(const RowsTree& t, int a, int4 fa, u32 entry, double 
             continue;
         }
         if (kind == 1) {
-            if (WRITE) m2l_col[c->m2l] = x;
+            if (WRITE) wt->m2l_col[c->m2l] = x;
             ++c->m2l;
         } else if (kind == 2) {
-            if (WRITE) p2p_col[c->p2p] = x;
+            if (WRITE) wt->p2p_col[c->p2p] = x;
             ++c->p2p;
         } else if (kind == 3) {
             for (int k = 0; k < 4; ++k) {
-                if (WRITE) inh_out[child_base[k] + c->inh[k]] = entry_with(x, second);
+                if (WRITE) {
+                    const u32 at = wt->child_at[k] + c->inh[k];
+                    wt->inh_next[at] = entry_with(x, second);
+                    wt->enode_next[at] = fa.z + k;
+                }
                 ++c->inh[k];
             }
         }
@@ -128,114 +143,194 @@ __device__ __forceinline__ u32 wave_sum_u32(u32 v) {
     return v;
 }
 
-__device__ __forceinline__ u32 wave_excl(u32 v, int lane, u32* total) {
-    u32 incl = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const u32 y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
-    }
-    *total = __shfl(incl, 63, 64);
-    return incl - v;
+// ------------------------------------------------------------------------------ per level
+// Entries of a level: inh[e] (the pair), enode[e] (its node), in node order; first[i] the
+// first entry of the level's i-th node (first[nodes] = the entry count); the count sits on
+// the device (lv[0]).  Every node has at least one entry (its self pair), so its first
+// entry's thread can write the node's row starts and its children's list starts.
+constexpr int kCols = 6;  // M2L, P2P, hand-downs to children 0..3
+constexpr unsigned kEntryBlocks = 2048;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr u32 kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ u32 level_entries(const u32* lv, u32 cap) {
+    const u32 n = lv[0];
+    return n < cap ? n : cap;
 }
 
-// Count step: per node of the level, its M2L and P2P row lengths and the number of
-// entries each child inherits.
-__global__ __launch_bounds__(64 * kRowsWaves) void rows_count_kernel(
-    RowsTree t, i64 lo, i64 hi, const u32* __restrict__ inh, const u32* __restrict__ inh_start,
-    double theta, u32* __restrict__ m2l_n, u32* __restrict__ p2p_n, u32* __restrict__ inh_n,
-    i64 next_lo, u32 cap_inh, u32* __restrict__ ecount, u32* __restrict__ flag) {
-    const int wave = threadIdx.x / 64, lane = td3_lane();
-    const i64 a = lo + (i64)blockIdx.x * kRowsWaves + wave;
-    if (a >= hi) return;  // uniform over the wavefront
-    // after an overflow at an earlier level the lists are incomplete: stop (the caller
-    // retries with more room, or takes the pair-list path)
+// Count step: every entry's walk, counted per column.
+__global__ __launch_bounds__(256) void entry_count_kernel(RowsTree t, const u32* __restrict__ inh,
+                                                          const int* __restrict__ enode,
+                                                          const u32* __restrict__ lv, u32 cap,
+                                                          double theta, u32* __restrict__ cnt,
+                                                          const u32* __restrict__ flag) {
     if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    const int4 fa = t.info[a];
-    // an overflowed list (flagged by the write step) is read only up to the capacity
-    const u32 s0 = inh_start[a - lo], s1 = inh_start[a - lo + 1];
-    const u32 e0 = s0 < cap_inh ? s0 : cap_inh, e1 = s1 < cap_inh ? s1 : cap_inh;
-    Counts tot = {0, 0, {0, 0, 0, 0}};
-    for (u32 b = e0; b < e1; b += 64) {
+    const u32 n = level_entries(lv, cap);
+    for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int a = enode[e];
         Counts c = {0, 0, {0, 0, 0, 0}};
-        if (b + lane < e1) {
-            walk_entry<false>(t, (int)a, fa, inh[b + lane], theta, &c, nullptr, nullptr,
-                              nullptr, nullptr);
-            u32* ec = ecount + 6 * (size_t)(b + lane);
-            ec[0] = c.m2l;
-            ec[1] = c.p2p;
-            for (int k = 0; k < 4; ++k) ec[2 + k] = c.inh[k];
-        }
-        tot.m2l += wave_sum_u32(c.m2l);
-        tot.p2p += wave_sum_u32(c.p2p);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) tot.inh[k] += wave_sum_u32(c.inh[k]);
+        walk_entry<false>(t, a, t.info[a], inh[e], theta, &c, nullptr);
+        u32* o = cnt + (size_t)kCols * e;
+        o[0] = c.m2l;
+        o[1] = c.p2p;
+        for (int k = 0; k < 4; ++k) o[2 + k] = c.inh[k];
     }
-    if (lane == 0) {
-        m2l_n[a - lo] = tot.m2l;
-        p2p_n[a - lo] = tot.p2p;
+}
+
+// Exclusive scan of the kCols columns over the level's entries, in three steps (sums of
+// tiles, scan of the sums, scan inside each tile); the entry count is read on the device.
+__global__ __launch_bounds__(kScanThreads) void col_sum_kernel(const u32* __restrict__ cnt,
+                                                               const u32* __restrict__ lv, u32 cap,
+                                                               u32* __restrict__ part) {
+    const u32 n = level_entries(lv, cap);
+    const u32 base = blockIdx.x * kScanTile;
+    if (base >= n) return;  // uniform over the block
+    __shared__ u32 s_sum[kScanThreads / 64][kCols];
+    u32 sum[kCols] = {0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < kScanItems; ++r) {
+        const u32 e = base + r * kScanThreads + threadIdx.x;
+        if (e < n)
+            for (int c = 0; c < kCols; ++c) sum[c] += cnt[(size_t)kCols * e + c];
+    }
+    for (int c = 0; c < kCols; ++c) sum[c] = wave_sum_u32(sum[c]);
+    if (td3_lane() == 0)
+        for (int c = 0; c < kCols; ++c) s_sum[threadIdx.x / 64][c] = sum[c];
+    __syncthreads();
+    if (threadIdx.x < kCols) {
+        u32 v = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) v += s_sum[w][threadIdx.x];
+        part[(size_t)kCols * blockIdx.x + threadIdx.x] = v;
+    }
+}
+
+// One block: exclusive scan of the tile sums per column (each thread a run of tiles);
+// totals[c] = the column's total.
+__global__ __launch_bounds__(256) void col_part_kernel(u32* __restrict__ part, const u32* __restrict__ lv,
+                                                       u32 cap, u32* __restrict__ totals) {
+    __shared__ u32 s_sum[256];
+    const u32 n = level_entries(lv, cap);
+    const u32 tiles = (n + kScanTile - 1) / kScanTile;
+    const u32 per = (tiles + 255) / 256;
+    const u32 i0 = threadIdx.x * per;
+    const u32 i1 = i0 + per < tiles ? i0 + per : tiles;
+    for (int c = 0; c < kCols; ++c) {
+        u32 sum = 0;
+        for (u32 i = i0; i < i1; ++i) sum += part[(size_t)kCols * i + c];
+        s_sum[threadIdx.x] = sum;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const u32 v = threadIdx.x >= (unsigned)off ? s_sum[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s_sum[threadIdx.x] += v;
+            __syncthreads();
+        }
+        u32 run = threadIdx.x > 0 ? s_sum[threadIdx.x - 1] : 0u;
+        for (u32 i = i0; i < i1; ++i) {
+            const u32 v = part[(size_t)kCols * i + c];
+            part[(size_t)kCols * i + c] = run;
+            run += v;
+        }
+        if (threadIdx.x == 255) totals[c] = s_sum[255];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void col_scan_kernel(u32* __restrict__ cnt,
+                                                                const u32* __restrict__ lv, u32 cap,
+                                                                const u32* __restrict__ part) {
+    const u32 n = level_entries(lv, cap);
+    const u32 base = blockIdx.x * kScanTile;
+    if (base >= n) return;  // uniform over the block
+    __shared__ u32 s_wave[kScanThreads / 64][kCols];
+    const int lane = td3_lane(), wave = threadIdx.x / 64;
+    u32 carry[kCols];
+    for (int c = 0; c < kCols; ++c) carry[c] = part[(size_t)kCols * blockIdx.x + c];
+    for (int r = 0; r < kScanItems; ++r) {
+        const u32 e = base + r * kScanThreads + threadIdx.x;
+        u32 v[kCols], incl[kCols];
+        for (int c = 0; c < kCols; ++c) {
+            v[c] = e < n ? cnt[(size_t)kCols * e + c] : 0u;
+            incl[c] = v[c];
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+            for (int c = 0; c < kCols; ++c) {
+                const u32 y = __shfl_up(incl[c], off, 64);
+                if (lane >= off) incl[c] += y;
+            }
+        if (lane == 63)
+            for (int c = 0; c < kCols; ++c) s_wave[wave][c] = incl[c];
+        __syncthreads();
+        for (int c = 0; c < kCols; ++c) {
+            u32 before = 0, all = 0;
+            for (int w = 0; w < kScanThreads / 64; ++w) {
+                const u32 y = s_wave[w][c];
+                before += w < wave ? y : 0u;
+                all += y;
+            }
+            if (e < n) cnt[(size_t)kCols * e + c] = carry[c] + before + incl[c] - v[c];
+            carry[c] += all;
+        }
+        __syncthreads();
+    }
+}
+
+// Write step: every entry's walk again, writing at its offsets; the first entry of a node
+// also writes the node's row starts and its children's list starts, entry 0 the next
+// level's entry count and row bases.
+__global__ __launch_bounds__(256) void entry_write_kernel(
+    RowsTree t, const u32* __restrict__ inh, const int* __restrict__ enode,
+    const u32* __restrict__ first, i64 lo, const u32* __restrict__ lv, u32* __restrict__ lv_next,
+    u32 cap, double theta, const u32* __restrict__ scan, const u32* __restrict__ totals,
+    u32* __restrict__ base, int level, u32* __restrict__ m2l_row, int* __restrict__ m2l_col,
+    u32* __restrict__ p2p_row, int* __restrict__ p2p_col, u32* __restrict__ inh_next,
+    int* __restrict__ enode_next, u32* __restrict__ first_next, i64 next_lo, i64 next_nodes,
+    u32 cap_m2l, u32 cap_p2p, u32* __restrict__ flag) {
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const u32 n = level_entries(lv, cap);
+    const u32 bm = base[2 * level], bp = base[2 * level + 1];
+    const u32 inh_total = totals[2] + totals[3] + totals[4] + totals[5];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        base[2 * level + 2] = bm + totals[0];
+        base[2 * level + 3] = bp + totals[1];
+        lv_next[0] = inh_total;
+        if (next_nodes > 0) first_next[next_nodes] = inh_total;
+    }
+    for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int a = enode[e];
+        const int4 fa = t.info[a];
+        const u32 f = first[a - lo], f1 = first[a - lo + 1];
+        const u32* se = scan + (size_t)kCols * e;
+        const u32* sf = scan + (size_t)kCols * f;
+        // children's lists: child k of a starts after every earlier child's entries
+        u32 child_at[4] = {0, 0, 0, 0};
+        if (fa.z >= 0) {
+            const u32* sl = f1 < n ? scan + (size_t)kCols * f1 : nullptr;
+            u32 start = sf[2] + sf[3] + sf[4] + sf[5];
+            for (int k = 0; k < 4; ++k) {
+                const u32 end_k = sl ? sl[2 + k] : totals[2 + k];
+                child_at[k] = start;
+                if (e == f) first_next[fa.z + k - next_lo] = start;
+                start += end_k - sf[2 + k];
+            }
+        }
+        if (e == f) {
+            m2l_row[a] = bm + sf[0];
+            p2p_row[a] = bp + sf[1];
+        }
+        Counts w = {bm + se[0], bp + se[1], {se[2] - sf[2], se[3] - sf[3], se[4] - sf[4], se[5] - sf[5]}};
+        // the outputs of this entry end where the next entry's begin
+        const u32* nx = e + 1 < n ? scan + (size_t)kCols * (e + 1) : totals;
+        bool fits = bm + nx[0] <= cap_m2l && bp + nx[1] <= cap_p2p;
         if (fa.z >= 0)
-            for (int k = 0; k < 4; ++k) inh_n[fa.z + k - next_lo] = tot.inh[k];
-    }
-}
-
-// Write step: every lane takes its entry's counts from the count step (its offsets inside
-// the node's rows and its children's lists) and walks the entry again, writing.
-__global__ __launch_bounds__(64 * kRowsWaves) void rows_write_kernel(
-    RowsTree t, i64 lo, i64 hi, const u32* __restrict__ inh, const u32* __restrict__ inh_start,
-    double theta, const u32* __restrict__ m2l_scan, const u32* __restrict__ p2p_scan,
-    const u32* __restrict__ base, int level, u32* __restrict__ m2l_row, int* __restrict__ m2l_col,
-    u32* __restrict__ p2p_row, int* __restrict__ p2p_col, u32* __restrict__ inh_out,
-    const u32* __restrict__ inh_out_start, i64 next_lo, u32 cap_m2l, u32 cap_p2p,
-    u32 cap_inh, const u32* __restrict__ ecount, u32* __restrict__ flag) {
-    const int wave = threadIdx.x / 64, lane = td3_lane();
-    const i64 a = lo + (i64)blockIdx.x * kRowsWaves + wave;
-    if (a >= hi) return;  // uniform over the wavefront
-    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    const int4 fa = t.info[a];
-    const u32 s0 = inh_start[a - lo], s1 = inh_start[a - lo + 1];
-    const u32 e0 = s0 < cap_inh ? s0 : cap_inh, e1 = s1 < cap_inh ? s1 : cap_inh;
-    u32 m2l_at = base[2 * level] + m2l_scan[a - lo];
-    u32 p2p_at = base[2 * level + 1] + p2p_scan[a - lo];
-    if (lane == 0) {
-        m2l_row[a] = m2l_at;
-        p2p_row[a] = p2p_at;
-    }
-    u32 child_at[4] = {0, 0, 0, 0};
-    if (fa.z >= 0)
-        for (int k = 0; k < 4; ++k) child_at[k] = inh_out_start[fa.z + k - next_lo];
-    for (u32 b = e0; b < e1; b += 64) {
-        const bool mine = b + lane < e1;
-        const u32 entry = mine ? inh[b + lane] : 0u;
-        Counts c = {0, 0, {0, 0, 0, 0}};
-        if (mine) {
-            const u32* ec = ecount + 6 * (size_t)(b + lane);
-            c.m2l = ec[0];
-            c.p2p = ec[1];
-            for (int k = 0; k < 4; ++k) c.inh[k] = ec[2 + k];
-        }
-        u32 tm, tp, ti[4];
-        const u32 om = wave_excl(c.m2l, lane, &tm);
-        const u32 op = wave_excl(c.p2p, lane, &tp);
-        u32 oi[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) oi[k] = wave_excl(c.inh[k], lane, &ti[k]);
-        const bool fits = m2l_at + tm <= cap_m2l && p2p_at + tp <= cap_p2p &&
-                          (fa.z < 0 || (child_at[0] + ti[0] <= cap_inh && child_at[1] + ti[1] <= cap_inh &&
-                                        child_at[2] + ti[2] <= cap_inh && child_at[3] + ti[3] <= cap_inh));
+            for (int k = 0; k < 4; ++k) fits = fits && child_at[k] + (nx[2 + k] - sf[2 + k]) <= cap;
         if (!fits) {
-            if (lane == 0) atomicOr(flag, 2u);
-            return;
+            atomicOr(flag, 2u);
+            continue;
         }
-        if (mine) {
-            Counts w = {m2l_at + om, p2p_at + op, {oi[0], oi[1], oi[2], oi[3]}};
-            walk_entry<true>(t, (int)a, fa, entry, theta, &w, m2l_col, p2p_col, inh_out,
-                             child_at);
-        }
-        m2l_at += tm;
-        p2p_at += tp;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) child_at[k] += ti[k];
+        WriteTargets wt = {m2l_col, p2p_col, inh_next, enode_next, child_at};
+        walk_entry<true>(t, a, fa, inh[e], theta, &w, &wt);
     }
 }
 
@@ -248,19 +343,13 @@ __global__ void rows_parent_kernel(const int4* __restrict__ info, i64 count, int
         for (int k = 0; k < 4; ++k) parent[c + k] = (int)node;
 }
 
-// base[2 (d + 1) + j] = base[2 d + j] + total_j (row starts of the next level).
-__global__ void rows_base_kernel(u32* base, int level, const u32* totals) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        base[2 * level + 2] = base[2 * level] + totals[0];
-        base[2 * level + 3] = base[2 * level + 1] + totals[1];
-    }
-}
-
-__global__ void rows_root_kernel(u32* inh, u32* inh_start, u32* base) {
+__global__ void rows_root_kernel(u32* inh, int* enode, u32* first, u32* lv, u32* base) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         inh[0] = 0u;  // the pair (root, root)
-        inh_start[0] = 0u;
-        inh_start[1] = 1u;
+        enode[0] = 0;
+        first[0] = 0u;
+        first[1] = 1u;
+        lv[0] = 1u;
         base[0] = 0u;
         base[1] = 0u;
     }
@@ -282,9 +371,8 @@ __global__ void rows_end_kernel(const u32* base, int levels, i64 count, u32* m2l
 }
 
 struct RowsWs {
-    u32 *inh_a, *inh_b, *start_a, *start_b, *m2l_n, *p2p_n, *m2l_scan, *p2p_scan, *totals,
-        *base, *flag, *scratch, *ecount;
-    int* parent;
+    u32 *inh_a, *inh_b, *first_a, *first_b, *cnt, *part, *totals, *lv, *base, *flag;
+    int *enode_a, *enode_b, *parent;
 };
 
 size_t rows_carve(RowsWs* w, void* p, i64 count, i64 width, i64 cap_inh, int levels) {
@@ -294,20 +382,20 @@ size_t rows_carve(RowsWs* w, void* p, i64 count, i64 width, i64 cap_inh, int lev
         off += (bytes + 255) & ~(size_t)255;
         return p ? (void*)((char*)p + o) : nullptr;
     };
+    const i64 tiles = (cap_inh + kScanTile - 1) / kScanTile;
     w->inh_a = (u32*)take(sizeof(u32) * (size_t)cap_inh);
     w->inh_b = (u32*)take(sizeof(u32) * (size_t)cap_inh);
-    w->start_a = (u32*)take(sizeof(u32) * (size_t)(width + 1));
-    w->start_b = (u32*)take(sizeof(u32) * (size_t)(width + 1));
-    w->m2l_n = (u32*)take(sizeof(u32) * (size_t)(width + 1));
-    w->p2p_n = (u32*)take(sizeof(u32) * (size_t)(width + 1));
-    w->m2l_scan = (u32*)take(sizeof(u32) * (size_t)(width + 1));
-    w->p2p_scan = (u32*)take(sizeof(u32) * (size_t)(width + 1));
-    w->totals = (u32*)take(sizeof(u32) * 4);
+    w->enode_a = (int*)take(sizeof(int) * (size_t)cap_inh);
+    w->enode_b = (int*)take(sizeof(int) * (size_t)cap_inh);
+    w->first_a = (u32*)take(sizeof(u32) * (size_t)(width + 1));
+    w->first_b = (u32*)take(sizeof(u32) * (size_t)(width + 1));
+    w->cnt = (u32*)take(sizeof(u32) * kCols * (size_t)cap_inh);
+    w->part = (u32*)take(sizeof(u32) * kCols * (size_t)tiles);
+    w->totals = (u32*)take(sizeof(u32) * kCols);
+    w->lv = (u32*)take(sizeof(u32) * 2);
     w->base = (u32*)take(sizeof(u32) * (size_t)(2 * levels + 4));
     w->flag = (u32*)take(sizeof(u32) * 4);
-    w->scratch = (u32*)take(td3sort::scratch_bytes(width + 1));
     w->parent = (int*)take(sizeof(int) * (size_t)count);
-    w->ecount = (u32*)take(sizeof(u32) * 6 * (size_t)cap_inh);
     return off;
 }
 
@@ -353,38 +441,45 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     hipLaunchKernelGGL(rows_parent_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s,
                        (const int4*)node_info, count, w.parent);
     TD3_LAUNCHED();
-    hipLaunchKernelGGL(rows_root_kernel, dim3(1), dim3(1), 0, s, w.inh_a, w.start_a, w.base);
+    hipLaunchKernelGGL(rows_root_kernel, dim3(1), dim3(1), 0, s, w.inh_a, w.enode_a, w.first_a,
+                       w.lv, w.base);
     TD3_LAUNCHED();
-    u32 *inh = w.inh_a, *inh_next = w.inh_b, *start = w.start_a, *start_next = w.start_b;
+    u32 *inh = w.inh_a, *inh_next = w.inh_b, *first = w.first_a, *first_next = w.first_b;
+    int *enode = w.enode_a, *enode_next = w.enode_b;
+    u32 *lv = w.lv, *lv_next = w.lv + 1;
+    const u32 cap = (u32)cap_inherit;
+    const unsigned tiles = (unsigned)((cap_inherit + kScanTile - 1) / kScanTile);
     for (int d = 0; d < n_levels; ++d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
         const i64 next_lo = hi, next_hi = d + 1 < n_levels ? levels_host[d + 2] : hi;
-        const i64 nodes = hi - lo, next = next_hi - next_lo;
-        const unsigned blocks = (unsigned)((nodes + kRowsWaves - 1) / kRowsWaves);
-        if (next > 0) TD3_HIP(hipMemsetAsync(start_next, 0, sizeof(u32) * (size_t)(next + 1), s));
-        TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(rows_count_kernel, dim3(blocks), dim3(64 * kRowsWaves), 0, s, t, lo, hi,
-                           inh, start, theta, w.m2l_n, w.p2p_n, start_next, next_lo,
-                           (u32)cap_inherit, w.ecount, w.flag));
+        const i64 next = next_hi - next_lo;
+        TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(entry_count_kernel, dim3(kEntryBlocks), dim3(256), 0, s, t, inh, enode, lv,
+                           cap, theta, w.cnt, w.flag));
         TD3_LAUNCHED();
-        if (int rc = td3sort::exclusive_scan_u32(w.m2l_n, w.m2l_scan, nodes, w.totals, w.scratch, s)) return rc;
-        if (int rc = td3sort::exclusive_scan_u32(w.p2p_n, w.p2p_scan, nodes, w.totals + 1, w.scratch, s)) return rc;
-        if (next > 0) {
-            // inherited-list starts of the next level, and the total at [next]
-            if (int rc = td3sort::exclusive_scan_u32(start_next, start_next, next + 1, nullptr, w.scratch, s)) return rc;
-        }
-        TD3_TIMED(s, "rows_write", hipLaunchKernelGGL(rows_write_kernel, dim3(blocks), dim3(64 * kRowsWaves), 0, s, t, lo, hi,
-                           inh, start, theta, w.m2l_scan, w.p2p_scan, w.base, d, m2l_row,
-                           (int*)m2l_col, p2p_row, (int*)p2p_col, inh_next, start_next, next_lo,
-                           (u32)cap_m2l, (u32)cap_p2p, (u32)cap_inherit, w.ecount, w.flag));
+        TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_sum_kernel, dim3(tiles), dim3(kScanThreads), 0, s, w.cnt, lv, cap, w.part));
         TD3_LAUNCHED();
-        hipLaunchKernelGGL(rows_base_kernel, dim3(1), dim3(1), 0, s, w.base, d, w.totals);
+        TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_part_kernel, dim3(1), dim3(256), 0, s, w.part, lv, cap, w.totals));
+        TD3_LAUNCHED();
+        TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_scan_kernel, dim3(tiles), dim3(kScanThreads), 0, s, w.cnt, lv, cap, w.part));
+        TD3_LAUNCHED();
+        TD3_TIMED(s, "rows_write", hipLaunchKernelGGL(entry_write_kernel, dim3(kEntryBlocks), dim3(256), 0, s, t, inh, enode, first,
+                           lo, lv, lv_next, cap, theta, w.cnt, w.totals, w.base, d, m2l_row,
+                           (int*)m2l_col, p2p_row, (int*)p2p_col, inh_next, enode_next,
+                           first_next, next_lo, next, (u32)cap_m2l, (u32)cap_p2p, w.flag));
         TD3_LAUNCHED();
         u32* tmp = inh;
         inh = inh_next;
         inh_next = tmp;
-        tmp = start;
-        start = start_next;
-        start_next = tmp;
+        tmp = first;
+        first = first_next;
+        first_next = tmp;
+        int* ti = enode;
+        enode = enode_next;
+        enode_next = ti;
+        tmp = lv;
+        lv = lv_next;
+        lv_next = tmp;
+        (void)hi;
     }
     hipLaunchKernelGGL(rows_end_kernel, dim3(1), dim3(1), 0, s, w.base, (int)n_levels, count,
                        m2l_row, p2p_row);
