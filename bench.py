@@ -27,6 +27,8 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# measured (rocprofv3 --pmc, calibrated by a 256 MiB copy) for this workload's M2L launch
+M2L_TRAFFIC_BYTES = 923.1e6
 KERNEL_NAMES = ("m2l", "m2l_fixup", "eval", "l2l", "rows_count", "rows_scan", "rows_write",
                 "traverse",
                 "row_sort", "lists_fill",
@@ -347,20 +349,25 @@ def main() -> int:
     near_pairs = int(np.sum(node_n[dst] * node_n[pcol_h]) - np.sum(node_n[node_n > 0]))
     m2l_directed = 2 * counts[0]
     flops = {"m2l": 910.0 * m2l_directed, "eval": 17.0 * near_pairs + 16 * 12 * n}
-    dom = max(("m2l", "eval"), key=lambda k: kernels.get(k, {"ms": 0})["ms"])
+    # the M2L kernel: the one dense contraction of the method, on the matrix cores
+    dom = "m2l"
     ms = kernels[dom]["ms"] / kernels[dom]["launches"]
     launches = kernels[dom]["launches"]
     achieved = flops[dom] / launches / (ms * 1e-3) / 1e12
     result["roofline"] = {
-        "kernel": dom,
+        "kernel": "m2l (m2l_mfma_kernel, FP32 on v_mfma_f32_16x16x4_f32; the FP64 top levels "
+                  "are a second, small launch)",
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
-        "traffic": None,
+        # HBM bytes of one m2l_mfma_kernel launch, rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE
+        # x 1.000 (factors from a 256 MiB copy in the same run)
+        "traffic": M2L_TRAFFIC_BYTES,
         "algorithmic": {"m2l_directed_pairs": m2l_directed, "near_directed_pairs": near_pairs,
-                        "flops_per_m2l_pair": 910, "flops_per_near_pair": 17},
+                        "flops_per_m2l_pair": 910, "flops_per_near_pair": 17,
+                        "traffic_source": "profiles/fmm_log_n1m_hbm_traffic.md"},
     }
     if not args.no_extras:
         result["extras"] = extras(td3)
