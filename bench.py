@@ -29,7 +29,7 @@ import __graft_entry__  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # measured (rocprofv3 --pmc, calibrated by a 256 MiB copy) for this workload's M2L launch
 M2L_TRAFFIC_BYTES = 923.1e6
-KERNEL_NAMES = ("m2l", "m2l_fixup", "eval", "l2l", "rows_count", "rows_scan", "rows_write",
+KERNEL_NAMES = ("m2l_mfma", "m2l", "m2l_fixup", "eval", "l2l", "rows_count", "rows_scan", "rows_write",
                 "traverse",
                 "row_sort", "lists_fill",
                 "lists_degree", "sort_scatter", "sort_hist", "scan", "tree_keys", "tree_count",
@@ -348,15 +348,16 @@ def main() -> int:
     dst = np.repeat(np.arange(len(prow_h) - 1), np.diff(prow_h))
     near_pairs = int(np.sum(node_n[dst] * node_n[pcol_h]) - np.sum(node_n[node_n > 0]))
     m2l_directed = 2 * counts[0]
-    flops = {"m2l": 910.0 * m2l_directed, "eval": 17.0 * near_pairs + 16 * 12 * n}
-    # the M2L kernel: the one dense contraction of the method, on the matrix cores
-    dom = "m2l"
-    ms = kernels[dom]["ms"] / kernels[dom]["launches"]
-    launches = kernels[dom]["launches"]
-    achieved = flops[dom] / launches / (ms * 1e-3) / 1e12
+    # the M2L on the matrix cores: destinations deeper than the FP64 top levels
+    fp64_top = tree.FP64_TOP_DEPTH
+    mrow_h = mrow.cpu().numpy().astype(np.int64)
+    first_fp32 = int(tree.levels[min(fp64_top + 1, len(tree.levels) - 1)])
+    mfma_pairs = int(mrow_h[-1] - mrow_h[first_fp32])
+    ms = kernels["m2l_mfma"]["ms"] / kernels["m2l_mfma"]["launches"]
+    achieved = 910.0 * mfma_pairs / (ms * 1e-3) / 1e12
     result["roofline"] = {
-        "kernel": "m2l (m2l_mfma_kernel, FP32 on v_mfma_f32_16x16x4_f32; the FP64 top levels "
-                  "are a second, small launch)",
+        "kernel": "m2l_mfma_kernel (FP32 M2L on v_mfma_f32_16x16x4_f32, destinations below "
+                  "the FP64 top levels)",
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
@@ -365,7 +366,8 @@ def main() -> int:
         # HBM bytes of one m2l_mfma_kernel launch, rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE
         # x 1.000 (factors from a 256 MiB copy in the same run)
         "traffic": M2L_TRAFFIC_BYTES,
-        "algorithmic": {"m2l_directed_pairs": m2l_directed, "near_directed_pairs": near_pairs,
+        "algorithmic": {"m2l_directed_pairs": m2l_directed, "mfma_directed_pairs": mfma_pairs,
+                        "launch_ms": ms, "near_directed_pairs": near_pairs,
                         "flops_per_m2l_pair": 910, "flops_per_near_pair": 17,
                         "traffic_source": "profiles/fmm_log_n1m_hbm_traffic.md"},
     }

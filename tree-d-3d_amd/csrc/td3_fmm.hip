@@ -620,7 +620,7 @@ int dispatch_m2l(hipStream_t s, int order, const int* col, const int* dst, const
     const int p1 = order + 1;
     if constexpr (sizeof(T) == 4) {
         if (p1 <= 16) {
-            TD3_TIMED(s, "m2l", hipLaunchKernelGGL(m2l_mfma_kernel, dim3(td3_blocks((node_hi - node_lo) * 64, 256)),
+            TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL(m2l_mfma_kernel, dim3(td3_blocks((node_hi - node_lo) * 64, 256)),
                                dim3(256), 0, s, col, row_start, node_lo, node_hi, dcenter, dhalf,
                                scenter, shalf, mre, mim, p1, coeff, lre, lim));
             TD3_LAUNCHED();
