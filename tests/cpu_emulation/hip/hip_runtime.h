@@ -345,24 +345,35 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     return td3emu_from<T>(all[td3emu::lane_id() ^ (unsigned)mask]);
 }
 
-// Matrix cores: v_mfma_f32_16x16x4_f32 on one wavefront.  Lane l holds A[l % 16][l / 16]
+// Matrix cores: v_mfma_{f32,f64}_16x16x4 on one wavefront.  Lane l holds A[l % 16][l / 16]
 // and B[l / 16][l % 16]; the result D[4 (l / 16) + r][l % 16] lands in element r of lane l.
-#define ext_vector_type(n) vector_size(sizeof(float) * (n))
-typedef float td3emu_f32x4 __attribute__((vector_size(16)));
-inline td3emu_f32x4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, td3emu_f32x4 c, int,
-                                                         int, int) {
+// ext_vector_type(4) becomes a 32-byte GCC vector: four doubles, or eight floats of which
+// the kernels use four.
+#define ext_vector_type(n) vector_size(8 * (n))
+typedef float td3emu_f32x4 __attribute__((vector_size(32)));
+typedef double td3emu_f64x4 __attribute__((vector_size(32)));
+template <typename V, typename T>
+inline V td3emu_mfma16(T a, T b, V c) {
     auto A = td3emu::exchange(td3emu_bits(a));
     auto B = td3emu::exchange(td3emu_bits(b));
     const unsigned lane = td3emu::lane_id();
     const unsigned col = lane % 16, row0 = 4 * (lane / 16);
-    td3emu_f32x4 d;
+    V d = c;
     for (unsigned r = 0; r < 4; ++r) {
-        float sum = c[r];
+        T sum = c[r];
         for (unsigned kk = 0; kk < 4; ++kk)
-            sum += td3emu_from<float>(A[kk * 16 + row0 + r]) * td3emu_from<float>(B[kk * 16 + col]);
+            sum += td3emu_from<T>(A[kk * 16 + row0 + r]) * td3emu_from<T>(B[kk * 16 + col]);
         d[r] = sum;
     }
     return d;
+}
+inline td3emu_f32x4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, td3emu_f32x4 c, int,
+                                                         int, int) {
+    return td3emu_mfma16<td3emu_f32x4, float>(a, b, c);
+}
+inline td3emu_f64x4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, td3emu_f64x4 c, int,
+                                                         int, int) {
+    return td3emu_mfma16<td3emu_f64x4, double>(a, b, c);
 }
 
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }

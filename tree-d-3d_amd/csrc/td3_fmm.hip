@@ -161,103 +161,138 @@ __global__ __launch_bounds__(256) void m2l_kernel(
 // A_0 ln t - S_0 (l = 0) and adds it to its running sum; the 16 lanes of each q are summed
 // at the end in a fixed butterfly, so the result is deterministic.
 typedef float td3_f32x4 __attribute__((ext_vector_type(4)));
+typedef double td3_f64x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ Cx<float> cpow_small(Cx<float> u, Cx<float> u2, Cx<float> u3, int q) {
+// The 16x16x4 product D += A B of one wavefront (A one element per lane, B one element
+// per lane, D four per lane), in FP32 or FP64.
+__device__ __forceinline__ td3_f32x4 mfma16(float a, float b, td3_f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ td3_f64x4 mfma16(double a, double b, td3_f64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+    typedef td3_f32x4 type;
+};
+template <>
+struct Vec4<double> {
+    typedef td3_f64x4 type;
+};
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cpow_small(Cx<T> u, Cx<T> u2, Cx<T> u3, int q) {
     // u^q for q in 0..3, selected without divergence
-    Cx<float> r = {1.0f, 0.0f};
+    Cx<T> r = {(T)1, (T)0};
     r = q == 1 ? u : r;
     r = q == 2 ? u2 : r;
     r = q == 3 ? u3 : r;
     return r;
 }
 
+__device__ __forceinline__ float rcp_of(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_of(double x) { return 1.0 / x; }
+
+template <typename T>
 __global__ __launch_bounds__(256) void m2l_mfma_kernel(
     const int* __restrict__ col, const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
     const double* __restrict__ dcenter, const double* __restrict__ dhalf,
     const double* __restrict__ scenter, const double* __restrict__ shalf,
-    const float* __restrict__ mre, const float* __restrict__ mim, int p1,
+    const T* __restrict__ mre, const T* __restrict__ mim, int p1,
     const double* __restrict__ coeff, double* __restrict__ lre, double* __restrict__ lim) {
+    typedef typename Vec4<T>::type V4;
+    constexpr bool kFP64 = sizeof(T) == 8;
     const int lane = td3_lane();
     const i64 d = node_lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
     if (d >= node_hi) return;  // uniform over the wavefront
     const u32 r0 = row_start[d], r1 = row_start[d + 1];
     if (r0 == r1) return;
     const int j = lane & 15, q = lane >> 4;
-    float ca[4];
+    T ca[4];
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
         const int k = 4 * kb + q;
-        ca[kb] = (k < p1 && j < p1) ? (float)coeff[k * p1 + j] : 0.0f;
+        ca[kb] = (k < p1 && j < p1) ? (T)coeff[k * p1 + j] : (T)0;
     }
     const double cx = dcenter[2 * d], cy = dcenter[2 * d + 1];
-    const float rt = (float)(1.4142135623730951 * dhalf[d]);
-    float acc_re[4] = {0.0f, 0.0f, 0.0f, 0.0f}, acc_im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const T rt = (T)(1.4142135623730951 * dhalf[d]);
+    T acc_re[4] = {(T)0, (T)0, (T)0, (T)0}, acc_im[4] = {(T)0, (T)0, (T)0, (T)0};
     for (u32 e0 = r0; e0 < r1; e0 += 16) {
         const u32 e = e0 + (u32)j;
         const bool valid = e < r1;
-        float x_re[4] = {0.0f, 0.0f, 0.0f, 0.0f}, x_im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        Cx<float> v = {0.0f, 0.0f};
-        float lnr = 0.0f;
+        T x_re[4] = {(T)0, (T)0, (T)0, (T)0}, x_im[4] = {(T)0, (T)0, (T)0, (T)0};
+        Cx<T> v = {(T)0, (T)0};
+        T lnr = (T)0, lni = (T)0;
         if (valid) {
             const u32 src = (u32)col[e];
-            // t = c_dst - c_src in FP64 (exact for nearby centres), then FP32
-            const float tr = (float)(cx - scenter[2 * src]);
-            const float ti = (float)(cy - scenter[2 * src + 1]);
-            const float t2 = fmaf(tr, tr, ti * ti);
-            const float inv = __builtin_amdgcn_rcpf(t2);
-            const float wr = tr * inv, wi = -ti * inv;
-            const float rs = (float)(1.4142135623730951 * shalf[src]);
-            const Cx<float> u = {rs * wr, rs * wi};
+            // t = c_dst - c_src in FP64 (exact for nearby centres)
+            const double tr64 = cx - scenter[2 * src], ti64 = cy - scenter[2 * src + 1];
+            const T tr = (T)tr64, ti = (T)ti64;
+            const T t2 = tr * tr + ti * ti;
+            const T inv = rcp_of(t2);
+            const T wr = tr * inv, wi = -ti * inv;
+            const T rs = (T)(1.4142135623730951 * shalf[src]);
+            const Cx<T> u = {rs * wr, rs * wi};
             v = {rt * wr, rt * wi};
-            const Cx<float> u2 = cmul(u, u);
-            const Cx<float> u3 = cmul(u2, u);
-            const Cx<float> u4 = cmul(u2, u2);
-            Cx<float> up = cpow_small(u, u2, u3, q);
+            const Cx<T> u2 = cmul(u, u);
+            const Cx<T> u3 = cmul(u2, u);
+            const Cx<T> u4 = cmul(u2, u2);
+            Cx<T> up = cpow_small(u, u2, u3, q);
             // rows of this lane's k = q + 4 kb: one address, immediate offsets
-            const float* ar = mre + (src * (u32)p1 + (u32)q);
-            const float* ai = mim + (src * (u32)p1 + (u32)q);
+            const T* ar = mre + (src * (u32)p1 + (u32)q);
+            const T* ai = mim + (src * (u32)p1 + (u32)q);
 #pragma unroll
             for (int kb = 0; kb < 4; ++kb) {
                 const int k = 4 * kb + q;
                 if (k < p1) {
-                    const Cx<float> x = cmul(Cx<float>{ar[4 * kb], ai[4 * kb]}, up);
+                    const Cx<T> x = cmul(Cx<T>{ar[4 * kb], ai[4 * kb]}, up);
                     x_re[kb] = x.re;
                     x_im[kb] = x.im;
                 }
                 up = cmul(up, u4);
             }
-            // ln |t| (v_log_f32 is log2)
-            lnr = 0.34657359027997264f * __builtin_amdgcn_logf(t2);
+            if (kFP64) {
+                if (q == 0) {
+                    lnr = (T)(0.5 * log(tr64 * tr64 + ti64 * ti64));
+                    lni = (T)atan2(ti64, tr64);
+                }
+            } else {
+                // ln |t| (v_log_f32 is log2)
+                lnr = (T)(0.34657359027997264f * __builtin_amdgcn_logf((float)t2));
+            }
         }
-        td3_f32x4 dre = {0.0f, 0.0f, 0.0f, 0.0f}, dim = {0.0f, 0.0f, 0.0f, 0.0f};
+        V4 dre = {(T)0, (T)0, (T)0, (T)0}, dim = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
-            dre = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[kb], x_re[kb], dre, 0, 0, 0);
-            dim = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[kb], x_im[kb], dim, 0, 0, 0);
+            dre = mfma16(ca[kb], x_re[kb], dre);
+            dim = mfma16(ca[kb], x_im[kb], dim);
         }
         if (valid) {
             // v^(4q), then one more factor of v per element
-            const Cx<float> v2 = cmul(v, v);
-            const Cx<float> v4 = cmul(v2, v2);
-            const Cx<float> v8 = cmul(v4, v4);
-            Cx<float> vp = {1.0f, 0.0f};
+            const Cx<T> v2 = cmul(v, v);
+            const Cx<T> v4 = cmul(v2, v2);
+            const Cx<T> v8 = cmul(v4, v4);
+            Cx<T> vp = {(T)1, (T)0};
             vp = q == 1 ? v4 : vp;
             vp = q == 2 ? v8 : vp;
             vp = q == 3 ? cmul(v8, v4) : vp;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int l = 4 * q + r;
-                const Cx<float> sl = {dre[r], dim[r]};
+                const Cx<T> sl = {dre[r], dim[r]};
                 if (l == 0) {
-                    // A_0 = X_0 (u^0 = 1 in lane q = 0) is real: the cell's charge.  Of
-                    // A_0 ln t only the real part A_0 ln|t| is kept: the imaginary part of
-                    // the constant term reaches neither the potential (-2 Re W), nor the
-                    // field (W'), nor, through L2L, any other coefficient.
-                    acc_re[0] += x_re[0] * lnr - sl.re;
-                    acc_im[0] -= sl.im;
+                    // A_0 = X_0 (u^0 = 1 in lane q = 0) is real: the cell's charge.  The
+                    // FP32 path keeps only the real part A_0 ln|t| of A_0 ln t: the
+                    // imaginary part of the constant term reaches neither the potential
+                    // (-2 Re W), nor the field (W'), nor, through L2L, any other coefficient.
+                    acc_re[0] += x_re[0] * lnr - x_im[0] * lni - sl.re;
+                    acc_im[0] += x_re[0] * lni + x_im[0] * lnr - sl.im;
                 } else if (l < p1) {
-                    const Cx<float> b = cmul(sl, vp);
-                    const float sign = (l & 1) ? 1.0f : -1.0f;
+                    const Cx<T> b = cmul(sl, vp);
+                    const T sign = (l & 1) ? (T)1 : (T)-1;
                     acc_re[r] += sign * b.re;
                     acc_im[r] += sign * b.im;
                 }
@@ -618,15 +653,6 @@ int dispatch_m2l(hipStream_t s, int order, const int* col, const int* dst, const
                  const double* dhalf, const double* scenter, const double* shalf, const T* mre,
                  const T* mim, const double* coeff, double* lre, double* lim, T* side) {
     const int p1 = order + 1;
-    if constexpr (sizeof(T) == 4) {
-        if (p1 <= 16) {
-            TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL(m2l_mfma_kernel, dim3(td3_blocks((node_hi - node_lo) * 64, 256)),
-                               dim3(256), 0, s, col, row_start, node_lo, node_hi, dcenter, dhalf,
-                               scenter, shalf, mre, mim, p1, coeff, lre, lim));
-            TD3_LAUNCHED();
-            return TD3_OK;
-        }
-    }
 #define TD3_M2L_CASE(PP)                                                                          \
     if (order <= PP)                                                                             \
         return launch_m2l<PP, T>(s, col, dst, row_start, e_lo, e_hi, node_lo, node_hi, dcenter,   \
@@ -681,6 +707,22 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
     TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
     hipStream_t s = (hipStream_t)stream;
     if (node_hi == node_lo) return TD3_OK;
+    if (order + 1 <= 16) {
+        // a wavefront per destination row on the matrix cores: no pair ranges needed
+        const dim3 grid(td3_blocks((node_hi - node_lo) * 64, 256));
+        if (precision == TD3_FP64)
+            TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<double>), grid, dim3(256), 0, s, col, row_start,
+                               node_lo, node_hi, dst_center, dst_half, src_center, src_half,
+                               (const double*)scaled_re, (const double*)scaled_im, (int)order + 1,
+                               coeff, local_re, local_im));
+        else
+            TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<float>), grid, dim3(256), 0, s, col, row_start,
+                               node_lo, node_hi, dst_center, dst_half, src_center, src_half,
+                               (const float*)scaled_re, (const float*)scaled_im, (int)order + 1,
+                               coeff, local_re, local_im));
+        TD3_LAUNCHED();
+        return TD3_OK;
+    }
     u32 bounds[2];
     TD3_HIP(hipMemcpyAsync(&bounds[0], row_start + node_lo, sizeof(u32), hipMemcpyDeviceToHost, s));
     TD3_HIP(hipMemcpyAsync(&bounds[1], row_start + node_hi, sizeof(u32), hipMemcpyDeviceToHost, s));
