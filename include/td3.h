@@ -182,8 +182,8 @@ int td3_traverse(const double *node_center, const double *node_half, const void 
  * rows hold the same pairs as td3_traverse + td3_lists (M2L both ways, mode 0; P2P both
  * ways with a self pair once, mode 1), each row in a fixed walk order (not sorted);
  * m2l_dst (may be NULL) gets the row's node of every M2L entry.  levels_host as
- * td3_moments.  out_counts (HOST int64[4]): M2L entries, P2P entries, stack overflow,
- * capacity overflow.  TD3_ERR_CAPACITY asks for larger cap_m2l / cap_p2p / cap_inherit
+ * td3_moments.  out_counts (HOST int64[4]): M2L entries, P2P entries, non-empty leaves
+ * (self pairs), capacity overflow.  TD3_ERR_CAPACITY asks for larger cap_m2l / cap_p2p / cap_inherit
  * (cap_inherit bounds the pairs handed to one level).  Any depth (the walks follow child
  * and parent links, no stack).  Synchronises the stream once.
  */

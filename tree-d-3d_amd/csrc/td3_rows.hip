@@ -334,13 +334,17 @@ __global__ __launch_bounds__(256) void entry_write_kernel(
     }
 }
 
-__global__ void rows_parent_kernel(const int4* __restrict__ info, i64 count, int* __restrict__ parent) {
+// Parent links, and the count of non-empty leaves (each holds one self pair in P2P).
+__global__ void rows_parent_kernel(const int4* __restrict__ info, i64 count, int* __restrict__ parent,
+                                   u32* __restrict__ leaves) {
     const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= count) return;
     if (node == 0) parent[0] = -1;
-    const int c = info[node].z;
-    if (c >= 0)
-        for (int k = 0; k < 4; ++k) parent[c + k] = (int)node;
+    const int4 f = info[node];
+    if (f.z >= 0)
+        for (int k = 0; k < 4; ++k) parent[f.z + k] = (int)node;
+    else if (f.y > f.x)
+        atomicAdd(leaves, 1u);
 }
 
 __global__ void rows_root_kernel(u32* inh, int* enode, u32* first, u32* lv, u32* base) {
@@ -439,7 +443,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     t.parent = w.parent;
     TD3_HIP(hipMemsetAsync(w.flag, 0, sizeof(u32) * 4, s));
     hipLaunchKernelGGL(rows_parent_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s,
-                       (const int4*)node_info, count, w.parent);
+                       (const int4*)node_info, count, w.parent, w.flag + 2);
     TD3_LAUNCHED();
     hipLaunchKernelGGL(rows_root_kernel, dim3(1), dim3(1), 0, s, w.inh_a, w.enode_a, w.first_a,
                        w.lv, w.base);
@@ -484,19 +488,19 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     hipLaunchKernelGGL(rows_end_kernel, dim3(1), dim3(1), 0, s, w.base, (int)n_levels, count,
                        m2l_row, p2p_row);
     TD3_LAUNCHED();
-    u32 host[4];
-    TD3_HIP(hipMemcpyAsync(host, w.flag, sizeof(u32) * 2, hipMemcpyDeviceToHost, s));
-    TD3_HIP(hipMemcpyAsync(host + 2, w.base + 2 * n_levels, sizeof(u32) * 2, hipMemcpyDeviceToHost, s));
+    u32 host[5];
+    TD3_HIP(hipMemcpyAsync(host, w.flag, sizeof(u32) * 3, hipMemcpyDeviceToHost, s));
+    TD3_HIP(hipMemcpyAsync(host + 3, w.base + 2 * n_levels, sizeof(u32) * 2, hipMemcpyDeviceToHost, s));
     TD3_HIP(hipStreamSynchronize(s));
-    out_counts[0] = host[2];
-    out_counts[1] = host[3];
-    out_counts[2] = 0;
+    out_counts[0] = host[3];
+    out_counts[1] = host[4];
+    out_counts[2] = host[2];
     out_counts[3] = host[0] & 2u ? 1 : 0;
     if (host[0] & 2u) {
-        td3_set_error("interaction rows overflowed (m2l %u, p2p %u)", host[2], host[3]);
+        td3_set_error("interaction rows overflowed (m2l %u, p2p %u)", host[3], host[4]);
         return TD3_ERR_CAPACITY;
     }
-    if (m2l_dst != nullptr && host[2] > 0) {
+    if (m2l_dst != nullptr && host[3] > 0) {
         hipLaunchKernelGGL(rows_dst_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, m2l_row,
                            count, (int*)m2l_dst);
         TD3_LAUNCHED();

@@ -244,14 +244,18 @@ class FlatQuadtree:
             raise ValueError("positions and charges must have the same length")
         self._d["pos"] = pos
         self._d["charge"] = charge
-        if n and not bool(torch.isfinite(pos).all()):
-            raise ValueError("Particle position must be finite")
         lo_x, hi_x = root_quad.xmid - root_quad.half, root_quad.xmid + root_quad.half
         lo_y, hi_y = root_quad.ymid - root_quad.half, root_quad.ymid + root_quad.half
-        if n and bool(
-            ((pos[0] < lo_x) | (pos[0] > hi_x) | (pos[1] < lo_y) | (pos[1] > hi_y)).any()
-        ):
-            raise ValueError("Particle lies outside tree bounds")
+        if n:
+            # both checks in one read-back
+            bad = torch.stack((
+                (~torch.isfinite(pos)).any(),
+                ((pos[0] < lo_x) | (pos[0] > hi_x) | (pos[1] < lo_y) | (pos[1] > hi_y)).any(),
+            )).cpu()
+            if bool(bad[0]):
+                raise ValueError("Particle position must be finite")
+            if bool(bad[1]):
+                raise ValueError("Particle lies outside tree bounds")
         self.leaf_capacity = int(leaf_capacity)
         self.max_depth = int(max_depth)
         self._root = (float(root_quad.xmid), float(root_quad.ymid), float(root_quad.half))
@@ -551,9 +555,7 @@ class FlatQuadtree:
                 continue
             _abi.check(code, "td3_rows")
             break
-        n_m2l, n_p2p = int(out[0]), int(out[1])
-        leaves = int(((self._d["node_child"] < 0)
-                      & (self._d["node_end"] > self._d["node_start"])).sum())
+        n_m2l, n_p2p, leaves = int(out[0]), int(out[1]), int(out[2])
         counts = (n_m2l // 2, (n_p2p + leaves) // 2)
         return (mrow, mcol[: max(n_m2l, 1)], mdst[: max(n_m2l, 1)]), \
             (prow, pcol[: max(n_p2p, 1)], None), counts
