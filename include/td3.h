@@ -222,8 +222,11 @@ int td3_scale_moments(int precision, const double *moment_re, const double *mome
  * row_start are td3_lists rows of the M2L pairs; coeff (order+1)^2 device table
  * (flat_tree.py:234-245, either truncation); scaled_re/im from td3_scale_moments of the
  * same precision; dst_* are the destination tree's cells (the rows), src_* the source
- * tree's (the columns; the same arrays for fmm, the target tree for fmm_at).  Rows of the range are WRITTEN (not accumulated); rows outside it are
- * untouched.  Reads two row offsets back (synchronises the stream).
+ * tree's (the columns; the same arrays for fmm, the target tree for fmm_at).  Rows of the
+ * range with pairs are WRITTEN (not accumulated); other rows are untouched.  FP32 up to
+ * order 15 runs on the matrix cores, a wavefront per row; otherwise a lane per pair, with
+ * the workspace (td3_m2l_workspace_bytes of at least the range's pair count) holding the
+ * partial sums of rows that span several wavefronts.  No host read-back.
  */
 size_t td3_m2l_workspace_bytes(int64_t pair_count, int64_t order, int precision);
 int td3_m2l(int precision, int64_t order, const int32_t *col, const int32_t *dst,

@@ -346,7 +346,9 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
 }
 
 // Matrix cores: v_mfma_{f32,f64}_16x16x4 on one wavefront.  Lane l holds A[l % 16][l / 16]
-// and B[l / 16][l % 16]; the result D[4 (l / 16) + r][l % 16] lands in element r of lane l.
+// and B[l / 16][l % 16]; element r of lane l of the result is D[4 (l / 16) + r][l % 16] for
+// f32 and D[l / 16 + 4 r][l % 16] for f64 (the layouts measured on gfx950 with one-hot
+// operands).
 // ext_vector_type(4) becomes a 32-byte GCC vector: four doubles, or eight floats of which
 // the kernels use four.
 #define ext_vector_type(n) vector_size(8 * (n))
@@ -357,12 +359,13 @@ inline V td3emu_mfma16(T a, T b, V c) {
     auto A = td3emu::exchange(td3emu_bits(a));
     auto B = td3emu::exchange(td3emu_bits(b));
     const unsigned lane = td3emu::lane_id();
-    const unsigned col = lane % 16, row0 = 4 * (lane / 16);
+    const unsigned col = lane % 16;
     V d = c;
     for (unsigned r = 0; r < 4; ++r) {
+        const unsigned row = sizeof(T) == 8 ? lane / 16 + 4 * r : 4 * (lane / 16) + r;
         T sum = c[r];
         for (unsigned kk = 0; kk < 4; ++kk)
-            sum += td3emu_from<T>(A[kk * 16 + row0 + r]) * td3emu_from<T>(B[kk * 16 + col]);
+            sum += td3emu_from<T>(A[kk * 16 + row]) * td3emu_from<T>(B[kk * 16 + col]);
         d[r] = sum;
     }
     return d;

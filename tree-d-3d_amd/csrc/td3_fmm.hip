@@ -37,22 +37,14 @@ __device__ __forceinline__ float shfl_up_d(float v, int off) { return __shfl_up(
 
 // ------------------------------------------------------------------------------ M2L
 template <int P, typename T>
-__global__ __launch_bounds__(256) void m2l_kernel(
+__device__ __forceinline__ void m2l_chunk(
     const int* __restrict__ col, const int* __restrict__ dst, const u32* __restrict__ row_start,
-    u32 e_lo, u32 e_hi, const double* __restrict__ dcenter, const double* __restrict__ dhalf,
-    const double* __restrict__ scenter, const double* __restrict__ shalf,
-    const T* __restrict__ mre, const T* __restrict__ mim, int p1, const double* __restrict__ coeff,
-    double* __restrict__ lre, double* __restrict__ lim, T* __restrict__ side) {
-    __shared__ T s_coeff[(P + 1) * (P + 1)];
-    for (int i = threadIdx.x; i < (P + 1) * (P + 1); i += blockDim.x) {
-        const int k = i / (P + 1), l = i % (P + 1);
-        s_coeff[i] = (k < p1 && l < p1) ? (T)coeff[k * p1 + l] : (T)0;
-    }
-    __syncthreads();
+    u32 w0, u32 e_hi, u32 chunk, const double* __restrict__ dcenter,
+    const double* __restrict__ dhalf, const double* __restrict__ scenter,
+    const double* __restrict__ shalf, const T* __restrict__ mre, const T* __restrict__ mim,
+    int p1, const T* s_coeff, double* __restrict__ lre, double* __restrict__ lim,
+    T* __restrict__ side) {
     const int lane = td3_lane();
-    const u32 wave = (u32)((blockIdx.x * blockDim.x + threadIdx.x) / 64);
-    const u32 w0 = e_lo + wave * 64;
-    if (w0 >= e_hi) return;
     const u32 e = w0 + lane;
     const bool valid = e < e_hi;
     const int d = valid ? dst[e] : -1 - lane;
@@ -136,18 +128,52 @@ __global__ __launch_bounds__(256) void m2l_kernel(
     const u32 rs0 = row_start[d], rs1 = row_start[d + 1];
     const bool before = rs0 < w0;
     const bool after = rs1 > w0 + 64;
+    // (loops over the register array with a compile-time bound keep it in registers)
     if (!before && !after) {
-        for (int l = 0; l < p1; ++l) {
-            lre[(i64)d * p1 + l] = (double)b[l].re;
-            lim[(i64)d * p1 + l] = (double)b[l].im;
+#pragma unroll
+        for (int l = 0; l <= P; ++l) {
+            if (l < p1) {
+                lre[(i64)d * p1 + l] = (double)b[l].re;
+                lim[(i64)d * p1 + l] = (double)b[l].im;
+            }
         }
     } else {
-        const i64 slot = ((i64)((w0 - e_lo) / 64) * 2 + (before ? 0 : 1)) * 2 * p1;
-        for (int l = 0; l < p1; ++l) {
-            side[slot + 2 * l] = b[l].re;
-            side[slot + 2 * l + 1] = b[l].im;
+        const i64 slot = ((i64)chunk * 2 + (before ? 0 : 1)) * 2 * p1;
+#pragma unroll
+        for (int l = 0; l <= P; ++l) {
+            if (l < p1) {
+                side[slot + 2 * l] = b[l].re;
+                side[slot + 2 * l + 1] = b[l].im;
+            }
         }
     }
+}
+
+// One lane per pair, 64 consecutive pairs (a chunk) per wavefront, over the rows
+// [node_lo, node_hi) (their bounds read on the device; the grid covers at most max_chunks
+// chunks, and wavefronts past the range exit).  A chunk's rows that end inside it are
+// written directly; the partial sums of rows that cross a chunk boundary go to side (two
+// slots per chunk) for m2l_fixup_kernel.
+template <int P, typename T>
+__global__ __launch_bounds__(256) void m2l_kernel(
+    const int* __restrict__ col, const int* __restrict__ dst, const u32* __restrict__ row_start,
+    i64 node_lo, i64 node_hi, u32 max_chunks, const double* __restrict__ dcenter,
+    const double* __restrict__ dhalf, const double* __restrict__ scenter,
+    const double* __restrict__ shalf, const T* __restrict__ mre, const T* __restrict__ mim,
+    int p1, const double* __restrict__ coeff, double* __restrict__ lre, double* __restrict__ lim,
+    T* __restrict__ side) {
+    const u32 e_lo = row_start[node_lo], e_hi = row_start[node_hi];
+    if (e_lo + (u64)blockIdx.x * (blockDim.x / 64) * 64 >= e_hi) return;  // the whole block
+    __shared__ T s_coeff[(P + 1) * (P + 1)];
+    for (int i = threadIdx.x; i < (P + 1) * (P + 1); i += blockDim.x) {
+        const int k = i / (P + 1), l = i % (P + 1);
+        s_coeff[i] = (k < p1 && l < p1) ? (T)coeff[k * p1 + l] : (T)0;
+    }
+    __syncthreads();
+    const u32 chunk = (u32)((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+    if (chunk >= max_chunks || e_lo + (u64)chunk * 64 >= e_hi) return;
+    m2l_chunk<P, T>(col, dst, row_start, e_lo + chunk * 64, e_hi, chunk, dcenter, dhalf, scenter,
+                    shalf, mre, mim, p1, s_coeff, lre, lim, side);
 }
 
 // ------------------------------------------------------------------------------ M2L on MFMA
@@ -211,6 +237,9 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
     const u32 r0 = row_start[d], r1 = row_start[d + 1];
     if (r0 == r1) return;
     const int j = lane & 15, q = lane >> 4;
+    // Element r of lane (j, q) of the product holds D[row_of(r)][j]:
+    // v_mfma_f32_16x16x4f32 gives row 4 q + r, v_mfma_f64_16x16x4f64 row q + 4 r.
+    auto row_of = [q](int r) { return kFP64 ? q + 4 * r : 4 * q + r; };
     T ca[4];
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
@@ -271,17 +300,24 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
             dim = mfma16(ca[kb], x_im[kb], dim);
         }
         if (valid) {
-            // v^(4q), then one more factor of v per element
+            // v^l for the first element's row l, then the step to the next element's
             const Cx<T> v2 = cmul(v, v);
             const Cx<T> v4 = cmul(v2, v2);
-            const Cx<T> v8 = cmul(v4, v4);
-            Cx<T> vp = {(T)1, (T)0};
-            vp = q == 1 ? v4 : vp;
-            vp = q == 2 ? v8 : vp;
-            vp = q == 3 ? cmul(v8, v4) : vp;
+            Cx<T> vp, step;
+            if (kFP64) {
+                vp = cpow_small(v, v2, cmul(v2, v), q);
+                step = v4;
+            } else {
+                const Cx<T> v8 = cmul(v4, v4);
+                vp = {(T)1, (T)0};
+                vp = q == 1 ? v4 : vp;
+                vp = q == 2 ? v8 : vp;
+                vp = q == 3 ? cmul(v8, v4) : vp;
+                step = v;
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int l = 4 * q + r;
+                const int l = row_of(r);
                 const Cx<T> sl = {dre[r], dim[r]};
                 if (l == 0) {
                     // A_0 = X_0 (u^0 = 1 in lane q = 0) is real: the cell's charge.  The
@@ -296,7 +332,7 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
                     acc_re[r] += sign * b.re;
                     acc_im[r] += sign * b.im;
                 }
-                vp = cmul(vp, v);
+                vp = cmul(vp, step);
             }
         }
     }
@@ -312,7 +348,7 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
     if (j == 0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int l = 4 * q + r;
+            const int l = row_of(r);
             if (l < p1) {
                 lre[d * p1 + l] = (double)acc_re[r];
                 lim[d * p1 + l] = (double)acc_im[r];
@@ -325,12 +361,13 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
 // plus the first-segment partials of the following waves, in order.
 template <typename T>
 __global__ void m2l_fixup_kernel(const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
-                                 u32 e_lo, int p1, const T* __restrict__ side,
-                                 double* __restrict__ lre, double* __restrict__ lim) {
+                                 int p1, const T* __restrict__ side, double* __restrict__ lre,
+                                 double* __restrict__ lim) {
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     const i64 node = node_lo + t / (2 * p1);
     const int j = (int)(t % (2 * p1));
     if (node >= node_hi) return;
+    const u32 e_lo = row_start[node_lo];
     const u32 rs0 = row_start[node], rs1 = row_start[node + 1];
     if (rs1 == rs0) return;
     const u32 wa = (rs0 - e_lo) / 64, wb = (rs1 - 1 - e_lo) / 64;
@@ -630,32 +667,30 @@ int launch_eval(int precision, hipStream_t s, const double* tpos, const i64* tpe
 }
 
 template <int P, typename T>
-int launch_m2l(hipStream_t s, const int* col, const int* dst, const u32* row_start, u32 e_lo,
-               u32 e_hi, i64 node_lo, i64 node_hi, const double* dcenter, const double* dhalf,
+int launch_m2l(hipStream_t s, const int* col, const int* dst, const u32* row_start, i64 node_lo,
+               i64 node_hi, u32 max_chunks, const double* dcenter, const double* dhalf,
                const double* scenter, const double* shalf, const T* mre, const T* mim, int p1,
                const double* coeff, double* lre, double* lim, T* side) {
-    if (e_hi <= e_lo) return TD3_OK;
-    const i64 waves = ((i64)(e_hi - e_lo) + 63) / 64;
-    TD3_TIMED(s, "m2l", hipLaunchKernelGGL((m2l_kernel<P, T>), dim3(td3_blocks(waves * 64, 256)), dim3(256), 0, s, col, dst,
-                       row_start, e_lo, e_hi, dcenter, dhalf, scenter, shalf, mre, mim, p1, coeff, lre,
-                       lim, side));
+    TD3_TIMED(s, "m2l", hipLaunchKernelGGL((m2l_kernel<P, T>), dim3(td3_blocks((i64)max_chunks * 64, 256)), dim3(256), 0, s, col, dst,
+                       row_start, node_lo, node_hi, max_chunks, dcenter, dhalf, scenter, shalf, mre,
+                       mim, p1, coeff, lre, lim, side));
     TD3_LAUNCHED();
     const i64 items = (node_hi - node_lo) * 2 * p1;
     TD3_TIMED(s, "m2l_fixup", hipLaunchKernelGGL((m2l_fixup_kernel<T>), dim3(td3_blocks(items, 256)), dim3(256), 0, s, row_start,
-                       node_lo, node_hi, e_lo, p1, side, lre, lim));
+                       node_lo, node_hi, p1, side, lre, lim));
     TD3_LAUNCHED();
     return TD3_OK;
 }
 
 template <typename T>
 int dispatch_m2l(hipStream_t s, int order, const int* col, const int* dst, const u32* row_start,
-                 u32 e_lo, u32 e_hi, i64 node_lo, i64 node_hi, const double* dcenter,
+                 i64 node_lo, i64 node_hi, u32 max_chunks, const double* dcenter,
                  const double* dhalf, const double* scenter, const double* shalf, const T* mre,
                  const T* mim, const double* coeff, double* lre, double* lim, T* side) {
     const int p1 = order + 1;
 #define TD3_M2L_CASE(PP)                                                                          \
     if (order <= PP)                                                                             \
-        return launch_m2l<PP, T>(s, col, dst, row_start, e_lo, e_hi, node_lo, node_hi, dcenter,   \
+        return launch_m2l<PP, T>(s, col, dst, row_start, node_lo, node_hi, max_chunks, dcenter,   \
                                  dhalf, scenter, shalf, mre, mim, p1, coeff, lre, lim, side);
     TD3_M2L_CASE(4)
     TD3_M2L_CASE(8)
@@ -707,36 +742,32 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
     TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
     hipStream_t s = (hipStream_t)stream;
     if (node_hi == node_lo) return TD3_OK;
-    if (order + 1 <= 16) {
-        // a wavefront per destination row on the matrix cores: no pair ranges needed
+    if (precision != TD3_FP64 && order + 1 <= 16) {
+        // FP32: a wavefront per destination row on the matrix cores.  (FP64 stays on the
+        // lane-per-pair kernel: on gfx950 v_mfma_f64_16x16x4f64 runs at the vector FP64
+        // rate, and the lane kernel does less work around the product.)
         const dim3 grid(td3_blocks((node_hi - node_lo) * 64, 256));
-        if (precision == TD3_FP64)
-            TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<double>), grid, dim3(256), 0, s, col, row_start,
-                               node_lo, node_hi, dst_center, dst_half, src_center, src_half,
-                               (const double*)scaled_re, (const double*)scaled_im, (int)order + 1,
-                               coeff, local_re, local_im));
-        else
-            TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<float>), grid, dim3(256), 0, s, col, row_start,
-                               node_lo, node_hi, dst_center, dst_half, src_center, src_half,
-                               (const float*)scaled_re, (const float*)scaled_im, (int)order + 1,
-                               coeff, local_re, local_im));
+        TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<float>), grid, dim3(256), 0, s, col, row_start,
+                           node_lo, node_hi, dst_center, dst_half, src_center, src_half,
+                           (const float*)scaled_re, (const float*)scaled_im, (int)order + 1,
+                           coeff, local_re, local_im));
         TD3_LAUNCHED();
         return TD3_OK;
     }
-    u32 bounds[2];
-    TD3_HIP(hipMemcpyAsync(&bounds[0], row_start + node_lo, sizeof(u32), hipMemcpyDeviceToHost, s));
-    TD3_HIP(hipMemcpyAsync(&bounds[1], row_start + node_hi, sizeof(u32), hipMemcpyDeviceToHost, s));
-    TD3_HIP(hipStreamSynchronize(s));
-    const i64 pairs = (i64)bounds[1] - (i64)bounds[0];
-    if (pairs <= 0) return TD3_OK;
-    TD3_ARG(workspace_bytes >= td3_m2l_workspace_bytes(pairs, order, precision), "workspace too small");
+    // the side buffer bounds the chunks: td3_m2l_workspace_bytes(pairs) holds two slots
+    // per 64 pairs (+1) of the rows [node_lo, node_hi)
+    const size_t word = precision == TD3_FP64 ? 8 : 4;
+    const size_t slot_pair = 2 * 2 * (size_t)(order + 1) * word;
+    TD3_ARG(workspace != nullptr && workspace_bytes >= slot_pair + 256, "workspace too small");
+    const size_t chunks = (workspace_bytes - 256) / slot_pair;
+    const u32 max_chunks = chunks > 0xffffffffull ? 0xffffffffu : (u32)chunks;
     if (precision == TD3_FP64)
-        return dispatch_m2l<double>(s, (int)order, col, dst, row_start, bounds[0], bounds[1], node_lo,
-                                    node_hi, dst_center, dst_half, src_center, src_half,
+        return dispatch_m2l<double>(s, (int)order, col, dst, row_start, node_lo, node_hi,
+                                    max_chunks, dst_center, dst_half, src_center, src_half,
                                     (const double*)scaled_re, (const double*)scaled_im, coeff,
                                     local_re, local_im, (double*)workspace);
-    return dispatch_m2l<float>(s, (int)order, col, dst, row_start, bounds[0], bounds[1], node_lo,
-                               node_hi, dst_center, dst_half, src_center, src_half,
+    return dispatch_m2l<float>(s, (int)order, col, dst, row_start, node_lo, node_hi, max_chunks,
+                               dst_center, dst_half, src_center, src_half,
                                (const float*)scaled_re, (const float*)scaled_im, coeff, local_re,
                                local_im, (float*)workspace);
 }
