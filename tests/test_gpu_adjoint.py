@@ -101,3 +101,31 @@ def test_energy_gradient_is_minus_the_force(td3):
     _, field = tree.fmm(softening=1e-3)
     force = q_h * field
     np.testing.assert_allclose(-g.cpu().numpy(), force, rtol=0, atol=1e-10 * np.abs(force).max())
+
+
+def test_symmetric_fmm_forces_are_the_energy_gradient(td3):
+    # smooth.py:411-422 at band 0 and tests/test_smooth.py:27-37: with the symmetric
+    # truncation (k + l <= p) the tree's forces are -dU/dx, U = 1/2 sum q phi, to central
+    # differences (h = 1e-6) within 1e-7 max|F|
+    rng = np.random.default_rng(7)
+    pos = rng.uniform(-0.9, 0.9, (2, 3000))
+    q = rng.choice([-1.0, 1.0], 3000)
+    kw = dict(kernel="log", order=12, symmetric=True, leaf_capacity=8)
+
+    def energy(p):
+        tree = td3.FlatQuadtree(p, q, td3.Quad(0, 0, 1), **kw)
+        phi, _ = tree.fmm(theta=0.5, softening=0.02)
+        return 0.5 * float(np.dot(q, phi))
+
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), **kw)
+    _, field = tree.fmm(theta=0.5, softening=0.02)
+    force = q * field
+    scale = np.abs(force).max()
+    h = 1e-6
+    for i in rng.choice(3000, 6, replace=False):
+        for axis in range(2):
+            pp, pm = pos.copy(), pos.copy()
+            pp[axis, i] += h
+            pm[axis, i] -= h
+            fd = -(energy(pp) - energy(pm)) / (2 * h)
+            assert abs(fd - force[axis, i]) < 1e-7 * scale, (i, axis, fd, force[axis, i])
