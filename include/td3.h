@@ -267,6 +267,50 @@ int td3_fmm_eval(int kernel, int precision, int64_t order, const double *sorted_
                  double *field, int64_t *status, void *stream);
 
 /*
+ * Log-kernel moments of sources that carry dipoles as well as charges: td3_moments
+ * (kernel TD3_LOG) plus a_n -= n D (z - c)^(n-1) per source, the moments of a dipole's
+ * potential -2 D.(z - z_s) / |z - z_s|^2 (sorted_dipole (2, N): x components, then y, in
+ * tree order).  No counterpart in the reference (which has no gradients): the source
+ * side of the transposed FMM for losses on fields (their gradient is a field of dipoles).
+ */
+int td3_log_moments_dipole(int64_t order, const double *sorted_pos, const double *sorted_charge,
+                           const double *sorted_dipole, int64_t n, const double *node_center,
+                           const void *node_info, int64_t node_count, const int64_t *levels_host,
+                           int64_t n_levels, const double *binomial, int64_t bsz,
+                           double *node_qabs, double *moment_re, double *moment_im, void *stream);
+
+/*
+ * td3_fmm_eval for the log kernel in FP64 with the adjoint's extras; no counterpart in the
+ * reference.  sorted_dipole (2, N, tree order, may be NULL): the sources' dipoles, near
+ * field -2 D.d / (r^2 + eps^2) in phi and 2 D / (r^2 + eps^2) - 4 (D.d) d / (r^2 + eps^2)^2
+ * in field (d = target - source; the far field comes with td3_log_moments_dipole's
+ * moments).  jacobian (3, N, may be NULL; not with dipoles): rows xx, xy, yy of dE/dx at
+ * every target, far field 2 W'' and near field 2 q (I / (r^2 + eps^2) - 2 d d^T /
+ * (r^2 + eps^2)^2) (the second-derivative kernel of the adjoint of a loss on fields).
+ */
+int td3_log_fmm_eval_ext(int64_t order, const double *sorted_pos, const int64_t *perm, int64_t n,
+                         const double *node_center, const double *node_half,
+                         const void *node_info, const double *local_re, const double *local_im,
+                         const uint32_t *p2p_row, const int32_t *p2p_col, const int32_t *leaf_of,
+                         const void *sources4, const double *sorted_dipole, double softening,
+                         double *phi, double *field, double *jacobian, int64_t *status,
+                         void *stream);
+
+/*
+ * td3_fmm_at_eval for the log kernel with td3_log_fmm_eval_ext's dipoles and Jacobian
+ * (jacobian (3, M) in target order); no counterpart in the reference.
+ */
+int td3_log_fmm_at_eval_ext(int64_t order, const double *sorted_targets,
+                            const int64_t *target_perm, const int64_t *exclude, int64_t m,
+                            const double *t_center, const double *t_half,
+                            const int32_t *t_leaf_of, const double *local_re,
+                            const double *local_im, const uint32_t *p2p_row,
+                            const int32_t *p2p_col, const int64_t *perm, int64_t n,
+                            const void *s_info, const void *s_sources4,
+                            const double *sorted_dipole, double softening, double *phi,
+                            double *field, double *jacobian, int64_t *status, void *stream);
+
+/*
  * (x, y, q, 0) float64 per sorted source (N x 32 bytes): the near field's source records
  * (td3_fmm_eval's sources4, td3_fmm_at_eval's s_sources4).  No counterpart in the
  * reference (it reads the (2, N) positions and the charges apart).

@@ -283,6 +283,76 @@ def case_rows():
                                       np.repeat(np.arange(t.node_count), np.diff(m_row.numpy())))
 
 
+def direct_log(targets, pos, q, dip=None, own=False):
+    """Potential, field and field Jacobian (xx, xy, yy) of log charges and dipoles, FP64
+    brute force (own index skipped when the targets are the sources)."""
+    d = targets[:, :, None] - pos[:, None, :]
+    r2 = d[0] ** 2 + d[1] ** 2
+    if own:
+        np.fill_diagonal(r2, np.inf)
+    inv = 1.0 / r2
+    phi = -(q * np.where(np.isinf(r2), 0.0, np.log(np.where(np.isinf(r2), 1.0, r2)))).sum(1)
+    field = (2 * q * inv * d).sum(2)
+    jac = np.stack(((2 * q * (inv - 2 * d[0] ** 2 * inv ** 2)).sum(1),
+                    (-4 * q * d[0] * d[1] * inv ** 2).sum(1),
+                    (2 * q * (inv - 2 * d[1] ** 2 * inv ** 2)).sum(1)))
+    if dip is not None:
+        pd = dip[0] * d[0] + dip[1] * d[1]
+        phi = phi - (2 * pd * inv).sum(1)
+        field = field + (2 * dip[:, None, :] * inv - 4 * pd * d * inv ** 2).sum(2)
+    return phi, field, jac
+
+
+def case_adjoint():
+    pos, q = cloud(700, 11)
+    rng = np.random.default_rng(12)
+    dip = rng.normal(0, 1, pos.shape)
+    probes = rng.uniform(-1, 1, (2, 300))
+    t = td3.FlatQuadtree(torch.tensor(pos), torch.tensor(q), td3.Quad(0, 0, 1), kernel="log",
+                         order=16, leaf_capacity=8)
+    phi, field, jac = (x.numpy() for x in t.fmm(theta=0.4, jacobian=True))
+    rphi, rfield, rjac = direct_log(pos, pos, q, own=True)
+    assert relative(phi, rphi) < 1e-8 and relative(field, rfield) < 1e-8
+    assert relative(jac, rjac) < 1e-7, relative(jac, rjac)
+    phi, field, jac = (x.numpy() for x in t.fmm_at(torch.tensor(probes), theta=0.4,
+                                                   jacobian=True))
+    rphi, rfield, rjac = direct_log(probes, pos, q)
+    assert relative(phi, rphi) < 1e-8 and relative(field, rfield) < 1e-8
+    assert relative(jac, rjac) < 1e-7, relative(jac, rjac)
+    t.set_charges(torch.tensor(q), dipoles=torch.tensor(dip))
+    phi, field = (x.numpy() for x in t.fmm(theta=0.4))
+    rphi, rfield, _ = direct_log(pos, pos, q, dip, own=True)
+    assert relative(phi, rphi) < 1e-7 and relative(field, rfield) < 1e-7, (
+        relative(phi, rphi), relative(field, rfield))
+    phi, field = (x.numpy() for x in t.fmm_at(torch.tensor(probes), theta=0.4))
+    rphi, rfield, _ = direct_log(probes, pos, q, dip)
+    assert relative(phi, rphi) < 1e-7 and relative(field, rfield) < 1e-7
+    # the gradients of a loss on potentials and fields against autograd of a direct sum
+    pos_t = torch.tensor(pos, requires_grad=True)
+    q_t = torch.tensor(q, requires_grad=True)
+    pr_t = torch.tensor(probes, requires_grad=True)
+    w = [torch.tensor(rng.normal(0, 1, s)) for s in ((700,), (2, 700), (300,), (2, 300))]
+    out = td3.log_fields(pos_t, q_t, pr_t, root=td3.Quad(0, 0, 1), order=16, theta=0.4)
+    loss = sum((a * b).sum() for a, b in zip(out, w))
+    got = torch.autograd.grad(loss, (pos_t, q_t, pr_t))
+    x = torch.tensor(pos, requires_grad=True)
+    c = torch.tensor(q, requires_grad=True)
+    p = torch.tensor(probes, requires_grad=True)
+    ref = []
+    for tgt, own in ((x, True), (p, False)):
+        d = tgt[:, :, None] - x[:, None, :]
+        r2 = d[0] ** 2 + d[1] ** 2
+        if own:
+            # the own pair has d = 0: r2 = 1 there adds nothing, nor to any derivative
+            r2 = r2 + torch.eye(700, dtype=torch.float64)
+        ref.append(-(c * torch.log(r2)).sum(1))
+        ref.append((2 * c * d / r2).sum(2))
+    rloss = sum((a * b).sum() for a, b in zip(ref, w))
+    want = torch.autograd.grad(rloss, (x, c, p))
+    for g, h in zip(got, want):
+        assert relative(g.numpy(), h.numpy()) < 1e-6, relative(g.numpy(), h.numpy())
+
+
 CASES = {name[5:]: fn for name, fn in globals().items() if name.startswith("case_")}
 
 if __name__ == "__main__":

@@ -129,3 +129,137 @@ def test_symmetric_fmm_forces_are_the_energy_gradient(td3):
             pm[axis, i] -= h
             fd = -(energy(pp) - energy(pm)) / (2 * h)
             assert abs(fd - force[axis, i]) < 1e-7 * scale, (i, axis, fd, force[axis, i])
+
+
+# ------------------------------------------------------------------ losses on fields
+def torch_fields(pos, q, targets, eps, own):
+    """Potential and field at ``targets`` of the charges ``q`` at ``pos`` (FP64 direct sum,
+    differentiable); ``own``: the targets are the sources, each excluding itself."""
+    d = targets[:, :, None] - pos[:, None, :]
+    r2 = d[0] ** 2 + d[1] ** 2 + eps * eps
+    keep = 1.0
+    if own:
+        # the own pair is masked out (r2 = 1 + eps^2 there keeps it finite)
+        eye = torch.eye(pos.shape[1], dtype=torch.float64, device=pos.device)
+        r2 = r2 + eye
+        keep = 1.0 - eye
+    return -((torch.log(r2) * keep) @ q), 2 * ((d / r2) * (q * keep)).sum(2)
+
+
+def test_dipole_sources_match_direct_sum(td3):
+    # a dipole D at s adds -2 D.d / r^2 to the potential at t (d = t - s)
+    rng = np.random.default_rng(21)
+    n = 6000
+    pos = rng.uniform(-1, 1, (2, n))
+    q = rng.choice([-1.0, 1.0], n)
+    dip = rng.normal(0, 1, (2, n))
+    probes = rng.uniform(-1, 1, (2, 700))
+    dev = torch.device("cuda")
+    print("dipole sources vs direct", flush=True)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=16)
+    tree.set_charges(q, dipoles=dip)
+    phi, field = tree.fmm(theta=0.5)
+    phi_p, field_p = tree.fmm_at(probes, theta=0.5)
+    x, c, D = (torch.tensor(a, device=dev) for a in (pos, q, dip))
+    for targets, own, got in ((x, True, (phi, field)), (torch.tensor(probes, device=dev), False,
+                                                        (phi_p, field_p))):
+        d = targets[:, :, None] - x[:, None, :]
+        r2 = d[0] ** 2 + d[1] ** 2
+        if own:
+            r2 = r2 + torch.eye(n, dtype=torch.float64, device=dev)
+        pd = (D[:, None, :] * d).sum(0)
+        rphi = -(torch.log(r2) @ c) - (2 * pd / r2).sum(1)
+        rfield = 2 * ((d / r2) * c).sum(2) + (2 * D[:, None, :] / r2 - 4 * pd * d / r2 ** 2).sum(2)
+        assert relative(torch.as_tensor(got[0], device=dev), rphi) < 1e-7
+        assert relative(torch.as_tensor(got[1], device=dev), rfield) < 1e-7
+    # set_charges without dipoles drops them
+    tree.set_charges(q)
+    phi0, _ = tree.fmm(theta=0.5)
+    rphi0, _ = torch_fields(x, c, x, 0.0, True)
+    assert relative(torch.as_tensor(phi0, device=dev), rphi0) < 1e-8
+
+
+def test_field_jacobian_matches_autograd_of_direct_sum(td3):
+    # J = dE/dx at a target, its own position moved alone: autograd of the direct field at
+    # copies of the positions, each excluding its own index
+    rng = np.random.default_rng(22)
+    n, m = 6000, 700
+    pos = rng.uniform(-1, 1, (2, n))
+    q = rng.choice([-1.0, 1.0], n)
+    probes = rng.uniform(-1, 1, (2, m))
+    dev = torch.device("cuda")
+    print("field jacobian vs autograd", flush=True)
+    x, c = torch.tensor(pos, device=dev), torch.tensor(q, device=dev)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=14)
+    for eps in (0.0, 0.01):
+        _, _, jac = tree.fmm(theta=0.5, softening=eps, jacobian=True)
+        _, _, jac_p = tree.fmm_at(probes, theta=0.5, softening=eps, jacobian=True)
+        for pts, own, got in ((pos, True, jac), (probes, False, jac_p)):
+            t = torch.tensor(pts, device=dev, requires_grad=True)
+            _, e = torch_fields(x, c, t, eps, own)
+            gx = torch.autograd.grad(e[0].sum(), t, retain_graph=True)[0]
+            gy = torch.autograd.grad(e[1].sum(), t)[0]
+            want = torch.stack((gx[0], gx[1], gy[1]))
+            assert relative(torch.as_tensor(got, device=dev), want) < 1e-6, eps
+            assert relative(gx[1], gy[0]) < 1e-12  # the Jacobian is symmetric
+
+
+def field_loss(phi, field, phi_p, field_p, q, w):
+    # potentials and fields at the sources and probes: energy, a weighted field term,
+    # mean squared probe potential and field
+    return (0.5 * torch.dot(q, phi) + (w * field).sum() + (phi_p ** 2).mean()
+            + (field_p ** 2).sum(0).mean())
+
+
+def test_field_loss_adjoint_matches_torch_autograd_of_direct_sum(td3):
+    pos_h, q_h, probes_h = uniform(8192, 512, 23)
+    w_h = np.random.default_rng(24).normal(0, 1e-3, (2, 8192))
+    dev = torch.device("cuda")
+    pos = torch.tensor(pos_h, device=dev, requires_grad=True)
+    q = torch.tensor(q_h, device=dev, requires_grad=True)
+    probes = torch.tensor(probes_h, device=dev, requires_grad=True)
+    w = torch.tensor(w_h, device=dev)
+    print("field-loss adjoint vs torch", flush=True)
+    out = td3.log_fields(pos, q, probes, root=td3.Quad(0, 0, 1), order=14)
+    got = torch.autograd.grad(field_loss(*out, q, w), (pos, q, probes))
+    phi, field = torch_fields(pos, q, pos, 0.0, True)
+    phi_p, field_p = torch_fields(pos, q, probes, 0.0, False)
+    assert relative(out[1].detach(), field.detach()) < 1e-7
+    want = torch.autograd.grad(field_loss(phi, field, phi_p, field_p, q, w), (pos, q, probes))
+    for g, h in zip(got, want):
+        assert relative(g, h) < 1e-5
+
+
+def test_field_loss_adjoint_matches_central_differences(td3):
+    pos_h, q_h, probes_h = uniform(20000, 1024, 25)
+    w_h = np.random.default_rng(26).normal(0, 1e-3, (2, 20000))
+    dev = torch.device("cuda")
+    w = torch.tensor(w_h, device=dev)
+    kw = dict(root=td3.Quad(0, 0, 1), softening=1e-3)
+
+    def loss(p, c):
+        out = td3.log_fields(p, c, torch.tensor(probes_h, device=dev), **kw)
+        return field_loss(*out, c, w)
+
+    pos = torch.tensor(pos_h, device=dev, requires_grad=True)
+    q = torch.tensor(q_h, device=dev, requires_grad=True)
+    print("field-loss adjoint vs central differences", flush=True)
+    g_pos, g_q = torch.autograd.grad(loss(pos, q), (pos, q))
+    scale_pos, scale_q = float(g_pos.abs().max()), float(g_q.abs().max())
+    h = 1e-6
+    qt = torch.tensor(q_h, device=dev)
+    for i in np.random.default_rng(27).choice(20000, 6, replace=False):
+        for axis in range(2):
+            pp, pm = pos_h.copy(), pos_h.copy()
+            pp[axis, i] += h
+            pm[axis, i] -= h
+            fd = (float(loss(torch.tensor(pp, device=dev), qt))
+                  - float(loss(torch.tensor(pm, device=dev), qt))) / (2 * h)
+            assert abs(fd - float(g_pos[axis, i])) < 1e-5 * scale_pos, (i, axis)
+        qp, qm = q_h.copy(), q_h.copy()
+        qp[i] += h
+        qm[i] -= h
+        pt = torch.tensor(pos_h, device=dev)
+        fd = (float(loss(pt, torch.tensor(qp, device=dev)))
+              - float(loss(pt, torch.tensor(qm, device=dev)))) / (2 * h)
+        assert abs(fd - float(g_q[i])) < 1e-5 * scale_q, i

@@ -10,8 +10,8 @@ from .constants import K_E
 from .flat_tree import MAX_ORDER, FlatNode, FlatQuadtree, direct_field
 from .quad import Quad
 from .simulation import ChargedBody, ElectrostaticSimulation
-from .adjoint import log_energy, log_potentials
+from .adjoint import log_energy, log_fields, log_potentials
 from .ensemble import Ensemble
 
 __all__ = ["K_E", "MAX_ORDER", "ChargedBody", "Ensemble", "ElectrostaticSimulation", "FlatNode",
-           "FlatQuadtree", "Quad", "direct_field", "log_energy", "log_potentials"]
+           "FlatQuadtree", "Quad", "direct_field", "log_energy", "log_fields", "log_potentials"]

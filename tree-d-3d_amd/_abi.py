@@ -61,6 +61,12 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_fmm_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _D, _C,
                       _P, _P, _P, _P], _C),
     "td3_pack_sources": ([_P, _P, _I, _P, _P], _C),
+    "td3_log_moments_dipole": ([_I, _P, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P],
+                               _C),
+    "td3_log_fmm_eval_ext": ([_I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P,
+                              _P, _P, _P], _C),
+    "td3_log_fmm_at_eval_ext": ([_I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P,
+                                 _P, _D, _P, _P, _P, _P, _P], _C),
     "td3_leaf_index": ([_P, _I, _P, _P], _C),
     "td3_dual_traverse": ([_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P, _Z,
                            _P, _P], _C),

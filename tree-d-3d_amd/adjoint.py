@@ -18,6 +18,15 @@ the potential of charges ``g`` placed at the targets, so the backward pass re-ru
 source tree's FMM with the charges ``g_src`` (`set_charges`: the tree and its interaction
 lists are kept, the expansions are recomputed) and runs a probe tree with charges
 ``g_probe`` at the sources (``fmm_at``).  The near field is recomputed, not stored.
+
+A loss on the fields ``E_t = -grad_t phi_t`` as well, with ``h = dL/dE`` at every target,
+adds a dipole ``h_t`` at each target to those charges (``sum_t h_t . dE_t/dq_s`` is the
+potential at ``s`` of dipoles ``h``: ``2 h . (t - s) / |t - s|^2``) and the field's
+Jacobian ``J = dE/dx`` (the second-derivative kernel) at the targets themselves:
+
+    dL/dq_s   = psi(s)                    (charges g and dipoles h at the targets)
+    dL/dx_s   = -g_s E_s + J_s h_s - q_s E^psi(s)
+    dL/dp     = -g_p E_p + J_p h_p
 """
 
 from __future__ import annotations
@@ -41,8 +50,7 @@ class _LogPotentials(torch.autograd.Function):
     @staticmethod
     def forward(ctx, positions, charges, probes, root, options):
         theta, softening = options["theta"], options["softening"]
-        tree_kw = dict(kernel="log", order=options["order"],
-                       leaf_capacity=options["leaf_capacity"], precision=options["precision"])
+        tree_kw = _tree_kw(options)
         pos = positions.detach().contiguous()
         q = charges.detach().contiguous()
         tree = FlatQuadtree(pos, q, root, **tree_kw)
@@ -59,31 +67,90 @@ class _LogPotentials(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g_src, g_probe):
         pos, q, pr, field, field_p = ctx.saved_tensors
-        theta, softening = ctx.options["theta"], ctx.options["softening"]
-        n = pos.shape[1]
-        g_src = (torch.zeros(n, dtype=torch.float64, device=pos.device) if g_src is None
-                 else g_src.to(torch.float64).contiguous())
-        m = pr.shape[1]
-        has_probe = m > 0 and g_probe is not None and bool((g_probe != 0).any())
-        # psi and E^psi at the sources from the charges g_src at the sources (own excluded)
-        tree = ctx.tree
-        tree.set_charges(g_src)
-        psi, e_psi = tree.fmm(theta=theta, softening=softening)
-        if has_probe:
-            # ... and from the charges g_probe at the probes
-            probe_tree = FlatQuadtree(pr, g_probe.to(torch.float64).contiguous(),
-                                      _covering_root(pr), **ctx.tree_kw)
-            psi_p, e_psi_p = probe_tree.fmm_at(pos, theta=theta, softening=softening)
-            psi = psi + psi_p
-            e_psi = e_psi + e_psi_p
-        grad_q = psi
+        g_src = _zeros_if_none(g_src, (pos.shape[1],), pos)
+        psi, e_psi = _transposed(ctx, pos, pr, g_src, None, g_probe, None)
         grad_pos = -g_src[None, :] * field - q[None, :] * e_psi
         grad_probes = None
-        if m and ctx.needs_input_grad[2]:
-            gp = torch.zeros(m, dtype=torch.float64, device=pos.device) if g_probe is None \
-                else g_probe.to(torch.float64)
+        if pr.shape[1] and ctx.needs_input_grad[2]:
+            gp = _zeros_if_none(g_probe, (pr.shape[1],), pr)
             grad_probes = -gp[None, :] * field_p
-        return grad_pos, grad_q, grad_probes, None, None
+        return grad_pos, psi, grad_probes, None, None
+
+
+class _LogFields(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, positions, charges, probes, root, options):
+        theta, softening = options["theta"], options["softening"]
+        tree_kw = _tree_kw(options)
+        pos = positions.detach().contiguous()
+        q = charges.detach().contiguous()
+        tree = FlatQuadtree(pos, q, root, **tree_kw)
+        phi, field, jac = tree.fmm(theta=theta, softening=softening, jacobian=True)
+        pr = probes.detach().contiguous()
+        if pr.shape[1]:
+            phi_p, field_p, jac_p = tree.fmm_at(pr, theta=theta, softening=softening,
+                                                jacobian=True)
+        else:
+            phi_p, field_p, jac_p = pr.new_zeros(0), pr.new_zeros((2, 0)), pr.new_zeros((3, 0))
+        ctx.save_for_backward(pos, q, pr, field, field_p, jac, jac_p)
+        ctx.tree, ctx.options, ctx.tree_kw = tree, options, tree_kw
+        return phi, field, phi_p, field_p
+
+    @staticmethod
+    def backward(ctx, g_src, h_src, g_probe, h_probe):
+        pos, q, pr, field, field_p, jac, jac_p = ctx.saved_tensors
+        g_src = _zeros_if_none(g_src, (pos.shape[1],), pos)
+        psi, e_psi = _transposed(ctx, pos, pr, g_src, h_src, g_probe, h_probe)
+        grad_pos = -g_src[None, :] * field - q[None, :] * e_psi
+        if h_src is not None:
+            grad_pos = grad_pos + _jacobian_times(jac, h_src.to(torch.float64))
+        grad_probes = None
+        if pr.shape[1] and ctx.needs_input_grad[2]:
+            gp = _zeros_if_none(g_probe, (pr.shape[1],), pr)
+            grad_probes = -gp[None, :] * field_p
+            if h_probe is not None:
+                grad_probes = grad_probes + _jacobian_times(jac_p, h_probe.to(torch.float64))
+        return grad_pos, psi, grad_probes, None, None
+
+
+def _tree_kw(options) -> dict:
+    return dict(kernel="log", order=options["order"], leaf_capacity=options["leaf_capacity"],
+                precision=options["precision"])
+
+
+def _zeros_if_none(g, shape, like):
+    if g is None:
+        return torch.zeros(shape, dtype=torch.float64, device=like.device)
+    return g.to(torch.float64).contiguous()
+
+
+def _nonzero(t) -> bool:
+    return t is not None and t.numel() > 0 and bool((t != 0).any())
+
+
+def _jacobian_times(jac, h):
+    """``J h`` for the symmetric Jacobians ``(3, M)`` (xx, xy, yy) and vectors ``(2, M)``."""
+    return torch.stack((jac[0] * h[0] + jac[1] * h[1], jac[1] * h[0] + jac[2] * h[1]))
+
+
+def _transposed(ctx, pos, pr, g_src, h_src, g_probe, h_probe):
+    """The transposed FMM: potential and field at the sources (own terms excluded) of
+    charges ``g`` and dipoles ``h`` at every target (the sources and the probes)."""
+    theta, softening = ctx.options["theta"], ctx.options["softening"]
+    tree = ctx.tree
+    dip = h_src.to(torch.float64).contiguous() if _nonzero(h_src) else None
+    tree.set_charges(g_src, dipoles=dip)
+    psi, e_psi = tree.fmm(theta=theta, softening=softening)
+    if pr.shape[1] and (_nonzero(g_probe) or _nonzero(h_probe)):
+        # ... and from the charges and dipoles at the probes
+        gp = _zeros_if_none(g_probe, (pr.shape[1],), pr)
+        probe_tree = FlatQuadtree(pr, gp, _covering_root(pr), **ctx.tree_kw)
+        if _nonzero(h_probe):
+            probe_tree.set_charges(gp, dipoles=h_probe.to(torch.float64).contiguous())
+        psi_p, e_psi_p = probe_tree.fmm_at(pos, theta=theta, softening=softening)
+        psi = psi + psi_p
+        e_psi = e_psi + e_psi_p
+    return psi, e_psi
 
 
 def log_potentials(positions, charges, probes=None, *, root=None, order: int = 12,
@@ -93,6 +160,13 @@ def log_potentials(positions, charges, probes=None, *, root=None, order: int = 1
     ``probes`` ``(2, M)``, differentiable in ``positions`` ``(2, N)``, ``charges`` ``(N,)``
     and ``probes``: the raw potential ``-sum q ln(r^2 + eps^2)`` of `FlatQuadtree.fmm`
     (softened near field).  ``root`` defaults to a square covering the sources."""
+    args = _prepare(positions, charges, probes, root, order, theta, softening, leaf_capacity,
+                    precision)
+    return _LogPotentials.apply(*args)
+
+
+def _prepare(positions, charges, probes, root, order, theta, softening, leaf_capacity,
+             precision):
     dev = _abi.device()
     positions = torch.as_tensor(positions, dtype=torch.float64, device=dev)
     charges = torch.as_tensor(charges, dtype=torch.float64, device=dev)
@@ -107,7 +181,19 @@ def log_potentials(positions, charges, probes=None, *, root=None, order: int = 1
         root = _covering_root(positions.detach())
     options = dict(order=int(order), theta=float(theta), softening=float(softening),
                    leaf_capacity=int(leaf_capacity), precision=precision)
-    return _LogPotentials.apply(positions, charges, probes, root, options)
+    return positions, charges, probes, root, options
+
+
+def log_fields(positions, charges, probes=None, *, root=None, order: int = 12,
+               theta: float = 0.5, softening: float = 0.0, leaf_capacity: int = 8,
+               precision: str = "fp64"):
+    """`log_potentials` with the fields: ``(phi (N,), E (2, N), phi_p (M,), E_p (2, M))`` at
+    the sources (own charge excluded) and the probes, all differentiable in ``positions``,
+    ``charges`` and ``probes``, so a loss may use potentials and fields.  The forward pass
+    also keeps the field's Jacobian at every target for the backward pass."""
+    args = _prepare(positions, charges, probes, root, order, theta, softening, leaf_capacity,
+                    precision)
+    return _LogFields.apply(*args)
 
 
 def log_energy(positions, charges, **options) -> torch.Tensor:

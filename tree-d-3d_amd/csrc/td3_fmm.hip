@@ -623,6 +623,99 @@ __global__ __launch_bounds__(256) void eval_kernel(
     }
 }
 
+// The log kernel's evaluation for adjoints, in FP64 (the far field as the locals were
+// built).  EXT 1: the sources also carry dipoles D (sdip: x components, then y, in tree
+// order), whose potential at d = target - source is -2 D.d / (r^2 + eps^2) and field
+// 2 D / (r^2 + eps^2) - 4 (D.d) d / (r^2 + eps^2)^2 (their far field is in the moments).
+// EXT 2: the Jacobian dE/dx of the field at each target, rows xx, xy, yy of jac; the far
+// field's is 2 W'' (E_x - i E_y = 2 W' is analytic), the near field's
+// 2 q (I / (r^2 + eps^2) - 2 d d^T / (r^2 + eps^2)^2).  Targets, exclusions and status as
+// in eval_kernel; a dipole counts as a source as a charge does.
+template <int MODE, int EXT>
+__global__ __launch_bounds__(256) void log_ext_eval_kernel(
+    const double* __restrict__ tpos, const i64* __restrict__ tperm, const i64* __restrict__ exclude,
+    i64 m, const double* __restrict__ tcenter, const double* __restrict__ thalf,
+    const double* __restrict__ lre, const double* __restrict__ lim, int p1,
+    const u32* __restrict__ prow, const int* __restrict__ pcol, const i64* __restrict__ sperm,
+    i64 n, const int4* __restrict__ sinfo, const int* __restrict__ leaf_of,
+    const double4* __restrict__ src4, const double* __restrict__ sdip, double eps2,
+    double* __restrict__ phi, double* __restrict__ field, double* __restrict__ jac,
+    i64* __restrict__ status) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const i64 leaf = leaf_of[k];
+    const double tx = tpos[k], ty = tpos[m + k];
+    const double tq = MODE == 0 ? src4[k].z : 0.0;
+    const i64 tskip = MODE == 1 ? exclude[tperm[k]] : -1;
+    double ap = 0.0, ax = 0.0, ay = 0.0, jxx = 0.0, jxy = 0.0, jyy = 0.0;
+    bool bad = false;
+    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
+    for (u32 r = r0; r < r1; ++r) {
+        const int4 sf = sinfo[pcol[r]];
+        for (int ks = sf.x; ks < sf.y; ++ks) {
+            if (MODE == 0 && ks == k) continue;
+            const double4 sv = src4[ks];
+            const double sq = sv.z;
+            const double px = EXT == 1 ? sdip[ks] : 0.0, py = EXT == 1 ? sdip[n + ks] : 0.0;
+            const bool live = sq != 0.0 || px != 0.0 || py != 0.0;
+            if (MODE == 1 && (!live || (tskip >= 0 && sperm[ks] == tskip))) continue;
+            const double dx = tx - sv.x, dy = ty - sv.y;
+            const double r2 = dx * dx + dy * dy + eps2;
+            if (r2 == 0.0) {
+                if (MODE == 1 || (tq != 0.0 && live)) bad = true;
+                continue;
+            }
+            ap -= sq * log(r2);
+            const double c = 2.0 * sq / r2;
+            ax += c * dx;
+            ay += c * dy;
+            if (EXT == 1) {
+                const double inv = 1.0 / r2;
+                const double pd = px * dx + py * dy;
+                const double c4 = 4.0 * pd * inv * inv;
+                ap -= 2.0 * pd * inv;
+                ax += 2.0 * px * inv - c4 * dx;
+                ay += 2.0 * py * inv - c4 * dy;
+            }
+            if (EXT == 2) {
+                const double c2 = 2.0 * c / r2;
+                jxx += c - c2 * dx * dx;
+                jxy -= c2 * dx * dy;
+                jyy += c - c2 * dy * dy;
+            }
+        }
+    }
+    const i64 i = tperm[k];
+    if (bad) td3_report(status, i + 1);
+    // Horner on the scaled local expansion at u' = (z - c) / r: W, W' and W'' / 2
+    const double inv_r = 1.0 / (1.4142135623730951 * thalf[leaf]);
+    const double ur = (tx - tcenter[2 * leaf]) * inv_r, ui = (ty - tcenter[2 * leaf + 1]) * inv_r;
+    const double* bre = lre + leaf * (i64)p1;
+    const double* bim = lim + leaf * (i64)p1;
+    double w_re = bre[p1 - 1], w_im = bim[p1 - 1];
+    double d_re = 0.0, d_im = 0.0, e_re = 0.0, e_im = 0.0;
+    for (int l = p1 - 2; l >= 0; --l) {
+        const double ne = e_re * ur - e_im * ui + d_re;
+        e_im = e_re * ui + e_im * ur + d_im;
+        e_re = ne;
+        const double nd = d_re * ur - d_im * ui + w_re;
+        d_im = d_re * ui + d_im * ur + w_im;
+        d_re = nd;
+        const double nw = w_re * ur - w_im * ui + bre[l];
+        w_im = w_re * ui + w_im * ur + bim[l];
+        w_re = nw;
+    }
+    phi[i] = ap - 2.0 * w_re;
+    field[i] = ax + 2.0 * d_re * inv_r;
+    field[m + i] = ay - 2.0 * d_im * inv_r;
+    if (EXT == 2) {
+        const double s = 4.0 * inv_r * inv_r;
+        jac[i] = jxx + s * e_re;
+        jac[m + i] = jxy - s * e_im;
+        jac[2 * m + i] = jyy - s * e_re;
+    }
+}
+
 // (x, y, q, 0) of every sorted source: one 32-byte load per source in the near field.
 __global__ void pack_sources_kernel(const double* __restrict__ pos, const double* __restrict__ charge,
                                     i64 n, double4* __restrict__ out) {
@@ -837,6 +930,70 @@ extern "C" int td3_fmm_at_eval(int kernel, int precision, int64_t order,
               p2p_col, sorted_pos, sorted_charge, perm, n, (const int4*)s_info, t_leaf_of,
               (const double4*)s_sources4, softening * softening, with_potential, phi, field,
               status);
+}
+
+namespace {
+template <int MODE>
+int launch_ext(hipStream_t s, const double* tpos, const i64* tperm, const i64* exclude, i64 m,
+               const double* tcenter, const double* thalf, const double* lre, const double* lim,
+               int order, const u32* prow, const int* pcol, const i64* sperm, i64 n,
+               const int4* sinfo, const int* leaf_of, const double4* src4, const double* sdip,
+               double eps2, double* phi, double* field, double* jac, i64* status) {
+    TD3_ARG(order >= 1 && order <= 64, "order");
+    TD3_ARG(leaf_of != nullptr, "leaf_of (td3_leaf_index of the target tree)");
+    TD3_ARG(src4 != nullptr, "sources4 (td3_pack_sources of the source tree)");
+    TD3_ARG(sdip == nullptr || jac == nullptr, "dipole sources with a Jacobian");
+    const unsigned blocks = td3_blocks(m, 256);
+    const int p1 = (int)order + 1;
+    if (sdip != nullptr)
+        TD3_TIMED(s, "eval_ext", hipLaunchKernelGGL((log_ext_eval_kernel<MODE, 1>), dim3(blocks), dim3(256), 0, s,
+                  tpos, tperm, exclude, m, tcenter, thalf, lre, lim, p1, prow, pcol, sperm, n,
+                  sinfo, leaf_of, src4, sdip, eps2, phi, field, jac, status));
+    else if (jac != nullptr)
+        TD3_TIMED(s, "eval_ext", hipLaunchKernelGGL((log_ext_eval_kernel<MODE, 2>), dim3(blocks), dim3(256), 0, s,
+                  tpos, tperm, exclude, m, tcenter, thalf, lre, lim, p1, prow, pcol, sperm, n,
+                  sinfo, leaf_of, src4, sdip, eps2, phi, field, jac, status));
+    else
+        TD3_TIMED(s, "eval_ext", hipLaunchKernelGGL((log_ext_eval_kernel<MODE, 0>), dim3(blocks), dim3(256), 0, s,
+                  tpos, tperm, exclude, m, tcenter, thalf, lre, lim, p1, prow, pcol, sperm, n,
+                  sinfo, leaf_of, src4, sdip, eps2, phi, field, jac, status));
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+}  // namespace
+
+extern "C" int td3_log_fmm_eval_ext(int64_t order, const double* sorted_pos, const int64_t* perm,
+                                    int64_t n, const double* node_center,
+                                    const double* node_half, const void* node_info,
+                                    const double* local_re, const double* local_im,
+                                    const uint32_t* p2p_row, const int32_t* p2p_col,
+                                    const int32_t* leaf_of, const void* sources4,
+                                    const double* sorted_dipole, double softening, double* phi,
+                                    double* field, double* jacobian, int64_t* status,
+                                    void* stream) {
+    TD3_ARG(n >= 1, "sizes");
+    return launch_ext<0>((hipStream_t)stream, sorted_pos, perm, nullptr, n, node_center,
+                         node_half, local_re, local_im, (int)order, p2p_row, p2p_col, perm, n,
+                         (const int4*)node_info, leaf_of, (const double4*)sources4,
+                         sorted_dipole, softening * softening, phi, field, jacobian, status);
+}
+
+extern "C" int td3_log_fmm_at_eval_ext(int64_t order, const double* sorted_targets,
+                                       const int64_t* target_perm, const int64_t* exclude,
+                                       int64_t m, const double* t_center, const double* t_half,
+                                       const int32_t* t_leaf_of, const double* local_re,
+                                       const double* local_im, const uint32_t* p2p_row,
+                                       const int32_t* p2p_col, const int64_t* perm, int64_t n,
+                                       const void* s_info, const void* s_sources4,
+                                       const double* sorted_dipole, double softening,
+                                       double* phi, double* field, double* jacobian,
+                                       int64_t* status, void* stream) {
+    TD3_ARG(m >= 1 && n >= 1, "sizes");
+    TD3_ARG(exclude != nullptr, "exclude");
+    return launch_ext<1>((hipStream_t)stream, sorted_targets, target_perm, exclude, m, t_center,
+                         t_half, local_re, local_im, (int)order, p2p_row, p2p_col, perm, n,
+                         (const int4*)s_info, t_leaf_of, (const double4*)s_sources4,
+                         sorted_dipole, softening * softening, phi, field, jacobian, status);
 }
 
 extern "C" int td3_leaf_index(const void* node_info, int64_t node_count, int32_t* leaf_of,

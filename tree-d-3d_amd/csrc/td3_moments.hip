@@ -12,9 +12,29 @@
 
 namespace {
 
-// a_n = sum_j q_j (z_j - c)^n for every leaf; qabs = sum |q_j|.
+// (x + i y)^k by binary powering, times (pr + i pi)
+__device__ __forceinline__ void cpow_times(double br, double bi, int k, double* pr, double* pi) {
+    while (k) {
+        if (k & 1) {
+            const double tr = *pr * br - *pi * bi;
+            *pi = *pr * bi + *pi * br;
+            *pr = tr;
+        }
+        k >>= 1;
+        if (k) {
+            const double tr = br * br - bi * bi;
+            bi = 2.0 * br * bi;
+            br = tr;
+        }
+    }
+}
+
+// a_n = sum_j q_j (z_j - c)^n for every leaf; qabs = sum |q_j|.  With dipoles D_j
+// (sdip (2, N), may be null) also a_n -= n D_j (z_j - c)^(n-1): the moments of the
+// potential -2 Re(D / (z - z_j)) of a dipole, the derivative of a charge's in z_j.
 __global__ void log_p2m_kernel(const double* __restrict__ spos, const double* __restrict__ scharge,
-                               i64 n, const double* __restrict__ center,
+                               const double* __restrict__ sdip, i64 n,
+                               const double* __restrict__ center,
                                const int4* __restrict__ info, i64 count, int p1,
                                double* __restrict__ qabs, double* __restrict__ mre,
                                double* __restrict__ mim) {
@@ -29,25 +49,17 @@ __global__ void log_p2m_kernel(const double* __restrict__ spos, const double* __
     for (int j = nf.x; j < nf.y; ++j) {
         const double q = scharge[j];
         const double dx = spos[j] - cx, dy = spos[n + j] - cy;
-        // q (dx + i dy)^k by binary powering
-        double pr = q, pi = 0.0, br = dx, bi = dy;
-        int e = k;
-        while (e) {
-            if (e & 1) {
-                const double tr = pr * br - pi * bi;
-                pi = pr * bi + pi * br;
-                pr = tr;
-            }
-            e >>= 1;
-            if (e) {
-                const double tr = br * br - bi * bi;
-                bi = 2.0 * br * bi;
-                br = tr;
-            }
-        }
+        double pr = q, pi = 0.0;
+        cpow_times(dx, dy, k, &pr, &pi);
         are += pr;
         aim += pi;
         qa += fabs(q);
+        if (sdip != nullptr && k > 0) {
+            double dr = -k * sdip[j], di = -k * sdip[n + j];
+            cpow_times(dx, dy, k - 1, &dr, &di);
+            are += dr;
+            aim += di;
+        }
     }
     mre[node * p1 + k] = are;
     mim[node * p1 + k] = aim;
@@ -159,12 +171,13 @@ extern "C" int td3_permute(const double* src, int64_t rows, int64_t ld_src, cons
     return TD3_OK;
 }
 
-extern "C" int td3_moments(int kernel, int64_t order, const double* sorted_pos,
-                           const double* sorted_charge, int64_t n, const double* node_center,
-                           const void* node_info, int64_t node_count, const int64_t* levels_host,
-                           int64_t n_levels, const double* binomial, int64_t bsz, double* node_q,
-                           double* node_qabs, double* node_p, double* node_s, double* moment_re,
-                           double* moment_im, void* stream) {
+namespace {
+int moments_impl(int kernel, int64_t order, const double* sorted_pos, const double* sorted_charge,
+                 const double* sorted_dipole, int64_t n, const double* node_center,
+                 const void* node_info, int64_t node_count, const int64_t* levels_host,
+                 int64_t n_levels, const double* binomial, int64_t bsz, double* node_q,
+                 double* node_qabs, double* node_p, double* node_s, double* moment_re,
+                 double* moment_im, void* stream) {
     TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(n >= 1 && node_count >= 1 && n_levels >= 1, "sizes");
     TD3_ARG(levels_host != nullptr && levels_host[n_levels] == node_count, "levels");
@@ -174,8 +187,8 @@ extern "C" int td3_moments(int kernel, int64_t order, const double* sorted_pos,
         TD3_ARG(order >= 1 && order <= 64 && bsz >= order + 1, "order");
         const int p1 = (int)order + 1;
         TD3_TIMED(s, "moments_p2m", hipLaunchKernelGGL(log_p2m_kernel, dim3(td3_blocks(node_count * p1, 256)), dim3(256), 0, s,
-                           sorted_pos, sorted_charge, n, node_center, info, node_count, p1, node_qabs,
-                           moment_re, moment_im));
+                           sorted_pos, sorted_charge, sorted_dipole, n, node_center, info, node_count,
+                           p1, node_qabs, moment_re, moment_im));
         TD3_LAUNCHED();
         for (i64 d = n_levels - 2; d >= 0; --d) {
             const i64 lo = levels_host[d], hi = levels_host[d + 1];
@@ -199,3 +212,30 @@ extern "C" int td3_moments(int kernel, int64_t order, const double* sorted_pos,
     }
     return TD3_OK;
 }
+}  // namespace
+
+extern "C" int td3_moments(int kernel, int64_t order, const double* sorted_pos,
+                           const double* sorted_charge, int64_t n, const double* node_center,
+                           const void* node_info, int64_t node_count, const int64_t* levels_host,
+                           int64_t n_levels, const double* binomial, int64_t bsz, double* node_q,
+                           double* node_qabs, double* node_p, double* node_s, double* moment_re,
+                           double* moment_im, void* stream) {
+    return moments_impl(kernel, order, sorted_pos, sorted_charge, nullptr, n, node_center,
+                        node_info, node_count, levels_host, n_levels, binomial, bsz, node_q,
+                        node_qabs, node_p, node_s, moment_re, moment_im, stream);
+}
+
+extern "C" int td3_log_moments_dipole(int64_t order, const double* sorted_pos,
+                                      const double* sorted_charge, const double* sorted_dipole,
+                                      int64_t n, const double* node_center, const void* node_info,
+                                      int64_t node_count, const int64_t* levels_host,
+                                      int64_t n_levels, const double* binomial, int64_t bsz,
+                                      double* node_qabs, double* moment_re, double* moment_im,
+                                      void* stream) {
+    TD3_ARG(sorted_dipole != nullptr, "sorted_dipole");
+    return moments_impl(TD3_LOG, order, sorted_pos, sorted_charge, sorted_dipole, n, node_center,
+                        node_info, node_count, levels_host, n_levels, binomial, bsz, nullptr,
+                        node_qabs, nullptr, nullptr, moment_re, moment_im, stream);
+}
+
+

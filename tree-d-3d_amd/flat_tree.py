@@ -387,13 +387,23 @@ class FlatQuadtree:
         bsz = 2 * self.order + 1
         binom = _device_table("binomial", _binomial_table, bsz)
         levels = np.ascontiguousarray(self.levels, dtype=np.int64)
-        _abi.call(
-            "td3_moments", _abi.LOG if self.kernel == "log" else _abi.COULOMB, self.order,
-            _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge), n,
-            _abi.ptr(self._d["node_center"]), _abi.ptr(self._node_info), c, levels.ctypes.data,
-            len(levels) - 1, _abi.ptr(binom), bsz, _abi.ptr(node_q), _abi.ptr(node_qabs),
-            _abi.ptr(node_p), _abi.ptr(node_s), _abi.ptr(mre), _abi.ptr(mim), _abi.stream(),
-        )
+        sdip = self.__dict__.get("_sdip")
+        if sdip is not None:
+            _abi.call(
+                "td3_log_moments_dipole", self.order, _abi.ptr(self._sorted_pos),
+                _abi.ptr(self._scharge), _abi.ptr(sdip), n, _abi.ptr(self._d["node_center"]),
+                _abi.ptr(self._node_info), c, levels.ctypes.data, len(levels) - 1,
+                _abi.ptr(binom), bsz, _abi.ptr(node_qabs), _abi.ptr(mre), _abi.ptr(mim),
+                _abi.stream(),
+            )
+        else:
+            _abi.call(
+                "td3_moments", _abi.LOG if self.kernel == "log" else _abi.COULOMB, self.order,
+                _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge), n,
+                _abi.ptr(self._d["node_center"]), _abi.ptr(self._node_info), c, levels.ctypes.data,
+                len(levels) - 1, _abi.ptr(binom), bsz, _abi.ptr(node_q), _abi.ptr(node_qabs),
+                _abi.ptr(node_p), _abi.ptr(node_s), _abi.ptr(mre), _abi.ptr(mim), _abi.stream(),
+            )
         if self.kernel == "log":
             node_q.copy_(mre[:, 0])
             node_p[:, 0] = mre[:, 1]
@@ -403,18 +413,41 @@ class FlatQuadtree:
         self._invalidate_host("node_q", "node_qabs", "node_p", "node_s", "moment_re",
                               "moment_im")
 
-    def set_charges(self, charges) -> None:
-        """flat_tree.py:853-868: new charges for the same particles; keeps cells and lists."""
+    def set_charges(self, charges, dipoles=None) -> None:
+        """flat_tree.py:853-868: new charges for the same particles; keeps cells and lists.
+
+        ``dipoles`` ``(2, N)`` (log kernel only; no counterpart in the reference) gives the
+        particles dipoles ``D`` as well, each adding ``-2 D.d / (r^2 + eps^2)`` to the
+        potential at ``d = target - particle`` (``-D`` times the gradient of a unit
+        charge's potential in its position) and minus its gradient to the field.  `fmm`
+        and `fmm_at` include them; a new `set_charges` without dipoles drops them.
+        """
         charges = _device(charges).reshape(-1)
-        if charges.shape[0] != self._d["pos"].shape[1]:
+        n = self._d["pos"].shape[1]
+        if charges.shape[0] != n:
             raise ValueError("charges must have one entry per particle")
+        dip = None
+        if dipoles is not None:
+            if self.kernel != "log":
+                raise ValueError("dipoles need the log kernel")
+            dip = _device(dipoles)
+            if dip.shape != (2, n):
+                raise ValueError("dipoles must have shape (2, N) (component-major)")
+            if not bool(torch.isfinite(dip).all()):
+                raise ValueError("dipoles must be finite")
         self._d["charge"].copy_(charges)
         self._invalidate_host("charge")
         self._cart = None
         self._scaled = {}
         self._src4 = None
-        if self._d["pos"].shape[1]:
+        self._sdip = None
+        if n:
             self._sorted_charge()
+            if dip is not None:
+                sdip = torch.empty((2, n), dtype=torch.float64, device=dip.device)
+                _abi.call("td3_permute", _abi.ptr(dip.contiguous()), 2, n,
+                          _abi.ptr(self._d["perm"]), n, _abi.ptr(sdip), _abi.stream())
+                self._sdip = sdip
             self._moments()
 
     @property
@@ -452,6 +485,8 @@ class FlatQuadtree:
         skipped for target ``i`` (-1: none); ``potential=False`` returns the potential as
         zeros (and skips the line-charge logarithms).
         """
+        if self.__dict__.get("_sdip") is not None:
+            raise ValueError("the walk takes charges only; dipoles are used by fmm and fmm_at")
         like_torch = _is_torch(targets) or self._like_torch
         t = _points(targets, "targets")
         m = t.shape[1]
@@ -685,14 +720,24 @@ class FlatQuadtree:
                   self.order, _abi.ptr(lre), _abi.ptr(lim), _abi.stream())
         return _out(lre, self._like_torch), _out(lim, self._like_torch)
 
+    @staticmethod
+    def _fmm_out(phi, field, jac, like_torch):
+        if jac is None:
+            return _out(phi, like_torch), _out(field, like_torch)
+        return _out(phi, like_torch), _out(field, like_torch), _out(jac, like_torch)
+
     def fmm(self, *, theta: float = 0.5, softening: float = 0.0, potential: bool = True,
-            precision=None):
+            precision=None, jacobian: bool = False):
         """Potential ``(N,)`` and field ``(2, N)`` at every source by the FMM, without ``k_e``.
 
         flat_tree.py:1461-1634: dual-tree traversal, M2L both ways, L2L, L2P and exact P2P
         between near leaves (softened; the far field of the log kernel is unsoftened).
         Each particle's own charge is excluded.  ``potential=False`` returns zeros for the
         potential and skips the near-field logarithms.
+
+        Log kernel only, no counterpart in the reference: ``jacobian=True`` also returns
+        ``dE/dx`` at every source, ``(3, N)`` rows xx, xy, yy; dipoles given to
+        `set_charges` are included.  Both evaluate the near field in FP64.
         """
         from ._utils import _finite
 
@@ -702,16 +747,33 @@ class FlatQuadtree:
         softening = _finite("softening", softening, minimum=0)
         n = self._d["pos"].shape[1]
         dev = self._d["pos"].device
+        sdip = self.__dict__.get("_sdip")
+        _check_jacobian(self, jacobian, sdip)
         phi = torch.zeros(n, dtype=torch.float64, device=dev)
         field = torch.zeros((2, n), dtype=torch.float64, device=dev)
+        jac = torch.zeros((3, n), dtype=torch.float64, device=dev) if jacobian else None
         if not n:
-            return _out(phi, self._like_torch), _out(field, self._like_torch)
+            return self._fmm_out(phi, field, jac, self._like_torch)
         if self.kernel != "log":
             return self._coulomb_fmm(theta, softening, potential, precision)
         prec = _precision(precision or self.precision)
         lre, lim = self._local_expansions(theta, prec)
         _, (prow, pcol, _), counts = self._lists(theta)
         status = torch.zeros(1, dtype=torch.int64, device=dev)
+        if jacobian or sdip is not None:
+            _abi.call(
+                "td3_log_fmm_eval_ext", self.order, _abi.ptr(self._sorted_pos),
+                _abi.ptr(self._d["perm"]), n, _abi.ptr(self._d["node_center"]),
+                _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), _abi.ptr(lre),
+                _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._leaf_index()),
+                _abi.ptr(self._sources4()), _abi.ptr(sdip), softening, _abi.ptr(phi),
+                _abi.ptr(field), _abi.ptr(jac), _abi.ptr(status), _abi.stream(),
+            )
+            _check(status)
+            if not potential:
+                phi.zero_()
+            self.fmm_pairs = counts
+            return self._fmm_out(phi, field, jac, self._like_torch)
         _abi.call(
             "td3_fmm_eval", _abi.LOG, prec, self.order, _abi.ptr(self._sorted_pos),
             _abi.ptr(self._scharge),
@@ -854,13 +916,15 @@ class FlatQuadtree:
         return tree, lists
 
     def fmm_at(self, targets, *, exclude=None, theta: float = 0.5, softening: float = 0.0,
-               potential: bool = True, leaf_capacity=None, precision=None):
+               potential: bool = True, leaf_capacity=None, precision=None,
+               jacobian: bool = False):
         """Potential ``(M,)`` and field ``(2, M)`` at ``targets`` ``(2, M)`` by the FMM.
 
         flat_tree.py:1152-1316: the targets get a tree of their own (``leaf_capacity``
         per leaf, by default the sources'), a dual traversal pairs its cells with the
         source cells, and local expansions run down the target tree to the targets.
         ``exclude[i]`` is a source index skipped for target ``i`` in the exact sums.
+        ``jacobian`` and dipoles as in `fmm` (``(3, M)`` at the targets).
         """
         from ._utils import _finite
 
@@ -873,10 +937,12 @@ class FlatQuadtree:
         if not bool(torch.isfinite(t).all()):
             raise ValueError("targets must be finite")
         m, n = t.shape[1], self._d["pos"].shape[1]
+        _check_jacobian(self, jacobian, self.__dict__.get("_sdip"))
         phi = torch.zeros(m, dtype=torch.float64, device=t.device)
         field = torch.zeros((2, m), dtype=torch.float64, device=t.device)
+        jac = torch.zeros((3, m), dtype=torch.float64, device=t.device) if jacobian else None
         if not m or not n:
-            return _out(phi, like_torch), _out(field, like_torch)
+            return self._fmm_out(phi, field, jac, like_torch)
         if exclude is None:
             ex = torch.full((m,), -1, dtype=torch.int64, device=t.device)
         else:
@@ -890,6 +956,22 @@ class FlatQuadtree:
         lre, lim = self._local_expansions(theta, prec, target=tree, m2l_rows=m2l_rows)
         status = torch.zeros(1, dtype=torch.int64, device=t.device)
         td = tree._d
+        sdip = self.__dict__.get("_sdip")
+        if jacobian or sdip is not None:
+            _abi.call(
+                "td3_log_fmm_at_eval_ext", self.order, _abi.ptr(tree._sorted_pos),
+                _abi.ptr(td["perm"]), _abi.ptr(ex), m, _abi.ptr(td["node_center"]),
+                _abi.ptr(td["node_half"]), _abi.ptr(tree._leaf_index()), _abi.ptr(lre),
+                _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._d["perm"]), n,
+                _abi.ptr(self._node_info), _abi.ptr(self._sources4()), _abi.ptr(sdip),
+                softening, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(jac), _abi.ptr(status),
+                _abi.stream(),
+            )
+            _check(status)
+            if not potential:
+                phi.zero_()
+            self.fmm_at_pairs = counts
+            return self._fmm_out(phi, field, jac, like_torch)
         _abi.call(
             "td3_fmm_at_eval", _abi.LOG, prec, self.order, _abi.ptr(tree._sorted_pos),
             _abi.ptr(td["perm"]),
@@ -929,6 +1011,13 @@ class FlatQuadtree:
         )
         _check(status)
         return _out(phi, like_torch), _out(field, like_torch)
+
+
+def _check_jacobian(tree, jacobian, sdip) -> None:
+    if jacobian and tree.kernel != "log":
+        raise ValueError("jacobian needs the log kernel")
+    if jacobian and sdip is not None:
+        raise ValueError("jacobian of the field of dipoles is not provided")
 
 
 def _m2l_coefficients(order: int, symmetric: bool) -> np.ndarray:
