@@ -179,9 +179,19 @@ def extra_config5(td3, n=262144, probes=8192):
         loss = (phi_p ** 2).mean() + 0.5 * torch.dot(q, phi)
         return torch.autograd.grad(loss, (pos, q))
 
+    def run_fields():
+        # the same with the probe fields in the loss: dipole sources in the transposed FMM
+        # and the field's Jacobian at the probes
+        phi, _, phi_p, field_p = td3.log_fields(pos, q, pr, root=td3.Quad(0, 0, 1))
+        loss = (phi_p ** 2).mean() + (field_p ** 2).sum(0).mean() + 0.5 * torch.dot(q, phi)
+        return torch.autograd.grad(loss, (pos, q))
+
     t, _ = _timed(run, repeats=2)
+    t_fields, _ = _timed(run_fields, repeats=2)
     return {"workload": f"log FMM N={n}, {probes} probes, order 12: loss + dL/dq + dL/dpos",
-            "ms": 1e3 * t}
+            "ms": 1e3 * t,
+            "fields_loss_ms": 1e3 * t_fields,
+            "fields_loss": "+ mean squared field at the probes (log_fields)"}
 
 
 def extras(td3):
