@@ -20,10 +20,12 @@
 // come from the same decision, evaluated with the same FP64 operations).
 //
 // Per level, one thread per inherited pair (the pairs of all the level's nodes, in node
-// order): count (each walk counted per column: M2L, P2P, hand-downs to each child), an
-// exclusive scan of the counts (which gives every write position, the rows of a node being
-// its pairs' outputs in order), write (each walk again).  A walk needs no stack: it moves
-// through the other cell's subtree by child and parent links, so any depth works.
+// order): count (each walk counted per column: M2L, P2P, hand-downs to each child -- the
+// same for all four -- with its first few outputs cached), an exclusive scan of the counts
+// (which gives every write position, the rows of a node being its pairs' outputs in
+// order), write (the cached outputs replayed; only longer walks walk again).  A walk needs
+// no stack: it moves through the other cell's subtree by child and parent links, so any
+// depth works.
 
 #include "td3_common.h"
 #include "td3_sort.h"
@@ -59,53 +61,105 @@ __device__ __forceinline__ int decide(const RowsTree& t, int a, int4 fa, int x, 
     return split_a ? 3 : 4;
 }
 
+// Per entry: M2L entries, P2P entries, and the entries handed to EACH child (a walk hands
+// every pair it splits on a's side to all four children, a self pair four to each).
 struct Counts {
-    u32 m2l, p2p, inh[4];
+    u32 m2l, p2p, inh;
 };
 
 __device__ __forceinline__ u32 entry_with(int x, bool second) {
     return (u32)x | (second ? kSecond : 0u);
 }
 
-// Where a writing walk puts its outputs: the rows, the next level's pairs and their nodes,
-// and the offsets of a's children's lists.
-struct WriteTargets {
+// A walk's outputs, in order, are events: M2L (x), P2P (x), or a hand-down of (x, side) to
+// each of a's four children.  The counting pass keeps the first kCached events of every
+// entry (code: x in bits 0-28, the kind in 29-30, the side in 31), so that the writing
+// pass replays them instead of walking again.
+constexpr int kCached = 8;
+constexpr u32 kEvM2L = 1u << 29, kEvP2P = 2u << 29, kEvSplit = 3u << 29, kEvKind = 3u << 29;
+
+// Counts per column and caches the events (cache[i * stride + e], i < kCached).
+struct CountSink {
+    Counts c;
+    u32 events;
+    u32* cache;
+    u32 stride;
+    __device__ void event(u32 code) {
+        if (events < (u32)kCached) cache[(size_t)events * stride] = code;
+        ++events;
+    }
+    __device__ void m2l(int x) {
+        ++c.m2l;
+        event((u32)x | kEvM2L);
+    }
+    __device__ void p2p(int x) {
+        ++c.p2p;
+        event((u32)x | kEvP2P);
+    }
+    __device__ void split(int x, bool second) {
+        ++c.inh;
+        event((u32)x | kEvSplit | (second ? kSecond : 0u));
+    }
+    __device__ void self_leaf(int a) { ++c.p2p; }
+    __device__ void self_children(int child0) { c.inh += 4; }
+};
+
+// Writes at the offsets it holds (advancing them): the rows (and the M2L entries' row
+// node), the next level's pairs and their nodes.
+struct WriteSink {
+    Counts c;
+    int a, first_child;
     int* m2l_col;
+    int* m2l_dst;
     int* p2p_col;
     u32* inh_next;
     int* enode_next;
-    const u32* child_at;
+    u32 child_base, child_stride;  // child k's list of a starts at child_base + k child_stride
+    __device__ void m2l(int x) {
+        m2l_col[c.m2l] = x;
+        if (m2l_dst != nullptr) m2l_dst[c.m2l] = a;
+        ++c.m2l;
+    }
+    __device__ void p2p(int x) { p2p_col[c.p2p++] = x; }
+    __device__ void split(int x, bool second) {
+        for (int k = 0; k < 4; ++k) {
+            const u32 at = child_base + k * child_stride + c.inh;
+            inh_next[at] = entry_with(x, second);
+            enode_next[at] = first_child + k;
+        }
+        ++c.inh;
+    }
+    __device__ void self_leaf(int x) { p2p(x); }
+    // the ten child pairs (c_i, c_j), i <= j: child k gets every sibling, itself as a self
+    // pair, c_j (j > k) with k first, c_i (i < k) with k second
+    __device__ void self_children(int child0) {
+        for (int k = 0; k < 4; ++k) {
+            const u32 at = child_base + k * child_stride + c.inh;
+            for (int j = 0; j < 4; ++j) {
+                inh_next[at + j] = (u32)(child0 + j) | (j < k ? kSecond : 0u);
+                enode_next[at + j] = child0 + k;
+            }
+        }
+        c.inh += 4;
+    }
 };
 
-// Walks one inherited entry of node a, counting into c (and, WRITE, writing at the offsets
-// c holds, which it advances).
-template <bool WRITE>
-__device__ void walk_entry(const RowsTree& t, int a, int4 fa, u32 entry, double theta, Counts* c,
-                           const WriteTargets* wt) {
-    if ((int)(entry & ~kSecond) == a) {
-        // self pair {a, a}
-        if (fa.y == fa.x) return;
-        if (fa.z < 0) {
-            if (WRITE) wt->p2p_col[c->p2p] = a;
-            ++c->p2p;
-            return;
-        }
-        // the ten child pairs (c_i, c_j), i <= j: child k gets every sibling, itself as a
-        // self pair, c_j (j > k) with k first, c_i (i < k) with k second
-        for (int k = 0; k < 4; ++k) {
-            if (WRITE) {
-                const u32 at = wt->child_at[k] + c->inh[k];
-                for (int j = 0; j < 4; ++j) {
-                    wt->inh_next[at + j] = (u32)(fa.z + j) | (j < k ? kSecond : 0u);
-                    wt->enode_next[at + j] = fa.z + k;
-                }
-            }
-            c->inh[k] += 4;
-        }
-        return;
-    }
-    // depth first over the subtree of the other cell x0, without a stack: children in
-    // order, back up through the parents once a fourth child is done
+// The self pair {a, a}: its leaf is one P2P entry, else the sibling pairs go down.
+template <class Sink>
+__device__ __forceinline__ void self_entry(int a, int4 fa, Sink& sink) {
+    if (fa.y == fa.x) return;
+    if (fa.z < 0)
+        sink.self_leaf(a);
+    else
+        sink.self_children(fa.z);
+}
+
+// Walks one inherited entry (not a self pair) of node a: depth first over the subtree of
+// the other cell x0, without a stack (children in order, back up through the parents once
+// a fourth child is done).
+template <class Sink>
+__device__ __forceinline__ void walk_entry(const RowsTree& t, int a, int4 fa, u32 entry, double theta,
+                           Sink& sink) {
     const int x0 = (int)(entry & ~kSecond);
     const bool second = (entry & kSecond) != 0;
     int x = x0;
@@ -115,25 +169,30 @@ __device__ void walk_entry(const RowsTree& t, int a, int4 fa, u32 entry, double 
             x = t.info[x].z;
             continue;
         }
-        if (kind == 1) {
-            if (WRITE) wt->m2l_col[c->m2l] = x;
-            ++c->m2l;
-        } else if (kind == 2) {
-            if (WRITE) wt->p2p_col[c->p2p] = x;
-            ++c->p2p;
-        } else if (kind == 3) {
-            for (int k = 0; k < 4; ++k) {
-                if (WRITE) {
-                    const u32 at = wt->child_at[k] + c->inh[k];
-                    wt->inh_next[at] = entry_with(x, second);
-                    wt->enode_next[at] = fa.z + k;
-                }
-                ++c->inh[k];
-            }
-        }
+        if (kind == 1)
+            sink.m2l(x);
+        else if (kind == 2)
+            sink.p2p(x);
+        else if (kind == 3)
+            sink.split(x, second);
         while (x != x0 && x - t.info[t.parent[x]].z == 3) x = t.parent[x];
         if (x == x0) break;
         ++x;
+    }
+}
+
+// Replays cached events into a writing sink.
+__device__ __forceinline__ void replay(const u32* cache, u32 stride, u32 events, WriteSink& sink) {
+    for (u32 i = 0; i < events; ++i) {
+        const u32 code = cache[(size_t)i * stride];
+        const int x = (int)(code & ~(kEvKind | kSecond));
+        const u32 kind = code & kEvKind;
+        if (kind == kEvM2L)
+            sink.m2l(x);
+        else if (kind == kEvP2P)
+            sink.p2p(x);
+        else
+            sink.split(x, (code & kSecond) != 0);
     }
 }
 
@@ -148,7 +207,7 @@ __device__ __forceinline__ u32 wave_sum_u32(u32 v) {
 // first entry of the level's i-th node (first[nodes] = the entry count); the count sits on
 // the device (lv[0]).  Every node has at least one entry (its self pair), so its first
 // entry's thread can write the node's row starts and its children's list starts.
-constexpr int kCols = 6;  // M2L, P2P, hand-downs to children 0..3
+constexpr int kCols = 3;  // M2L, P2P, hand-downs to each child
 constexpr unsigned kEntryBlocks = 2048;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
@@ -164,17 +223,23 @@ __global__ __launch_bounds__(256) void entry_count_kernel(RowsTree t, const u32*
                                                           const int* __restrict__ enode,
                                                           const u32* __restrict__ lv, u32 cap,
                                                           double theta, u32* __restrict__ cnt,
+                                                          u32* __restrict__ cache,
                                                           const u32* __restrict__ flag) {
     if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const u32 n = level_entries(lv, cap);
     for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         const int a = enode[e];
-        Counts c = {0, 0, {0, 0, 0, 0}};
-        walk_entry<false>(t, a, t.info[a], inh[e], theta, &c, nullptr);
+        const int4 fa = t.info[a];
+        const u32 entry = inh[e];
+        CountSink sink = {{0, 0, 0}, 0u, cache + e, cap};
+        if ((int)(entry & ~kSecond) == a)
+            self_entry(a, fa, sink);
+        else
+            walk_entry(t, a, fa, entry, theta, sink);
         u32* o = cnt + (size_t)kCols * e;
-        o[0] = c.m2l;
-        o[1] = c.p2p;
-        for (int k = 0; k < 4; ++k) o[2 + k] = c.inh[k];
+        o[0] = sink.c.m2l;
+        o[1] = sink.c.p2p;
+        o[2] = sink.c.inh;
     }
 }
 
@@ -187,7 +252,7 @@ __global__ __launch_bounds__(kScanThreads) void col_sum_kernel(const u32* __rest
     const u32 base = blockIdx.x * kScanTile;
     if (base >= n) return;  // uniform over the block
     __shared__ u32 s_sum[kScanThreads / 64][kCols];
-    u32 sum[kCols] = {0, 0, 0, 0, 0, 0};
+    u32 sum[kCols] = {0, 0, 0};
     for (int r = 0; r < kScanItems; ++r) {
         const u32 e = base + r * kScanThreads + threadIdx.x;
         if (e < n)
@@ -284,13 +349,14 @@ __global__ __launch_bounds__(256) void entry_write_kernel(
     const u32* __restrict__ first, i64 lo, const u32* __restrict__ lv, u32* __restrict__ lv_next,
     u32 cap, double theta, const u32* __restrict__ scan, const u32* __restrict__ totals,
     u32* __restrict__ base, int level, u32* __restrict__ m2l_row, int* __restrict__ m2l_col,
-    u32* __restrict__ p2p_row, int* __restrict__ p2p_col, u32* __restrict__ inh_next,
-    int* __restrict__ enode_next, u32* __restrict__ first_next, i64 next_lo, i64 next_nodes,
-    u32 cap_m2l, u32 cap_p2p, u32* __restrict__ flag) {
+    int* __restrict__ m2l_dst, u32* __restrict__ p2p_row, int* __restrict__ p2p_col,
+    u32* __restrict__ inh_next, int* __restrict__ enode_next, u32* __restrict__ first_next,
+    i64 next_lo, i64 next_nodes, u32 cap_m2l, u32 cap_p2p, const u32* __restrict__ cache,
+    u32* __restrict__ flag) {
     if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const u32 n = level_entries(lv, cap);
     const u32 bm = base[2 * level], bp = base[2 * level + 1];
-    const u32 inh_total = totals[2] + totals[3] + totals[4] + totals[5];
+    const u32 inh_total = 4 * totals[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         base[2 * level + 2] = bm + totals[0];
         base[2 * level + 3] = bp + totals[1];
@@ -303,34 +369,42 @@ __global__ __launch_bounds__(256) void entry_write_kernel(
         const u32 f = first[a - lo], f1 = first[a - lo + 1];
         const u32* se = scan + (size_t)kCols * e;
         const u32* sf = scan + (size_t)kCols * f;
-        // children's lists: child k of a starts after every earlier child's entries
-        u32 child_at[4] = {0, 0, 0, 0};
+        // children's lists: a hands h entries to each child, child k's list after the
+        // earlier children's
+        u32 child_base = 0, h = 0;
         if (fa.z >= 0) {
-            const u32* sl = f1 < n ? scan + (size_t)kCols * f1 : nullptr;
-            u32 start = sf[2] + sf[3] + sf[4] + sf[5];
-            for (int k = 0; k < 4; ++k) {
-                const u32 end_k = sl ? sl[2 + k] : totals[2 + k];
-                child_at[k] = start;
-                if (e == f) first_next[fa.z + k - next_lo] = start;
-                start += end_k - sf[2 + k];
-            }
+            h = (f1 < n ? scan[(size_t)kCols * f1 + 2] : totals[2]) - sf[2];
+            child_base = 4 * sf[2];
+            if (e == f)
+                for (int k = 0; k < 4; ++k) first_next[fa.z + k - next_lo] = child_base + k * h;
         }
         if (e == f) {
             m2l_row[a] = bm + sf[0];
             p2p_row[a] = bp + sf[1];
         }
-        Counts w = {bm + se[0], bp + se[1], {se[2] - sf[2], se[3] - sf[3], se[4] - sf[4], se[5] - sf[5]}};
+        const Counts at = {bm + se[0], bp + se[1], se[2] - sf[2]};
         // the outputs of this entry end where the next entry's begin
         const u32* nx = e + 1 < n ? scan + (size_t)kCols * (e + 1) : totals;
         bool fits = bm + nx[0] <= cap_m2l && bp + nx[1] <= cap_p2p;
-        if (fa.z >= 0)
-            for (int k = 0; k < 4; ++k) fits = fits && child_at[k] + (nx[2 + k] - sf[2 + k]) <= cap;
+        if (fa.z >= 0) fits = fits && child_base + 3 * h + (nx[2] - sf[2]) <= cap;
         if (!fits) {
             atomicOr(flag, 2u);
             continue;
         }
-        WriteTargets wt = {m2l_col, p2p_col, inh_next, enode_next, child_at};
-        walk_entry<true>(t, a, fa, inh[e], theta, &w, &wt);
+        WriteSink sink = {at, a, fa.z, m2l_col, m2l_dst, p2p_col, inh_next, enode_next,
+                          child_base, h};
+        const u32 entry = inh[e];
+        if ((int)(entry & ~kSecond) == a) {
+            self_entry(a, fa, sink);
+        } else {
+            // a walk's events: one per M2L, P2P or hand-down (each hand-down is one entry
+            // in every child's column)
+            const u32 events = (nx[0] - se[0]) + (nx[1] - se[1]) + (nx[2] - se[2]);
+            if (events <= (u32)kCached)
+                replay(cache + e, cap, events, sink);
+            else
+                walk_entry(t, a, fa, entry, theta, sink);
+        }
     }
 }
 
@@ -359,13 +433,6 @@ __global__ void rows_root_kernel(u32* inh, int* enode, u32* first, u32* lv, u32*
     }
 }
 
-// dst of every M2L entry (the row's node), for the lane-per-pair FP64 M2L
-__global__ void rows_dst_kernel(const u32* __restrict__ row, i64 count, int* __restrict__ dst) {
-    const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= count) return;
-    for (u32 k = row[r]; k < row[r + 1]; ++k) dst[k] = (int)r;
-}
-
 __global__ void rows_end_kernel(const u32* base, int levels, i64 count, u32* m2l_row,
                                 u32* p2p_row) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -375,7 +442,7 @@ __global__ void rows_end_kernel(const u32* base, int levels, i64 count, u32* m2l
 }
 
 struct RowsWs {
-    u32 *inh_a, *inh_b, *first_a, *first_b, *cnt, *part, *totals, *lv, *base, *flag;
+    u32 *inh_a, *inh_b, *first_a, *first_b, *cnt, *cache, *part, *totals, *lv, *base, *flag;
     int *enode_a, *enode_b, *parent;
 };
 
@@ -394,6 +461,7 @@ size_t rows_carve(RowsWs* w, void* p, i64 count, i64 width, i64 cap_inh, int lev
     w->first_a = (u32*)take(sizeof(u32) * (size_t)(width + 1));
     w->first_b = (u32*)take(sizeof(u32) * (size_t)(width + 1));
     w->cnt = (u32*)take(sizeof(u32) * kCols * (size_t)cap_inh);
+    w->cache = (u32*)take(sizeof(u32) * kCached * (size_t)cap_inh);
     w->part = (u32*)take(sizeof(u32) * kCols * (size_t)tiles);
     w->totals = (u32*)take(sizeof(u32) * kCols);
     w->lv = (u32*)take(sizeof(u32) * 2);
@@ -426,7 +494,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
                         int64_t* out_counts, void* stream) {
     TD3_ARG(levels_host != nullptr && n_levels >= 1 && n_levels <= 64, "levels");
     const i64 count = levels_host[n_levels];
-    TD3_ARG(count >= 1 && count < 0x7fffffff, "node count");
+    TD3_ARG(count >= 1 && count < (1 << 29), "node count");  // event codes hold 29 bits
     TD3_ARG(cap_m2l >= 1 && cap_m2l < 0xffffffffll && cap_p2p >= 1 && cap_p2p < 0xffffffffll &&
                 cap_inherit >= 16 && cap_inherit < 0xffffffffll, "capacities");
     TD3_ARG(out_counts != nullptr, "out_counts");
@@ -458,7 +526,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
         const i64 next_lo = hi, next_hi = d + 1 < n_levels ? levels_host[d + 2] : hi;
         const i64 next = next_hi - next_lo;
         TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(entry_count_kernel, dim3(kEntryBlocks), dim3(256), 0, s, t, inh, enode, lv,
-                           cap, theta, w.cnt, w.flag));
+                           cap, theta, w.cnt, w.cache, w.flag));
         TD3_LAUNCHED();
         TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_sum_kernel, dim3(tiles), dim3(kScanThreads), 0, s, w.cnt, lv, cap, w.part));
         TD3_LAUNCHED();
@@ -468,8 +536,9 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
         TD3_LAUNCHED();
         TD3_TIMED(s, "rows_write", hipLaunchKernelGGL(entry_write_kernel, dim3(kEntryBlocks), dim3(256), 0, s, t, inh, enode, first,
                            lo, lv, lv_next, cap, theta, w.cnt, w.totals, w.base, d, m2l_row,
-                           (int*)m2l_col, p2p_row, (int*)p2p_col, inh_next, enode_next,
-                           first_next, next_lo, next, (u32)cap_m2l, (u32)cap_p2p, w.flag));
+                           (int*)m2l_col, (int*)m2l_dst, p2p_row, (int*)p2p_col, inh_next,
+                           enode_next, first_next, next_lo, next, (u32)cap_m2l, (u32)cap_p2p,
+                           w.cache, w.flag));
         TD3_LAUNCHED();
         u32* tmp = inh;
         inh = inh_next;
@@ -499,11 +568,6 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     if (host[0] & 2u) {
         td3_set_error("interaction rows overflowed (m2l %u, p2p %u)", host[3], host[4]);
         return TD3_ERR_CAPACITY;
-    }
-    if (m2l_dst != nullptr && host[3] > 0) {
-        hipLaunchKernelGGL(rows_dst_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, m2l_row,
-                           count, (int*)m2l_dst);
-        TD3_LAUNCHED();
     }
     return TD3_OK;
 }
