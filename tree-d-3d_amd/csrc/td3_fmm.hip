@@ -381,6 +381,301 @@ __global__ void m2l_fixup_kernel(const u32* __restrict__ row_start, i64 node_lo,
         lre[node * p1 + l] = v;
 }
 
+// ------------------------------------------------------------------------------ M2L by class
+// Pairs of cells whose sizes are equal or differ by a factor 2 sit at offsets t = c_dst -
+// c_src = 2 h_min delta with 2 delta integral (cell centres lie on the finer cell's grid),
+// and |2 delta| <= 9 under the acceptance rule: such a pair's scaled operator
+// B_l = sum_k T[l][k] A_k depends on (size ratio, delta) only, up to the constant
+// A_0 ln(2 h_min) of B_0.  So same-class pairs of 16 destinations make one 16x16 operator
+// times a 16x16 block of moments on the matrix cores, with no per-pair powers:
+//   T[l][k] = (-1)^(l+1) coeff[k][l] u^k v^l (l >= 1),  T[0][k] = -coeff[k][0] u^k
+//   (+ ln delta at k = 0),  u = r_src / t,  v = r_dst / t  (r = sqrt(2) h).
+// A tile is 16 destinations at the same position among their siblings (so that in regular
+// regions their rows share their classes), each row sorted by class beforehand; the wave
+// steps through the union of the 16 rows' classes in order.  Other pairs (larger size
+// ratios, or other trees' cells) take the same product one destination at a time, with
+// the operator formed on the fly.
+constexpr int kClassDelta = 9;
+constexpr int kClassSide = 2 * kClassDelta + 1;
+constexpr u32 kClasses = 3 * kClassSide * kClassSide;
+constexpr u32 kGeneric = 0xffffu;
+constexpr int kClassTable = 256;  // floats per class and part: [kb][q][l]
+
+// 0: h_src = h_dst, 1: h_src = 2 h_dst, 2: h_src = h_dst / 2
+__device__ __forceinline__ u32 m2l_class_of(double tx, double ty, double hd, double hs) {
+    int ratio;
+    double hmin;
+    if (hs == hd) {
+        ratio = 0;
+        hmin = hd;
+    } else if (hs == 2.0 * hd) {
+        ratio = 1;
+        hmin = hd;
+    } else if (2.0 * hs == hd) {
+        ratio = 2;
+        hmin = hs;
+    } else {
+        return kGeneric;
+    }
+    const double fx = tx / hmin, fy = ty / hmin;  // 2 delta
+    const double rx = rint(fx), ry = rint(fy);
+    if (fabs(fx - rx) > 1e-9 || fabs(fy - ry) > 1e-9 || fabs(rx) > kClassDelta ||
+        fabs(ry) > kClassDelta)
+        return kGeneric;
+    return (u32)((ratio * kClassSide + (int)rx + kClassDelta) * kClassSide + (int)ry +
+                 kClassDelta);
+}
+
+// The operators of every class in FP64, stored FP32 as [class][kb][q][l] (k = 4 kb + q):
+// lane (l, q) of a wavefront reads its A operand for K-block kb at one offset.
+__global__ void m2l_class_table_kernel(const double* __restrict__ coeff, int p1,
+                                       float* __restrict__ tre, float* __restrict__ tim) {
+    const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (i64)kClasses * kClassTable) return;
+    const u32 cls = (u32)(idx / kClassTable);
+    const int rem = (int)(idx % kClassTable);
+    const int kb = rem / 64, q = (rem / 16) % 4, l = rem % 16, k = 4 * kb + q;
+    const int ratio = (int)(cls / (kClassSide * kClassSide));
+    const int dx2 = (int)((cls / kClassSide) % kClassSide) - kClassDelta;
+    const int dy2 = (int)(cls % kClassSide) - kClassDelta;
+    double re = 0.0, im = 0.0;
+    if ((dx2 != 0 || dy2 != 0) && k < p1 && l < p1) {
+        // delta = (dx2 + i dy2) / 2; u = (sqrt(2)/2) (h_src/h_min) / delta, v likewise
+        const double dr = 0.5 * dx2, di = 0.5 * dy2, d2 = dr * dr + di * di;
+        const double wr = dr / d2, wi = -di / d2;  // 1 / delta
+        const double s = 0.7071067811865476;
+        const double fs = ratio == 1 ? 2.0 : 1.0, fd = ratio == 2 ? 2.0 : 1.0;
+        double ur = s * fs * wr, ui = s * fs * wi, vr = s * fd * wr, vi = s * fd * wi;
+        double pr = coeff[k * p1 + l], pi = 0.0;
+        for (int i = 0; i < k; ++i) {
+            const double t = pr * ur - pi * ui;
+            pi = pr * ui + pi * ur;
+            pr = t;
+        }
+        for (int i = 0; i < l; ++i) {
+            const double t = pr * vr - pi * vi;
+            pi = pr * vi + pi * vr;
+            pr = t;
+        }
+        const double sign = l == 0 ? -1.0 : ((l & 1) ? 1.0 : -1.0);
+        re = sign * pr;
+        im = sign * pi;
+        if (k == 0 && l == 0) {
+            re += 0.5 * log(d2);
+            im += atan2(di, dr);
+        }
+    }
+    tre[idx] = (float)re;
+    tim[idx] = (float)im;
+}
+
+// Each row of [node_lo, node_hi) re-sorted by class (then source): key[e], src[e] at the
+// row's own positions.  A wavefront per row; rows up to kNetworkRow (64) entries by a
+// bitonic network across the lanes, longer ones by one lane (the CPU build of the tests
+// lowers the bound, so that both ways run there).
+#ifndef TD3_CLASS_NETWORK_ROW
+#define TD3_CLASS_NETWORK_ROW 64
+#endif
+constexpr int kNetworkRow = TD3_CLASS_NETWORK_ROW;
+__global__ __launch_bounds__(256) void m2l_classify_kernel(
+    const u32* __restrict__ row_start, const int* __restrict__ col, i64 node_lo, i64 node_hi,
+    const double* __restrict__ center, const double* __restrict__ half, u32* __restrict__ key,
+    int* __restrict__ src) {
+    const i64 d = node_lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    if (d >= node_hi) return;  // uniform over the wavefront
+    const int lane = td3_lane();
+    const u32 r0 = row_start[d], r1 = row_start[d + 1];
+    const int len = (int)(r1 - r0);
+    if (len == 0) return;
+    const double cx = center[2 * d], cy = center[2 * d + 1], hd = half[d];
+    auto class_at = [&](u32 e) {
+        const int s = col[e];
+        return m2l_class_of(cx - center[2 * s], cy - center[2 * s + 1], hd, half[s]);
+    };
+    if (len <= kNetworkRow) {
+        u64 v = ~0ull;
+        if (lane < len) {
+            const u32 e = r0 + (u32)lane;
+            v = ((u64)class_at(e) << 32) | (u32)col[e];
+        }
+#pragma unroll
+        for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const u32 lo = __shfl_xor((u32)v, stride, 64);
+                const u32 hi = __shfl_xor((u32)(v >> 32), stride, 64);
+                const u64 o = ((u64)hi << 32) | lo;
+                const bool up = (lane & size) == 0;
+                const bool lower = (lane & stride) == 0;
+                v = (lower == up) ? (v < o ? v : o) : (v > o ? v : o);
+            }
+        }
+        if (lane < len) {
+            key[r0 + lane] = (u32)(v >> 32);
+            src[r0 + lane] = (int)(u32)v;
+        }
+        return;
+    }
+    if (lane != 0) return;
+    // insertion sort in place (long rows are rare: a cell beside many small ones)
+    for (u32 e = r0; e < r1; ++e) {
+        const u64 v = ((u64)class_at(e) << 32) | (u32)col[e];
+        u32 i = e;
+        while (i > r0) {
+            const u64 w = ((u64)key[i - 1] << 32) | (u32)src[i - 1];
+            if (w <= v) break;
+            key[i] = key[i - 1];
+            src[i] = src[i - 1];
+            --i;
+        }
+        key[i] = (u32)(v >> 32);
+        src[i] = (int)(u32)v;
+    }
+}
+
+__device__ __forceinline__ u32 wave_min_u32(u32 v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const u32 o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Tiles over the sibling groups of [node_lo, node_hi) (node_lo - 1 divisible by 4: nodes
+// >= 1 come in groups of four siblings): wavefront t takes position k = t % 4 of the
+// 16 groups 16 (t / 4) .. + 15.  FP32 scaled moments in, scaled locals written (rows
+// without pairs untouched).
+__global__ __launch_bounds__(256) void m2l_class_kernel(
+    const u32* __restrict__ row_start, const u32* __restrict__ key, const int* __restrict__ src,
+    i64 node_lo, i64 node_hi, const double* __restrict__ center,
+    const double* __restrict__ half, const float* __restrict__ mre,
+    const float* __restrict__ mim, int p1, const double* __restrict__ coeff,
+    const float* __restrict__ tre, const float* __restrict__ tim, double* __restrict__ lre,
+    double* __restrict__ lim) {
+    const i64 t = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = td3_lane(), j = lane & 15, q = lane >> 4;
+    const i64 g0 = (node_lo - 1) / 4;
+    const i64 d = 1 + 4 * (g0 + 16 * (t / 4) + j) + (t % 4);
+    const bool has = d >= node_lo && d < node_hi;
+    if (!__ballot(has)) return;  // uniform
+    u32 cur = has ? row_start[d] : 0u;
+    const u32 end = has ? row_start[d + 1] : 0u;
+    // A_0 ln(2 h_min) of B_0: lane (j, q = 0) holds element (l = 0, dest j)
+    const double hd = has ? half[d] : 1.0;
+    const float ln2h = (float)log(2.0 * hd), ln_h = (float)log(hd);
+    td3_f32x4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
+    while (true) {
+        const u32 mykey = cur < end ? key[cur] : 0xffffffffu;
+        const u32 c = wave_min_u32(mykey);
+        if (c >= kGeneric) break;
+        const float* ar = tre + (size_t)c * kClassTable + lane;
+        const float* ai = tim + (size_t)c * kClassTable + lane;
+        float a_re[4], a_im[4], x_re[4] = {0.f, 0.f, 0.f, 0.f}, x_im[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            a_re[kb] = ar[64 * kb];
+            a_im[kb] = ai[64 * kb];
+        }
+        const bool mine = mykey == c;
+        if (mine) {
+            const int s = src[cur];
+            const float* xr = mre + (size_t)s * p1 + q;
+            const float* xi = mim + (size_t)s * p1 + q;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                if (4 * kb + q < p1) {
+                    x_re[kb] = xr[4 * kb];
+                    x_im[kb] = xi[4 * kb];
+                }
+            }
+            ++cur;
+        }
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            dre = mfma16(a_re[kb], x_re[kb], dre);
+            dre = mfma16(-a_im[kb], x_im[kb], dre);
+            dim = mfma16(a_re[kb], x_im[kb], dim);
+            dim = mfma16(a_im[kb], x_re[kb], dim);
+        }
+        if (mine && q == 0) {
+            // + A_0 ln(2 h_min): h_min = h_dst unless the source is the smaller cell
+            const float lh = c >= 2u * kClassSide * kClassSide ? ln_h : ln2h;
+            dre[0] += x_re[0] * lh;
+            dim[0] += x_im[0] * lh;
+        }
+    }
+    // the remaining (generic) pairs, one destination column at a time
+    while (true) {
+        const u64 pending = __ballot(cur < end);
+        if (!pending) break;
+        const int js = (__ffsll((unsigned long long)pending) - 1) & 15;
+        const u32 ce = __shfl(cur, js, 64);
+        const int s = src[ce];
+        const i64 ds = 1 + 4 * (g0 + 16 * (t / 4) + js) + (t % 4);
+        const double tx = center[2 * ds] - center[2 * s], ty = center[2 * ds + 1] - center[2 * s + 1];
+        const double t2 = tx * tx + ty * ty;
+        const double wr = tx / t2, wi = -ty / t2;
+        const double rs = 1.4142135623730951 * half[s], rd = 1.4142135623730951 * half[ds];
+        const Cx<float> u = {(float)(rs * wr), (float)(rs * wi)};
+        const Cx<float> v = {(float)(rd * wr), (float)(rd * wi)};
+        // this lane's operator entries T[l][4 kb + q], l = lane % 16
+        const int l = j;
+        Cx<float> vl = {1.f, 0.f}, vp = v;
+        for (int e = l; e > 0; e >>= 1) {
+            if (e & 1) vl = cmul(vl, vp);
+            vp = cmul(vp, vp);
+        }
+        const Cx<float> u2 = cmul(u, u);
+        const Cx<float> u4 = cmul(u2, u2);
+        Cx<float> uk = cpow_small(u, u2, cmul(u2, u), q);
+        const float sign = l == 0 ? -1.f : ((l & 1) ? 1.f : -1.f);
+        float a_re[4], a_im[4], x_re[4] = {0.f, 0.f, 0.f, 0.f}, x_im[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            const int k = 4 * kb + q;
+            const float cf = (k < p1 && l < p1) ? sign * (float)coeff[k * p1 + l] : 0.f;
+            const Cx<float> e = cmul(uk, vl);
+            a_re[kb] = cf * e.re;
+            a_im[kb] = cf * e.im;
+            uk = cmul(uk, u4);
+        }
+        if (lane == 0) {
+            // + ln t at (l, k) = (0, 0)
+            a_re[0] += (float)(0.5 * log(t2));
+            a_im[0] += (float)atan2(ty, tx);
+        }
+        if (j == js) {
+            const float* xr = mre + (size_t)s * p1 + q;
+            const float* xi = mim + (size_t)s * p1 + q;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                if (4 * kb + q < p1) {
+                    x_re[kb] = xr[4 * kb];
+                    x_im[kb] = xi[4 * kb];
+                }
+            }
+            ++cur;
+        }
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            dre = mfma16(a_re[kb], x_re[kb], dre);
+            dre = mfma16(-a_im[kb], x_im[kb], dre);
+            dim = mfma16(a_re[kb], x_im[kb], dim);
+            dim = mfma16(a_im[kb], x_re[kb], dim);
+        }
+    }
+    if (!has || end == row_start[d]) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int l = 4 * q + r;
+        if (l < p1) {
+            lre[d * p1 + l] = (double)dre[r];
+            lim[d * p1 + l] = (double)dim[r];
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------ scaling
 // A_k = a_k / r^k (r = sqrt(2) h), as T.
 template <typename T>
@@ -800,10 +1095,20 @@ int dispatch_m2l(hipStream_t s, int order, const int* col, const int* dst, const
 
 }  // namespace
 
+namespace {
+// the class path's workspace: sorted keys and sources per pair, the operator tables
+size_t class_bytes(i64 pair_count) {
+    return (size_t)pair_count * 8 + 2 * (size_t)kClasses * kClassTable * 4 + 1024;
+}
+}  // namespace
+
 extern "C" size_t td3_m2l_workspace_bytes(int64_t pair_count, int64_t order, int precision) {
     const i64 waves = (pair_count + 63) / 64 + 1;
     const size_t word = precision == TD3_FP64 ? 8 : 4;
-    return (size_t)waves * 2 * 2 * (size_t)(order + 1) * word + 256;
+    const size_t side = (size_t)waves * 2 * 2 * (size_t)(order + 1) * word + 256;
+    if (precision == TD3_FP64 || order + 1 > 16) return side;
+    const size_t cls = class_bytes(pair_count);
+    return side > cls ? side : cls;
 }
 
 extern "C" int td3_scale_moments(int precision, const double* moment_re, const double* moment_im,
@@ -835,6 +1140,32 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
     TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
     hipStream_t s = (hipStream_t)stream;
     if (node_hi == node_lo) return TD3_OK;
+    if (precision != TD3_FP64 && order + 1 <= 16 && dst_center == src_center &&
+        node_lo >= 1 && (node_lo - 1) % 4 == 0 && (node_hi - 1) % 4 == 0) {
+        // FP32 within one tree: by class on the matrix cores (rows sorted by class first)
+        const i64 pairs_cap = (i64)((workspace_bytes - 2 * (size_t)kClasses * kClassTable * 4 -
+                                     1024) / 8);
+        TD3_ARG(workspace != nullptr && workspace_bytes >= class_bytes(0), "workspace too small");
+        char* ws = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+        float* tre = (float*)ws;
+        float* tim = tre + (size_t)kClasses * kClassTable;
+        u32* key = (u32*)(tim + (size_t)kClasses * kClassTable);
+        int* src = (int*)(key + pairs_cap);
+        TD3_TIMED(s, "m2l_class_table", hipLaunchKernelGGL(m2l_class_table_kernel, dim3(td3_blocks((i64)kClasses * kClassTable, 256)), dim3(256), 0, s,
+                  coeff, (int)order + 1, tre, tim));
+        TD3_LAUNCHED();
+        TD3_TIMED(s, "m2l_classify", hipLaunchKernelGGL(m2l_classify_kernel, dim3(td3_blocks((node_hi - node_lo) * 64, 256)), dim3(256), 0, s,
+                  row_start, col, node_lo, node_hi, dst_center, dst_half, key, src));
+        TD3_LAUNCHED();
+        const i64 groups = (node_hi - node_lo + 3) / 4;
+        const i64 tiles = 4 * ((groups + 15) / 16);
+        TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL(m2l_class_kernel, dim3(td3_blocks(tiles * 64, 256)), dim3(256), 0, s,
+                  row_start, key, src, node_lo, node_hi, dst_center, dst_half,
+                  (const float*)scaled_re, (const float*)scaled_im, (int)order + 1, coeff, tre,
+                  tim, local_re, local_im));
+        TD3_LAUNCHED();
+        return TD3_OK;
+    }
     if (precision != TD3_FP64 && order + 1 <= 16) {
         // FP32: a wavefront per destination row on the matrix cores.  (FP64 stays on the
         // lane-per-pair kernel: on gfx950 v_mfma_f64_16x16x4f64 runs at the vector FP64
