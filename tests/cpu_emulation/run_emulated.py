@@ -172,7 +172,9 @@ def case_m2l_class():
         t.FP64_TOP_DEPTH = 0
         lre64, lim64 = t.local_expansions(0.5, precision="fp64")
         lre32, lim32 = t.local_expansions(0.5, precision="mixed")
-        for a, b in ((lre32, lre64), (lim32, lim64)):
+        # (Im B_0 aside: the FP32 per-pair path leaves out Im(A_0 ln t), which reaches
+        # neither the potential nor the field)
+        for a, b in ((lre32, lre64), (np.asarray(lim32)[:, 1:], np.asarray(lim64)[:, 1:])):
             a, b = np.asarray(a), np.asarray(b)
             # per coefficient: relative to the coefficient's largest value over the cells
             err = np.abs(a - b).max(0) / np.maximum(np.abs(b).max(0), 1e-300)

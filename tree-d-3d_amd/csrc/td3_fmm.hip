@@ -222,19 +222,34 @@ __device__ __forceinline__ Cx<T> cpow_small(Cx<T> u, Cx<T> u2, Cx<T> u3, int q) 
 __device__ __forceinline__ float rcp_of(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp_of(double x) { return 1.0 / x; }
 
+// generic_key (may be null): the row's pairs are only those from its first key >= kGeneric
+// on (rows sorted by m2l_classify_kernel); accumulate: the row's sum is added to the
+// locals instead of written.
 template <typename T>
 __global__ __launch_bounds__(256) void m2l_mfma_kernel(
     const int* __restrict__ col, const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
     const double* __restrict__ dcenter, const double* __restrict__ dhalf,
     const double* __restrict__ scenter, const double* __restrict__ shalf,
     const T* __restrict__ mre, const T* __restrict__ mim, int p1,
-    const double* __restrict__ coeff, double* __restrict__ lre, double* __restrict__ lim) {
+    const double* __restrict__ coeff, double* __restrict__ lre, double* __restrict__ lim,
+    const u32* __restrict__ generic_key, int accumulate) {
     typedef typename Vec4<T>::type V4;
     constexpr bool kFP64 = sizeof(T) == 8;
     const int lane = td3_lane();
     const i64 d = node_lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
     if (d >= node_hi) return;  // uniform over the wavefront
-    const u32 r0 = row_start[d], r1 = row_start[d + 1];
+    u32 r0 = row_start[d];
+    const u32 r1 = row_start[d + 1];
+    if (generic_key != nullptr) {
+        // the first generic entry (all lanes agree; the row is sorted)
+        u32 first = r1;
+        for (u32 b = r0; b < r1 && first == r1; b += 64) {
+            const u32 e = b + (u32)lane;
+            const u64 m = __ballot(e < r1 && generic_key[e] >= 0xffffu);
+            if (m) first = b + (u32)(__ffsll((unsigned long long)m) - 1);
+        }
+        r0 = first;
+    }
     if (r0 == r1) return;
     const int j = lane & 15, q = lane >> 4;
     // Element r of lane (j, q) of the product holds D[row_of(r)][j]:
@@ -350,8 +365,9 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
         for (int r = 0; r < 4; ++r) {
             const int l = row_of(r);
             if (l < p1) {
-                lre[d * p1 + l] = (double)acc_re[r];
-                lim[d * p1 + l] = (double)acc_im[r];
+                const double br = (double)acc_re[r], bi = (double)acc_im[r];
+                lre[d * p1 + l] = accumulate ? lre[d * p1 + l] + br : br;
+                lim[d * p1 + l] = accumulate ? lim[d * p1 + l] + bi : bi;
             }
         }
     }
@@ -470,13 +486,18 @@ __global__ void m2l_class_table_kernel(const double* __restrict__ coeff, int p1,
 }
 
 // Each row of [node_lo, node_hi) re-sorted by class (then source): key[e], src[e] at the
-// row's own positions.  A wavefront per row; rows up to kNetworkRow (64) entries by a
-// bitonic network across the lanes, longer ones by one lane (the CPU build of the tests
-// lowers the bound, so that both ways run there).
+// row's own positions.  A wavefront per row: rows up to kNetworkRow (64) entries by a
+// bitonic network across the lanes, up to kRankRow by ranks counted in LDS (the keys are
+// distinct: a source appears once per row), longer ones by one lane.  (The CPU build of
+// the tests lowers both bounds, so that every way runs there.)
 #ifndef TD3_CLASS_NETWORK_ROW
 #define TD3_CLASS_NETWORK_ROW 64
 #endif
+#ifndef TD3_CLASS_RANK_ROW
+#define TD3_CLASS_RANK_ROW 1024
+#endif
 constexpr int kNetworkRow = TD3_CLASS_NETWORK_ROW;
+constexpr int kRankRow = TD3_CLASS_RANK_ROW;
 __global__ __launch_bounds__(256) void m2l_classify_kernel(
     const u32* __restrict__ row_start, const int* __restrict__ col, i64 node_lo, i64 node_hi,
     const double* __restrict__ center, const double* __restrict__ half, u32* __restrict__ key,
@@ -516,8 +537,26 @@ __global__ __launch_bounds__(256) void m2l_classify_kernel(
         }
         return;
     }
+    if (len <= kRankRow) {
+        __shared__ u64 s_rank[4][kRankRow];
+        u64* buf = s_rank[(threadIdx.x / 64) & 3];
+        for (int i = lane; i < len; i += 64) {
+            const u32 e = r0 + (u32)i;
+            buf[i] = ((u64)class_at(e) << 32) | (u32)col[e];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int i = lane; i < len; i += 64) {
+            const u64 v = buf[i];
+            int rank = 0;
+            for (int m = 0; m < len; ++m) rank += buf[m] < v ? 1 : 0;
+            key[r0 + rank] = (u32)(v >> 32);
+            src[r0 + rank] = (int)(u32)v;
+        }
+        return;
+    }
     if (lane != 0) return;
-    // insertion sort in place (long rows are rare: a cell beside many small ones)
+    // insertion sort in place (rows this long are rare: a cell beside many small ones)
     for (u32 e = r0; e < r1; ++e) {
         const u64 v = ((u64)class_at(e) << 32) | (u32)col[e];
         u32 i = e;
@@ -544,134 +583,136 @@ __device__ __forceinline__ u32 wave_min_u32(u32 v) {
 
 // Tiles over the sibling groups of [node_lo, node_hi) (node_lo - 1 divisible by 4: nodes
 // >= 1 come in groups of four siblings): wavefront t takes position k = t % 4 of the
-// 16 groups 16 (t / 4) .. + 15.  FP32 scaled moments in, scaled locals written (rows
-// without pairs untouched).
+// 16 groups 16 (t / 4) .. + 15.  FP32 scaled moments in; each pair's product in FP32 from
+// zero, the sum over a row in FP64; the row's classed pairs written (rows without pairs
+// untouched; their other pairs are added by m2l_mfma_kernel afterwards).  The 16 rows'
+// keys and sources are staged in LDS, and the next class's operator and moments are
+// loaded while the matrix cores work on the current one.
+constexpr int kStage = 1024;  // staged entries per wavefront
+
 __global__ __launch_bounds__(256) void m2l_class_kernel(
-    const u32* __restrict__ row_start, const u32* __restrict__ key, const int* __restrict__ src,
-    i64 node_lo, i64 node_hi, const double* __restrict__ center,
+    const u32* __restrict__ row_start, const u32* __restrict__ key,
+    const int* __restrict__ src, i64 node_lo, i64 node_hi,
     const double* __restrict__ half, const float* __restrict__ mre,
-    const float* __restrict__ mim, int p1, const double* __restrict__ coeff,
-    const float* __restrict__ tre, const float* __restrict__ tim, double* __restrict__ lre,
-    double* __restrict__ lim) {
+    const float* __restrict__ mim, int p1, const float* __restrict__ tre,
+    const float* __restrict__ tim, double* __restrict__ lre, double* __restrict__ lim) {
+    __shared__ u32 s_key[4][kStage];
+    __shared__ int s_src[4][kStage];
     const i64 t = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const int lane = td3_lane(), j = lane & 15, q = lane >> 4;
+    const int lane = td3_lane(), j = lane & 15, q = lane >> 4, w = (threadIdx.x / 64) & 3;
     const i64 g0 = (node_lo - 1) / 4;
     const i64 d = 1 + 4 * (g0 + 16 * (t / 4) + j) + (t % 4);
     const bool has = d >= node_lo && d < node_hi;
     if (!__ballot(has)) return;  // uniform
-    u32 cur = has ? row_start[d] : 0u;
-    const u32 end = has ? row_start[d + 1] : 0u;
-    // A_0 ln(2 h_min) of B_0: lane (j, q = 0) holds element (l = 0, dest j)
+    const u32 r0 = has ? row_start[d] : 0u;
+    const u32 r1 = has ? row_start[d + 1] : 0u;
+    // stage: row j at offset off_j (exclusive prefix of the 16 lengths)
+    const u32 len_all = r1 - r0;
+    u32 off = len_all;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        const u32 y = __shfl_up(off, o, 16);
+        if (j >= o) off += y;
+    }
+    const u32 total = __shfl(off, 15, 64);
+    off -= len_all;
+    const bool staged = total <= (u32)kStage;
+    // len: the row's classed pairs (a prefix: generic keys sort last)
+    u32 len = 0;
+    for (u32 i = (u32)q; i < len_all; i += 4) {
+        const u32 k = key[r0 + i];
+        len += k < kGeneric ? 1u : 0u;
+        if (staged) {
+            s_key[w][off + i] = k;
+            s_src[w][off + i] = src[r0 + i];
+        }
+    }
+    len += __shfl_xor(len, 16, 64);
+    len += __shfl_xor(len, 32, 64);
+    if (staged) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    auto key_at = [&](u32 i) { return staged ? s_key[w][off + i] : key[r0 + i]; };
+    auto src_at = [&](u32 i) { return staged ? s_src[w][off + i] : src[r0 + i]; };
     const double hd = has ? half[d] : 1.0;
-    const float ln2h = (float)log(2.0 * hd), ln_h = (float)log(hd);
-    td3_f32x4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
-    while (true) {
-        const u32 mykey = cur < end ? key[cur] : 0xffffffffu;
-        const u32 c = wave_min_u32(mykey);
-        if (c >= kGeneric) break;
+    const double ln2h = log(2.0 * hd), ln_h = log(hd);
+    double acc_re[4] = {0.0, 0.0, 0.0, 0.0}, acc_im[4] = {0.0, 0.0, 0.0, 0.0};
+    // one class: its operator (A operand) and, for the lanes whose row has it, the source's
+    // moments (B operand)
+    struct Step {
+        float a_re[4], a_im[4], x_re[4], x_im[4];
+    };
+    auto load = [&](u32 c, bool mine, u32 i, Step& st) {
         const float* ar = tre + (size_t)c * kClassTable + lane;
         const float* ai = tim + (size_t)c * kClassTable + lane;
-        float a_re[4], a_im[4], x_re[4] = {0.f, 0.f, 0.f, 0.f}, x_im[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
-            a_re[kb] = ar[64 * kb];
-            a_im[kb] = ai[64 * kb];
+            st.a_re[kb] = ar[64 * kb];
+            st.a_im[kb] = ai[64 * kb];
+            st.x_re[kb] = 0.f;
+            st.x_im[kb] = 0.f;
         }
-        const bool mine = mykey == c;
         if (mine) {
-            const int s = src[cur];
+            const int s = src_at(i);
             const float* xr = mre + (size_t)s * p1 + q;
             const float* xi = mim + (size_t)s * p1 + q;
 #pragma unroll
             for (int kb = 0; kb < 4; ++kb) {
                 if (4 * kb + q < p1) {
-                    x_re[kb] = xr[4 * kb];
-                    x_im[kb] = xi[4 * kb];
+                    st.x_re[kb] = xr[4 * kb];
+                    st.x_im[kb] = xi[4 * kb];
                 }
             }
-            ++cur;
         }
+    };
+    u32 cur = 0;
+    u32 my = cur < len ? key_at(cur) : 0xffffffffu;
+    u32 c = wave_min_u32(my);
+    if (c >= kGeneric) c = 0xffffffffu;
+    Step now;
+    bool mine = my == c;
+    if (c != 0xffffffffu) load(c, mine, cur, now);
+    while (c != 0xffffffffu) {
+        // the next class, and its loads, before this one's products
+        const u32 nxt = cur + (mine ? 1u : 0u);
+        const u32 my_n = nxt < len ? key_at(nxt) : 0xffffffffu;
+        u32 c_n = wave_min_u32(my_n);
+        if (c_n >= kGeneric) c_n = 0xffffffffu;
+        const bool mine_n = my_n == c_n;
+        Step later;
+        if (c_n != 0xffffffffu) load(c_n, mine_n, nxt, later);
+        td3_f32x4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
-            dre = mfma16(a_re[kb], x_re[kb], dre);
-            dre = mfma16(-a_im[kb], x_im[kb], dre);
-            dim = mfma16(a_re[kb], x_im[kb], dim);
-            dim = mfma16(a_im[kb], x_re[kb], dim);
+            dre = mfma16(now.a_re[kb], now.x_re[kb], dre);
+            dre = mfma16(-now.a_im[kb], now.x_im[kb], dre);
+            dim = mfma16(now.a_re[kb], now.x_im[kb], dim);
+            dim = mfma16(now.a_im[kb], now.x_re[kb], dim);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            acc_re[r] += (double)dre[r];
+            acc_im[r] += (double)dim[r];
         }
         if (mine && q == 0) {
             // + A_0 ln(2 h_min): h_min = h_dst unless the source is the smaller cell
-            const float lh = c >= 2u * kClassSide * kClassSide ? ln_h : ln2h;
-            dre[0] += x_re[0] * lh;
-            dim[0] += x_im[0] * lh;
+            const double lh = c >= 2u * kClassSide * kClassSide ? ln_h : ln2h;
+            acc_re[0] += (double)now.x_re[0] * lh;
+            acc_im[0] += (double)now.x_im[0] * lh;
         }
+        now = later;
+        c = c_n;
+        mine = mine_n;
+        cur = nxt;
     }
-    // the remaining (generic) pairs, one destination column at a time
-    while (true) {
-        const u64 pending = __ballot(cur < end);
-        if (!pending) break;
-        const int js = (__ffsll((unsigned long long)pending) - 1) & 15;
-        const u32 ce = __shfl(cur, js, 64);
-        const int s = src[ce];
-        const i64 ds = 1 + 4 * (g0 + 16 * (t / 4) + js) + (t % 4);
-        const double tx = center[2 * ds] - center[2 * s], ty = center[2 * ds + 1] - center[2 * s + 1];
-        const double t2 = tx * tx + ty * ty;
-        const double wr = tx / t2, wi = -ty / t2;
-        const double rs = 1.4142135623730951 * half[s], rd = 1.4142135623730951 * half[ds];
-        const Cx<float> u = {(float)(rs * wr), (float)(rs * wi)};
-        const Cx<float> v = {(float)(rd * wr), (float)(rd * wi)};
-        // this lane's operator entries T[l][4 kb + q], l = lane % 16
-        const int l = j;
-        Cx<float> vl = {1.f, 0.f}, vp = v;
-        for (int e = l; e > 0; e >>= 1) {
-            if (e & 1) vl = cmul(vl, vp);
-            vp = cmul(vp, vp);
-        }
-        const Cx<float> u2 = cmul(u, u);
-        const Cx<float> u4 = cmul(u2, u2);
-        Cx<float> uk = cpow_small(u, u2, cmul(u2, u), q);
-        const float sign = l == 0 ? -1.f : ((l & 1) ? 1.f : -1.f);
-        float a_re[4], a_im[4], x_re[4] = {0.f, 0.f, 0.f, 0.f}, x_im[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            const int k = 4 * kb + q;
-            const float cf = (k < p1 && l < p1) ? sign * (float)coeff[k * p1 + l] : 0.f;
-            const Cx<float> e = cmul(uk, vl);
-            a_re[kb] = cf * e.re;
-            a_im[kb] = cf * e.im;
-            uk = cmul(uk, u4);
-        }
-        if (lane == 0) {
-            // + ln t at (l, k) = (0, 0)
-            a_re[0] += (float)(0.5 * log(t2));
-            a_im[0] += (float)atan2(ty, tx);
-        }
-        if (j == js) {
-            const float* xr = mre + (size_t)s * p1 + q;
-            const float* xi = mim + (size_t)s * p1 + q;
-#pragma unroll
-            for (int kb = 0; kb < 4; ++kb) {
-                if (4 * kb + q < p1) {
-                    x_re[kb] = xr[4 * kb];
-                    x_im[kb] = xi[4 * kb];
-                }
-            }
-            ++cur;
-        }
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            dre = mfma16(a_re[kb], x_re[kb], dre);
-            dre = mfma16(-a_im[kb], x_im[kb], dre);
-            dim = mfma16(a_re[kb], x_im[kb], dim);
-            dim = mfma16(a_im[kb], x_re[kb], dim);
-        }
-    }
-    if (!has || end == row_start[d]) return;
+    if (!has || r1 == r0) return;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int l = 4 * q + r;
         if (l < p1) {
-            lre[d * p1 + l] = (double)dre[r];
-            lim[d * p1 + l] = (double)dim[r];
+            lre[d * p1 + l] = acc_re[r];
+            lim[d * p1 + l] = acc_im[r];
         }
     }
 }
@@ -1160,9 +1201,14 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
         const i64 groups = (node_hi - node_lo + 3) / 4;
         const i64 tiles = 4 * ((groups + 15) / 16);
         TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL(m2l_class_kernel, dim3(td3_blocks(tiles * 64, 256)), dim3(256), 0, s,
-                  row_start, key, src, node_lo, node_hi, dst_center, dst_half,
-                  (const float*)scaled_re, (const float*)scaled_im, (int)order + 1, coeff, tre,
-                  tim, local_re, local_im));
+                  row_start, key, src, node_lo, node_hi, dst_half, (const float*)scaled_re,
+                  (const float*)scaled_im, (int)order + 1, tre, tim, local_re, local_im));
+        TD3_LAUNCHED();
+        // the other pairs (each row's generic suffix), added pair by pair
+        TD3_TIMED(s, "m2l_generic", hipLaunchKernelGGL((m2l_mfma_kernel<float>), dim3(td3_blocks((node_hi - node_lo) * 64, 256)), dim3(256), 0, s,
+                  src, row_start, node_lo, node_hi, dst_center, dst_half, src_center, src_half,
+                  (const float*)scaled_re, (const float*)scaled_im, (int)order + 1, coeff,
+                  local_re, local_im, key, 1));
         TD3_LAUNCHED();
         return TD3_OK;
     }
@@ -1174,7 +1220,7 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
         TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<float>), grid, dim3(256), 0, s, col, row_start,
                            node_lo, node_hi, dst_center, dst_half, src_center, src_half,
                            (const float*)scaled_re, (const float*)scaled_im, (int)order + 1,
-                           coeff, local_re, local_im));
+                           coeff, local_re, local_im, nullptr, 0));
         TD3_LAUNCHED();
         return TD3_OK;
     }

@@ -35,29 +35,32 @@ namespace {
 constexpr u32 kSecond = 0x80000000u;
 
 struct RowsTree {
-    const double* center;
-    const double* half;
-    const int4* info;  // {start, end, child, depth}
+    const double4* geo;  // {centre x, centre y, half, 0}: one load per visited cell
+    const int4* info;    // {start, end, child, depth}
     const int* parent;
+    int sib0;            // the first non-root node: siblings come in fours from here
 };
 
-// The decision for the pair {a, x} (a is `a_second` ? second : first).  Returns 0 drop,
-// 1 M2L, 2 P2P, 3 split a, 4 split x.
-__device__ __forceinline__ int decide(const RowsTree& t, int a, int4 fa, int x, bool a_second,
-                                      double theta) {
+// The decision for the pair {a, x} (a is `a_second` ? second : first; ga, fa its geometry
+// and info).  Returns 0 drop, 1 M2L, 2 P2P, 3 split a, 4 split x (x's first child in
+// *child).
+__device__ __forceinline__ int decide(const RowsTree& t, double4 ga, int4 fa, int x,
+                                      bool a_second, double theta, int* child) {
     const int4 fx = t.info[x];
     if (fx.y == fx.x || fa.y == fa.x) return 0;
+    const double4 gx = t.geo[x];
     const bool leaf_a = fa.z < 0, leaf_x = fx.z < 0;
-    const int f = a_second ? x : a, s = a_second ? a : x;
-    const double dx = __dadd_rn(t.center[2 * f], -t.center[2 * s]);
-    const double dy = __dadd_rn(t.center[2 * f + 1], -t.center[2 * s + 1]);
+    const double4 gf = a_second ? gx : ga, gs = a_second ? ga : gx;
+    const double dx = __dadd_rn(gf.x, -gs.x);
+    const double dy = __dadd_rn(gf.y, -gs.y);
     const double distance = __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
-    const double radii = __dmul_rn(1.4142135623730951, __dadd_rn(t.half[f], t.half[s]));
+    const double radii = __dmul_rn(1.4142135623730951, __dadd_rn(gf.z, gs.z));
     if (radii < __dmul_rn(theta, distance)) return 1;
     if (leaf_a && leaf_x) return 2;
     const bool leaf_f = a_second ? leaf_x : leaf_a, leaf_s = a_second ? leaf_a : leaf_x;
-    const bool split_first = leaf_s || (!leaf_f && t.half[f] >= t.half[s]);
+    const bool split_first = leaf_s || (!leaf_f && gf.z >= gs.z);
     const bool split_a = split_first != a_second;
+    *child = fx.z;
     return split_a ? 3 : 4;
 }
 
@@ -164,11 +167,13 @@ __device__ __forceinline__ void walk_entry(const RowsTree& t, int a, int4 fa, u3
                            Sink& sink) {
     const int x0 = (int)(entry & ~kSecond);
     const bool second = (entry & kSecond) != 0;
+    const double4 ga = t.geo[a];
     int x = x0;
     while (true) {
-        const int kind = decide(t, a, fa, x, second, theta);
+        int child = -1;
+        const int kind = decide(t, ga, fa, x, second, theta, &child);
         if (kind == 4) {
-            x = t.info[x].z;
+            x = child;
             continue;
         }
         if (kind == 1)
@@ -177,7 +182,7 @@ __device__ __forceinline__ void walk_entry(const RowsTree& t, int a, int4 fa, u3
             sink.p2p(x);
         else if (kind == 3)
             sink.split(x, second);
-        while (x != x0 && x - t.info[t.parent[x]].z == 3) x = t.parent[x];
+        while (x != x0 && ((x - t.sib0) & 3) == 3) x = t.parent[x];
         if (x == x0) break;
         ++x;
     }
@@ -212,7 +217,7 @@ __device__ __forceinline__ u32 wave_sum_u32(u32 v) {
 constexpr int kCols = 3;  // M2L, P2P, hand-downs to each child
 constexpr unsigned kEntryBlocks = 2048;
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 4;
 constexpr u32 kScanTile = kScanThreads * kScanItems;
 
 __device__ __forceinline__ u32 level_entries(const u32* lv, u32 cap) {
@@ -220,16 +225,33 @@ __device__ __forceinline__ u32 level_entries(const u32* lv, u32 cap) {
     return n < cap ? n : cap;
 }
 
-// Count step: every entry's walk, counted per column.
-__global__ __launch_bounds__(256) void entry_count_kernel(RowsTree t, const u32* __restrict__ inh,
-                                                          const int* __restrict__ enode,
-                                                          const u32* __restrict__ lv, u32 cap,
-                                                          double theta, u32* __restrict__ cnt,
-                                                          u32* __restrict__ cache,
-                                                          const u32* __restrict__ flag) {
+// Count step: every entry's walk, counted per column; block b takes the tile of kScanTile
+// entries from b kScanTile and writes the tile's column sums to part.
+__device__ __forceinline__ void block_sum3(u32 v[kCols], u32 (*s_sum)[kCols]) {
+    for (int c = 0; c < kCols; ++c) v[c] = wave_sum_u32(v[c]);
+    if (td3_lane() == 0)
+        for (int c = 0; c < kCols; ++c) s_sum[threadIdx.x / 64][c] = v[c];
+    __syncthreads();
+    for (int c = 0; c < kCols; ++c) {
+        u32 t = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) t += s_sum[w][c];
+        v[c] = t;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void entry_count_kernel(
+    RowsTree t, const u32* __restrict__ inh, const int* __restrict__ enode,
+    const u32* __restrict__ lv, u32 cap, double theta, u32* __restrict__ cnt,
+    u32* __restrict__ cache, u32* __restrict__ part, const u32* __restrict__ flag) {
     if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const u32 n = level_entries(lv, cap);
-    for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    __shared__ u32 s_sum[kScanThreads / 64][kCols];
+    for (u32 tile = blockIdx.x; tile * kScanTile < n; tile += gridDim.x) {
+    const u32 base = tile * kScanTile;
+    u32 sum[kCols] = {0, 0, 0};
+    for (int r = 0; r < kScanItems; ++r) {
+        const u32 e = base + r * kScanThreads + threadIdx.x;
+        if (e >= n) continue;
         const int a = enode[e];
         const int4 fa = t.info[a];
         const u32 entry = inh[e];
@@ -242,77 +264,40 @@ __global__ __launch_bounds__(256) void entry_count_kernel(RowsTree t, const u32*
         o[0] = sink.c.m2l;
         o[1] = sink.c.p2p;
         o[2] = sink.c.inh;
+        sum[0] += sink.c.m2l;
+        sum[1] += sink.c.p2p;
+        sum[2] += sink.c.inh;
+    }
+    block_sum3(sum, s_sum);
+    if (threadIdx.x < kCols) part[(size_t)kCols * tile + threadIdx.x] = sum[threadIdx.x];
+    __syncthreads();  // s_sum is reused by the next tile
     }
 }
 
-// Exclusive scan of the kCols columns over the level's entries, in three steps (sums of
-// tiles, scan of the sums, scan inside each tile); the entry count is read on the device.
-__global__ __launch_bounds__(kScanThreads) void col_sum_kernel(const u32* __restrict__ cnt,
-                                                               const u32* __restrict__ lv, u32 cap,
-                                                               u32* __restrict__ part) {
-    const u32 n = level_entries(lv, cap);
-    const u32 base = blockIdx.x * kScanTile;
-    if (base >= n) return;  // uniform over the block
-    __shared__ u32 s_sum[kScanThreads / 64][kCols];
-    u32 sum[kCols] = {0, 0, 0};
-    for (int r = 0; r < kScanItems; ++r) {
-        const u32 e = base + r * kScanThreads + threadIdx.x;
-        if (e < n)
-            for (int c = 0; c < kCols; ++c) sum[c] += cnt[(size_t)kCols * e + c];
-    }
-    for (int c = 0; c < kCols; ++c) sum[c] = wave_sum_u32(sum[c]);
-    if (td3_lane() == 0)
-        for (int c = 0; c < kCols; ++c) s_sum[threadIdx.x / 64][c] = sum[c];
-    __syncthreads();
-    if (threadIdx.x < kCols) {
-        u32 v = 0;
-        for (int w = 0; w < kScanThreads / 64; ++w) v += s_sum[w][threadIdx.x];
-        part[(size_t)kCols * blockIdx.x + threadIdx.x] = v;
-    }
-}
-
-// One block: exclusive scan of the tile sums per column (each thread a run of tiles);
-// totals[c] = the column's total.
-__global__ __launch_bounds__(256) void col_part_kernel(u32* __restrict__ part, const u32* __restrict__ lv,
-                                                       u32 cap, u32* __restrict__ totals) {
-    __shared__ u32 s_sum[256];
-    const u32 n = level_entries(lv, cap);
-    const u32 tiles = (n + kScanTile - 1) / kScanTile;
-    const u32 per = (tiles + 255) / 256;
-    const u32 i0 = threadIdx.x * per;
-    const u32 i1 = i0 + per < tiles ? i0 + per : tiles;
-    for (int c = 0; c < kCols; ++c) {
-        u32 sum = 0;
-        for (u32 i = i0; i < i1; ++i) sum += part[(size_t)kCols * i + c];
-        s_sum[threadIdx.x] = sum;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const u32 v = threadIdx.x >= (unsigned)off ? s_sum[threadIdx.x - off] : 0u;
-            __syncthreads();
-            s_sum[threadIdx.x] += v;
-            __syncthreads();
-        }
-        u32 run = threadIdx.x > 0 ? s_sum[threadIdx.x - 1] : 0u;
-        for (u32 i = i0; i < i1; ++i) {
-            const u32 v = part[(size_t)kCols * i + c];
-            part[(size_t)kCols * i + c] = run;
-            run += v;
-        }
-        if (threadIdx.x == 255) totals[c] = s_sum[255];
-        __syncthreads();
-    }
-}
-
+// Exclusive scan of the columns inside each tile, carried by the sums of the tiles before
+// it (each block adds them up itself); block 0 also writes the columns' totals.
 __global__ __launch_bounds__(kScanThreads) void col_scan_kernel(u32* __restrict__ cnt,
                                                                 const u32* __restrict__ lv, u32 cap,
-                                                                const u32* __restrict__ part) {
+                                                                const u32* __restrict__ part,
+                                                                u32* __restrict__ totals,
+                                                                const u32* __restrict__ flag) {
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const u32 n = level_entries(lv, cap);
-    const u32 base = blockIdx.x * kScanTile;
-    if (base >= n) return;  // uniform over the block
+    __shared__ u32 s_sum[kScanThreads / 64][kCols];
     __shared__ u32 s_wave[kScanThreads / 64][kCols];
     const int lane = td3_lane(), wave = threadIdx.x / 64;
-    u32 carry[kCols];
-    for (int c = 0; c < kCols; ++c) carry[c] = part[(size_t)kCols * blockIdx.x + c];
+    const u32 tiles = (n + kScanTile - 1) / kScanTile;
+    for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const u32 base = tile * kScanTile;
+    u32 carry[kCols] = {0, 0, 0};
+    const u32 upto = tile == 0 ? tiles : tile;
+    for (u32 i = threadIdx.x; i < upto; i += kScanThreads)
+        for (int c = 0; c < kCols; ++c) carry[c] += part[(size_t)kCols * i + c];
+    block_sum3(carry, s_sum);
+    if (tile == 0) {
+        if (threadIdx.x < kCols) totals[threadIdx.x] = carry[threadIdx.x];
+        for (int c = 0; c < kCols; ++c) carry[c] = 0;
+    }
     for (int r = 0; r < kScanItems; ++r) {
         const u32 e = base + r * kScanThreads + threadIdx.x;
         u32 v[kCols], incl[kCols];
@@ -340,6 +325,7 @@ __global__ __launch_bounds__(kScanThreads) void col_scan_kernel(u32* __restrict_
             carry[c] += all;
         }
         __syncthreads();
+    }
     }
 }
 
@@ -423,6 +409,12 @@ __global__ void rows_parent_kernel(const int4* __restrict__ info, i64 count, int
         atomicAdd(leaves, 1u);
 }
 
+__global__ void rows_geo_kernel(const double* __restrict__ center, const double* __restrict__ half,
+                                i64 count, double4* __restrict__ geo) {
+    const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node < count) geo[node] = make_double4(center[2 * node], center[2 * node + 1], half[node], 0.0);
+}
+
 __global__ void rows_root_kernel(u32* inh, int* enode, u32* first, u32* lv, u32* base) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         inh[0] = 0u;  // the pair (root, root)
@@ -446,6 +438,7 @@ __global__ void rows_end_kernel(const u32* base, int levels, i64 count, u32* m2l
 struct RowsWs {
     u32 *inh_a, *inh_b, *first_a, *first_b, *cnt, *cache, *part, *totals, *lv, *base, *flag;
     int *enode_a, *enode_b, *parent;
+    double4* geo;
 };
 
 size_t rows_carve(RowsWs* w, void* p, i64 count, i64 width, i64 cap_inh, int levels) {
@@ -470,6 +463,7 @@ size_t rows_carve(RowsWs* w, void* p, i64 count, i64 width, i64 cap_inh, int lev
     w->base = (u32*)take(sizeof(u32) * (size_t)(2 * levels + 4));
     w->flag = (u32*)take(sizeof(u32) * 4);
     w->parent = (int*)take(sizeof(int) * (size_t)count);
+    w->geo = (double4*)take(sizeof(double4) * (size_t)count);
     return off;
 }
 
@@ -507,10 +501,13 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     const i64 width = widest_level(levels_host, (int)n_levels);
     rows_carve(&w, workspace, count, width, cap_inherit, (int)n_levels);
     RowsTree t;
-    t.center = node_center;
-    t.half = node_half;
+    t.geo = w.geo;
     t.info = (const int4*)node_info;
     t.parent = w.parent;
+    t.sib0 = (int)(levels_host[1] - levels_host[0]);
+    hipLaunchKernelGGL(rows_geo_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_center,
+                       node_half, count, w.geo);
+    TD3_LAUNCHED();
     TD3_HIP(hipMemsetAsync(w.flag, 0, sizeof(u32) * 4, s));
     hipLaunchKernelGGL(rows_parent_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s,
                        (const int4*)node_info, count, w.parent, w.flag + 2);
@@ -523,18 +520,18 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     u32 *lv = w.lv, *lv_next = w.lv + 1;
     const u32 cap = (u32)cap_inherit;
     const unsigned tiles = (unsigned)((cap_inherit + kScanTile - 1) / kScanTile);
+    // blocks stride over the tiles: enough to fill the chip, few to launch when a level
+    // is small (the entry counts stay on the device)
+    const unsigned grid_tiles = tiles < kEntryBlocks ? tiles : kEntryBlocks;
     for (int d = 0; d < n_levels; ++d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
         const i64 next_lo = hi, next_hi = d + 1 < n_levels ? levels_host[d + 2] : hi;
         const i64 next = next_hi - next_lo;
-        TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(entry_count_kernel, dim3(kEntryBlocks), dim3(256), 0, s, t, inh, enode, lv,
-                           cap, theta, w.cnt, w.cache, w.flag));
+        TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(entry_count_kernel, dim3(grid_tiles), dim3(kScanThreads), 0, s, t, inh, enode, lv,
+                           cap, theta, w.cnt, w.cache, w.part, w.flag));
         TD3_LAUNCHED();
-        TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_sum_kernel, dim3(tiles), dim3(kScanThreads), 0, s, w.cnt, lv, cap, w.part));
-        TD3_LAUNCHED();
-        TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_part_kernel, dim3(1), dim3(256), 0, s, w.part, lv, cap, w.totals));
-        TD3_LAUNCHED();
-        TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_scan_kernel, dim3(tiles), dim3(kScanThreads), 0, s, w.cnt, lv, cap, w.part));
+        TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_scan_kernel, dim3(grid_tiles), dim3(kScanThreads), 0, s, w.cnt, lv, cap, w.part,
+                           w.totals, w.flag));
         TD3_LAUNCHED();
         TD3_TIMED(s, "rows_write", hipLaunchKernelGGL(entry_write_kernel, dim3(kEntryBlocks), dim3(256), 0, s, t, inh, enode, first,
                            lo, lv, lv_next, cap, theta, w.cnt, w.totals, w.base, d, m2l_row,
