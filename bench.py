@@ -29,8 +29,7 @@ import __graft_entry__  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # measured (rocprofv3 --pmc, calibrated by a 256 MiB copy) for this workload's M2L launch
 M2L_TRAFFIC_BYTES = 923.1e6
-KERNEL_NAMES = ("m2l_mfma", "m2l_classify", "m2l_class_table", "m2l", "m2l_fixup", "eval", "l2l",
-                "rows_count", "rows_scan", "rows_write",
+KERNEL_NAMES = ("m2l_mfma", "m2l", "m2l_fixup", "eval", "l2l", "rows_count", "rows_scan", "rows_write",
                 "traverse",
                 "row_sort", "lists_fill",
                 "lists_degree", "sort_scatter", "sort_hist", "scan", "tree_keys", "tree_count",
@@ -367,15 +366,15 @@ def main() -> int:
     ms = kernels["m2l_mfma"]["ms"] / kernels["m2l_mfma"]["launches"]
     achieved = 910.0 * mfma_pairs / (ms * 1e-3) / 1e12
     result["roofline"] = {
-        "kernel": "m2l_class_kernel (FP32 M2L by offset class on v_mfma_f32_16x16x4_f32, "
-                  "destinations below the FP64 top levels)",
+        "kernel": "m2l_mfma_kernel (FP32 M2L on v_mfma_f32_16x16x4_f32, destinations below "
+                  "the FP64 top levels)",
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
-        # HBM bytes of one m2l_class_kernel launch, rocprofv3 FETCH_SIZE x 2.000 +
-        # WRITE_SIZE x 1.000 (factors from a 256 MiB copy in the same run)
+        # HBM bytes of one m2l_mfma_kernel launch, rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE
+        # x 1.000 (factors from a 256 MiB copy in the same run)
         "traffic": M2L_TRAFFIC_BYTES,
         "algorithmic": {"m2l_directed_pairs": m2l_directed, "mfma_directed_pairs": mfma_pairs,
                         "launch_ms": ms, "near_directed_pairs": near_pairs,

@@ -7,6 +7,6 @@ ROOT=$(cd "$HERE/../.." && pwd)
 OUT="$HERE/libtd3_emu.so"
 SRCS=$(ls "$ROOT"/tree-d-3d_amd/csrc/*.hip)
 g++ -std=c++20 -O1 -g -fPIC -shared -pthread -fsanitize=address -fno-omit-frame-pointer \
-    -DTD3_TRAVERSE_BLOCKS=4 -DTD3_CLASS_NETWORK_ROW=12 -DTD3_CLASS_RANK_ROW=30 -I"$HERE" -I"$ROOT/tree-d-3d_amd/csrc" -I"$ROOT/include" \
+    -DTD3_TRAVERSE_BLOCKS=4 -I"$HERE" -I"$ROOT/tree-d-3d_amd/csrc" -I"$ROOT/include" \
     -Wno-unused-result -x c++ $SRCS -o "$OUT"
 echo "$OUT"

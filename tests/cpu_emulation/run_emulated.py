@@ -159,33 +159,6 @@ def case_fmm():
         assert relative(got[1], want[1]) < 1e-12
 
 
-def case_m2l_class():
-    # FP32 M2L by class from depth 1 down (the product, the class table, the sort by class
-    # and the generic pairs) against the FP64 locals of the same tree
-    rng = np.random.default_rng(17)
-    pos = np.concatenate((rng.uniform(-1, 1, (2, 1500)),
-                          rng.normal([[0.3], [-0.2]], 0.02, (2, 1500))), axis=1).clip(-1, 1)
-    q = rng.choice([-1.0, 1.0], 3000)
-    for order, symmetric in ((12, False), (7, True)):
-        t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=order,
-                             leaf_capacity=4, symmetric=symmetric)
-        t.FP64_TOP_DEPTH = 0
-        lre64, lim64 = t.local_expansions(0.5, precision="fp64")
-        lre32, lim32 = t.local_expansions(0.5, precision="mixed")
-        # (Im B_0 aside: the FP32 per-pair path leaves out Im(A_0 ln t), which reaches
-        # neither the potential nor the field)
-        for a, b in ((lre32, lre64), (np.asarray(lim32)[:, 1:], np.asarray(lim64)[:, 1:])):
-            a, b = np.asarray(a), np.asarray(b)
-            # per coefficient: relative to the coefficient's largest value over the cells
-            err = np.abs(a - b).max(0) / np.maximum(np.abs(b).max(0), 1e-300)
-            assert err.max() < 1e-5, err
-        p64 = t.fmm(theta=0.5, softening=1e-3)
-        p32 = t.fmm(theta=0.5, softening=1e-3, precision="mixed")
-        # every level's M2L in FP32 here (the top levels included, whose terms are the
-        # largest): FP32 rounding, a misplaced term would be orders larger
-        assert relative(p32[0], p64[0]) < 5e-6 and relative(p32[1], p64[1]) < 5e-6
-
-
 def case_fmm_at():
     rng = np.random.default_rng(13)
     pos, q = cloud(250, 13)

@@ -222,34 +222,19 @@ __device__ __forceinline__ Cx<T> cpow_small(Cx<T> u, Cx<T> u2, Cx<T> u3, int q) 
 __device__ __forceinline__ float rcp_of(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp_of(double x) { return 1.0 / x; }
 
-// generic_key (may be null): the row's pairs are only those from its first key >= kGeneric
-// on (rows sorted by m2l_classify_kernel); accumulate: the row's sum is added to the
-// locals instead of written.
 template <typename T>
 __global__ __launch_bounds__(256) void m2l_mfma_kernel(
     const int* __restrict__ col, const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
     const double* __restrict__ dcenter, const double* __restrict__ dhalf,
     const double* __restrict__ scenter, const double* __restrict__ shalf,
     const T* __restrict__ mre, const T* __restrict__ mim, int p1,
-    const double* __restrict__ coeff, double* __restrict__ lre, double* __restrict__ lim,
-    const u32* __restrict__ generic_key, int accumulate) {
+    const double* __restrict__ coeff, double* __restrict__ lre, double* __restrict__ lim) {
     typedef typename Vec4<T>::type V4;
     constexpr bool kFP64 = sizeof(T) == 8;
     const int lane = td3_lane();
     const i64 d = node_lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
     if (d >= node_hi) return;  // uniform over the wavefront
-    u32 r0 = row_start[d];
-    const u32 r1 = row_start[d + 1];
-    if (generic_key != nullptr) {
-        // the first generic entry (all lanes agree; the row is sorted)
-        u32 first = r1;
-        for (u32 b = r0; b < r1 && first == r1; b += 64) {
-            const u32 e = b + (u32)lane;
-            const u64 m = __ballot(e < r1 && generic_key[e] >= 0xffffu);
-            if (m) first = b + (u32)(__ffsll((unsigned long long)m) - 1);
-        }
-        r0 = first;
-    }
+    const u32 r0 = row_start[d], r1 = row_start[d + 1];
     if (r0 == r1) return;
     const int j = lane & 15, q = lane >> 4;
     // Element r of lane (j, q) of the product holds D[row_of(r)][j]:
@@ -365,9 +350,8 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
         for (int r = 0; r < 4; ++r) {
             const int l = row_of(r);
             if (l < p1) {
-                const double br = (double)acc_re[r], bi = (double)acc_im[r];
-                lre[d * p1 + l] = accumulate ? lre[d * p1 + l] + br : br;
-                lim[d * p1 + l] = accumulate ? lim[d * p1 + l] + bi : bi;
+                lre[d * p1 + l] = (double)acc_re[r];
+                lim[d * p1 + l] = (double)acc_im[r];
             }
         }
     }
@@ -395,326 +379,6 @@ __global__ void m2l_fixup_kernel(const u32* __restrict__ row_start, i64 node_lo,
         lim[node * p1 + l] = v;
     else
         lre[node * p1 + l] = v;
-}
-
-// ------------------------------------------------------------------------------ M2L by class
-// Pairs of cells whose sizes are equal or differ by a factor 2 sit at offsets t = c_dst -
-// c_src = 2 h_min delta with 2 delta integral (cell centres lie on the finer cell's grid),
-// and |2 delta| <= 9 under the acceptance rule: such a pair's scaled operator
-// B_l = sum_k T[l][k] A_k depends on (size ratio, delta) only, up to the constant
-// A_0 ln(2 h_min) of B_0.  So same-class pairs of 16 destinations make one 16x16 operator
-// times a 16x16 block of moments on the matrix cores, with no per-pair powers:
-//   T[l][k] = (-1)^(l+1) coeff[k][l] u^k v^l (l >= 1),  T[0][k] = -coeff[k][0] u^k
-//   (+ ln delta at k = 0),  u = r_src / t,  v = r_dst / t  (r = sqrt(2) h).
-// A tile is 16 destinations at the same position among their siblings (so that in regular
-// regions their rows share their classes), each row sorted by class beforehand; the wave
-// steps through the union of the 16 rows' classes in order.  Other pairs (larger size
-// ratios, or other trees' cells) take the same product one destination at a time, with
-// the operator formed on the fly.
-constexpr int kClassDelta = 9;
-constexpr int kClassSide = 2 * kClassDelta + 1;
-constexpr u32 kClasses = 3 * kClassSide * kClassSide;
-constexpr u32 kGeneric = 0xffffu;
-constexpr int kClassTable = 256;  // floats per class and part: [kb][q][l]
-
-// 0: h_src = h_dst, 1: h_src = 2 h_dst, 2: h_src = h_dst / 2
-__device__ __forceinline__ u32 m2l_class_of(double tx, double ty, double hd, double hs) {
-    int ratio;
-    double hmin;
-    if (hs == hd) {
-        ratio = 0;
-        hmin = hd;
-    } else if (hs == 2.0 * hd) {
-        ratio = 1;
-        hmin = hd;
-    } else if (2.0 * hs == hd) {
-        ratio = 2;
-        hmin = hs;
-    } else {
-        return kGeneric;
-    }
-    const double fx = tx / hmin, fy = ty / hmin;  // 2 delta
-    const double rx = rint(fx), ry = rint(fy);
-    if (fabs(fx - rx) > 1e-9 || fabs(fy - ry) > 1e-9 || fabs(rx) > kClassDelta ||
-        fabs(ry) > kClassDelta)
-        return kGeneric;
-    return (u32)((ratio * kClassSide + (int)rx + kClassDelta) * kClassSide + (int)ry +
-                 kClassDelta);
-}
-
-// The operators of every class in FP64, stored FP32 as [class][kb][q][l] (k = 4 kb + q):
-// lane (l, q) of a wavefront reads its A operand for K-block kb at one offset.
-__global__ void m2l_class_table_kernel(const double* __restrict__ coeff, int p1,
-                                       float* __restrict__ tre, float* __restrict__ tim) {
-    const i64 idx = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (i64)kClasses * kClassTable) return;
-    const u32 cls = (u32)(idx / kClassTable);
-    const int rem = (int)(idx % kClassTable);
-    const int kb = rem / 64, q = (rem / 16) % 4, l = rem % 16, k = 4 * kb + q;
-    const int ratio = (int)(cls / (kClassSide * kClassSide));
-    const int dx2 = (int)((cls / kClassSide) % kClassSide) - kClassDelta;
-    const int dy2 = (int)(cls % kClassSide) - kClassDelta;
-    double re = 0.0, im = 0.0;
-    if ((dx2 != 0 || dy2 != 0) && k < p1 && l < p1) {
-        // delta = (dx2 + i dy2) / 2; u = (sqrt(2)/2) (h_src/h_min) / delta, v likewise
-        const double dr = 0.5 * dx2, di = 0.5 * dy2, d2 = dr * dr + di * di;
-        const double wr = dr / d2, wi = -di / d2;  // 1 / delta
-        const double s = 0.7071067811865476;
-        const double fs = ratio == 1 ? 2.0 : 1.0, fd = ratio == 2 ? 2.0 : 1.0;
-        double ur = s * fs * wr, ui = s * fs * wi, vr = s * fd * wr, vi = s * fd * wi;
-        double pr = coeff[k * p1 + l], pi = 0.0;
-        for (int i = 0; i < k; ++i) {
-            const double t = pr * ur - pi * ui;
-            pi = pr * ui + pi * ur;
-            pr = t;
-        }
-        for (int i = 0; i < l; ++i) {
-            const double t = pr * vr - pi * vi;
-            pi = pr * vi + pi * vr;
-            pr = t;
-        }
-        const double sign = l == 0 ? -1.0 : ((l & 1) ? 1.0 : -1.0);
-        re = sign * pr;
-        im = sign * pi;
-        if (k == 0 && l == 0) {
-            re += 0.5 * log(d2);
-            im += atan2(di, dr);
-        }
-    }
-    tre[idx] = (float)re;
-    tim[idx] = (float)im;
-}
-
-// Each row of [node_lo, node_hi) re-sorted by class (then source): key[e], src[e] at the
-// row's own positions.  A wavefront per row: rows up to kNetworkRow (64) entries by a
-// bitonic network across the lanes, up to kRankRow by ranks counted in LDS (the keys are
-// distinct: a source appears once per row), longer ones by one lane.  (The CPU build of
-// the tests lowers both bounds, so that every way runs there.)
-#ifndef TD3_CLASS_NETWORK_ROW
-#define TD3_CLASS_NETWORK_ROW 64
-#endif
-#ifndef TD3_CLASS_RANK_ROW
-#define TD3_CLASS_RANK_ROW 1024
-#endif
-constexpr int kNetworkRow = TD3_CLASS_NETWORK_ROW;
-constexpr int kRankRow = TD3_CLASS_RANK_ROW;
-__global__ __launch_bounds__(256) void m2l_classify_kernel(
-    const u32* __restrict__ row_start, const int* __restrict__ col, i64 node_lo, i64 node_hi,
-    const double* __restrict__ center, const double* __restrict__ half, u32* __restrict__ key,
-    int* __restrict__ src) {
-    const i64 d = node_lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    if (d >= node_hi) return;  // uniform over the wavefront
-    const int lane = td3_lane();
-    const u32 r0 = row_start[d], r1 = row_start[d + 1];
-    const int len = (int)(r1 - r0);
-    if (len == 0) return;
-    const double cx = center[2 * d], cy = center[2 * d + 1], hd = half[d];
-    auto class_at = [&](u32 e) {
-        const int s = col[e];
-        return m2l_class_of(cx - center[2 * s], cy - center[2 * s + 1], hd, half[s]);
-    };
-    if (len <= kNetworkRow) {
-        u64 v = ~0ull;
-        if (lane < len) {
-            const u32 e = r0 + (u32)lane;
-            v = ((u64)class_at(e) << 32) | (u32)col[e];
-        }
-#pragma unroll
-        for (int size = 2; size <= 64; size <<= 1) {
-#pragma unroll
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                const u32 lo = __shfl_xor((u32)v, stride, 64);
-                const u32 hi = __shfl_xor((u32)(v >> 32), stride, 64);
-                const u64 o = ((u64)hi << 32) | lo;
-                const bool up = (lane & size) == 0;
-                const bool lower = (lane & stride) == 0;
-                v = (lower == up) ? (v < o ? v : o) : (v > o ? v : o);
-            }
-        }
-        if (lane < len) {
-            key[r0 + lane] = (u32)(v >> 32);
-            src[r0 + lane] = (int)(u32)v;
-        }
-        return;
-    }
-    if (len <= kRankRow) {
-        __shared__ u64 s_rank[4][kRankRow];
-        u64* buf = s_rank[(threadIdx.x / 64) & 3];
-        for (int i = lane; i < len; i += 64) {
-            const u32 e = r0 + (u32)i;
-            buf[i] = ((u64)class_at(e) << 32) | (u32)col[e];
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        for (int i = lane; i < len; i += 64) {
-            const u64 v = buf[i];
-            int rank = 0;
-            for (int m = 0; m < len; ++m) rank += buf[m] < v ? 1 : 0;
-            key[r0 + rank] = (u32)(v >> 32);
-            src[r0 + rank] = (int)(u32)v;
-        }
-        return;
-    }
-    if (lane != 0) return;
-    // insertion sort in place (rows this long are rare: a cell beside many small ones)
-    for (u32 e = r0; e < r1; ++e) {
-        const u64 v = ((u64)class_at(e) << 32) | (u32)col[e];
-        u32 i = e;
-        while (i > r0) {
-            const u64 w = ((u64)key[i - 1] << 32) | (u32)src[i - 1];
-            if (w <= v) break;
-            key[i] = key[i - 1];
-            src[i] = src[i - 1];
-            --i;
-        }
-        key[i] = (u32)(v >> 32);
-        src[i] = (int)(u32)v;
-    }
-}
-
-__device__ __forceinline__ u32 wave_min_u32(u32 v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const u32 o = __shfl_xor(v, off, 64);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
-// Tiles over the sibling groups of [node_lo, node_hi) (node_lo - 1 divisible by 4: nodes
-// >= 1 come in groups of four siblings): wavefront t takes position k = t % 4 of the
-// 16 groups 16 (t / 4) .. + 15.  FP32 scaled moments in; each pair's product in FP32 from
-// zero, the sum over a row in FP64; the row's classed pairs written (rows without pairs
-// untouched; their other pairs are added by m2l_mfma_kernel afterwards).  The 16 rows'
-// keys and sources are staged in LDS, and the next class's operator and moments are
-// loaded while the matrix cores work on the current one.
-constexpr int kStage = 1024;  // staged entries per wavefront
-
-__global__ __launch_bounds__(256) void m2l_class_kernel(
-    const u32* __restrict__ row_start, const u32* __restrict__ key,
-    const int* __restrict__ src, i64 node_lo, i64 node_hi,
-    const double* __restrict__ half, const float* __restrict__ mre,
-    const float* __restrict__ mim, int p1, const float* __restrict__ tre,
-    const float* __restrict__ tim, double* __restrict__ lre, double* __restrict__ lim) {
-    __shared__ u32 s_key[4][kStage];
-    __shared__ int s_src[4][kStage];
-    const i64 t = ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const int lane = td3_lane(), j = lane & 15, q = lane >> 4, w = (threadIdx.x / 64) & 3;
-    const i64 g0 = (node_lo - 1) / 4;
-    const i64 d = 1 + 4 * (g0 + 16 * (t / 4) + j) + (t % 4);
-    const bool has = d >= node_lo && d < node_hi;
-    if (!__ballot(has)) return;  // uniform
-    const u32 r0 = has ? row_start[d] : 0u;
-    const u32 r1 = has ? row_start[d + 1] : 0u;
-    // stage: row j at offset off_j (exclusive prefix of the 16 lengths)
-    const u32 len_all = r1 - r0;
-    u32 off = len_all;
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-        const u32 y = __shfl_up(off, o, 16);
-        if (j >= o) off += y;
-    }
-    const u32 total = __shfl(off, 15, 64);
-    off -= len_all;
-    const bool staged = total <= (u32)kStage;
-    // len: the row's classed pairs (a prefix: generic keys sort last)
-    u32 len = 0;
-    for (u32 i = (u32)q; i < len_all; i += 4) {
-        const u32 k = key[r0 + i];
-        len += k < kGeneric ? 1u : 0u;
-        if (staged) {
-            s_key[w][off + i] = k;
-            s_src[w][off + i] = src[r0 + i];
-        }
-    }
-    len += __shfl_xor(len, 16, 64);
-    len += __shfl_xor(len, 32, 64);
-    if (staged) {
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    }
-    auto key_at = [&](u32 i) { return staged ? s_key[w][off + i] : key[r0 + i]; };
-    auto src_at = [&](u32 i) { return staged ? s_src[w][off + i] : src[r0 + i]; };
-    const double hd = has ? half[d] : 1.0;
-    const double ln2h = log(2.0 * hd), ln_h = log(hd);
-    double acc_re[4] = {0.0, 0.0, 0.0, 0.0}, acc_im[4] = {0.0, 0.0, 0.0, 0.0};
-    // one class: its operator (A operand) and, for the lanes whose row has it, the source's
-    // moments (B operand)
-    struct Step {
-        float a_re[4], a_im[4], x_re[4], x_im[4];
-    };
-    auto load = [&](u32 c, bool mine, u32 i, Step& st) {
-        const float* ar = tre + (size_t)c * kClassTable + lane;
-        const float* ai = tim + (size_t)c * kClassTable + lane;
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            st.a_re[kb] = ar[64 * kb];
-            st.a_im[kb] = ai[64 * kb];
-            st.x_re[kb] = 0.f;
-            st.x_im[kb] = 0.f;
-        }
-        if (mine) {
-            const int s = src_at(i);
-            const float* xr = mre + (size_t)s * p1 + q;
-            const float* xi = mim + (size_t)s * p1 + q;
-#pragma unroll
-            for (int kb = 0; kb < 4; ++kb) {
-                if (4 * kb + q < p1) {
-                    st.x_re[kb] = xr[4 * kb];
-                    st.x_im[kb] = xi[4 * kb];
-                }
-            }
-        }
-    };
-    u32 cur = 0;
-    u32 my = cur < len ? key_at(cur) : 0xffffffffu;
-    u32 c = wave_min_u32(my);
-    if (c >= kGeneric) c = 0xffffffffu;
-    Step now;
-    bool mine = my == c;
-    if (c != 0xffffffffu) load(c, mine, cur, now);
-    while (c != 0xffffffffu) {
-        // the next class, and its loads, before this one's products
-        const u32 nxt = cur + (mine ? 1u : 0u);
-        const u32 my_n = nxt < len ? key_at(nxt) : 0xffffffffu;
-        u32 c_n = wave_min_u32(my_n);
-        if (c_n >= kGeneric) c_n = 0xffffffffu;
-        const bool mine_n = my_n == c_n;
-        Step later;
-        if (c_n != 0xffffffffu) load(c_n, mine_n, nxt, later);
-        td3_f32x4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            dre = mfma16(now.a_re[kb], now.x_re[kb], dre);
-            dre = mfma16(-now.a_im[kb], now.x_im[kb], dre);
-            dim = mfma16(now.a_re[kb], now.x_im[kb], dim);
-            dim = mfma16(now.a_im[kb], now.x_re[kb], dim);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            acc_re[r] += (double)dre[r];
-            acc_im[r] += (double)dim[r];
-        }
-        if (mine && q == 0) {
-            // + A_0 ln(2 h_min): h_min = h_dst unless the source is the smaller cell
-            const double lh = c >= 2u * kClassSide * kClassSide ? ln_h : ln2h;
-            acc_re[0] += (double)now.x_re[0] * lh;
-            acc_im[0] += (double)now.x_im[0] * lh;
-        }
-        now = later;
-        c = c_n;
-        mine = mine_n;
-        cur = nxt;
-    }
-    if (!has || r1 == r0) return;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int l = 4 * q + r;
-        if (l < p1) {
-            lre[d * p1 + l] = acc_re[r];
-            lim[d * p1 + l] = acc_im[r];
-        }
-    }
 }
 
 // ------------------------------------------------------------------------------ scaling
@@ -1136,20 +800,10 @@ int dispatch_m2l(hipStream_t s, int order, const int* col, const int* dst, const
 
 }  // namespace
 
-namespace {
-// the class path's workspace: sorted keys and sources per pair, the operator tables
-size_t class_bytes(i64 pair_count) {
-    return (size_t)pair_count * 8 + 2 * (size_t)kClasses * kClassTable * 4 + 1024;
-}
-}  // namespace
-
 extern "C" size_t td3_m2l_workspace_bytes(int64_t pair_count, int64_t order, int precision) {
     const i64 waves = (pair_count + 63) / 64 + 1;
     const size_t word = precision == TD3_FP64 ? 8 : 4;
-    const size_t side = (size_t)waves * 2 * 2 * (size_t)(order + 1) * word + 256;
-    if (precision == TD3_FP64 || order + 1 > 16) return side;
-    const size_t cls = class_bytes(pair_count);
-    return side > cls ? side : cls;
+    return (size_t)waves * 2 * 2 * (size_t)(order + 1) * word + 256;
 }
 
 extern "C" int td3_scale_moments(int precision, const double* moment_re, const double* moment_im,
@@ -1181,37 +835,6 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
     TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
     hipStream_t s = (hipStream_t)stream;
     if (node_hi == node_lo) return TD3_OK;
-    if (precision != TD3_FP64 && order + 1 <= 16 && dst_center == src_center &&
-        node_lo >= 1 && (node_lo - 1) % 4 == 0 && (node_hi - 1) % 4 == 0) {
-        // FP32 within one tree: by class on the matrix cores (rows sorted by class first)
-        const i64 pairs_cap = (i64)((workspace_bytes - 2 * (size_t)kClasses * kClassTable * 4 -
-                                     1024) / 8);
-        TD3_ARG(workspace != nullptr && workspace_bytes >= class_bytes(0), "workspace too small");
-        char* ws = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
-        float* tre = (float*)ws;
-        float* tim = tre + (size_t)kClasses * kClassTable;
-        u32* key = (u32*)(tim + (size_t)kClasses * kClassTable);
-        int* src = (int*)(key + pairs_cap);
-        TD3_TIMED(s, "m2l_class_table", hipLaunchKernelGGL(m2l_class_table_kernel, dim3(td3_blocks((i64)kClasses * kClassTable, 256)), dim3(256), 0, s,
-                  coeff, (int)order + 1, tre, tim));
-        TD3_LAUNCHED();
-        TD3_TIMED(s, "m2l_classify", hipLaunchKernelGGL(m2l_classify_kernel, dim3(td3_blocks((node_hi - node_lo) * 64, 256)), dim3(256), 0, s,
-                  row_start, col, node_lo, node_hi, dst_center, dst_half, key, src));
-        TD3_LAUNCHED();
-        const i64 groups = (node_hi - node_lo + 3) / 4;
-        const i64 tiles = 4 * ((groups + 15) / 16);
-        TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL(m2l_class_kernel, dim3(td3_blocks(tiles * 64, 256)), dim3(256), 0, s,
-                  row_start, key, src, node_lo, node_hi, dst_half, (const float*)scaled_re,
-                  (const float*)scaled_im, (int)order + 1, tre, tim, local_re, local_im));
-        TD3_LAUNCHED();
-        // the other pairs (each row's generic suffix), added pair by pair
-        TD3_TIMED(s, "m2l_generic", hipLaunchKernelGGL((m2l_mfma_kernel<float>), dim3(td3_blocks((node_hi - node_lo) * 64, 256)), dim3(256), 0, s,
-                  src, row_start, node_lo, node_hi, dst_center, dst_half, src_center, src_half,
-                  (const float*)scaled_re, (const float*)scaled_im, (int)order + 1, coeff,
-                  local_re, local_im, key, 1));
-        TD3_LAUNCHED();
-        return TD3_OK;
-    }
     if (precision != TD3_FP64 && order + 1 <= 16) {
         // FP32: a wavefront per destination row on the matrix cores.  (FP64 stays on the
         // lane-per-pair kernel: on gfx950 v_mfma_f64_16x16x4f64 runs at the vector FP64
@@ -1220,7 +843,7 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
         TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<float>), grid, dim3(256), 0, s, col, row_start,
                            node_lo, node_hi, dst_center, dst_half, src_center, src_half,
                            (const float*)scaled_re, (const float*)scaled_im, (int)order + 1,
-                           coeff, local_re, local_im, nullptr, 0));
+                           coeff, local_re, local_im));
         TD3_LAUNCHED();
         return TD3_OK;
     }
