@@ -225,8 +225,10 @@ __device__ __forceinline__ u32 level_entries(const u32* lv, u32 cap) {
     return n < cap ? n : cap;
 }
 
-// Count step: every entry's walk, counted per column; block b takes the tile of kScanTile
-// entries from b kScanTile and writes the tile's column sums to part.
+// Count step: every entry's walk, counted per column; a block of kScanTile threads takes a
+// tile of kScanTile entries, one each (the walks are long and uneven: one per thread
+// keeps every level's walks in parallel), and writes the tile's column sums to part.
+template <int WAVES>
 __device__ __forceinline__ void block_sum3(u32 v[kCols], u32 (*s_sum)[kCols]) {
     for (int c = 0; c < kCols; ++c) v[c] = wave_sum_u32(v[c]);
     if (td3_lane() == 0)
@@ -234,24 +236,23 @@ __device__ __forceinline__ void block_sum3(u32 v[kCols], u32 (*s_sum)[kCols]) {
     __syncthreads();
     for (int c = 0; c < kCols; ++c) {
         u32 t = 0;
-        for (int w = 0; w < kScanThreads / 64; ++w) t += s_sum[w][c];
+        for (int w = 0; w < WAVES; ++w) t += s_sum[w][c];
         v[c] = t;
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void entry_count_kernel(
+__global__ __launch_bounds__(kScanTile) void entry_count_kernel(
     RowsTree t, const u32* __restrict__ inh, const int* __restrict__ enode,
     const u32* __restrict__ lv, u32 cap, double theta, u32* __restrict__ cnt,
     u32* __restrict__ cache, u32* __restrict__ part, const u32* __restrict__ flag) {
     if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const u32 n = level_entries(lv, cap);
-    __shared__ u32 s_sum[kScanThreads / 64][kCols];
+    __shared__ u32 s_sum[kScanTile / 64][kCols];
     for (u32 tile = blockIdx.x; tile * kScanTile < n; tile += gridDim.x) {
-    const u32 base = tile * kScanTile;
     u32 sum[kCols] = {0, 0, 0};
-    for (int r = 0; r < kScanItems; ++r) {
-        const u32 e = base + r * kScanThreads + threadIdx.x;
-        if (e >= n) continue;
+    {
+        const u32 e = tile * kScanTile + threadIdx.x;
+        if (e < n) {
         const int a = enode[e];
         const int4 fa = t.info[a];
         const u32 entry = inh[e];
@@ -267,8 +268,9 @@ __global__ __launch_bounds__(kScanThreads) void entry_count_kernel(
         sum[0] += sink.c.m2l;
         sum[1] += sink.c.p2p;
         sum[2] += sink.c.inh;
+        }
     }
-    block_sum3(sum, s_sum);
+    block_sum3<kScanTile / 64>(sum, s_sum);
     if (threadIdx.x < kCols) part[(size_t)kCols * tile + threadIdx.x] = sum[threadIdx.x];
     __syncthreads();  // s_sum is reused by the next tile
     }
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(kScanThreads) void col_scan_kernel(u32* __restrict_
     const u32 upto = tile == 0 ? tiles : tile;
     for (u32 i = threadIdx.x; i < upto; i += kScanThreads)
         for (int c = 0; c < kCols; ++c) carry[c] += part[(size_t)kCols * i + c];
-    block_sum3(carry, s_sum);
+    block_sum3<kScanThreads / 64>(carry, s_sum);
     if (tile == 0) {
         if (threadIdx.x < kCols) totals[threadIdx.x] = carry[threadIdx.x];
         for (int c = 0; c < kCols; ++c) carry[c] = 0;
@@ -527,7 +529,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
         const i64 next_lo = hi, next_hi = d + 1 < n_levels ? levels_host[d + 2] : hi;
         const i64 next = next_hi - next_lo;
-        TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(entry_count_kernel, dim3(grid_tiles), dim3(kScanThreads), 0, s, t, inh, enode, lv,
+        TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(entry_count_kernel, dim3(grid_tiles), dim3(kScanTile), 0, s, t, inh, enode, lv,
                            cap, theta, w.cnt, w.cache, w.part, w.flag));
         TD3_LAUNCHED();
         TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_scan_kernel, dim3(grid_tiles), dim3(kScanThreads), 0, s, w.cnt, lv, cap, w.part,

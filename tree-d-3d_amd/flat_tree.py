@@ -757,6 +757,14 @@ class FlatQuadtree:
         if self.kernel != "log":
             return self._coulomb_fmm(theta, softening, potential, precision)
         prec = _precision(precision or self.precision)
+        if ("lists", theta) not in self._traversals:
+            # queue the work that needs no interaction rows ahead of them: the GPU runs it
+            # while the host waits for the rows' sizes
+            self._leaf_index()
+            self._sources4()
+            self._scaled_moments(_abi.FP64)
+            if prec != _abi.FP64:
+                self._scaled_moments(prec)
         lre, lim = self._local_expansions(theta, prec)
         _, (prow, pcol, _), counts = self._lists(theta)
         status = torch.zeros(1, dtype=torch.int64, device=dev)
