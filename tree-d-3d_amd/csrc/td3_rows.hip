@@ -46,9 +46,11 @@ struct RowsTree {
 // *child).
 __device__ __forceinline__ int decide(const RowsTree& t, double4 ga, int4 fa, int x,
                                       bool a_second, double theta, int* child) {
+    // both loads issued together (the geometry is not needed for an empty cell, but
+    // waiting for the count first would serialise two memory latencies per visit)
     const int4 fx = t.info[x];
-    if (fx.y == fx.x || fa.y == fa.x) return 0;
     const double4 gx = t.geo[x];
+    if (fx.y == fx.x || fa.y == fa.x) return 0;
     const bool leaf_a = fa.z < 0, leaf_x = fx.z < 0;
     const double4 gf = a_second ? gx : ga, gs = a_second ? ga : gx;
     const double dx = __dadd_rn(gf.x, -gs.x);
@@ -219,15 +221,20 @@ constexpr unsigned kEntryBlocks = 2048;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 4;
 constexpr u32 kScanTile = kScanThreads * kScanItems;
+// The counting pass sums tiles of kCountTile entries (one per thread: walks are long and
+// uneven, so small blocks retire early) and adds each sum into the super-tile of
+// kSuperTiles tiles (integer sums: the same whatever the order); a scan tile's carry is the
+// super-tiles before it plus at most kSuperTiles - 1 tiles.
+constexpr u32 kCountTile = 256;
+constexpr u32 kSuperTiles = 16;
+constexpr u32 kCountPerScan = kScanTile / kCountTile;
 
 __device__ __forceinline__ u32 level_entries(const u32* lv, u32 cap) {
     const u32 n = lv[0];
     return n < cap ? n : cap;
 }
 
-// Count step: every entry's walk, counted per column; a block of kScanTile threads takes a
-// tile of kScanTile entries, one each (the walks are long and uneven: one per thread
-// keeps every level's walks in parallel), and writes the tile's column sums to part.
+// Count step: every entry's walk, counted per column, in tiles of kCountTile entries.
 template <int WAVES>
 __device__ __forceinline__ void block_sum3(u32 v[kCols], u32 (*s_sum)[kCols]) {
     for (int c = 0; c < kCols; ++c) v[c] = wave_sum_u32(v[c]);
@@ -241,38 +248,40 @@ __device__ __forceinline__ void block_sum3(u32 v[kCols], u32 (*s_sum)[kCols]) {
     }
 }
 
-__global__ __launch_bounds__(kScanTile) void entry_count_kernel(
+__global__ __launch_bounds__(kCountTile) void entry_count_kernel(
     RowsTree t, const u32* __restrict__ inh, const int* __restrict__ enode,
     const u32* __restrict__ lv, u32 cap, double theta, u32* __restrict__ cnt,
-    u32* __restrict__ cache, u32* __restrict__ part, const u32* __restrict__ flag) {
+    u32* __restrict__ cache, u32* __restrict__ part, u32* __restrict__ super,
+    const u32* __restrict__ flag) {
     if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const u32 n = level_entries(lv, cap);
-    __shared__ u32 s_sum[kScanTile / 64][kCols];
-    for (u32 tile = blockIdx.x; tile * kScanTile < n; tile += gridDim.x) {
-    u32 sum[kCols] = {0, 0, 0};
-    {
-        const u32 e = tile * kScanTile + threadIdx.x;
+    __shared__ u32 s_sum[kCountTile / 64][kCols];
+    for (u32 tile = blockIdx.x; tile * kCountTile < n; tile += gridDim.x) {
+        u32 sum[kCols] = {0, 0, 0};
+        const u32 e = tile * kCountTile + threadIdx.x;
         if (e < n) {
-        const int a = enode[e];
-        const int4 fa = t.info[a];
-        const u32 entry = inh[e];
-        CountSink sink = {{0, 0, 0}, 0u, cache + e, cap};
-        if ((int)(entry & ~kSecond) == a)
-            self_entry(a, fa, sink);
-        else
-            walk_entry(t, a, fa, entry, theta, sink);
-        u32* o = cnt + (size_t)kCols * e;
-        o[0] = sink.c.m2l;
-        o[1] = sink.c.p2p;
-        o[2] = sink.c.inh;
-        sum[0] += sink.c.m2l;
-        sum[1] += sink.c.p2p;
-        sum[2] += sink.c.inh;
+            const int a = enode[e];
+            const int4 fa = t.info[a];
+            const u32 entry = inh[e];
+            CountSink sink = {{0, 0, 0}, 0u, cache + e, cap};
+            if ((int)(entry & ~kSecond) == a)
+                self_entry(a, fa, sink);
+            else
+                walk_entry(t, a, fa, entry, theta, sink);
+            u32* o = cnt + (size_t)kCols * e;
+            o[0] = sink.c.m2l;
+            o[1] = sink.c.p2p;
+            o[2] = sink.c.inh;
+            sum[0] = sink.c.m2l;
+            sum[1] = sink.c.p2p;
+            sum[2] = sink.c.inh;
         }
-    }
-    block_sum3<kScanTile / 64>(sum, s_sum);
-    if (threadIdx.x < kCols) part[(size_t)kCols * tile + threadIdx.x] = sum[threadIdx.x];
-    __syncthreads();  // s_sum is reused by the next tile
+        block_sum3<kCountTile / 64>(sum, s_sum);
+        if (threadIdx.x < kCols) {
+            part[(size_t)kCols * tile + threadIdx.x] = sum[threadIdx.x];
+            atomicAdd(super + (size_t)kCols * (tile / kSuperTiles) + threadIdx.x, sum[threadIdx.x]);
+        }
+        __syncthreads();  // s_sum is reused by the next tile
     }
 }
 
@@ -281,6 +290,7 @@ __global__ __launch_bounds__(kScanTile) void entry_count_kernel(
 __global__ __launch_bounds__(kScanThreads) void col_scan_kernel(u32* __restrict__ cnt,
                                                                 const u32* __restrict__ lv, u32 cap,
                                                                 const u32* __restrict__ part,
+                                                                const u32* __restrict__ super,
                                                                 u32* __restrict__ totals,
                                                                 const u32* __restrict__ flag) {
     if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
@@ -292,9 +302,19 @@ __global__ __launch_bounds__(kScanThreads) void col_scan_kernel(u32* __restrict_
     for (u32 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const u32 base = tile * kScanTile;
     u32 carry[kCols] = {0, 0, 0};
-    const u32 upto = tile == 0 ? tiles : tile;
-    for (u32 i = threadIdx.x; i < upto; i += kScanThreads)
-        for (int c = 0; c < kCols; ++c) carry[c] += part[(size_t)kCols * i + c];
+    if (tile == 0) {
+        // the totals: every super-tile
+        const u32 supers = (tiles * kCountPerScan + kSuperTiles - 1) / kSuperTiles;
+        for (u32 i = threadIdx.x; i < supers; i += kScanThreads)
+            for (int c = 0; c < kCols; ++c) carry[c] += super[(size_t)kCols * i + c];
+    } else {
+        // the super-tiles, then the count tiles, before this tile's first count tile
+        const u32 first = tile * kCountPerScan, whole = first / kSuperTiles;
+        for (u32 i = threadIdx.x; i < whole; i += kScanThreads)
+            for (int c = 0; c < kCols; ++c) carry[c] += super[(size_t)kCols * i + c];
+        for (u32 i = whole * kSuperTiles + threadIdx.x; i < first; i += kScanThreads)
+            for (int c = 0; c < kCols; ++c) carry[c] += part[(size_t)kCols * i + c];
+    }
     block_sum3<kScanThreads / 64>(carry, s_sum);
     if (tile == 0) {
         if (threadIdx.x < kCols) totals[threadIdx.x] = carry[threadIdx.x];
@@ -438,7 +458,9 @@ __global__ void rows_end_kernel(const u32* base, int levels, i64 count, u32* m2l
 }
 
 struct RowsWs {
-    u32 *inh_a, *inh_b, *first_a, *first_b, *cnt, *cache, *part, *totals, *lv, *base, *flag;
+    u32 *inh_a, *inh_b, *first_a, *first_b, *cnt, *cache, *part, *super, *totals, *lv, *base,
+        *flag;
+    i64 super_stride;  // u32 per level
     int *enode_a, *enode_b, *parent;
     double4* geo;
 };
@@ -459,7 +481,10 @@ size_t rows_carve(RowsWs* w, void* p, i64 count, i64 width, i64 cap_inh, int lev
     w->first_b = (u32*)take(sizeof(u32) * (size_t)(width + 1));
     w->cnt = (u32*)take(sizeof(u32) * kCols * (size_t)cap_inh);
     w->cache = (u32*)take(sizeof(u32) * kCached * (size_t)cap_inh);
-    w->part = (u32*)take(sizeof(u32) * kCols * (size_t)tiles);
+    const i64 count_tiles = (cap_inh + kCountTile - 1) / kCountTile;
+    w->part = (u32*)take(sizeof(u32) * kCols * (size_t)count_tiles);
+    w->super_stride = (count_tiles + kSuperTiles - 1) / kSuperTiles * kCols;
+    w->super = (u32*)take(sizeof(u32) * (size_t)w->super_stride * (size_t)levels);
     w->totals = (u32*)take(sizeof(u32) * kCols);
     w->lv = (u32*)take(sizeof(u32) * 2);
     w->base = (u32*)take(sizeof(u32) * (size_t)(2 * levels + 4));
@@ -511,6 +536,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
                        node_half, count, w.geo);
     TD3_LAUNCHED();
     TD3_HIP(hipMemsetAsync(w.flag, 0, sizeof(u32) * 4, s));
+    TD3_HIP(hipMemsetAsync(w.super, 0, sizeof(u32) * (size_t)w.super_stride * n_levels, s));
     hipLaunchKernelGGL(rows_parent_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s,
                        (const int4*)node_info, count, w.parent, w.flag + 2);
     TD3_LAUNCHED();
@@ -529,11 +555,12 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
         const i64 next_lo = hi, next_hi = d + 1 < n_levels ? levels_host[d + 2] : hi;
         const i64 next = next_hi - next_lo;
-        TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(entry_count_kernel, dim3(grid_tiles), dim3(kScanTile), 0, s, t, inh, enode, lv,
-                           cap, theta, w.cnt, w.cache, w.part, w.flag));
+        u32* super = w.super + (size_t)w.super_stride * d;
+        TD3_TIMED(s, "rows_count", hipLaunchKernelGGL(entry_count_kernel, dim3(kEntryBlocks), dim3(kCountTile), 0, s, t, inh, enode, lv,
+                           cap, theta, w.cnt, w.cache, w.part, super, w.flag));
         TD3_LAUNCHED();
         TD3_TIMED(s, "rows_scan", hipLaunchKernelGGL(col_scan_kernel, dim3(grid_tiles), dim3(kScanThreads), 0, s, w.cnt, lv, cap, w.part,
-                           w.totals, w.flag));
+                           super, w.totals, w.flag));
         TD3_LAUNCHED();
         TD3_TIMED(s, "rows_write", hipLaunchKernelGGL(entry_write_kernel, dim3(kEntryBlocks), dim3(256), 0, s, t, inh, enode, first,
                            lo, lv, lv_next, cap, theta, w.cnt, w.totals, w.base, d, m2l_row,
