@@ -416,9 +416,6 @@ __global__ void unscale_locals_kernel(double* __restrict__ lre, double* __restri
 // kernels/fmm_downward/fmm_downward_pyccel.py:15-64 in scaled form: with s = c_child - c_parent,
 // B^child_m += sum_{l >= m} C(l, m) B_l (r_c / r_p)^m (s / r_p)^(l - m).  One thread per
 // (child, m) of the children of the nodes [lo, hi).
-// P: a compile-time bound on the order (the loads of the parent's coefficients issue
-// together), or 0 for any order.
-template <int P>
 __global__ void l2l_kernel(const double* __restrict__ center, const double* __restrict__ half,
                            const int4* __restrict__ info, i64 lo, i64 hi, int p1,
                            const double* __restrict__ binomial, int bsz, double* __restrict__ lre,
@@ -436,38 +433,14 @@ __global__ void l2l_kernel(const double* __restrict__ center, const double* __re
     const double sr = (center[2 * child] - center[2 * parent]) / rp;
     const double si = (center[2 * child + 1] - center[2 * parent + 1]) / rp;
     double acc_r = 0.0, acc_i = 0.0, pr = 1.0, pi = 0.0;
-    if (P > 0) {
-        double br[P + 1], bi[P + 1];
-#pragma unroll
-        for (int l = 0; l <= P; ++l) {
-            br[l] = 0.0;
-            bi[l] = 0.0;
-            if (l >= m && l < p1) {
-                const double c = binomial[l * bsz + m];
-                br[l] = lre[parent * p1 + l] * c;
-                bi[l] = lim[parent * p1 + l] * c;
-            }
-        }
-#pragma unroll
-        for (int l = 0; l <= P; ++l) {
-            if (l >= m && l < p1) {
-                acc_r += br[l] * pr - bi[l] * pi;
-                acc_i += br[l] * pi + bi[l] * pr;
-                const double tr = pr * sr - pi * si;
-                pi = pr * si + pi * sr;
-                pr = tr;
-            }
-        }
-    } else {
-        for (int l = m; l < p1; ++l) {
-            const double c = binomial[l * bsz + m];
-            const double br = lre[parent * p1 + l] * c, bi = lim[parent * p1 + l] * c;
-            acc_r += br * pr - bi * pi;
-            acc_i += br * pi + bi * pr;
-            const double tr = pr * sr - pi * si;
-            pi = pr * si + pi * sr;
-            pr = tr;
-        }
+    for (int l = m; l < p1; ++l) {
+        const double c = binomial[l * bsz + m];
+        const double br = lre[parent * p1 + l] * c, bi = lim[parent * p1 + l] * c;
+        acc_r += br * pr - bi * pi;
+        acc_i += br * pi + bi * pr;
+        const double tr = pr * sr - pi * si;
+        pi = pr * si + pi * sr;
+        pr = tr;
     }
     double f = 1.0;
     const double ratio = rc / rp;
@@ -901,19 +874,9 @@ extern "C" int td3_l2l(int64_t order, const double* node_center, const double* n
     const int p1 = (int)order + 1;
     for (i64 d = 0; d + 1 < n_levels; ++d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
-        const dim3 grid(td3_blocks((hi - lo) * 4 * p1, 256));
-        if (p1 <= 13)
-            TD3_TIMED(s, "l2l", hipLaunchKernelGGL((l2l_kernel<12>), grid, dim3(256), 0, s,
-                      node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
-                      local_re, local_im));
-        else if (p1 <= 17)
-            TD3_TIMED(s, "l2l", hipLaunchKernelGGL((l2l_kernel<16>), grid, dim3(256), 0, s,
-                      node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
-                      local_re, local_im));
-        else
-            TD3_TIMED(s, "l2l", hipLaunchKernelGGL((l2l_kernel<0>), grid, dim3(256), 0, s,
-                      node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
-                      local_re, local_im));
+        TD3_TIMED(s, "l2l", hipLaunchKernelGGL(l2l_kernel, dim3(td3_blocks((hi - lo) * 4 * p1, 256)), dim3(256), 0, s,
+                           node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
+                           local_re, local_im));
         TD3_LAUNCHED();
     }
     return TD3_OK;

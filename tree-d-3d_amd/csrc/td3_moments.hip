@@ -67,7 +67,7 @@ __global__ void log_p2m_kernel(const double* __restrict__ spos, const double* __
 }
 
 // a_n += sum_m C(n, m) a_m(child) s^(n-m), s = c_child - c, for the nodes [lo, hi).
-__global__ void log_m2m_any_kernel(const double* __restrict__ center, const int4* __restrict__ info,
+__global__ void log_m2m_kernel(const double* __restrict__ center, const int4* __restrict__ info,
                                i64 lo, i64 hi, int p1, const double* __restrict__ binomial,
                                int bsz, double* __restrict__ qabs, double* __restrict__ mre,
                                double* __restrict__ mim) {
@@ -94,57 +94,6 @@ __global__ void log_m2m_any_kernel(const double* __restrict__ center, const int4
             const double tr = pr * sr - pi * si;
             pi = pr * si + pi * sr;
             pr = tr;
-        }
-        are += acc_r;
-        aim += acc_i;
-    }
-    mre[node * p1 + k] = are;
-    mim[node * p1 + k] = aim;
-    if (k == 0) qabs[node] = qa;
-}
-
-// The same for orders up to P, unrolled: the loads of a child's moments and binomials
-// issue together instead of one term at a time.
-template <int P>
-__global__ void log_m2m_kernel(const double* __restrict__ center, const int4* __restrict__ info,
-                               i64 lo, i64 hi, int p1, const double* __restrict__ binomial,
-                               int bsz, double* __restrict__ qabs, double* __restrict__ mre,
-                               double* __restrict__ mim) {
-    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (hi - lo) * p1) return;
-    const i64 node = lo + t / p1;
-    const int k = (int)(t % p1);
-    const int4 nf = info[node];
-    if (nf.z < 0) return;
-    const double cx = center[2 * node], cy = center[2 * node + 1];
-    double are = 0.0, aim = 0.0, qa = 0.0;
-    for (int c = nf.z; c < nf.z + 4; ++c) {
-        qa += qabs[c];
-        const int4 cf = info[c];
-        if (cf.y == cf.x) continue;
-        const double sr = center[2 * c] - cx, si = center[2 * c + 1] - cy;
-        double br[P + 1], bi[P + 1];
-#pragma unroll
-        for (int m = 0; m <= P; ++m) {
-            br[m] = 0.0;
-            bi[m] = 0.0;
-            if (m <= k) {
-                const double b = binomial[k * bsz + m];
-                br[m] = mre[(i64)c * p1 + m] * b;
-                bi[m] = mim[(i64)c * p1 + m] * b;
-            }
-        }
-        // m = k .. 0 with s^(k-m) built upward, as the loop above
-        double pr = 1.0, pi = 0.0, acc_r = 0.0, acc_i = 0.0;
-#pragma unroll
-        for (int m = P; m >= 0; --m) {
-            if (m <= k) {
-                acc_r += br[m] * pr - bi[m] * pi;
-                acc_i += br[m] * pi + bi[m] * pr;
-                const double tr = pr * sr - pi * si;
-                pi = pr * si + pi * sr;
-                pr = tr;
-            }
         }
         are += acc_r;
         aim += acc_i;
@@ -243,19 +192,9 @@ int moments_impl(int kernel, int64_t order, const double* sorted_pos, const doub
         TD3_LAUNCHED();
         for (i64 d = n_levels - 2; d >= 0; --d) {
             const i64 lo = levels_host[d], hi = levels_host[d + 1];
-            const dim3 grid(td3_blocks((hi - lo) * p1, 256));
-            if (p1 <= 13)
-                TD3_TIMED(s, "moments_m2m", hipLaunchKernelGGL((log_m2m_kernel<12>), grid, dim3(256), 0, s,
-                          node_center, info, lo, hi, p1, binomial, (int)bsz, node_qabs, moment_re,
-                          moment_im));
-            else if (p1 <= 17)
-                TD3_TIMED(s, "moments_m2m", hipLaunchKernelGGL((log_m2m_kernel<16>), grid, dim3(256), 0, s,
-                          node_center, info, lo, hi, p1, binomial, (int)bsz, node_qabs, moment_re,
-                          moment_im));
-            else
-                TD3_TIMED(s, "moments_m2m", hipLaunchKernelGGL(log_m2m_any_kernel, grid, dim3(256), 0, s,
-                          node_center, info, lo, hi, p1, binomial, (int)bsz, node_qabs, moment_re,
-                          moment_im));
+            TD3_TIMED(s, "moments_m2m", hipLaunchKernelGGL(log_m2m_kernel, dim3(td3_blocks((hi - lo) * p1, 256)), dim3(256), 0, s,
+                               node_center, info, lo, hi, p1, binomial, (int)bsz, node_qabs, moment_re,
+                               moment_im));
             TD3_LAUNCHED();
         }
         return TD3_OK;
