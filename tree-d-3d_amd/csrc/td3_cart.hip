@@ -70,14 +70,16 @@ __global__ void cart_m2m_kernel(const double* __restrict__ center, const int4* _
         const int4 cf = info[c];
         if (cf.y == cf.x) continue;
         const double sx = center[2 * c] - cx, sy = center[2 * c + 1] - cy;
-        for (int bx = 0; bx <= gx; ++bx) {
-            double px = 1.0;
-            for (int k = 0; k < gx - bx; ++k) px *= sx;
-            for (int by = 0; by <= gy; ++by) {
-                double py = 1.0;
-                for (int k = 0; k < gy - by; ++k) py *= sy;
-                acc += binom[gx * bsz + bx] * binom[gy * bsz + by] * mom[(i64)c * nc + cidx(bx, by)] * px * py;
+        // s^(g - b) built up as b runs down from g (no power loops per term)
+        double px = 1.0;
+        for (int bx = gx; bx >= 0; --bx) {
+            const double fx = binom[gx * bsz + bx] * px;
+            double py = 1.0;
+            for (int by = gy; by >= 0; --by) {
+                acc += fx * binom[gy * bsz + by] * mom[(i64)c * nc + cidx(bx, by)] * py;
+                py *= sy;
             }
+            px *= sx;
         }
     }
     mom[node * nc + i] = acc;
@@ -293,14 +295,16 @@ __global__ void cart_l2l_kernel(const double* __restrict__ center, const int4* _
     const double sx = center[2 * child] - center[2 * parent];
     const double sy = center[2 * child + 1] - center[2 * parent + 1];
     double acc = 0.0;
+    // s^(g - b) built up with g (no power loops per term)
+    double px = 1.0;
     for (int gx = bx; gx <= p; ++gx) {
-        double px = 1.0;
-        for (int k = 0; k < gx - bx; ++k) px *= sx;
+        const double fx = binom[gx * bsz + bx] * px;
+        double py = 1.0;
         for (int gy = by; gx + gy <= p; ++gy) {
-            double py = 1.0;
-            for (int k = 0; k < gy - by; ++k) py *= sy;
-            acc += binom[gx * bsz + bx] * binom[gy * bsz + by] * local[parent * nc + cidx(gx, gy)] * px * py;
+            acc += fx * binom[gy * bsz + by] * local[parent * nc + cidx(gx, gy)] * py;
+            py *= sy;
         }
+        px *= sx;
     }
     local[child * nc + i] += acc;
 }
