@@ -449,6 +449,49 @@ __global__ void l2l_kernel(const double* __restrict__ center, const double* __re
     lim[child * p1 + m] += acc_i * f;
 }
 
+// The same for p + 1 <= 16 coefficients, 16 lanes per (parent, child): lane l loads the
+// parent's coefficient l once and the others come by shuffles within the 16.  Same terms
+// in the same order as l2l_kernel.
+__global__ void l2l16_kernel(const double* __restrict__ center, const double* __restrict__ half,
+                             const int4* __restrict__ info, i64 lo, i64 hi, int p1,
+                             const double* __restrict__ binomial, int bsz,
+                             double* __restrict__ lre, double* __restrict__ lim) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    const i64 parent = lo + t / 64;
+    const int m = (int)(t % 16);
+    if (parent >= hi) return;  // uniform over the 64 lanes of a parent
+    const int4 pf = info[parent];
+    if (pf.z < 0) return;
+    const i64 child = pf.z + (t % 64) / 16;
+    const double my_re = m < p1 ? lre[parent * p1 + m] : 0.0;
+    const double my_im = m < p1 ? lim[parent * p1 + m] : 0.0;
+    const double rp = 1.4142135623730951 * half[parent];
+    const double rc = 1.4142135623730951 * half[child];
+    const double sr = (center[2 * child] - center[2 * parent]) / rp;
+    const double si = (center[2 * child + 1] - center[2 * parent + 1]) / rp;
+    double acc_r = 0.0, acc_i = 0.0, pr = 1.0, pi = 0.0;
+    for (int j = 0; j < 16; ++j) {
+        const int l = m + j;
+        const double b_re = __shfl(my_re, l < 16 ? l : 15, 16);
+        const double b_im = __shfl(my_im, l < 16 ? l : 15, 16);
+        if (l < p1) {
+            const double c = binomial[l * bsz + m];
+            const double br = b_re * c, bi = b_im * c;
+            acc_r += br * pr - bi * pi;
+            acc_i += br * pi + bi * pr;
+            const double tr = pr * sr - pi * si;
+            pi = pr * si + pi * sr;
+            pr = tr;
+        }
+    }
+    if (m >= p1) return;
+    double f = 1.0;
+    const double ratio = rc / rp;
+    for (int i = 0; i < m; ++i) f *= ratio;
+    lre[child * p1 + m] += acc_r * f;
+    lim[child * p1 + m] += acc_i * f;
+}
+
 // ------------------------------------------------------------------------------ L2P + P2P
 // kernels/fmm_evaluate_host/fmm_evaluate_host_pyccel.py:14-100: L2P W = sum_l b_l u^l,
 // phi += -2 Re W, E_x - i E_y += 2 W'; near field phi -= q ln(r^2 + eps^2),
@@ -874,9 +917,14 @@ extern "C" int td3_l2l(int64_t order, const double* node_center, const double* n
     const int p1 = (int)order + 1;
     for (i64 d = 0; d + 1 < n_levels; ++d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
-        TD3_TIMED(s, "l2l", hipLaunchKernelGGL(l2l_kernel, dim3(td3_blocks((hi - lo) * 4 * p1, 256)), dim3(256), 0, s,
-                           node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
-                           local_re, local_im));
+        if (p1 <= 16)
+            TD3_TIMED(s, "l2l", hipLaunchKernelGGL(l2l16_kernel, dim3(td3_blocks((hi - lo) * 64, 256)), dim3(256), 0, s,
+                      node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
+                      local_re, local_im));
+        else
+            TD3_TIMED(s, "l2l", hipLaunchKernelGGL(l2l_kernel, dim3(td3_blocks((hi - lo) * 4 * p1, 256)), dim3(256), 0, s,
+                      node_center, node_half, (const int4*)node_info, lo, hi, p1, binomial, (int)bsz,
+                      local_re, local_im));
         TD3_LAUNCHED();
     }
     return TD3_OK;
