@@ -85,6 +85,51 @@ __global__ void cart_m2m_kernel(const double* __restrict__ center, const int4* _
     mom[node * nc + i] = acc;
 }
 
+// The same with a wavefront per node (orders up to 13: nc <= kCartStage): the four
+// children's moments are staged in LDS once, instead of every (node, i) thread
+// reloading up to (g_x + 1)(g_y + 1) of them per child.  Same terms in the same order.
+constexpr int kCartStage = 105;  // csize(13)
+__global__ __launch_bounds__(256) void cart_m2m_lds_kernel(
+    const double* __restrict__ center, const int4* __restrict__ info, i64 lo, i64 hi, int p,
+    int nc, const double* __restrict__ binom, int bsz, double* __restrict__ mom) {
+    __shared__ double s_m[4][4][kCartStage];
+    const int w = (int)(threadIdx.x / 64), lane = td3_lane();
+    const i64 node = lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    if (node >= hi) return;  // uniform over the wavefront
+    const int4 nf = info[node];
+    if (nf.z < 0) return;
+    for (int c = 0; c < 4; ++c) {
+        const int4 cf = info[nf.z + c];
+        for (int i = lane; i < nc; i += 64)
+            s_m[w][c][i] = cf.y == cf.x ? 0.0 : mom[(i64)(nf.z + c) * nc + i];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const double cx = center[2 * node], cy = center[2 * node + 1];
+    for (int i = lane; i < nc; i += 64) {
+        int deg = 0;
+        while (csize(deg) <= i) ++deg;
+        const int gy = i - deg * (deg + 1) / 2, gx = deg - gy;
+        double acc = 0.0;
+        for (int c = 0; c < 4; ++c) {
+            const int4 cf = info[nf.z + c];
+            if (cf.y == cf.x) continue;
+            const double sx = center[2 * (nf.z + c)] - cx, sy = center[2 * (nf.z + c) + 1] - cy;
+            double px = 1.0;
+            for (int bx = gx; bx >= 0; --bx) {
+                const double fx = binom[gx * bsz + bx] * px;
+                double py = 1.0;
+                for (int by = gy; by >= 0; --by) {
+                    acc += fx * binom[gy * bsz + by] * s_m[w][c][cidx(bx, by)] * py;
+                    py *= sy;
+                }
+                px *= sx;
+            }
+        }
+        mom[node * nc + i] = acc;
+    }
+}
+
 // m'_a = (-1)^|a| m_a / a!
 __global__ void cart_prime_kernel(const double* __restrict__ mom, i64 count, int nc,
                                   double* __restrict__ out) {
@@ -309,6 +354,43 @@ __global__ void cart_l2l_kernel(const double* __restrict__ center, const int4* _
     local[child * nc + i] += acc;
 }
 
+// The same with a wavefront per parent: its coefficients staged in LDS once for the four
+// children's 4 nc outputs.
+__global__ __launch_bounds__(256) void cart_l2l_lds_kernel(
+    const double* __restrict__ center, const int4* __restrict__ info, i64 lo, i64 hi, int p,
+    int nc, const double* __restrict__ binom, int bsz, double* __restrict__ local) {
+    __shared__ double s_l[4][kCartStage];
+    const int w = (int)(threadIdx.x / 64), lane = td3_lane();
+    const i64 parent = lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    if (parent >= hi) return;  // uniform over the wavefront
+    const int4 pf = info[parent];
+    if (pf.z < 0) return;
+    for (int i = lane; i < nc; i += 64) s_l[w][i] = local[parent * nc + i];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int o = lane; o < 4 * nc; o += 64) {
+        const i64 child = pf.z + o / nc;
+        const int i = o % nc;
+        int deg = 0;
+        while (csize(deg) <= i) ++deg;
+        const int by = i - deg * (deg + 1) / 2, bx = deg - by;
+        const double sx = center[2 * child] - center[2 * parent];
+        const double sy = center[2 * child + 1] - center[2 * parent + 1];
+        double acc = 0.0;
+        double px = 1.0;
+        for (int gx = bx; gx <= p; ++gx) {
+            const double fx = binom[gx * bsz + bx] * px;
+            double py = 1.0;
+            for (int gy = by; gx + gy <= p; ++gy) {
+                acc += fx * binom[gy * bsz + by] * s_l[w][cidx(gx, gy)] * py;
+                py *= sy;
+            }
+            px *= sx;
+        }
+        local[child * nc + i] += acc;
+    }
+}
+
 template <int P>
 int launch_cart_m2l(hipStream_t s, const int* col, const int* dst, const u32* row_start, u32 e_lo,
                     u32 e_hi, i64 node_lo, i64 node_hi, const double* dcenter,
@@ -353,8 +435,12 @@ extern "C" int td3_cart_moments(int64_t order, const double* sorted_pos, const d
     TD3_LAUNCHED();
     for (i64 d = n_levels - 2; d >= 0; --d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
-        TD3_TIMED(s, "cart_m2m", hipLaunchKernelGGL(cart_m2m_kernel, dim3(td3_blocks((hi - lo) * nc, 256)), dim3(256), 0, s,
-                           node_center, info, lo, hi, p, nc, binomial, (int)bsz, moments));
+        if (nc <= kCartStage)
+            TD3_TIMED(s, "cart_m2m", hipLaunchKernelGGL(cart_m2m_lds_kernel, dim3(td3_blocks((hi - lo) * 64, 256)), dim3(256), 0, s,
+                      node_center, info, lo, hi, p, nc, binomial, (int)bsz, moments));
+        else
+            TD3_TIMED(s, "cart_m2m", hipLaunchKernelGGL(cart_m2m_kernel, dim3(td3_blocks((hi - lo) * nc, 256)), dim3(256), 0, s,
+                      node_center, info, lo, hi, p, nc, binomial, (int)bsz, moments));
         TD3_LAUNCHED();
     }
     hipLaunchKernelGGL(cart_prime_kernel, dim3(td3_blocks(node_count * nc, 256)), dim3(256), 0, s, moments,
@@ -420,8 +506,12 @@ extern "C" int td3_cart_l2l(int64_t order, const double* node_center, const void
     const int p = (int)order, nc = csize(p);
     for (i64 d = 0; d + 1 < n_levels; ++d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
-        TD3_TIMED(s, "cart_l2l", hipLaunchKernelGGL(cart_l2l_kernel, dim3(td3_blocks((hi - lo) * 4 * nc, 256)), dim3(256), 0, s,
-                           node_center, (const int4*)node_info, lo, hi, p, nc, binomial, (int)bsz, local));
+        if (nc <= kCartStage)
+            TD3_TIMED(s, "cart_l2l", hipLaunchKernelGGL(cart_l2l_lds_kernel, dim3(td3_blocks((hi - lo) * 64, 256)), dim3(256), 0, s,
+                      node_center, (const int4*)node_info, lo, hi, p, nc, binomial, (int)bsz, local));
+        else
+            TD3_TIMED(s, "cart_l2l", hipLaunchKernelGGL(cart_l2l_kernel, dim3(td3_blocks((hi - lo) * 4 * nc, 256)), dim3(256), 0, s,
+                      node_center, (const int4*)node_info, lo, hi, p, nc, binomial, (int)bsz, local));
         TD3_LAUNCHED();
     }
     return TD3_OK;
