@@ -185,7 +185,11 @@ int td3_traverse(const double *node_center, const double *node_half, const void 
  * td3_moments.  out_counts (HOST int64[4]): M2L entries, P2P entries, non-empty leaves
  * (self pairs), capacity overflow.  TD3_ERR_CAPACITY asks for larger cap_m2l / cap_p2p / cap_inherit
  * (cap_inherit bounds the pairs handed to one level).  Any depth (the walks follow child
- * and parent links, no stack).  Synchronises the stream once.
+ * and parent links, no stack).  Synchronises the stream once, unless device_counts
+ * (DEVICE int64[4], may be NULL) is given: then the same four counts are written on the
+ * device, with no read-back and no synchronisation (out_counts may then be NULL), and
+ * on overflow every row is left empty, so the rows can be evaluated safely before the
+ * caller reads the counts and builds again with larger capacities.
  */
 size_t td3_rows_workspace_bytes(const int64_t *levels_host, int64_t n_levels,
                                 int64_t cap_inherit);
@@ -193,7 +197,8 @@ int td3_rows(const double *node_center, const double *node_half, const void *nod
              const int64_t *levels_host, int64_t n_levels, double theta, int64_t cap_m2l,
              int64_t cap_p2p, int64_t cap_inherit, uint32_t *m2l_row, int32_t *m2l_col,
              int32_t *m2l_dst, uint32_t *p2p_row, int32_t *p2p_col, void *workspace,
-             size_t workspace_bytes, int64_t *out_counts, void *stream);
+             size_t workspace_bytes, int64_t *out_counts, int64_t *device_counts,
+             void *stream);
 
 /*
  * Rows by node of a pair list (kernels/pair_csr/pair_csr_pyccel.py:12-45): mode 0 lists
@@ -224,9 +229,10 @@ int td3_scale_moments(int precision, const double *moment_re, const double *mome
  * same precision; dst_* are the destination tree's cells (the rows), src_* the source
  * tree's (the columns; the same arrays for fmm, the target tree for fmm_at).  Rows of the
  * range with pairs are WRITTEN (not accumulated); other rows are untouched.  FP32 up to
- * order 15 runs on the matrix cores, a wavefront per row; otherwise a lane per pair, with
- * the workspace (td3_m2l_workspace_bytes of at least the range's pair count) holding the
- * partial sums of rows that span several wavefronts.  No host read-back.
+ * order 15 runs on the matrix cores, a wavefront per row, and so does FP64 up to order 15
+ * when workspace is NULL; otherwise a lane per pair, with the workspace
+ * (td3_m2l_workspace_bytes of at least the range's pair count) holding the partial sums
+ * of rows that span several wavefronts.  No host read-back.
  */
 size_t td3_m2l_workspace_bytes(int64_t pair_count, int64_t order, int precision);
 int td3_m2l(int precision, int64_t order, const int32_t *col, const int32_t *dst,

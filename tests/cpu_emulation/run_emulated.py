@@ -159,6 +159,19 @@ def case_fmm():
         assert relative(got[1], want[1]) < 1e-12
 
 
+def case_rows_overflow():
+    # rows built without a read-back overflow their capacities: every row is left empty,
+    # the FMM notices with its status and builds them again
+    pos, q = cloud(400, 3)
+    t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10, leaf_capacity=4)
+    t._ROW_CAPS = (1, 1, 1)
+    got = t.fmm(theta=0.5, softening=1e-3, precision="mixed")
+    r = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10, leaf_capacity=4)
+    want = r.fmm(theta=0.5, softening=1e-3, precision="mixed")
+    assert t.fmm_pairs == r.fmm_pairs
+    assert relative(got[0], want[0]) < 1e-14 and relative(got[1], want[1]) < 1e-14
+
+
 def case_fmm_at():
     rng = np.random.default_rng(13)
     pos, q = cloud(250, 13)

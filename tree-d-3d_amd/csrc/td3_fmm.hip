@@ -878,15 +878,23 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
     TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
     hipStream_t s = (hipStream_t)stream;
     if (node_hi == node_lo) return TD3_OK;
-    if (precision != TD3_FP64 && order + 1 <= 16) {
-        // FP32: a wavefront per destination row on the matrix cores.  (FP64 stays on the
-        // lane-per-pair kernel: on gfx950 v_mfma_f64_16x16x4f64 runs at the vector FP64
-        // rate, and the lane kernel does less work around the product.)
+    if (order + 1 <= 16 && (precision != TD3_FP64 || workspace == nullptr)) {
+        // a wavefront per destination row on the matrix cores: FP32, or FP64 when no
+        // workspace is given (a grid by rows, so the pair count need not be known; for a
+        // whole tree the lane-per-pair kernel is the faster FP64 one: on gfx950
+        // v_mfma_f64_16x16x4f64 runs at the vector FP64 rate, and that kernel does less
+        // work around the product)
         const dim3 grid(td3_blocks((node_hi - node_lo) * 64, 256));
-        TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<float>), grid, dim3(256), 0, s, col, row_start,
-                           node_lo, node_hi, dst_center, dst_half, src_center, src_half,
-                           (const float*)scaled_re, (const float*)scaled_im, (int)order + 1,
-                           coeff, local_re, local_im));
+        if (precision == TD3_FP64)
+            TD3_TIMED(s, "m2l_mfma64", hipLaunchKernelGGL((m2l_mfma_kernel<double>), grid, dim3(256), 0, s, col, row_start,
+                      node_lo, node_hi, dst_center, dst_half, src_center, src_half,
+                      (const double*)scaled_re, (const double*)scaled_im, (int)order + 1,
+                      coeff, local_re, local_im));
+        else
+            TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<float>), grid, dim3(256), 0, s, col, row_start,
+                      node_lo, node_hi, dst_center, dst_half, src_center, src_half,
+                      (const float*)scaled_re, (const float*)scaled_im, (int)order + 1,
+                      coeff, local_re, local_im));
         TD3_LAUNCHED();
         return TD3_OK;
     }

@@ -449,6 +449,25 @@ __global__ void rows_root_kernel(u32* inh, int* enode, u32* first, u32* lv, u32*
     }
 }
 
+__global__ void rows_empty_kernel(const u32* __restrict__ flag, i64 count,
+                                  u32* __restrict__ m2l_row, u32* __restrict__ p2p_row) {
+    if (!(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 2u)) return;
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > count) return;
+    m2l_row[i] = 0u;
+    p2p_row[i] = 0u;
+}
+
+// {M2L entries, P2P entries, non-empty leaves, overflow}
+__global__ void rows_counts_kernel(const u32* base, int levels, const u32* flag, int64_t* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        out[0] = base[2 * levels];
+        out[1] = base[2 * levels + 1];
+        out[2] = flag[2];
+        out[3] = (flag[0] & 2u) ? 1 : 0;
+    }
+}
+
 __global__ void rows_end_kernel(const u32* base, int levels, i64 count, u32* m2l_row,
                                 u32* p2p_row) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -514,13 +533,13 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
                         int64_t cap_m2l, int64_t cap_p2p, int64_t cap_inherit,
                         uint32_t* m2l_row, int32_t* m2l_col, int32_t* m2l_dst, uint32_t* p2p_row,
                         int32_t* p2p_col, void* workspace, size_t workspace_bytes,
-                        int64_t* out_counts, void* stream) {
+                        int64_t* out_counts, int64_t* device_counts, void* stream) {
     TD3_ARG(levels_host != nullptr && n_levels >= 1 && n_levels <= 64, "levels");
     const i64 count = levels_host[n_levels];
     TD3_ARG(count >= 1 && count < (1 << 29), "node count");  // event codes hold 29 bits
     TD3_ARG(cap_m2l >= 1 && cap_m2l < 0xffffffffll && cap_p2p >= 1 && cap_p2p < 0xffffffffll &&
                 cap_inherit >= 16 && cap_inherit < 0xffffffffll, "capacities");
-    TD3_ARG(out_counts != nullptr, "out_counts");
+    TD3_ARG(out_counts != nullptr || device_counts != nullptr, "out_counts");
     TD3_ARG(workspace_bytes >= td3_rows_workspace_bytes(levels_host, n_levels, cap_inherit),
             "workspace too small");
     hipStream_t s = (hipStream_t)stream;
@@ -585,6 +604,17 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     hipLaunchKernelGGL(rows_end_kernel, dim3(1), dim3(1), 0, s, w.base, (int)n_levels, count,
                        m2l_row, p2p_row);
     TD3_LAUNCHED();
+    if (device_counts != nullptr) {
+        // no read-back: the counts go to the device, and an overflow leaves every row
+        // empty (safe to evaluate; the caller checks the counts and builds again)
+        hipLaunchKernelGGL(rows_empty_kernel, dim3(td3_blocks(count + 1, 256)), dim3(256), 0, s,
+                           w.flag, count, m2l_row, p2p_row);
+        TD3_LAUNCHED();
+        hipLaunchKernelGGL(rows_counts_kernel, dim3(1), dim3(1), 0, s, w.base, (int)n_levels,
+                           w.flag, device_counts);
+        TD3_LAUNCHED();
+        return TD3_OK;
+    }
     u32 host[5];
     TD3_HIP(hipMemcpyAsync(host, w.flag, sizeof(u32) * 3, hipMemcpyDeviceToHost, s));
     TD3_HIP(hipMemcpyAsync(host + 3, w.base + 2 * n_levels, sizeof(u32) * 2, hipMemcpyDeviceToHost, s));

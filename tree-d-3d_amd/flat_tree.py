@@ -562,14 +562,32 @@ class FlatQuadtree:
         self._traversals[key] = lists
         return lists
 
-    def _rows(self, theta: float):
-        """td3_rows: every node's rows from the pairs it inherits, level by level."""
+    def _rows(self, theta: float, deferred: bool = False):
+        """td3_rows: every node's rows from the pairs it inherits, level by level.
+
+        ``deferred``: no read-back; returns the rows at full capacity and their counts on
+        the device ``(m2l, p2p, leaves, overflow)`` for `_accept_rows` to check later (an
+        overflow leaves every row empty)."""
         dev = self._d["pos"].device
         c = self.node_count
         levels = np.ascontiguousarray(self.levels, dtype=np.int64)
         n_levels = len(levels) - 1
-        caps = [96 * c + 64, 40 * c + 64, 24 * c + 64]
+        caps = [f * c + 64 for f in self._ROW_CAPS]
         out = np.zeros(4, dtype=np.int64)
+        if deferred:
+            i32 = dict(dtype=torch.int32, device=dev)
+            mrow, prow = torch.empty(c + 1, **i32), torch.empty(c + 1, **i32)
+            mcol, mdst = torch.empty(caps[0], **i32), torch.empty(caps[0], **i32)
+            pcol = torch.empty(caps[1], **i32)
+            dcounts = torch.zeros(4, dtype=torch.int64, device=dev)
+            nbytes = _abi.lib().td3_rows_workspace_bytes(levels.ctypes.data, n_levels, caps[2])
+            ws = _abi.workspace(nbytes)
+            _abi.call("td3_rows", _abi.ptr(self._d["node_center"]), _abi.ptr(self._d["node_half"]),
+                      _abi.ptr(self._node_info), levels.ctypes.data, n_levels, float(theta),
+                      caps[0], caps[1], caps[2], _abi.ptr(mrow), _abi.ptr(mcol), _abi.ptr(mdst),
+                      _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(ws), nbytes, None,
+                      _abi.ptr(dcounts), _abi.stream())
+            return (mrow, mcol, mdst), (prow, pcol, None), dcounts
         while True:
             i32 = dict(dtype=torch.int32, device=dev)
             mrow, prow = torch.empty(c + 1, **i32), torch.empty(c + 1, **i32)
@@ -581,7 +599,7 @@ class FlatQuadtree:
                 _abi.ptr(self._d["node_center"]), _abi.ptr(self._d["node_half"]),
                 _abi.ptr(self._node_info), levels.ctypes.data, n_levels, float(theta), caps[0],
                 caps[1], caps[2], _abi.ptr(mrow), _abi.ptr(mcol), _abi.ptr(mdst),
-                _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(ws), nbytes, out.ctypes.data,
+                _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(ws), nbytes, out.ctypes.data, None,
                 _abi.stream(),
             )
             if code == _abi.TD3_ERR_CAPACITY:
@@ -590,8 +608,13 @@ class FlatQuadtree:
                 continue
             _abi.check(code, "td3_rows")
             break
+        return self._sliced_rows((mrow, mcol, mdst), (prow, pcol, None), out)
+
+    @staticmethod
+    def _sliced_rows(m2l_rows, p2p_rows, out):
         n_m2l, n_p2p, leaves = int(out[0]), int(out[1]), int(out[2])
         counts = (n_m2l // 2, (n_p2p + leaves) // 2)
+        (mrow, mcol, mdst), (prow, pcol, _) = m2l_rows, p2p_rows
         return (mrow, mcol[: max(n_m2l, 1)], mdst[: max(n_m2l, 1)]), \
             (prow, pcol[: max(n_p2p, 1)], None), counts
 
@@ -675,6 +698,10 @@ class FlatQuadtree:
     #: largest: a cell's net charge grows with its size)
     FP64_TOP_DEPTH = 5
 
+    #: first capacities of the interaction rows per node (M2L, P2P entries, pairs handed
+    #: to one level): exceeded, the rows are built again with twice the room
+    _ROW_CAPS = (96, 40, 24)
+
     def _local_expansions(self, theta: float, precision: int, target=None, m2l_rows=None):
         """Scaled local expansions B_l = b_l r^l of every cell of ``target`` (this tree by
         default): M2L from this tree's cells along ``m2l_rows``, then L2L down ``target``."""
@@ -698,8 +725,14 @@ class FlatQuadtree:
             if hi <= lo:
                 continue
             are, aim = self._scaled_moments(prec)
-            nbytes = _abi.lib().td3_m2l_workspace_bytes(int(mcol.shape[0]), self.order, prec)
-            ws = _abi.workspace(nbytes)
+            if precision != _abi.FP64 and self.order + 1 <= 16:
+                # mixed: both ranges on the matrix cores, gridded by rows (no workspace,
+                # no pair count needed)
+                nbytes, ws = 0, None
+            else:
+                nbytes = _abi.lib().td3_m2l_workspace_bytes(int(mcol.shape[0]), self.order,
+                                                            prec)
+                ws = _abi.workspace(nbytes)
             _abi.call("td3_m2l", prec, self.order, _abi.ptr(mcol), _abi.ptr(mdst),
                       _abi.ptr(mrow), lo, hi, _abi.ptr(td["node_center"]),
                       _abi.ptr(td["node_half"]), _abi.ptr(self._d["node_center"]),
@@ -757,6 +790,7 @@ class FlatQuadtree:
         if self.kernel != "log":
             return self._coulomb_fmm(theta, softening, potential, precision)
         prec = _precision(precision or self.precision)
+        pending = None
         if ("lists", theta) not in self._traversals:
             # queue the work that needs no interaction rows ahead of them: the GPU runs it
             # while the host waits for the rows' sizes
@@ -765,8 +799,17 @@ class FlatQuadtree:
             self._scaled_moments(_abi.FP64)
             if prec != _abi.FP64:
                 self._scaled_moments(prec)
-        lre, lim = self._local_expansions(theta, prec)
-        _, (prow, pcol, _), counts = self._lists(theta)
+            if prec != _abi.FP64 and self.order + 1 <= 16:
+                # mixed precision needs no row counts before its last read-back: the rows
+                # are built and used at full capacity and checked with the status
+                pending = self._rows(theta, deferred=True)
+        if pending is not None:
+            m2l_rows, p2p_rows, dcounts = pending
+            lre, lim = self._local_expansions(theta, prec, m2l_rows=m2l_rows)
+            prow, pcol = p2p_rows[0], p2p_rows[1]
+        else:
+            lre, lim = self._local_expansions(theta, prec)
+            _, (prow, pcol, _), counts = self._lists(theta)
         status = torch.zeros(1, dtype=torch.int64, device=dev)
         if jacobian or sdip is not None:
             _abi.call(
@@ -792,7 +835,21 @@ class FlatQuadtree:
             1 if potential else 0, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status),
             _abi.stream(),
         )
-        _check(status)
+        if pending is not None:
+            host = torch.cat((status, dcounts)).cpu().numpy()
+            if host[4]:
+                # the rows overflowed (every row was left empty): build them again with
+                # room to spare (read back, capacities doubled until they fit), then
+                # evaluate again on them
+                self._lists(theta)
+                return self.fmm(theta=theta, softening=softening, potential=potential,
+                                precision=precision)
+            lists = self._sliced_rows(m2l_rows, p2p_rows, host[1:])
+            self._traversals[("lists", theta)] = lists
+            counts = lists[2]
+            _check(host[:1])
+        else:
+            _check(status)
         self.fmm_pairs = counts
         return _out(phi, self._like_torch), _out(field, self._like_torch)
 
