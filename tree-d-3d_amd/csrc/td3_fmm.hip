@@ -532,8 +532,11 @@ __global__ __launch_bounds__(256) void eval_kernel(
         bool bad = false;
         const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
         if (PREC == TD3_FP64) {
+            // the next partner leaf's range is loaded while this one's sources are summed
+            int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
             for (u32 r = r0; r < r1; ++r) {
-                const int4 sf = sinfo[pcol[r]];
+                const int4 sf = sf_next;
+                if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
                 for (int ks = sf.x; ks < sf.y; ++ks) {
                     const double4 sv = src4[ks];
                     const double sq = sv.z;
@@ -563,8 +566,11 @@ __global__ __launch_bounds__(256) void eval_kernel(
             // Differences in FP64 (exact for nearby particles, whatever cells hold them),
             // kernel in FP32; FP32 sums over one partner leaf, FP64 sums over the leaves.
             const float eps2f = (float)eps2;
+            // the next partner leaf's range is loaded while this one's sources are summed
+            int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
             for (u32 r = r0; r < r1; ++r) {
-                const int4 sf = sinfo[pcol[r]];
+                const int4 sf = sf_next;
+                if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
                 float fp = 0.0f, fx = 0.0f, fy = 0.0f;
                 for (int ks = sf.x; ks < sf.y; ++ks) {
                     const double4 sv = src4[ks];
