@@ -72,7 +72,9 @@ int td3_direct(int kernel, int precision, const double *targets, int64_t m,
  * levels int64[67] (levels[d]:levels[d+1] are the nodes of depth d); perm (N,) int64;
  * sorted_pos (2, N) = pos[:, perm] (may be NULL); node_info (C,) int4 {start, end, child,
  * depth} for the other kernels (may be NULL).  out_sizes (HOST int64[3]) gets
- * {node_count, n_levels, needed capacity}.  Returns TD3_ERR_CAPACITY when capacity <
+ * {node_count, n_levels, needed capacity}; levels_host (HOST int64[67], may be NULL) the
+ * same level starts as levels, so that the caller needs no read-back of its own.  Returns
+ * TD3_ERR_CAPACITY when capacity <
  * node_count (call again with out_sizes[2]).  Synchronises the stream once (one
  * read-back of the split counts).  Supports N < 2^31 and trees of depth <= 64.
  */
@@ -97,13 +99,13 @@ int td3_forest_build(const double *pos, int64_t n, int64_t systems,
                      int64_t *node_start, int64_t *node_end, int64_t *node_child,
                      int64_t *node_depth, int64_t *levels, int64_t *perm, double *sorted_pos,
                      void *node_info, void *workspace, size_t workspace_bytes,
-                     int64_t *out_sizes, void *stream);
+                     int64_t *out_sizes, int64_t *levels_host, void *stream);
 int td3_tree_build(const double *pos, int64_t n, double xmid, double ymid, double half,
                    int64_t leaf_capacity, int64_t max_depth, int64_t capacity, double *node_center,
                    double *node_half, int64_t *node_start, int64_t *node_end, int64_t *node_child,
                    int64_t *node_depth, int64_t *levels, int64_t *perm, double *sorted_pos,
                    void *node_info, void *workspace, size_t workspace_bytes, int64_t *out_sizes,
-                   void *stream);
+                   int64_t *levels_host, void *stream);
 
 /* dst[r, k] = src[r * ld_src + perm[k]] for r < rows (<= 8), k < n: tree-order copies
  * (flat_tree.py:879-889 _sorted_sources). */

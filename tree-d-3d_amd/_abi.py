@@ -39,9 +39,9 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_direct": ([_C, _C, _P, _I, _P, _P, _I, _P, _D, _P, _P, _P, _P, _Z, _P], _C),
     "td3_tree_workspace_bytes": ([_I, _I], _Z),
     "td3_tree_build": ([_P, _I, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                        _P, _Z, _P, _P], _C),
+                        _P, _Z, _P, _P, _P], _C),
     "td3_forest_build": ([_P, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                          _P, _P, _Z, _P, _P], _C),
+                          _P, _P, _Z, _P, _P, _P], _C),
     "td3_permute": ([_P, _I, _I, _P, _I, _P, _P], _C),
     "td3_moments": ([_C, _I, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P],
                     _C),

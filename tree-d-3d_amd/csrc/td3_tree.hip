@@ -370,7 +370,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                double* node_center, double* node_half, int64_t* node_start, int64_t* node_end,
                int64_t* node_child, int64_t* node_depth, int64_t* levels, int64_t* perm,
                double* sorted_pos, void* node_info, void* workspace, size_t workspace_bytes,
-               int64_t* out_sizes, hipStream_t s) {
+               int64_t* out_sizes, int64_t* levels_host, hipStream_t s) {
     TD3_ARG(n > 0 && n < (int64_t)0x7fffffff, "n must be in [1, 2^31)");
     TD3_ARG(leaf_capacity >= 1 && max_depth >= 1, "leaf_capacity and max_depth must be positive");
     TD3_ARG(capacity >= roots_n, "capacity");
@@ -452,6 +452,14 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     out_sizes[0] = count;
     out_sizes[1] = n_levels;
     out_sizes[2] = count;
+    if (levels_host != nullptr) {
+        // the level starts, from the splitting nodes per depth already read back: each
+        // has four children one depth down
+        levels_host[0] = 0;
+        levels_host[1] = roots_n;
+        for (int d = 0; d + 2 <= n_levels; ++d)
+            levels_host[d + 2] = levels_host[d + 1] + 4 * (i64)host_small[d];
+    }
     if (count > capacity) {
         td3_set_error("node capacity %lld < %lld", (long long)capacity, (long long)count);
         return TD3_ERR_CAPACITY;
@@ -502,7 +510,7 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
                               int64_t* node_end, int64_t* node_child, int64_t* node_depth,
                               int64_t* levels, int64_t* perm, double* sorted_pos, void* node_info,
                               void* workspace, size_t workspace_bytes, int64_t* out_sizes,
-                              void* stream) {
+                              int64_t* levels_host, void* stream) {
     Root rt;
     rt.roots = nullptr;
     rt.cx = xmid;
@@ -511,7 +519,7 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     return build_impl(pos, n, 1, rt, nullptr, nullptr, leaf_capacity, max_depth, capacity,
                       node_center, node_half, node_start, node_end, node_child, node_depth, levels,
                       perm, sorted_pos, node_info, workspace, workspace_bytes, out_sizes,
-                      (hipStream_t)stream);
+                      levels_host, (hipStream_t)stream);
 }
 
 extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
@@ -521,7 +529,8 @@ extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
                                 int64_t* node_start, int64_t* node_end, int64_t* node_child,
                                 int64_t* node_depth, int64_t* levels, int64_t* perm,
                                 double* sorted_pos, void* node_info, void* workspace,
-                                size_t workspace_bytes, int64_t* out_sizes, void* stream) {
+                                size_t workspace_bytes, int64_t* out_sizes,
+                                int64_t* levels_host, void* stream) {
     TD3_ARG(systems >= 1 && systems <= (1 << 24), "systems");
     TD3_ARG(particle_system != nullptr && system_start != nullptr && roots != nullptr,
             "particle_system, system_start and roots");
@@ -531,5 +540,5 @@ extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
     return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
                       capacity, node_center, node_half, node_start, node_end, node_child,
                       node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
-                      out_sizes, (hipStream_t)stream);
+                      out_sizes, levels_host, (hipStream_t)stream);
 }

@@ -61,6 +61,7 @@ class _Forest(FlatQuadtree):
         systems = self._roots.shape[0]
         capacity = 4 * (2 * n // self.leaf_capacity + 16 * systems)
         out = np.zeros(3, dtype=np.int64)
+        levels_host = np.zeros(67, dtype=np.int64)
         while True:
             f64 = dict(dtype=torch.float64, device=dev)
             i64 = dict(dtype=torch.int64, device=dev)
@@ -82,7 +83,7 @@ class _Forest(FlatQuadtree):
                 _abi.ptr(center), _abi.ptr(half), _abi.ptr(start), _abi.ptr(end),
                 _abi.ptr(child), _abi.ptr(depth), _abi.ptr(levels), _abi.ptr(perm),
                 _abi.ptr(sorted_pos), _abi.ptr(info), _abi.ptr(ws), nbytes, out.ctypes.data,
-                _abi.stream(),
+                levels_host.ctypes.data, _abi.stream(),
             )
             if code == _abi.TD3_ERR_CAPACITY:
                 capacity = max(2 * capacity, int(out[2]))
@@ -91,7 +92,7 @@ class _Forest(FlatQuadtree):
             break
         count, n_levels = int(out[0]), int(out[1])
         self.node_count = count
-        self.levels = levels[: n_levels + 1].cpu().numpy()
+        self.levels = levels_host[: n_levels + 1].copy()
         self._d.update(perm=perm, node_center=center[:count], node_half=half[:count],
                        node_start=start[:count], node_end=end[:count], node_child=child[:count],
                        node_depth=depth[:count])

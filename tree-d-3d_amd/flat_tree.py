@@ -318,6 +318,7 @@ class FlatQuadtree:
         capacity = 4 * (2 * n // self.leaf_capacity + 16)
         xmid, ymid, half = self._root
         out = np.zeros(3, dtype=np.int64)
+        levels_host = np.zeros(67, dtype=np.int64)
         while True:
             f64 = dict(dtype=torch.float64, device=dev)
             i64 = dict(dtype=torch.int64, device=dev)
@@ -338,7 +339,7 @@ class FlatQuadtree:
                 _abi.ptr(center), _abi.ptr(half_arr), _abi.ptr(start), _abi.ptr(end),
                 _abi.ptr(child), _abi.ptr(depth), _abi.ptr(levels), _abi.ptr(perm),
                 _abi.ptr(sorted_pos), _abi.ptr(info), _abi.ptr(ws), nbytes,
-                out.ctypes.data, _abi.stream(),
+                out.ctypes.data, levels_host.ctypes.data, _abi.stream(),
             )
             if code == _abi.TD3_ERR_CAPACITY:
                 capacity = max(2 * capacity, int(out[2]))
@@ -347,7 +348,7 @@ class FlatQuadtree:
             break
         count, n_levels = int(out[0]), int(out[1])
         self.node_count = count
-        self.levels = levels[: n_levels + 1].cpu().numpy()
+        self.levels = levels_host[: n_levels + 1].copy()
         self._d.update(
             perm=perm,
             node_center=center[:count],
