@@ -153,6 +153,7 @@ def extra_config4(td3, systems=512, steps=100):
     import torch
 
     e0 = ens.total_energies()
+    waves = ens.autotune()  # the walk's launch shape, timed on this ensemble
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -162,6 +163,7 @@ def extra_config4(td3, systems=512, steps=100):
     drift = np.abs(ens.total_energies() - e0) / np.abs(e0)
     return {"workload": f"{systems} systems, sum N = {n}, {steps} steps, method tree",
             "ms_per_step": 1e3 * t / steps, "particle_steps_per_s": n * steps / t,
+            "walk_waves_per_block": waves,
             "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))}}
 
 

@@ -162,6 +162,14 @@ int td3_walk(int kernel, int precision, const double *targets, int64_t m, const 
              double *phi, double *field, int64_t *status, void *stream);
 
 /*
+ * The walk's launch shape: wavefronts (64 targets each) per block, 1, 2, 4 (default) or 8,
+ * for every later td3_walk of the process; 0 only asks.  Returns the previous value.  No
+ * counterpart in the reference: the knob of this build's tuner (Ensemble.autotune times
+ * the choices on the ensemble's own step and keeps the fastest).
+ */
+int td3_walk_waves(int waves);
+
+/*
  * Dual-tree traversal of one tree with itself (the FMM's interaction lists).  Replaces
  * flat_tree.py:1318-1358 and kernels/fmm_traverse (fmm_traverse_cuda.cu:14-97) with the
  * rules of kernels/fmm_traverse/fmm_traverse_pyccel.py:15-106.  Writes the mutual M2L

@@ -81,3 +81,25 @@ def test_ensemble_roots_grow_and_energy_drift_is_small(td3):
     assert roots[0, 2] == 1.0 and roots[1, 2] > 1.0
     drift = np.abs(ens.total_energies() - e0) / np.abs(e0)
     assert drift[0] < 1e-3 and drift[1] < 1e-3, drift
+
+
+def test_autotuned_walk_shapes_give_the_same_fields(td3):
+    # every launch shape of the walk computes the same walk (a wavefront's targets and
+    # traversal do not depend on the wavefronts per block): bit-identical fields
+    abi = sys.modules["tree_d_3d_amd._abi"]
+    sims = gaussian_systems(td3, [2000, 700, 5000, 33], 9)
+    ens = td3.Ensemble(sims)
+    print("walk launch shapes", flush=True)
+    previous = abi.lib().td3_walk_waves(0)
+    try:
+        fields = []
+        for waves in (1, 2, 4, 8):
+            abi.lib().td3_walk_waves(waves)
+            phi, field = ens.compute_fields().walk(ens.theta, ens.softening)
+            fields.append((phi.clone(), field.clone()))
+        for phi, field in fields[1:]:
+            assert torch.equal(phi, fields[0][0]) and torch.equal(field, fields[0][1])
+        chosen = ens.autotune(repeats=1)
+        assert chosen in (1, 2, 4, 8) and abi.lib().td3_walk_waves(0) == chosen
+    finally:
+        abi.lib().td3_walk_waves(previous)

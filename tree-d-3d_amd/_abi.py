@@ -46,6 +46,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_moments": ([_C, _I, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P],
                     _C),
     "td3_target_order_workspace_bytes": ([_I], _Z),
+    "td3_walk_waves": ([_C], _C),
     "td3_target_order": ([_P, _I, _D, _D, _D, _P, _P, _Z, _P], _C),
     "td3_walk": ([_C, _C, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I,
                   _I, _P, _D, _D, _C, _P, _P, _P, _P], _C),

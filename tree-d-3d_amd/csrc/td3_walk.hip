@@ -36,17 +36,17 @@ struct WalkTree {
     int p1;
 };
 
-template <int KERNEL, int PREC>
-__global__ __launch_bounds__(64 * kWaves) void walk_kernel(
+template <int KERNEL, int PREC, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void walk_kernel(
     WalkTree tr, const double* __restrict__ targets, i64 m, const i64* __restrict__ order,
     const i64* __restrict__ exclude, const int* __restrict__ target_root, double theta,
     double eps2, int with_potential, int stack_size, double* __restrict__ phi,
     double* __restrict__ field, i64* __restrict__ status) {
-    __shared__ int s_node[kWaves][kMaxStack];
-    __shared__ u64 s_mask[kWaves][kMaxStack];
+    __shared__ int s_node[WAVES][kMaxStack];
+    __shared__ u64 s_mask[WAVES][kMaxStack];
     const int wave = threadIdx.x / TD3_WAVE;
     const int lane = td3_lane();
-    const i64 slot = ((i64)blockIdx.x * kWaves + wave) * TD3_WAVE + lane;
+    const i64 slot = ((i64)blockIdx.x * WAVES + wave) * TD3_WAVE + lane;
     const bool active = slot < m;
     const i64 ti = active ? order[slot] : 0;
     const double tx = active ? targets[ti] : 0.0;
@@ -261,14 +261,43 @@ __global__ void widen_kernel(const u32* __restrict__ v, i64 n, i64* __restrict__
     if (i < n) out[i] = (i64)v[i];
 }
 
+template <int KERNEL, int PREC, int WAVES>
+void launch_walk_w(hipStream_t s, const WalkTree& tr, const double* targets, i64 m,
+                   const i64* order, const i64* exclude, const int* target_root, double theta,
+                   double eps2, int with_potential, int stack_size, double* phi, double* field,
+                   i64* status) {
+    const unsigned blocks = td3_blocks(m, 64 * WAVES);
+    TD3_TIMED(s, "walk", hipLaunchKernelGGL((walk_kernel<KERNEL, PREC, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, tr, targets,
+                       m, order, exclude, target_root, theta, eps2, with_potential, stack_size, phi,
+                       field, status));
+}
+
+// The walk's wavefronts per block (1, 2, 4 or 8): a launch shape chosen by the caller's
+// tuning (td3_walk_waves), the same for every walk of the process.
+int g_walk_waves = kWaves;
+
 template <int KERNEL, int PREC>
-void launch_walk(unsigned blocks, hipStream_t s, const WalkTree& tr, const double* targets, i64 m,
+void launch_walk(hipStream_t s, const WalkTree& tr, const double* targets, i64 m,
                  const i64* order, const i64* exclude, const int* target_root, double theta,
                  double eps2, int with_potential, int stack_size, double* phi, double* field,
                  i64* status) {
-    TD3_TIMED(s, "walk", hipLaunchKernelGGL((walk_kernel<KERNEL, PREC>), dim3(blocks), dim3(64 * kWaves), 0, s, tr, targets,
-                       m, order, exclude, target_root, theta, eps2, with_potential, stack_size, phi,
-                       field, status));
+    switch (g_walk_waves) {
+        case 1:
+            launch_walk_w<KERNEL, PREC, 1>(s, tr, targets, m, order, exclude, target_root, theta,
+                                           eps2, with_potential, stack_size, phi, field, status);
+            break;
+        case 2:
+            launch_walk_w<KERNEL, PREC, 2>(s, tr, targets, m, order, exclude, target_root, theta,
+                                           eps2, with_potential, stack_size, phi, field, status);
+            break;
+        case 8:
+            launch_walk_w<KERNEL, PREC, 8>(s, tr, targets, m, order, exclude, target_root, theta,
+                                           eps2, with_potential, stack_size, phi, field, status);
+            break;
+        default:
+            launch_walk_w<KERNEL, PREC, 4>(s, tr, targets, m, order, exclude, target_root, theta,
+                                           eps2, with_potential, stack_size, phi, field, status);
+    }
 }
 
 }  // namespace
@@ -339,23 +368,29 @@ extern "C" int td3_walk(int kernel, int precision, const double* targets, int64_
                            : (int)(3 * tree_depth + 8) < kMaxStack ? (int)(3 * tree_depth + 8)
                                                                    : kMaxStack;
     const double eps2 = softening * softening;
-    const unsigned blocks = td3_blocks(m, 64 * kWaves);
     hipStream_t s = (hipStream_t)stream;
     if (kernel == TD3_LOG) {
         if (precision == TD3_FP64)
-            launch_walk<TD3_LOG, TD3_FP64>(blocks, s, tr, targets, m, order, exclude, target_root, theta, eps2,
+            launch_walk<TD3_LOG, TD3_FP64>(s, tr, targets, m, order, exclude, target_root, theta, eps2,
                                            with_potential, stack_size, phi, field, status);
         else
-            launch_walk<TD3_LOG, TD3_MIXED>(blocks, s, tr, targets, m, order, exclude, target_root, theta, eps2,
+            launch_walk<TD3_LOG, TD3_MIXED>(s, tr, targets, m, order, exclude, target_root, theta, eps2,
                                             with_potential, stack_size, phi, field, status);
     } else {
         if (precision == TD3_FP64)
-            launch_walk<TD3_COULOMB, TD3_FP64>(blocks, s, tr, targets, m, order, exclude, target_root, theta, eps2,
+            launch_walk<TD3_COULOMB, TD3_FP64>(s, tr, targets, m, order, exclude, target_root, theta, eps2,
                                                with_potential, stack_size, phi, field, status);
         else
-            launch_walk<TD3_COULOMB, TD3_MIXED>(blocks, s, tr, targets, m, order, exclude, target_root, theta,
+            launch_walk<TD3_COULOMB, TD3_MIXED>(s, tr, targets, m, order, exclude, target_root, theta,
                                                 eps2, with_potential, stack_size, phi, field, status);
     }
     TD3_LAUNCHED();
     return TD3_OK;
+}
+
+extern "C" int td3_walk_waves(int waves) {
+    TD3_ARG(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8, "waves");
+    const int previous = g_walk_waves;
+    if (waves != 0) g_walk_waves = waves;
+    return previous;
 }

@@ -245,6 +245,30 @@ class Ensemble:
         self._current = True
         return self.forest
 
+    def autotune(self, repeats: int = 3) -> int:
+        """Times the walk's launch shapes (wavefronts of 64 targets per block: 1, 2, 4, 8)
+        on this ensemble's own forest and keeps the fastest for every later walk of the
+        process (`td3_walk_waves`); returns it.  Every shape computes the same walk (each
+        wavefront's targets and traversal are the same), so the fields do not change."""
+        import time
+
+        forest = self.compute_fields()
+        best, best_t = 4, float("inf")
+        for waves in (1, 2, 4, 8):
+            _abi.lib().td3_walk_waves(waves)
+            forest.walk(self.theta, self.softening)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(repeats):
+                forest.walk(self.theta, self.softening)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            if t < best_t:
+                best, best_t = waves, t
+        _abi.lib().td3_walk_waves(best)
+        self.walk_waves = best
+        return best
+
     # ------------------------------------------------------------------ dynamics
     def _push(self, first_half: bool) -> None:
         _abi.call("td3_push", 1 if first_half else 0, self.num_particles,
