@@ -374,10 +374,11 @@ class FlatQuadtree:
         dev = self._d["pos"].device
         c, n = self.node_count, self._d["pos"].shape[1]
         f64 = dict(dtype=torch.float64, device=dev)
-        node_q = torch.zeros(c, **f64)
-        node_qabs = torch.zeros(c, **f64)
-        node_p = torch.zeros((c, 2), **f64)
-        node_s = torch.zeros((c, 3), **f64)
+        # written in full by the kernels (log: q and p from the moments; s stays zero)
+        node_q = torch.empty(c, **f64)
+        node_qabs = torch.empty(c, **f64)
+        node_p = torch.empty((c, 2), **f64)
+        node_s = (torch.zeros if self.kernel == "log" else torch.empty)((c, 3), **f64)
         p1 = self.order + 1
         if self.kernel == "log":
             mre = torch.empty((c, p1), **f64)
@@ -660,7 +661,8 @@ class FlatQuadtree:
         if cached is not None:
             return cached
         n = self._d["pos"].shape[1]
-        src4 = torch.zeros((max(n, 1), 4), dtype=torch.float64, device=self._d["pos"].device)
+        src4 = (torch.empty if n else torch.zeros)((max(n, 1), 4), dtype=torch.float64,
+                                                    device=self._d["pos"].device)
         _abi.call("td3_pack_sources", _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge), n,
                   _abi.ptr(src4), _abi.stream())
         self._src4 = src4
@@ -672,7 +674,8 @@ class FlatQuadtree:
         if cached is not None:
             return cached
         n = self._d["pos"].shape[1]
-        leaf_of = torch.zeros(max(n, 1), dtype=torch.int32, device=self._d["pos"].device)
+        leaf_of = (torch.empty if n else torch.zeros)(max(n, 1), dtype=torch.int32,
+                                                       device=self._d["pos"].device)
         _abi.call("td3_leaf_index", _abi.ptr(self._node_info), self.node_count,
                   _abi.ptr(leaf_of), _abi.stream())
         self._leaf_of = leaf_of
@@ -783,9 +786,11 @@ class FlatQuadtree:
         dev = self._d["pos"].device
         sdip = self.__dict__.get("_sdip")
         _check_jacobian(self, jacobian, sdip)
-        phi = torch.zeros(n, dtype=torch.float64, device=dev)
-        field = torch.zeros((2, n), dtype=torch.float64, device=dev)
-        jac = torch.zeros((3, n), dtype=torch.float64, device=dev) if jacobian else None
+        # every target's values are written by the evaluation
+        alloc = torch.empty if n else torch.zeros
+        phi = alloc(n, dtype=torch.float64, device=dev)
+        field = alloc((2, n), dtype=torch.float64, device=dev)
+        jac = alloc((3, n), dtype=torch.float64, device=dev) if jacobian else None
         if not n:
             return self._fmm_out(phi, field, jac, self._like_torch)
         if self.kernel != "log":
