@@ -31,6 +31,8 @@ _abi._LIB = None
 _abi.device = lambda: torch.device("cpu")
 _abi.stream = lambda: 0
 _abi.ptr = lambda t: None if t is None else t.data_ptr()
+# workspaces start as junk (0xa5 bytes), so that a kernel relying on zeroed scratch fails
+_abi.workspace = lambda nbytes: torch.full((max(int(nbytes), 8),), 0xA5, dtype=torch.uint8)
 
 
 def relative(a, b):

@@ -437,15 +437,17 @@ __global__ void rows_geo_kernel(const double* __restrict__ center, const double*
     if (node < count) geo[node] = make_double4(center[2 * node], center[2 * node + 1], half[node], 0.0);
 }
 
-__global__ void rows_root_kernel(u32* inh, int* enode, u32* first, u32* lv, u32* base) {
+// Every level's row bases start at zero: after an overflow the writes stop, and the
+// counts read back are then those of the levels written so far (a lower bound).
+__global__ void rows_root_kernel(u32* inh, int* enode, u32* first, u32* lv, u32* base,
+                                 int levels) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         inh[0] = 0u;  // the pair (root, root)
         enode[0] = 0;
         first[0] = 0u;
         first[1] = 1u;
         lv[0] = 1u;
-        base[0] = 0u;
-        base[1] = 0u;
+        for (int k = 0; k < 2 * levels + 2; ++k) base[k] = 0u;
     }
 }
 
@@ -560,7 +562,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
                        (const int4*)node_info, count, w.parent, w.flag + 2);
     TD3_LAUNCHED();
     hipLaunchKernelGGL(rows_root_kernel, dim3(1), dim3(1), 0, s, w.inh_a, w.enode_a, w.first_a,
-                       w.lv, w.base);
+                       w.lv, w.base, (int)n_levels);
     TD3_LAUNCHED();
     u32 *inh = w.inh_a, *inh_next = w.inh_b, *first = w.first_a, *first_next = w.first_b;
     int *enode = w.enode_a, *enode_next = w.enode_b;
