@@ -161,10 +161,30 @@ def extra_config4(td3, systems=512, steps=100):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     drift = np.abs(ens.total_energies() - e0) / np.abs(e0)
+    # the same steps as one recorded HIP graph per step (Ensemble.run): the first step
+    # direct, the second recording and replaying, the rest timed replays
+    rec = td3.Ensemble.from_arrays(pos, np.zeros((2, n)), np.ones(n), np.ones(n), sizes, roots,
+                                   theta=0.5, softening=1e-3, k_e=1.0, dt=1e-4, B_z=1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec.run(2, graph=True)
+    torch.cuda.synchronize()
+    t_record = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rec.run(steps - 2, graph=True)
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    same = all(torch.equal(getattr(rec, k), getattr(ens, k))
+               for k in ("positions", "velocities", "forces"))
     return {"workload": f"{systems} systems, sum N = {n}, {steps} steps, method tree",
             "ms_per_step": 1e3 * t / steps, "particle_steps_per_s": n * steps / t,
             "walk_waves_per_block": waves,
-            "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))}}
+            "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))},
+            "graph": {"ms_per_step": 1e3 * t_graph / (steps - 2),
+                      "particle_steps_per_s": n * (steps - 2) / t_graph,
+                      "first_two_steps_ms": 1e3 * t_record,
+                      "node_capacity": rec.graph_capacity,
+                      "states_equal_step": bool(same)}}
 
 
 def extra_config5(td3, n=262144, probes=8192):

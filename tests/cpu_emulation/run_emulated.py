@@ -269,6 +269,61 @@ def case_forest():
     assert forest.node_count == total, (forest.node_count, total)
 
 
+def case_forest_resident():
+    """The step without read-backs (the one a HIP graph records) against Ensemble.step: the
+    same forest arrays and the same states, bit for bit; a capacity too small for the nodes
+    makes every root a leaf (exact sums per system) and is reported."""
+    from tree_d_3d_amd import ensemble
+
+    rng = np.random.default_rng(7)
+    sizes = [300, 17, 1, 520, 64, 9]
+    n = sum(sizes)
+    pos = np.concatenate([rng.normal(0, 0.1, (2, k)) for k in sizes], axis=1)
+    roots = np.array([[0.0, 0.0, 1.0]] * len(sizes))
+    args = (pos, np.zeros((2, n)), np.ones(n), np.ones(n), sizes, roots)
+    kw = dict(theta=0.5, softening=1e-3, k_e=1.0, dt=1e-4, B_z=1.0)
+    fixed = np.zeros(n, dtype=bool)
+    fixed[5] = fixed[400] = True
+    eager = ensemble.Ensemble.from_arrays(*args, fixed=fixed, **kw)
+    resident = ensemble.Ensemble.from_arrays(*args, fixed=fixed, **kw)
+    for _ in range(3):
+        eager.step()
+        resident.step_resident()
+        for name in ("positions", "velocities", "forces"):
+            a, b = getattr(eager, name).numpy(), getattr(resident, name).numpy()
+            assert np.array_equal(a, b), (name, np.abs(a - b).max())
+        assert np.array_equal(eager.potential_energies(), resident.potential_energies())
+        r, f = resident._resident, eager.forest
+        count = int(r.sizes[0])
+        assert count == f.node_count and int(r.sizes[2]) == 0
+        lv = r.levels.numpy()
+        assert np.array_equal(lv[: len(f.levels)], f.levels) and np.all(lv[len(f.levels):] == count)
+        for mine, name in ((r.center, "node_center"), (r.start, "node_start"),
+                           (r.end, "node_end"), (r.child, "node_child"), (r.perm, "perm"),
+                           (r.node_q, "node_q"), (r.node_s, "node_s")):
+            want = f._d[name].numpy()
+            assert np.array_equal(mine.numpy()[: len(want)], want), name
+    # too few nodes: the roots alone, as leaves: direct sums per system
+    small = ensemble.Ensemble.from_arrays(*args, **kw)
+    small.compute_fields()
+    small._resident = ensemble._Resident(small, len(sizes) + 3)
+    small.graph_capacity = len(sizes) + 3
+    before = small.positions.clone()
+    small.step_resident()
+    assert small._resident is None and small.graph_capacity > 2 * (len(sizes) + 3)
+    # the step's fields came from the new positions by exact sums within each system
+    after = small.positions
+    starts = np.concatenate(([0], np.cumsum(sizes)))
+    phi = small._potential.numpy()
+    for s_i in range(len(sizes)):
+        sl = slice(starts[s_i], starts[s_i + 1])
+        k = starts[s_i + 1] - starts[s_i]
+        want, _ = td3.direct_field(after[:, sl], after[:, sl], np.ones(k), softening=1e-3,
+                                   exclude=np.arange(k))
+        assert relative(phi[sl], np.asarray(want)) < 1e-13, s_i
+    assert not torch.equal(before, after)
+
+
 def sorted_rows(row, col):
     row = row.numpy().astype(np.int64)
     col = col.numpy()[: row[-1]].astype(np.int64)

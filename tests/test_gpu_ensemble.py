@@ -103,3 +103,71 @@ def test_autotuned_walk_shapes_give_the_same_fields(td3):
         assert chosen in (1, 2, 4, 8) and abi.lib().td3_walk_waves(0) == chosen
     finally:
         abi.lib().td3_walk_waves(previous)
+
+
+def _same_state(a, b):
+    for name in ("positions", "velocities", "forces"):
+        x, y = getattr(a, name), getattr(b, name)
+        assert torch.equal(x, y), (name, float((x - y).abs().max()))
+    a.compute_fields()
+    b.compute_fields()
+    assert torch.equal(a._potential, b._potential) and torch.equal(a._electric, b._electric)
+    # per-system sums (index_add_ adds in no fixed order): to rounding
+    np.testing.assert_allclose(a.total_energies(), b.total_energies(), rtol=1e-12)
+
+
+def test_graph_replays_equal_steps(td3):
+    """Ensemble.run(graph=True) (one step direct, then a recorded HIP graph replayed) gives
+    the states of Ensemble.step, bit for bit: after a first run, after every single replay
+    with the other ensemble's eager steps in between, and after a long run."""
+    rng = np.random.default_rng(12)
+    sizes = [int(k) for k in np.exp(rng.uniform(np.log(200), np.log(6000), 24)).round()]
+    eager = td3.Ensemble(gaussian_systems(td3, sizes, seed=12))
+    graph = td3.Ensemble(gaussian_systems(td3, sizes, seed=12))
+    print(f"graph run: {len(sizes)} systems, N = {sum(sizes)}", flush=True)
+    graph.run(3, graph=True)
+    for _ in range(3):
+        eager.step()
+    _same_state(eager, graph)
+    assert graph._graph is not None and graph.iteration == 3
+    for _ in range(3):
+        eager.step()
+        graph.run(1)
+        _same_state(eager, graph)
+    graph.run(10)
+    for _ in range(10):
+        eager.step()
+    _same_state(eager, graph)
+    assert graph.iteration == eager.iteration == 16
+    assert graph.current_time == eager.current_time
+    # eager steps of the same ensemble between replays: the record reads their fields
+    graph.step()
+    eager.step()
+    graph.run(2)
+    eager.step()
+    eager.step()
+    _same_state(eager, graph)
+
+
+def test_resident_step_overflow_sums_each_system_directly(td3):
+    """A node capacity too small for the forest: the resident step makes every root a leaf
+    (exact sums per system), reports it, and the next recording has room."""
+    ens_mod = sys.modules["tree_d_3d_amd.ensemble"]
+    sizes = [700, 40, 1300]
+    ens = td3.Ensemble(gaussian_systems(td3, sizes, seed=3))
+    ens.compute_fields()
+    ens._resident = ens_mod._Resident(ens, len(sizes) + 1)
+    ens.graph_capacity = len(sizes) + 1
+    print("resident step with 4 nodes of capacity", flush=True)
+    ens.step_resident()
+    assert ens._resident is None and ens.graph_capacity > 8
+    starts = np.concatenate(([0], np.cumsum(sizes)))
+    for s in range(len(sizes)):
+        sl = slice(int(starts[s]), int(starts[s + 1]))
+        k = sizes[s]
+        pos = ens.positions[:, sl]
+        want, _ = td3.direct_field(pos, pos, np.ones(k), softening=1e-3, exclude=np.arange(k))
+        got = ens._potential[sl]
+        assert float((got - want).abs().max() / want.abs().max()) < 1e-13
+    ens.run(3)
+    assert ens._resident is not None and int(ens._resident.sizes[3]) == 0

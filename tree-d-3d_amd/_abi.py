@@ -42,9 +42,12 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                         _P, _Z, _P, _P, _P], _C),
     "td3_forest_build": ([_P, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                           _P, _P, _Z, _P, _P, _P], _C),
+    "td3_forest_build_nosync": ([_P, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P,
+                                 _P, _P, _P, _P, _Z, _P, _P], _C),
     "td3_permute": ([_P, _I, _I, _P, _I, _P, _P], _C),
     "td3_moments": ([_C, _I, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P],
                     _C),
+    "td3_coulomb_moments_nosync": ([_P, _P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P], _C),
     "td3_target_order_workspace_bytes": ([_I], _Z),
     "td3_walk_waves": ([_C], _C),
     "td3_target_order": ([_P, _I, _D, _D, _D, _P, _P, _Z, _P], _C),

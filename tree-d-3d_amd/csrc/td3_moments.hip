@@ -152,15 +152,15 @@ __global__ void log_m2m16_kernel(const double* __restrict__ center, const int4* 
     if (k == 0) qabs[node] = qa;
 }
 
-// Coulomb moments (q, |q|, p, raw S) of the nodes [lo, hi): leaves from their particles
-// (leaves_only) or parents from their children.
-__global__ void coulomb_kernel(const double* __restrict__ spos, const double* __restrict__ scharge,
-                               i64 n, const double* __restrict__ center,
-                               const int4* __restrict__ info, i64 lo, i64 hi, int leaves_only,
-                               double* __restrict__ nq, double* __restrict__ nqabs,
-                               double* __restrict__ np, double* __restrict__ ns) {
-    const i64 node = lo + (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= hi) return;
+// Coulomb moments (q, |q|, p, raw S) of one node: a leaf from its particles (leaves_only)
+// or a parent from its children.
+__device__ __forceinline__ void coulomb_node(const double* __restrict__ spos,
+                                             const double* __restrict__ scharge, i64 n,
+                                             const double* __restrict__ center,
+                                             const int4* __restrict__ info, i64 node,
+                                             int leaves_only, double* __restrict__ nq,
+                                             double* __restrict__ nqabs, double* __restrict__ np,
+                                             double* __restrict__ ns) {
     const int4 nf = info[node];
     const bool leaf = nf.z < 0;
     if (leaf != (leaves_only != 0)) return;
@@ -198,6 +198,33 @@ __global__ void coulomb_kernel(const double* __restrict__ spos, const double* __
     ns[3 * node] = sxx;
     ns[3 * node + 1] = sxy;
     ns[3 * node + 2] = syy;
+}
+
+// The nodes [lo, hi), a thread each.
+__global__ void coulomb_kernel(const double* __restrict__ spos, const double* __restrict__ scharge,
+                               i64 n, const double* __restrict__ center,
+                               const int4* __restrict__ info, i64 lo, i64 hi, int leaves_only,
+                               double* __restrict__ nq, double* __restrict__ nqabs,
+                               double* __restrict__ np, double* __restrict__ ns) {
+    const i64 node = lo + (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= hi) return;
+    coulomb_node(spos, scharge, n, center, info, node, leaves_only, nq, nqabs, np, ns);
+}
+
+// The nodes [levels[d], levels[d + 1]) with the level starts read on the device (d < 0:
+// every node, [0, levels[66])), grid-stride: a fixed grid whatever the level's width.
+__global__ void coulomb_level_kernel(const double* __restrict__ spos,
+                                     const double* __restrict__ scharge, i64 n,
+                                     const double* __restrict__ center,
+                                     const int4* __restrict__ info, const i64* __restrict__ levels,
+                                     int d, int leaves_only, double* __restrict__ nq,
+                                     double* __restrict__ nqabs, double* __restrict__ np,
+                                     double* __restrict__ ns) {
+    const i64 lo = d < 0 ? 0 : levels[d];
+    const i64 hi = d < 0 ? levels[66] : levels[d + 1];
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 node = lo + (i64)blockIdx.x * blockDim.x + threadIdx.x; node < hi; node += stride)
+        coulomb_node(spos, scharge, n, center, info, node, leaves_only, nq, nqabs, np, ns);
 }
 
 __global__ void permute_kernel(const double* __restrict__ src, const i64* __restrict__ perm, i64 n,
@@ -293,3 +320,28 @@ extern "C" int td3_log_moments_dipole(int64_t order, const double* sorted_pos,
 }
 
 
+
+extern "C" int td3_coulomb_moments_nosync(const double* sorted_pos, const double* sorted_charge,
+                                          int64_t n, const double* node_center,
+                                          const void* node_info, int64_t capacity,
+                                          const int64_t* levels, int64_t max_depth,
+                                          double* node_q, double* node_qabs, double* node_p,
+                                          double* node_s, void* stream) {
+    TD3_ARG(n >= 1 && capacity >= 1, "sizes");
+    TD3_ARG(max_depth >= 1 && max_depth <= 64, "max_depth");
+    TD3_ARG(levels != nullptr && node_info != nullptr, "levels, node_info");
+    hipStream_t s = (hipStream_t)stream;
+    const int4* info = (const int4*)node_info;
+    const i64 cap_blocks = td3_blocks(capacity, 256);
+    const unsigned grid = (unsigned)(cap_blocks < 2048 ? cap_blocks : 2048);
+    TD3_TIMED(s, "moments_coulomb", hipLaunchKernelGGL(coulomb_level_kernel, dim3(grid), dim3(256), 0, s, sorted_pos, sorted_charge, n,
+                       node_center, info, levels, -1, 1, node_q, node_qabs, node_p, node_s));
+    TD3_LAUNCHED();
+    // parents sit at depths max_depth - 1 .. 0; levels past the tree's depth are empty
+    for (int d = (int)max_depth - 1; d >= 0; --d) {
+        TD3_TIMED(s, "moments_coulomb", hipLaunchKernelGGL(coulomb_level_kernel, dim3(grid), dim3(256), 0, s, sorted_pos, sorted_charge,
+                           n, node_center, info, levels, d, 0, node_q, node_qabs, node_p, node_s));
+        TD3_LAUNCHED();
+    }
+    return TD3_OK;
+}

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void count_kernel(const u64* __restrict__ khi,
 // One (depth, particle) entry per splitting node, in particle order.
 __global__ void entries_kernel(const u64* __restrict__ khi, const u64* __restrict__ klo, i64 n,
                                int nkey, Keys kl, const u32* __restrict__ cnt, const u32* __restrict__ off,
-                               u64* __restrict__ ekey, u32* __restrict__ eval) {
+                               i64 ne_cap, u64* __restrict__ ekey, u32* __restrict__ eval) {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const u32 c = cnt[i];
@@ -149,6 +149,7 @@ __global__ void entries_kernel(const u64* __restrict__ khi, const u64* __restric
     if (i > 0) prev = lcp_levels(khi[i - 1], klo ? klo[i - 1] : 0ull, khi[i], klo ? klo[i] : 0ull, nkey, kl);
     const u32 o = off[i];
     for (u32 j = 0; j < c; ++j) {
+        if ((i64)o + j >= ne_cap) return;  // more splitting nodes than the capacity holds
         ekey[o + j] = (u64)(prev + 1 + (int)j);
         eval[o + j] = (u32)i;
     }
@@ -175,6 +176,58 @@ __global__ void levels_kernel(const u32* __restrict__ depth_count, i64 roots_n,
     bstart[kMaxLevels + 1] = b;
 }
 
+// The build without a host read-back (td3_forest_build_nosync): node count, level count
+// and level starts from the per-depth counts on the device.  sizes = {node count, levels,
+// needed capacity when it overflows (else 0), largest need of any overflow so far (kept
+// until the caller clears it)}.  Every level start past the deepest level is the node
+// count, so those levels are empty.  When the nodes do not fit, the forest is the roots
+// alone, each a leaf.
+__global__ void levels_nosync_kernel(const u32* __restrict__ depth_count, i64 roots_n,
+                                     i64 capacity, i64* __restrict__ levels,
+                                     u32* __restrict__ bstart, i64* __restrict__ sizes) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    i64 count = roots_n;
+    int n_levels = 1;
+    for (int d = 0; d <= kMaxLevels; ++d) {
+        const u32 c = depth_count[d];
+        if (c) {
+            count += 4 * (i64)c;
+            n_levels = d + 2;
+        }
+    }
+    const bool over = count > capacity;
+    levels[0] = 0;
+    levels[1] = roots_n;
+    i64 lv = roots_n;
+    u32 b = 0;
+    for (int d = 0; d <= kMaxLevels; ++d) {
+        bstart[d] = b;
+        const u32 c = depth_count[d];
+        b += c;
+        if (!over) lv += 4 * (i64)c;
+        levels[d + 2] = lv;
+    }
+    bstart[kMaxLevels + 1] = b;
+    sizes[0] = over ? roots_n : count;
+    sizes[1] = over ? 1 : n_levels;
+    sizes[2] = over ? count : 0;
+    if (over && count > sizes[3]) sizes[3] = count;
+}
+
+__global__ void zero_u32_kernel(u32* __restrict__ p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0u;
+}
+
+// Entries [s_total, ne) get the depth key 255, which sorts after every real depth.
+__global__ void pad_entries_kernel(u64* __restrict__ ekey, u32* __restrict__ eval, i64 ne,
+                                   const u32* __restrict__ s_total) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne || k < (i64)*s_total) return;
+    ekey[k] = 255ull;
+    eval[k] = 0u;
+}
+
 // The root of every system: its particle range [sys_start[s], sys_start[s + 1]) (the whole
 // set when sys_start is null) and, when it splits, its children from the rank of its
 // depth-0 entry.
@@ -184,12 +237,13 @@ __global__ void roots_kernel(i64 roots_n, const i64* __restrict__ sys_start, i64
                              double* __restrict__ node_center, double* __restrict__ node_half,
                              i64* __restrict__ node_start, i64* __restrict__ node_end,
                              i64* __restrict__ node_child, i64* __restrict__ node_depth,
-                             int4* __restrict__ node_info) {
+                             int4* __restrict__ node_info, const i64* __restrict__ sizes) {
     const i64 sys = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (sys >= roots_n) return;
     const i64 start = sys_start ? sys_start[sys] : 0;
     const i64 end = sys_start ? sys_start[sys + 1] : n;
-    const bool split = end - start > cap && max_depth > 0;
+    const bool over = sizes != nullptr && sizes[2] != 0;
+    const bool split = end - start > cap && max_depth > 0 && !over;
     i64 child = -1;
     if (split) {
         // depth-0 entries are the first bucket, in particle order
@@ -223,8 +277,12 @@ __global__ void nodes_kernel(const u64* __restrict__ khi, const u64* __restrict_
                              const i64* __restrict__ levels, double* __restrict__ node_center,
                              double* __restrict__ node_half, i64* __restrict__ node_start,
                              i64* __restrict__ node_end, i64* __restrict__ node_child,
-                             i64* __restrict__ node_depth, int4* __restrict__ node_info) {
+                             i64* __restrict__ node_depth, int4* __restrict__ node_info,
+                             const u32* __restrict__ s_limit, const i64* __restrict__ sizes) {
     const i64 g = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    // without a read-back: the entry count on the device, bounded by the entry capacity
+    // (s_total), and nothing when the nodes overflow
+    if (s_limit != nullptr && (sizes[2] != 0 || g >= (i64)*s_limit)) return;
     if (g >= s_total) return;
     const int d = (int)edepth[g];
     const i64 i = (i64)epos[g];
@@ -295,8 +353,10 @@ __global__ void nodes_kernel(const u64* __restrict__ khi, const u64* __restrict_
 // For every leaf, its start position, scattered to the input order of its particles.
 __global__ void leaf_key_kernel(const i64* __restrict__ node_start, const i64* __restrict__ node_end,
                                 const i64* __restrict__ node_child, i64 count,
-                                const u32* __restrict__ sorted_idx, u64* __restrict__ key2) {
+                                const u32* __restrict__ sorted_idx, u64* __restrict__ key2,
+                                const i64* __restrict__ sizes) {
     const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sizes != nullptr && node >= sizes[0]) return;
     if (node >= count || node_child[node] >= 0) return;
     const i64 s = node_start[node], e = node_end[node];
     for (i64 k = s; k < e; ++k) key2[sorted_idx[k]] = (u64)s;
@@ -370,11 +430,13 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                double* node_center, double* node_half, int64_t* node_start, int64_t* node_end,
                int64_t* node_child, int64_t* node_depth, int64_t* levels, int64_t* perm,
                double* sorted_pos, void* node_info, void* workspace, size_t workspace_bytes,
-               int64_t* out_sizes, int64_t* levels_host, hipStream_t s) {
+               int64_t* out_sizes, int64_t* levels_host, int64_t* sizes_dev, hipStream_t s) {
+    // sizes_dev set: the build without a host read-back (td3_forest_build_nosync)
+    const bool nosync = sizes_dev != nullptr;
     TD3_ARG(n > 0 && n < (int64_t)0x7fffffff, "n must be in [1, 2^31)");
     TD3_ARG(leaf_capacity >= 1 && max_depth >= 1, "leaf_capacity and max_depth must be positive");
     TD3_ARG(capacity >= roots_n, "capacity");
-    TD3_ARG(out_sizes != nullptr, "out_sizes");
+    TD3_ARG(nosync || out_sizes != nullptr, "out_sizes");
     TD3_ARG(levels != nullptr, "levels (int64[67])");
     TreeWs w;
     const size_t need = carve(&w, nullptr, n, capacity);
@@ -385,6 +447,8 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     kl.l1 = (64 - kl.sbits) / 2;
     const int max_levels = kl.l1 + 32 < kMaxLevels ? kl.l1 + 32 : kMaxLevels;
     TD3_ARG(kl.sbits <= 24, "at most 2^24 systems");
+    // without a read-back the keys hold every level a cell can split at
+    TD3_ARG(!nosync || max_depth <= max_levels, "max_depth beyond the key's levels");
     const int nlev = (int)(max_depth < max_levels ? max_depth : max_levels);
     const int max_split = (int)(max_depth - 1 < max_levels ? max_depth - 1 : max_levels);
     u32* depth_count = w.small;          // [65]
@@ -398,6 +462,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     // stable), and the topology depends only on prefixes no longer than the deepest split.
     int stages[3], nstage = 0;
     for (int k : {16, kl.l1, nlev}) {
+        if (nosync && k != nlev) continue;
         const int key = k < nlev ? k : nlev;
         if (nstage == 0 || key > stages[nstage - 1]) stages[nstage++] = key;
     }
@@ -426,10 +491,17 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                 return rc;
         }
         // 3. splitting-node starts
-        TD3_HIP(hipMemsetAsync(w.small, 0, sizeof(u32) * 96, s));
+        if (nosync) {
+            // a fill kernel, not a memset: this build may be recorded into a HIP graph
+            hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(128), 0, s, w.small, 96);
+            TD3_LAUNCHED();
+        } else {
+            TD3_HIP(hipMemsetAsync(w.small, 0, sizeof(u32) * 96, s));
+        }
         TD3_TIMED(s, "tree_count", hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
                            (i64)leaf_capacity, max_split, nkey, kl, w.cnt, depth_count, flag));
         TD3_LAUNCHED();
+        if (nosync) break;  // the keys hold every level: the flag cannot be set
         TD3_HIP(hipMemcpyAsync(host_small, w.small, sizeof(u32) * 81, hipMemcpyDeviceToHost, s));
         TD3_HIP(hipStreamSynchronize(s));
         if (!host_small[80]) break;
@@ -438,6 +510,45 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                           (long long)max_depth);
             return TD3_ERR_ARG;
         }
+    }
+    if (nosync) {
+        // 4'. the same steps at the capacity's sizes, every count read on the device
+        const i64 ne = entries_cap(capacity);
+        const u64* klo_n = deep ? w.klo : nullptr;
+        if (int rc = td3sort::exclusive_scan_u32(w.cnt, w.off, n, s_total, w.scratch, s)) return rc;
+        hipLaunchKernelGGL(entries_kernel, dim3(nb), dim3(256), 0, s, w.khi, klo_n, n, nkey, kl,
+                           w.cnt, w.off, ne, w.ekey, w.eval);
+        TD3_LAUNCHED();
+        hipLaunchKernelGGL(pad_entries_kernel, dim3(td3_blocks(ne, 256)), dim3(256), 0, s, w.ekey,
+                           w.eval, ne, (const u32*)s_total);
+        TD3_LAUNCHED();
+        if (int rc = td3sort::sort_pairs(w.ekey, w.eval, nullptr, w.ekey_alt, w.eval_alt, nullptr,
+                                         ne, 0, 8, false, w.scratch, s))
+            return rc;
+        hipLaunchKernelGGL(levels_nosync_kernel, dim3(1), dim3(64), 0, s, depth_count, roots_n,
+                           (i64)capacity, levels, bstart, sizes_dev);
+        TD3_LAUNCHED();
+        hipLaunchKernelGGL(roots_kernel, dim3(td3_blocks(roots_n, 256)), dim3(256), 0, s, roots_n,
+                           sys_start, (i64)n, (i64)leaf_capacity, (int)max_depth, rt, w.eval,
+                           depth_count, levels, node_center, node_half, node_start, node_end,
+                           node_child, node_depth, (int4*)node_info, (const i64*)sizes_dev);
+        TD3_LAUNCHED();
+        TD3_TIMED(s, "tree_nodes", hipLaunchKernelGGL(nodes_kernel, dim3(td3_blocks(ne, 256)), dim3(256), 0, s, w.khi, klo_n, n,
+                           (i64)leaf_capacity, (int)max_depth, nkey, kl, rt, w.ekey, w.eval,
+                           ne, bstart, depth_count, levels, node_center, node_half, node_start,
+                           node_end, node_child, node_depth, (int4*)node_info,
+                           (const u32*)s_total, (const i64*)sizes_dev));
+        TD3_LAUNCHED();
+        TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(capacity, 256)), dim3(256), 0, s, node_start,
+                           node_end, node_child, (i64)capacity, w.idx, w.khi_alt,
+                           (const i64*)sizes_dev));
+        TD3_LAUNCHED();
+        if (int rc = td3sort::sort_pairs(w.khi_alt, w.idx_alt, nullptr, w.khi, w.idx, nullptr, n, 0,
+                                         bits_for(n), true, w.scratch, s))
+            return rc;
+        TD3_TIMED(s, "tree_gather", hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, w.idx_alt, pos, n, perm, sorted_pos));
+        TD3_LAUNCHED();
+        return TD3_OK;
     }
     // node count and level count
     i64 count = roots_n, s_sum = 0;
@@ -470,7 +581,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
         // offsets into a separate array: entries_kernel needs the counts and the offsets
         if (int rc = td3sort::exclusive_scan_u32(w.cnt, w.off, n, s_total, w.scratch, s)) return rc;
         hipLaunchKernelGGL(entries_kernel, dim3(nb), dim3(256), 0, s, w.khi, klo, n, nkey, kl, w.cnt,
-                           w.off, w.ekey, w.eval);
+                           w.off, (i64)INT64_MAX, w.ekey, w.eval);
         TD3_LAUNCHED();
         if (int rc = td3sort::sort_pairs(w.ekey, w.eval, nullptr, w.ekey_alt, w.eval_alt, nullptr,
                                          s_sum, 0, 8, false, w.scratch, s))
@@ -481,18 +592,19 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     hipLaunchKernelGGL(roots_kernel, dim3(td3_blocks(roots_n, 256)), dim3(256), 0, s, roots_n,
                        sys_start, (i64)n, (i64)leaf_capacity, (int)max_depth, rt, w.eval,
                        depth_count, levels, node_center, node_half, node_start, node_end,
-                       node_child, node_depth, (int4*)node_info);
+                       node_child, node_depth, (int4*)node_info, (const i64*)nullptr);
     TD3_LAUNCHED();
     if (s_sum > 0) {
         TD3_TIMED(s, "tree_nodes", hipLaunchKernelGGL(nodes_kernel, dim3(td3_blocks(s_sum, 256)), dim3(256), 0, s, w.khi, klo, n,
                            (i64)leaf_capacity, (int)max_depth, nkey, kl, rt, w.ekey, w.eval,
                            s_sum, bstart, depth_count, levels, node_center, node_half, node_start,
-                           node_end, node_child, node_depth, (int4*)node_info));
+                           node_end, node_child, node_depth, (int4*)node_info,
+                           (const u32*)nullptr, (const i64*)nullptr));
         TD3_LAUNCHED();
     }
     // 5. input order inside each leaf: sort the input indices by their leaf's start
     TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_start,
-                       node_end, node_child, count, w.idx, w.khi_alt));
+                       node_end, node_child, count, w.idx, w.khi_alt, (const i64*)nullptr));
     TD3_LAUNCHED();
     if (int rc = td3sort::sort_pairs(w.khi_alt, w.idx_alt, nullptr, w.khi, w.idx, nullptr, n, 0,
                                      bits_for(n), true, w.scratch, s))
@@ -519,7 +631,7 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     return build_impl(pos, n, 1, rt, nullptr, nullptr, leaf_capacity, max_depth, capacity,
                       node_center, node_half, node_start, node_end, node_child, node_depth, levels,
                       perm, sorted_pos, node_info, workspace, workspace_bytes, out_sizes,
-                      levels_host, (hipStream_t)stream);
+                      levels_host, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
@@ -540,5 +652,27 @@ extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
     return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
                       capacity, node_center, node_half, node_start, node_end, node_child,
                       node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
-                      out_sizes, levels_host, (hipStream_t)stream);
+                      out_sizes, levels_host, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int td3_forest_build_nosync(const double* pos, int64_t n, int64_t systems,
+                                       const int32_t* particle_system, const int64_t* system_start,
+                                       const double* roots, int64_t leaf_capacity,
+                                       int64_t max_depth, int64_t capacity, double* node_center,
+                                       double* node_half, int64_t* node_start, int64_t* node_end,
+                                       int64_t* node_child, int64_t* node_depth, int64_t* levels,
+                                       int64_t* perm, double* sorted_pos, void* node_info,
+                                       void* workspace, size_t workspace_bytes, int64_t* sizes,
+                                       void* stream) {
+    TD3_ARG(systems >= 1 && systems <= (1 << 24), "systems");
+    TD3_ARG(particle_system != nullptr && system_start != nullptr && roots != nullptr,
+            "particle_system, system_start and roots");
+    TD3_ARG(sizes != nullptr && levels != nullptr && node_info != nullptr, "sizes, levels, node_info");
+    Root rt;
+    rt.roots = roots;
+    rt.cx = rt.cy = rt.h = 0.0;
+    return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
+                      capacity, node_center, node_half, node_start, node_end, node_child,
+                      node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
+                      nullptr, nullptr, sizes, (hipStream_t)stream);
 }

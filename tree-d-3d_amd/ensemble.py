@@ -12,6 +12,14 @@ of them at once:
   starting from the roots of its targets' systems (`td3_walk` with ``target_root``);
 * one push launch (`td3_push`) for every particle.
 
+`Ensemble.run(steps, graph=True)` takes the same steps through a HIP graph: the step's
+launches are recorded once (torch's `CUDAGraph` on the library's stream) and replayed, so
+that a step costs one graph launch and no host work.  The recorded step reads nothing back
+to the host: the forest is built at a fixed node capacity with its counts and level
+starts on the device (`td3_forest_build_nosync`), the moments read their level ranges
+there (`td3_coulomb_moments_nosync`), and every array is allocated before the recording.
+It computes the same trees, fields and states as `step`, bit for bit.
+
 Each system keeps the reference's semantics: its root grows to hold its particles
 (simulation.py:625-639), its fields are cached between the end of one step and the
 start of the next (one evaluation per step), fixed particles act as sources only, and
@@ -123,6 +131,84 @@ class _Forest(FlatQuadtree):
         return phi, field
 
 
+class _Resident:
+    """Every array one step of an ensemble touches, allocated once at a fixed node
+    capacity, and the step itself as launches that allocate nothing and read nothing back
+    (so that it can be recorded into a HIP graph)."""
+
+    def __init__(self, ens: "Ensemble", capacity: int):
+        dev, n = ens.positions.device, ens.num_particles
+        f64 = dict(dtype=torch.float64, device=dev)
+        i64 = dict(dtype=torch.int64, device=dev)
+        self.capacity = int(capacity)
+        c = self.capacity
+        self.center = torch.empty((c, 2), **f64)
+        self.half = torch.empty(c, **f64)
+        self.start = torch.empty(c, **i64)
+        self.end = torch.empty(c, **i64)
+        self.child = torch.empty(c, **i64)
+        self.depth = torch.empty(c, **i64)
+        self.info = torch.empty((c, 4), dtype=torch.int32, device=dev)
+        self.levels = torch.zeros(67, **i64)
+        self.sizes = torch.zeros(4, **i64)  # count, levels, need on overflow, largest need
+        self.perm = torch.empty(n, **i64)
+        self.sorted_pos = torch.empty((2, n), **f64)
+        self.scharge = torch.empty(n, **f64)
+        self.node_q = torch.empty(c, **f64)
+        self.node_qabs = torch.empty(c, **f64)
+        self.node_p = torch.empty((c, 2), **f64)
+        self.node_s = torch.empty((c, 3), **f64)
+        self.moment = torch.zeros((1, COULOMB_ORDER + 1), **f64)
+        self.ws_bytes = _abi.lib().td3_tree_workspace_bytes(n, c)
+        self.ws = _abi.workspace(self.ws_bytes)
+        self.phi = torch.empty(n, **f64)
+        self.field = torch.empty((2, n), **f64)
+        self.status = torch.zeros(1, **i64)
+        self.grown = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.everyone = torch.arange(n, device=dev)
+        # fields are written in place into these from now on
+        self.kq = ens.k_e * ens.charges[None, :]
+        self.electric = ens._electric.clone()
+        self.potential = ens._potential.clone()
+
+    def step(self, ens: "Ensemble") -> None:
+        """simulation.py:800-835 for every system, as `Ensemble.step` with current fields."""
+        n, st = ens.num_particles, _abi.stream()
+        ens._push(first_half=True, electric=self.electric)
+        _abi.call("td3_system_bounds", _abi.ptr(ens.positions), n, ens.systems,
+                  _abi.ptr(ens.system_start), _abi.ptr(ens.roots), _abi.ptr(self.grown), st)
+        _abi.call(
+            "td3_forest_build_nosync", _abi.ptr(ens.positions), n, ens.systems,
+            _abi.ptr(ens.system), _abi.ptr(ens.system_start), _abi.ptr(ens.roots),
+            ens.leaf_capacity, ens.max_depth, self.capacity, _abi.ptr(self.center),
+            _abi.ptr(self.half), _abi.ptr(self.start), _abi.ptr(self.end), _abi.ptr(self.child),
+            _abi.ptr(self.depth), _abi.ptr(self.levels), _abi.ptr(self.perm),
+            _abi.ptr(self.sorted_pos), _abi.ptr(self.info), _abi.ptr(self.ws), self.ws_bytes,
+            _abi.ptr(self.sizes), st)
+        _abi.call("td3_permute", _abi.ptr(ens.charges), 1, n, _abi.ptr(self.perm), n,
+                  _abi.ptr(self.scharge), st)
+        _abi.call("td3_coulomb_moments_nosync", _abi.ptr(self.sorted_pos),
+                  _abi.ptr(self.scharge), n, _abi.ptr(self.center), _abi.ptr(self.info),
+                  self.capacity, _abi.ptr(self.levels), ens.max_depth, _abi.ptr(self.node_q),
+                  _abi.ptr(self.node_qabs), _abi.ptr(self.node_p), _abi.ptr(self.node_s), st)
+        self.phi.zero_()
+        self.field.zero_()
+        _abi.call(
+            "td3_walk", _abi.COULOMB, _abi.FP64, _abi.ptr(ens.positions), n,
+            _abi.ptr(self.perm), _abi.ptr(self.everyone), _abi.ptr(self.sorted_pos),
+            _abi.ptr(self.scharge), _abi.ptr(self.perm), n, _abi.ptr(self.center),
+            _abi.ptr(self.half), _abi.ptr(self.info), _abi.ptr(self.node_qabs),
+            _abi.ptr(self.node_q), _abi.ptr(self.node_p), _abi.ptr(self.node_s),
+            _abi.ptr(self.moment), _abi.ptr(self.moment), COULOMB_ORDER, ens.max_depth,
+            _abi.ptr(ens.system), float(ens.theta), float(ens.softening), 1,
+            _abi.ptr(self.phi), _abi.ptr(self.field), _abi.ptr(self.status), st)
+        # the fields of compute_fields, with the same roundings, in place
+        torch.mul(self.phi, ens.k_e, out=self.potential)
+        torch.mul(self.kq, self.field, out=self.electric)
+        self.electric.masked_fill_(ens.fixed[None, :], 0.0)
+        ens._push(first_half=False, electric=self.electric)
+
+
 class Ensemble:
     """S independent Coulomb systems (``ElectrostaticSimulation`` semantics with
     ``method="tree"``) advanced together; see the module docstring."""
@@ -212,6 +298,10 @@ class Ensemble:
         self._potential = torch.zeros_like(self.charges)
         self._current = False
         self.forest = None
+        self._resident = None  # fixed-capacity arrays of the resident / recorded step
+        self._graph = None
+        self._graph_key = None
+        self.graph_capacity = None
 
     @property
     def num_particles(self) -> int:
@@ -270,10 +360,11 @@ class Ensemble:
         return best
 
     # ------------------------------------------------------------------ dynamics
-    def _push(self, first_half: bool) -> None:
+    def _push(self, first_half: bool, electric: torch.Tensor | None = None) -> None:
+        electric = self._electric if electric is None else electric
         _abi.call("td3_push", 1 if first_half else 0, self.num_particles,
                   _abi.ptr(self.positions), _abi.ptr(self.velocities), _abi.ptr(self.forces),
-                  _abi.ptr(self._electric), _abi.ptr(self.masses), _abi.ptr(self.charges),
+                  _abi.ptr(electric), _abi.ptr(self.masses), _abi.ptr(self.charges),
                   _abi.ptr(self._fixed_u8), float(self.dt), float(self.B_z), _abi.stream())
 
     def step(self, dt=None):
@@ -288,6 +379,89 @@ class Ensemble:
         self.current_time += self.dt
         self.iteration += 1
         return forest
+
+    def _ready_resident(self) -> _Resident:
+        """The fixed-capacity arrays of the recorded step, holding the current fields; made
+        with room for twice the nodes of the current forest (or the build's own estimate,
+        or the capacity an overflow asked for)."""
+        self.dt = _finite("dt", self.dt, positive=True)
+        forest = self.compute_fields()
+        r = self._resident
+        if r is None:
+            n = self.num_particles
+            capacity = max(4 * (2 * n // self.leaf_capacity + 16 * self.systems),
+                           2 * int(forest.node_count) if forest is not None else 0,
+                           self.graph_capacity or 0)
+            self.graph_capacity = capacity
+            r = self._resident = _Resident(self, capacity)
+            self._graph = None
+        elif self._electric is not r.electric:
+            # fields computed by step() since: the recorded step reads these arrays
+            r.electric.copy_(self._electric)
+            r.potential.copy_(self._potential)
+        self._electric, self._potential = r.electric, r.potential
+        return r
+
+    def _after_resident(self, steps: int) -> None:
+        """Bookkeeping after resident steps, then the read-back of their reports: a step
+        whose nodes overflowed the capacity had its systems summed directly (each root a
+        leaf: exact fields), and the arrays are made again with room for them; a
+        coincident pair or a too-deep tree raises."""
+        r = self._resident
+        self._current, self.forest = True, None
+        for _ in range(steps):  # the clock of `step`, one addition per step
+            self.current_time += self.dt
+        self.iteration += steps
+        need = int(r.sizes[3])
+        status = r.status.clone()
+        r.status.zero_()
+        if need:
+            self.graph_capacity = 2 * need
+            self._resident, self._graph = None, None
+        if not bool(torch.isfinite(self.positions).all()):
+            raise ValueError("Particle positions must be finite; reduce dt")
+        _check(status)
+
+    def step_resident(self) -> None:
+        """One step through the launches that `run(graph=True)` records, run directly:
+        the same state as `step`, from fixed-capacity arrays with no read-back inside."""
+        self._ready_resident().step(self)
+        self._after_resident(1)
+
+    def run(self, steps: int, *, graph: bool = True) -> None:
+        """``steps`` steps.  ``graph``: the step is recorded once into a HIP graph (after
+        one step taken directly, which loads every kernel) and replayed for the rest; the
+        record is kept for later calls while ``dt``, ``B_z``, ``theta`` and ``softening``
+        stay.  Reports are read back once at the end (`step_resident`).  ``graph=False``
+        takes ``step()`` ``steps`` times."""
+        steps = int(steps)
+        if steps < 0:
+            raise ValueError("steps must be non-negative")
+        if not graph:
+            for _ in range(steps):
+                self.step()
+            return
+        if steps == 0:
+            return
+        key = (self.dt, self.B_z, self.theta, self.softening)
+        if self._graph is not None and self._graph_key != key:
+            self._graph = None
+        done = 0
+        if self._graph is None:
+            self.step_resident()
+            done = 1
+            if steps == 1:
+                return
+            r = self._ready_resident()
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                r.step(self)
+            self._graph, self._graph_key = g, key
+        self._ready_resident()
+        for _ in range(steps - done):
+            self._graph.replay()
+        self._after_resident(steps - done)
 
     # ------------------------------------------------------------------ diagnostics
     def _per_system(self, values: torch.Tensor) -> np.ndarray:
