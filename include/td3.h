@@ -111,22 +111,25 @@ int td3_tree_build(const double *pos, int64_t n, double xmid, double ymid, doubl
  * td3_forest_build without a host read-back or a memset, so that an ensemble step can be
  * recorded into a HIP graph and replayed (simulation.py:800-835 steps one system with a
  * host-synchronous build; no counterpart in the reference).  Node arrays, node_info and
- * levels (int64[67], device) are at `capacity`; the keys hold every level up to max_depth
- * (<= the key's level count), so no stage is chosen on the host.  sizes (int64[4], device):
- * node count, level count, the needed capacity when the nodes overflow `capacity` (else
- * 0), and the largest such need of any call so far (never cleared here: the caller zeroes
- * it and reads it after its replays).  levels[d] past the deepest level equal the node
- * count (empty levels).  On an overflow the forest is the roots alone, each a leaf, so the
- * walk then sums every system directly.  Same tree as td3_forest_build otherwise; same
- * workspace size.
+ * levels (int64[67], device) are at `capacity`.  The keys hold key_levels levels (at most
+ * the key's level count): a caller that saw the tree's depth passes a few more levels than
+ * it had, so no stage is chosen on the host.  sizes (int64[6], device): node count, level
+ * count, the needed capacity when the nodes overflow `capacity` (-1 when the key is too
+ * short, else 0), the largest such need of any call so far, 1 when a cell would split
+ * below the key's levels (else 0), and 1 when that happened in any call so far (the "so
+ * far" entries are never cleared here: the caller zeroes them and reads them after its
+ * replays).  levels[d] past the deepest level equal the node count (empty levels).  On an
+ * overflow or a short key the forest is the roots alone, each a leaf, so the walk then
+ * sums every system directly.  Same tree as td3_forest_build otherwise (no cell is deeper
+ * than key_levels); same workspace size.
  */
 int td3_forest_build_nosync(const double *pos, int64_t n, int64_t systems,
                             const int32_t *particle_system, const int64_t *system_start,
                             const double *roots, int64_t leaf_capacity, int64_t max_depth,
-                            int64_t capacity, double *node_center, double *node_half,
-                            int64_t *node_start, int64_t *node_end, int64_t *node_child,
-                            int64_t *node_depth, int64_t *levels, int64_t *perm,
-                            double *sorted_pos, void *node_info, void *workspace,
+                            int64_t capacity, int64_t key_levels, double *node_center,
+                            double *node_half, int64_t *node_start, int64_t *node_end,
+                            int64_t *node_child, int64_t *node_depth, int64_t *levels,
+                            int64_t *perm, double *sorted_pos, void *node_info, void *workspace,
                             size_t workspace_bytes, int64_t *sizes, void *stream);
 
 /* dst[r, k] = src[r * ld_src + perm[k]] for r < rows (<= 8), k < n: tree-order copies
@@ -153,8 +156,8 @@ int td3_moments(int kernel, int64_t order, const double *sorted_pos, const doubl
 /*
  * Coulomb moments of td3_moments (same values, bit for bit) with the level starts read on
  * the device (levels int64[67] from td3_forest_build_nosync): a fixed grid per level for
- * the depths max_depth - 1 .. 0, no host read-back, so it can be recorded into a HIP
- * graph.  Arrays at `capacity` nodes.  Replaces kernels/quadtree_moments
+ * the depths max_depth - 1 .. 0 (max_depth: a bound on the tree's depth, such as the
+ * build's key_levels), no host read-back, so it can be recorded into a HIP graph.  Arrays at `capacity` nodes.  Replaces kernels/quadtree_moments
  * (flat_tree.py:663-744) for a captured ensemble step.
  */
 int td3_coulomb_moments_nosync(const double *sorted_pos, const double *sorted_charge, int64_t n,

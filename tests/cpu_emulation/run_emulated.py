@@ -303,25 +303,34 @@ def case_forest_resident():
                            (r.node_q, "node_q"), (r.node_s, "node_s")):
             want = f._d[name].numpy()
             assert np.array_equal(mine.numpy()[: len(want)], want), name
-    # too few nodes: the roots alone, as leaves: direct sums per system
-    small = ensemble.Ensemble.from_arrays(*args, **kw)
-    small.compute_fields()
-    small._resident = ensemble._Resident(small, len(sizes) + 3)
-    small.graph_capacity = len(sizes) + 3
-    before = small.positions.clone()
-    small.step_resident()
-    assert small._resident is None and small.graph_capacity > 2 * (len(sizes) + 3)
-    # the step's fields came from the new positions by exact sums within each system
-    after = small.positions
+    assert resident._resident.key_levels == 16 and not resident._full_key
+    # too few nodes, or keys too short for the depth: the roots alone, as leaves (direct
+    # sums per system), reported; the next resident arrays have room / every level
     starts = np.concatenate(([0], np.cumsum(sizes)))
-    phi = small._potential.numpy()
-    for s_i in range(len(sizes)):
-        sl = slice(starts[s_i], starts[s_i + 1])
-        k = starts[s_i + 1] - starts[s_i]
-        want, _ = td3.direct_field(after[:, sl], after[:, sl], np.ones(k), softening=1e-3,
-                                   exclude=np.arange(k))
-        assert relative(phi[sl], np.asarray(want)) < 1e-13, s_i
-    assert not torch.equal(before, after)
+    for capacity, key in ((len(sizes) + 3, 48), (4 * n, 2)):
+        small = ensemble.Ensemble.from_arrays(*args, **kw)
+        small.compute_fields()
+        small._resident = ensemble._Resident(small, capacity, key)
+        small.graph_capacity = capacity
+        before = small.positions.clone()
+        small.step_resident()
+        assert small._resident is None
+        if key == 2:
+            assert small._full_key and small.graph_capacity == capacity
+        else:
+            assert not small._full_key and small.graph_capacity > 2 * capacity
+        after = small.positions
+        phi = small._potential.numpy()
+        for s_i in range(len(sizes)):
+            sl = slice(starts[s_i], starts[s_i + 1])
+            k = starts[s_i + 1] - starts[s_i]
+            want, _ = td3.direct_field(after[:, sl], after[:, sl], np.ones(k), softening=1e-3,
+                                       exclude=np.arange(k))
+            assert relative(phi[sl], np.asarray(want)) < 1e-13, s_i
+        assert not torch.equal(before, after)
+        small.step_resident()
+        assert small._resident.key_levels == (48 if key == 2 else 16)
+        assert int(small._resident.sizes[2]) == 0
 
 
 def sorted_rows(row, col):

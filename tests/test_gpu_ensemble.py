@@ -156,7 +156,7 @@ def test_resident_step_overflow_sums_each_system_directly(td3):
     sizes = [700, 40, 1300]
     ens = td3.Ensemble(gaussian_systems(td3, sizes, seed=3))
     ens.compute_fields()
-    ens._resident = ens_mod._Resident(ens, len(sizes) + 1)
+    ens._resident = ens_mod._Resident(ens, len(sizes) + 1, 48)
     ens.graph_capacity = len(sizes) + 1
     print("resident step with 4 nodes of capacity", flush=True)
     ens.step_resident()
@@ -171,3 +171,29 @@ def test_resident_step_overflow_sums_each_system_directly(td3):
         assert float((got - want).abs().max() / want.abs().max()) < 1e-13
     ens.run(3)
     assert ens._resident is not None and int(ens._resident.sizes[3]) == 0
+
+
+def test_resident_step_short_key_is_reported_and_recorded_again(td3):
+    """Keys of too few levels for the forest's depth: exact sums per system for that step,
+    then the step is recorded again with keys of every level, and replays equal step()."""
+    ens_mod = sys.modules["tree_d_3d_amd.ensemble"]
+    sizes = [900, 2000, 50]
+    eager = td3.Ensemble(gaussian_systems(td3, sizes, seed=8))
+    ens = td3.Ensemble(gaussian_systems(td3, sizes, seed=8))
+    ens.compute_fields()
+    ens._resident = ens_mod._Resident(ens, 1 << 16, 3)
+    print("resident step with 3-level keys", flush=True)
+    ens.step_resident()
+    eager.step()
+    assert ens._full_key and ens._resident is None
+    a, b = ens._potential, eager._potential
+    assert float((a - b).abs().max() / b.abs().max()) < 1e-2  # exact sums vs the walk
+    ens.positions.copy_(eager.positions)
+    ens.velocities.copy_(eager.velocities)
+    ens.roots.copy_(eager.roots)
+    ens._current = False
+    ens.run(4)
+    for _ in range(4):
+        eager.step()
+    assert ens._resident.key_levels == 48
+    _same_state(eager, ens)

@@ -42,8 +42,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                         _P, _Z, _P, _P, _P], _C),
     "td3_forest_build": ([_P, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                           _P, _P, _Z, _P, _P, _P], _C),
-    "td3_forest_build_nosync": ([_P, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P,
-                                 _P, _P, _P, _P, _Z, _P, _P], _C),
+    "td3_forest_build_nosync": ([_P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P,
+                                 _P, _P, _P, _P, _P, _Z, _P, _P], _C),
     "td3_permute": ([_P, _I, _I, _P, _I, _P, _P], _C),
     "td3_moments": ([_C, _I, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P],
                     _C),

@@ -178,13 +178,15 @@ __global__ void levels_kernel(const u32* __restrict__ depth_count, i64 roots_n,
 
 // The build without a host read-back (td3_forest_build_nosync): node count, level count
 // and level starts from the per-depth counts on the device.  sizes = {node count, levels,
-// needed capacity when it overflows (else 0), largest need of any overflow so far (kept
-// until the caller clears it)}.  Every level start past the deepest level is the node
-// count, so those levels are empty.  When the nodes do not fit, the forest is the roots
-// alone, each a leaf.
+// needed capacity when it overflows (else 0), largest need of any overflow so far, 1 when
+// a cell would split below the key's levels (else 0), the same for any call so far}; the
+// "so far" entries are kept until the caller clears them.  Every level start past the
+// deepest level is the node count, so those levels are empty.  When the nodes do not fit,
+// or the key is too short, the forest is the roots alone, each a leaf.
 __global__ void levels_nosync_kernel(const u32* __restrict__ depth_count, i64 roots_n,
-                                     i64 capacity, i64* __restrict__ levels,
-                                     u32* __restrict__ bstart, i64* __restrict__ sizes) {
+                                     i64 capacity, const u32* __restrict__ short_key,
+                                     i64* __restrict__ levels, u32* __restrict__ bstart,
+                                     i64* __restrict__ sizes) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     i64 count = roots_n;
     int n_levels = 1;
@@ -195,7 +197,8 @@ __global__ void levels_nosync_kernel(const u32* __restrict__ depth_count, i64 ro
             n_levels = d + 2;
         }
     }
-    const bool over = count > capacity;
+    const bool too_short = *short_key != 0;
+    const bool over = count > capacity || too_short;
     levels[0] = 0;
     levels[1] = roots_n;
     i64 lv = roots_n;
@@ -210,8 +213,11 @@ __global__ void levels_nosync_kernel(const u32* __restrict__ depth_count, i64 ro
     bstart[kMaxLevels + 1] = b;
     sizes[0] = over ? roots_n : count;
     sizes[1] = over ? 1 : n_levels;
-    sizes[2] = over ? count : 0;
-    if (over && count > sizes[3]) sizes[3] = count;
+    const i64 need = !too_short && count > capacity ? count : 0;
+    sizes[2] = over ? (need ? need : -1) : 0;
+    if (need > sizes[3]) sizes[3] = need;
+    sizes[4] = too_short ? 1 : 0;
+    if (too_short) sizes[5] = 1;
 }
 
 __global__ void zero_u32_kernel(u32* __restrict__ p, int n) {
@@ -430,8 +436,10 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                double* node_center, double* node_half, int64_t* node_start, int64_t* node_end,
                int64_t* node_child, int64_t* node_depth, int64_t* levels, int64_t* perm,
                double* sorted_pos, void* node_info, void* workspace, size_t workspace_bytes,
-               int64_t* out_sizes, int64_t* levels_host, int64_t* sizes_dev, hipStream_t s) {
-    // sizes_dev set: the build without a host read-back (td3_forest_build_nosync)
+               int64_t* out_sizes, int64_t* levels_host, int64_t* sizes_dev,
+               int64_t key_levels, hipStream_t s) {
+    // sizes_dev set: the build without a host read-back (td3_forest_build_nosync), its keys
+    // holding key_levels levels
     const bool nosync = sizes_dev != nullptr;
     TD3_ARG(n > 0 && n < (int64_t)0x7fffffff, "n must be in [1, 2^31)");
     TD3_ARG(leaf_capacity >= 1 && max_depth >= 1, "leaf_capacity and max_depth must be positive");
@@ -447,8 +455,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     kl.l1 = (64 - kl.sbits) / 2;
     const int max_levels = kl.l1 + 32 < kMaxLevels ? kl.l1 + 32 : kMaxLevels;
     TD3_ARG(kl.sbits <= 24, "at most 2^24 systems");
-    // without a read-back the keys hold every level a cell can split at
-    TD3_ARG(!nosync || max_depth <= max_levels, "max_depth beyond the key's levels");
+    TD3_ARG(!nosync || key_levels >= 1, "key_levels must be positive");
     const int nlev = (int)(max_depth < max_levels ? max_depth : max_levels);
     const int max_split = (int)(max_depth - 1 < max_levels ? max_depth - 1 : max_levels);
     u32* depth_count = w.small;          // [65]
@@ -462,7 +469,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     // stable), and the topology depends only on prefixes no longer than the deepest split.
     int stages[3], nstage = 0;
     for (int k : {16, kl.l1, nlev}) {
-        if (nosync && k != nlev) continue;
+        if (nosync) k = key_levels;
         const int key = k < nlev ? k : nlev;
         if (nstage == 0 || key > stages[nstage - 1]) stages[nstage++] = key;
     }
@@ -501,7 +508,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
         TD3_TIMED(s, "tree_count", hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
                            (i64)leaf_capacity, max_split, nkey, kl, w.cnt, depth_count, flag));
         TD3_LAUNCHED();
-        if (nosync) break;  // the keys hold every level: the flag cannot be set
+        if (nosync) break;  // a set flag is read on the device (levels_nosync_kernel)
         TD3_HIP(hipMemcpyAsync(host_small, w.small, sizeof(u32) * 81, hipMemcpyDeviceToHost, s));
         TD3_HIP(hipStreamSynchronize(s));
         if (!host_small[80]) break;
@@ -526,7 +533,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                                          ne, 0, 8, false, w.scratch, s))
             return rc;
         hipLaunchKernelGGL(levels_nosync_kernel, dim3(1), dim3(64), 0, s, depth_count, roots_n,
-                           (i64)capacity, levels, bstart, sizes_dev);
+                           (i64)capacity, (const u32*)flag, levels, bstart, sizes_dev);
         TD3_LAUNCHED();
         hipLaunchKernelGGL(roots_kernel, dim3(td3_blocks(roots_n, 256)), dim3(256), 0, s, roots_n,
                            sys_start, (i64)n, (i64)leaf_capacity, (int)max_depth, rt, w.eval,
@@ -631,7 +638,7 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     return build_impl(pos, n, 1, rt, nullptr, nullptr, leaf_capacity, max_depth, capacity,
                       node_center, node_half, node_start, node_end, node_child, node_depth, levels,
                       perm, sorted_pos, node_info, workspace, workspace_bytes, out_sizes,
-                      levels_host, nullptr, (hipStream_t)stream);
+                      levels_host, nullptr, 0, (hipStream_t)stream);
 }
 
 extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
@@ -652,13 +659,14 @@ extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
     return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
                       capacity, node_center, node_half, node_start, node_end, node_child,
                       node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
-                      out_sizes, levels_host, nullptr, (hipStream_t)stream);
+                      out_sizes, levels_host, nullptr, 0, (hipStream_t)stream);
 }
 
 extern "C" int td3_forest_build_nosync(const double* pos, int64_t n, int64_t systems,
                                        const int32_t* particle_system, const int64_t* system_start,
                                        const double* roots, int64_t leaf_capacity,
-                                       int64_t max_depth, int64_t capacity, double* node_center,
+                                       int64_t max_depth, int64_t capacity,
+                                       int64_t key_levels, double* node_center,
                                        double* node_half, int64_t* node_start, int64_t* node_end,
                                        int64_t* node_child, int64_t* node_depth, int64_t* levels,
                                        int64_t* perm, double* sorted_pos, void* node_info,
@@ -674,5 +682,5 @@ extern "C" int td3_forest_build_nosync(const double* pos, int64_t n, int64_t sys
     return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
                       capacity, node_center, node_half, node_start, node_end, node_child,
                       node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
-                      nullptr, nullptr, sizes, (hipStream_t)stream);
+                      nullptr, nullptr, sizes, key_levels, (hipStream_t)stream);
 }

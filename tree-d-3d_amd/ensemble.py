@@ -136,8 +136,9 @@ class _Resident:
     capacity, and the step itself as launches that allocate nothing and read nothing back
     (so that it can be recorded into a HIP graph)."""
 
-    def __init__(self, ens: "Ensemble", capacity: int):
+    def __init__(self, ens: "Ensemble", capacity: int, key_levels: int):
         dev, n = ens.positions.device, ens.num_particles
+        self.key_levels = int(key_levels)
         f64 = dict(dtype=torch.float64, device=dev)
         i64 = dict(dtype=torch.int64, device=dev)
         self.capacity = int(capacity)
@@ -150,7 +151,8 @@ class _Resident:
         self.depth = torch.empty(c, **i64)
         self.info = torch.empty((c, 4), dtype=torch.int32, device=dev)
         self.levels = torch.zeros(67, **i64)
-        self.sizes = torch.zeros(4, **i64)  # count, levels, need on overflow, largest need
+        # count, levels, need on overflow, largest need, short key, short key so far
+        self.sizes = torch.zeros(6, **i64)
         self.perm = torch.empty(n, **i64)
         self.sorted_pos = torch.empty((2, n), **f64)
         self.scharge = torch.empty(n, **f64)
@@ -180,7 +182,8 @@ class _Resident:
         _abi.call(
             "td3_forest_build_nosync", _abi.ptr(ens.positions), n, ens.systems,
             _abi.ptr(ens.system), _abi.ptr(ens.system_start), _abi.ptr(ens.roots),
-            ens.leaf_capacity, ens.max_depth, self.capacity, _abi.ptr(self.center),
+            ens.leaf_capacity, ens.max_depth, self.capacity, self.key_levels,
+            _abi.ptr(self.center),
             _abi.ptr(self.half), _abi.ptr(self.start), _abi.ptr(self.end), _abi.ptr(self.child),
             _abi.ptr(self.depth), _abi.ptr(self.levels), _abi.ptr(self.perm),
             _abi.ptr(self.sorted_pos), _abi.ptr(self.info), _abi.ptr(self.ws), self.ws_bytes,
@@ -189,7 +192,8 @@ class _Resident:
                   _abi.ptr(self.scharge), st)
         _abi.call("td3_coulomb_moments_nosync", _abi.ptr(self.sorted_pos),
                   _abi.ptr(self.scharge), n, _abi.ptr(self.center), _abi.ptr(self.info),
-                  self.capacity, _abi.ptr(self.levels), ens.max_depth, _abi.ptr(self.node_q),
+                  self.capacity, _abi.ptr(self.levels), min(self.key_levels, ens.max_depth),
+                  _abi.ptr(self.node_q),
                   _abi.ptr(self.node_qabs), _abi.ptr(self.node_p), _abi.ptr(self.node_s), st)
         self.phi.zero_()
         self.field.zero_()
@@ -299,6 +303,8 @@ class Ensemble:
         self._current = False
         self.forest = None
         self._resident = None  # fixed-capacity arrays of the resident / recorded step
+        self._full_key = False  # a step found the forest deeper than its key
+        self._levels_seen = 0  # the deepest forest built so far (level count)
         self._graph = None
         self._graph_key = None
         self.graph_capacity = None
@@ -327,6 +333,7 @@ class Ensemble:
         self._expand_roots()
         self.forest = _Forest(self.positions, self.charges, self.system, self.system_start,
                               self.roots, self.leaf_capacity, self.max_depth)
+        self._levels_seen = max(self._levels_seen, len(self.forest.levels) - 1)
         phi, field = self.forest.walk(self.theta, self.softening)
         mobile = ~self.fixed
         self._potential = self.k_e * phi
@@ -383,7 +390,8 @@ class Ensemble:
     def _ready_resident(self) -> _Resident:
         """The fixed-capacity arrays of the recorded step, holding the current fields; made
         with room for twice the nodes of the current forest (or the build's own estimate,
-        or the capacity an overflow asked for)."""
+        or the capacity an overflow asked for) and keys of four levels more than its depth
+        (at least 16; every level after a too-short key)."""
         self.dt = _finite("dt", self.dt, positive=True)
         forest = self.compute_fields()
         r = self._resident
@@ -393,7 +401,10 @@ class Ensemble:
                            2 * int(forest.node_count) if forest is not None else 0,
                            self.graph_capacity or 0)
             self.graph_capacity = capacity
-            r = self._resident = _Resident(self, capacity)
+            key = self.max_depth
+            if self._levels_seen and not self._full_key:
+                key = min(self.max_depth, max(16, self._levels_seen + 4))
+            r = self._resident = _Resident(self, capacity, key)
             self._graph = None
         elif self._electric is not r.electric:
             # fields computed by step() since: the recorded step reads these arrays
@@ -412,11 +423,18 @@ class Ensemble:
         for _ in range(steps):  # the clock of `step`, one addition per step
             self.current_time += self.dt
         self.iteration += steps
-        need = int(r.sizes[3])
+        sizes = r.sizes.cpu()
+        need, short = int(sizes[3]), int(sizes[5])
+        if int(sizes[2]) == 0:
+            self._levels_seen = max(self._levels_seen, int(sizes[1]))
         status = r.status.clone()
         r.status.zero_()
-        if need:
-            self.graph_capacity = 2 * need
+        if short and r.key_levels >= self.max_depth:
+            raise RuntimeError("the tree needs more levels than its keys hold; lower max_depth")
+        if need or short:
+            if need:
+                self.graph_capacity = 2 * need
+            self._full_key = self._full_key or bool(short)
             self._resident, self._graph = None, None
         if not bool(torch.isfinite(self.positions).all()):
             raise ValueError("Particle positions must be finite; reduce dt")
