@@ -197,3 +197,34 @@ def test_resident_step_short_key_is_reported_and_recorded_again(td3):
         eager.step()
     assert ens._resident.key_levels == 48
     _same_state(eager, ens)
+
+
+def test_simulation_run_graph_equals_step(td3):
+    """ElectrostaticSimulation.run(graph=True) (the step recorded as a HIP graph through a
+    one-system ensemble) gives step()'s states bit for bit, across calls, and keeps the
+    bodies' views; other methods refuse it."""
+    a = gaussian_systems(td3, [3000], seed=21)[0]
+    b = gaussian_systems(td3, [3000], seed=21)[0]
+    print("simulation run(graph=True)", flush=True)
+    for _ in range(5):
+        a.step()
+    b.run(5, graph=True)
+    assert torch.equal(a.positions, b.positions) and torch.equal(a.velocities, b.velocities)
+    for _ in range(3):
+        a.step()
+    b.run(3)
+    assert torch.equal(a.positions, b.positions) and torch.equal(a.velocities, b.velocities)
+    # a step() between runs, then a run again: the record reads the new state
+    a.step()
+    b.step()
+    for _ in range(2):
+        a.step()
+    b.run(2)
+    assert torch.equal(a.positions, b.positions) and torch.equal(a.velocities, b.velocities)
+    assert b.iteration == a.iteration == 11 and b.current_time == a.current_time
+    assert np.array_equal(np.asarray(a.bodies[7].pos), np.asarray(b.bodies[7].pos))
+    ea, eb = a.get_total_energy(), b.get_total_energy()
+    assert abs(ea - eb) <= 1e-12 * abs(ea)
+    c = gaussian_systems(td3, [500], seed=1, method="direct")[0]
+    with pytest.raises(ValueError):
+        c.run(2, graph=True)

@@ -294,6 +294,62 @@ class ElectrostaticSimulation:
         self.iteration += 1
         return tree
 
+    def run(self, steps: int, *, graph: bool = True):
+        """``steps`` steps of `step`.  ``graph``: the step is recorded once as a HIP graph
+        and replayed (`Ensemble.run` on a one-system ensemble of this simulation, kept for
+        later calls), which gives the same states bit for bit; it needs the Coulomb
+        Barnes-Hut walk in FP64 (``method`` resolving to ``"tree"``).  No counterpart in the
+        reference, which steps on the host (simulation.py:800-835)."""
+        steps = int(steps)
+        if steps < 0:
+            raise ValueError("steps must be non-negative")
+        if not graph:
+            for _ in range(steps):
+                self.step()
+            return self._tree
+        if self.kernel != "coulomb" or self.precision != "fp64" or \
+                self._resolve_method() != "tree":
+            raise ValueError("graph=True records the Coulomb Barnes-Hut walk step "
+                             "(kernel 'coulomb', method 'tree', precision 'fp64')")
+        from .ensemble import Ensemble
+
+        self.dt = _finite("dt", self.dt, positive=True)
+        ens = getattr(self, "_graph_ensemble", None)
+        if ens is None or ens.num_particles != self.num_particles:
+            ens = self._graph_ensemble = Ensemble([self])
+        else:
+            same = (torch.equal(ens.positions, self.positions)
+                    and torch.equal(ens.velocities, self.velocities)
+                    and torch.equal(ens.charges, self.charges)
+                    and torch.equal(ens.fixed, self.fixed) and torch.equal(ens.masses, self.masses)
+                    and ens.k_e == self.k_e)
+            if not same:
+                ens._graph = ens._resident = None
+                ens.positions.copy_(self.positions)
+                ens.velocities.copy_(self.velocities)
+                ens.charges.copy_(self.charges)
+                ens.masses.copy_(self.masses)
+                ens.fixed.copy_(self.fixed)
+                ens._fixed_u8.copy_(self.fixed)
+                ens.k_e = self.k_e
+                ens._current = False
+            b = self.bounds
+            ens.roots.copy_(torch.tensor([[b.xmid, b.ymid, b.half]], dtype=torch.float64))
+            ens.dt, ens.B_z, ens.theta, ens.softening = self.dt, self.B_z, self.theta, \
+                self.softening
+        ens.run(steps, graph=True)
+        self.positions.copy_(ens.positions)
+        self.velocities.copy_(ens.velocities)
+        self.forces.copy_(ens.forces)
+        root = ens.roots[0].tolist()
+        self.bounds = Quad(*root)
+        for _ in range(steps):
+            self.current_time += self.dt
+        self.iteration += steps
+        self._state = None  # fields are evaluated again when asked for
+        self._tree = None
+        return None
+
     # ------------------------------------------------------------------ diagnostics
     def get_kinetic_energy(self) -> float:
         return float(0.5 * torch.sum(self.masses * torch.sum(self.velocities ** 2, dim=0)))
