@@ -304,6 +304,10 @@ inline int __builtin_amdgcn_readfirstlane(int v) {
     auto all = td3emu::exchange((unsigned long long)(unsigned)v);
     return (int)(unsigned)all[0];
 }
+inline int __builtin_amdgcn_readlane(int v, int lane) {
+    auto all = td3emu::exchange((unsigned long long)(unsigned)v);
+    return (int)(unsigned)all[lane];
+}
 
 inline unsigned long long __ballot(int pred) {
     auto all = td3emu::exchange(pred ? 1ull : 0ull);
@@ -415,6 +419,7 @@ inline float __frsqrt_rn(float a) { return 1.0f / std::sqrt(a); }
 inline float __builtin_amdgcn_logf(float a) { return std::log2(a); }
 inline float __builtin_amdgcn_rcpf(float a) { return 1.0f / a; }
 inline float __builtin_amdgcn_rsqf(float a) { return 1.0f / std::sqrt(a); }
+inline double rsqrt(double a) { return 1.0 / std::sqrt(a); }
 using std::atan2;
 using std::fabs;
 using std::log;

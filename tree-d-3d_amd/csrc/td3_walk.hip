@@ -36,6 +36,17 @@ struct WalkTree {
     int p1;
 };
 
+// Lane i's value in every lane (i the same in all lanes): two v_readlane_b32.
+__device__ __forceinline__ u64 lane_bits(u64 b, int i) {
+    const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)b, i);
+    const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(b >> 32), i);
+    return ((u64)hi << 32) | lo;
+}
+__device__ __forceinline__ double lane_value(double v, int i) {
+    return __builtin_bit_cast(double, lane_bits(__builtin_bit_cast(u64, v), i));
+}
+__device__ __forceinline__ i64 lane_value(i64 v, int i) { return (i64)lane_bits((u64)v, i); }
+
 template <int KERNEL, int PREC, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
     WalkTree tr, const double* __restrict__ targets, i64 m, const i64* __restrict__ order,
@@ -93,16 +104,34 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
         const int4 nf = tr.info[node];
         const bool mine = (mask & me) != 0;
         if (nf.z < 0) {
-            if (mine && !bad) {
-                for (int k = nf.x; k < nf.y; ++k) {
-                    const double c = tr.scharge[k];
-                    if (tr.perm[k] == skip || c == 0.0) continue;
-                    const double dx = tx - tr.spos[k];
-                    const double dy = ty - tr.spos[tr.n + k];
+            // The leaf's sources are loaded once, a lane each, and handed to every lane in
+            // turn (v_readlane): one round of loads per 64 sources instead of four dependent
+            // loads per source.  Each target adds the same terms in the same order.
+            const int k0 = __builtin_amdgcn_readfirstlane(nf.x);
+            const int k1 = __builtin_amdgcn_readfirstlane(nf.y);
+            for (int base = k0; base < k1; base += TD3_WAVE) {
+                const int kl = base + lane;
+                double lc = 0.0, lx = 0.0, ly = 0.0;
+                i64 lp = -1;
+                if (kl < k1) {
+                    lc = tr.scharge[kl];
+                    lp = tr.perm[kl];
+                    lx = tr.spos[kl];
+                    ly = tr.spos[tr.n + kl];
+                }
+                const int cnt = k1 - base < TD3_WAVE ? k1 - base : TD3_WAVE;
+                for (int i = 0; i < cnt; ++i) {
+                    const double c = lane_value(lc, i);
+                    const i64 pk = lane_value(lp, i);
+                    const double sx = lane_value(lx, i), sy = lane_value(ly, i);
+                    if (!mine || bad) continue;
+                    if (pk == skip || c == 0.0) continue;
+                    const double dx = tx - sx;
+                    const double dy = ty - sy;
                     const double r2 = dx * dx + dy * dy + eps2;
                     if (r2 == 0.0) {
                         bad = true;
-                        break;
+                        continue;
                     }
                     if (PREC == TD3_FP64) {
                         if (KERNEL == TD3_LOG) {
@@ -111,7 +140,9 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
                             ex += inv * dx;
                             ey += inv * dy;
                         } else {
-                            const double inv_r = 1.0 / sqrt(r2);
+                            // HIP's FP64 rsqrt (1 ulp) in place of 1 / sqrt (two
+                            // correctly rounded operations, each an iterative sequence)
+                            const double inv_r = rsqrt(r2);
                             const double cr3 = c * inv_r * inv_r * inv_r;
                             pot += c * inv_r;
                             ex += cr3 * dx;
@@ -194,7 +225,7 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
                     ey -= 2.0 * (double)f_im / h;
                 }
             } else {
-                const double inv_r = 1.0 / sqrt(d2 + eps2);
+                const double inv_r = rsqrt(d2 + eps2);
                 const double inv_r2 = inv_r * inv_r;
                 const double inv_r3 = inv_r2 * inv_r;
                 const double inv_r5 = inv_r3 * inv_r2;
