@@ -268,6 +268,9 @@ def main() -> int:
                          "--pmc runs: the copy calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other configurations (ensemble, fmm_at, adjoint, walk)")
+    ap.add_argument("--extra", choices=["config1", "config3", "config4", "config5"],
+                    help="run only this other configuration and print its JSON (for "
+                         "profiling it); the metric line is not printed")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -278,6 +281,12 @@ def main() -> int:
 
     td3 = __graft_entry__.load_package()
     from tree_d_3d_amd import _abi
+
+    if args.extra:
+        fn = {"config1": extra_config1, "config3": extra_config3, "config4": extra_config4,
+              "config5": extra_config5}[args.extra]
+        print(json.dumps({args.extra: fn(td3)}), flush=True)
+        return 0
 
     n = args.n
     pos_h, q_h = make_inputs(n)
