@@ -44,7 +44,7 @@ def assert_same_tree(tree, ref):
 
 
 @pytest.mark.parametrize("name", list(clouds()))
-@pytest.mark.parametrize("leaf", [1, 6])
+@pytest.mark.parametrize("leaf", [1, 6, 8, 40])
 def test_tree_bit_identical(td3, oracle, name, leaf):
     pos = clouds()[name]
     q = np.ones(pos.shape[1])

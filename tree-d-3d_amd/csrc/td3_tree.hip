@@ -368,6 +368,28 @@ __global__ void leaf_key_kernel(const i64* __restrict__ node_start, const i64* _
     for (i64 k = s; k < e; ++k) key2[sorted_idx[k]] = (u64)s;
 }
 
+// Input order inside each leaf of at most kRankLeaf particles, without a sort: a leaf's
+// particles (input indices sorted_idx[s..e), all different) each go to the position of
+// their rank among them.  Writes order[s..e) for every leaf, so all of order[0, n).
+constexpr int kRankLeaf = 32;
+__global__ void leaf_rank_kernel(const i64* __restrict__ node_start, const i64* __restrict__ node_end,
+                                 const i64* __restrict__ node_child, i64 count,
+                                 const u32* __restrict__ sorted_idx, u32* __restrict__ order) {
+    const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= count || node_child[node] >= 0) return;
+    const i64 s = node_start[node], e = node_end[node];
+    u32 v[kRankLeaf];
+    // the caller launches this only when every leaf holds at most kRankLeaf particles; the
+    // bound keeps every access inside v whatever the input
+    const int k = (int)(e - s < kRankLeaf ? e - s : kRankLeaf);
+    for (int a = 0; a < k; ++a) v[a] = sorted_idx[s + a];
+    for (int a = 0; a < k; ++a) {
+        int rank = 0;
+        for (int b = 0; b < k; ++b) rank += v[b] < v[a] ? 1 : 0;
+        order[s + rank] = v[a];
+    }
+}
+
 __global__ void gather_kernel(const u32* __restrict__ order, const double* __restrict__ pos, i64 n,
                               i64* __restrict__ perm, double* __restrict__ sorted_pos) {
     const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -609,13 +631,21 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                            (const u32*)nullptr, (const i64*)nullptr));
         TD3_LAUNCHED();
     }
-    // 5. input order inside each leaf: sort the input indices by their leaf's start
-    TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_start,
-                       node_end, node_child, count, w.idx, w.khi_alt, (const i64*)nullptr));
-    TD3_LAUNCHED();
-    if (int rc = td3sort::sort_pairs(w.khi_alt, w.idx_alt, nullptr, w.khi, w.idx, nullptr, n, 0,
-                                     bits_for(n), true, w.scratch, s))
-        return rc;
+    // 5. input order inside each leaf.  A leaf holds more than leaf_capacity particles only
+    // at max_depth: when no node is that deep and leaves are small, each leaf ranks its
+    // own; otherwise the input indices are sorted by their leaf's start.
+    if (leaf_capacity <= kRankLeaf && n_levels - 1 < max_depth) {
+        TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_rank_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s,
+                           node_start, node_end, node_child, count, w.idx, w.idx_alt));
+        TD3_LAUNCHED();
+    } else {
+        TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_start,
+                           node_end, node_child, count, w.idx, w.khi_alt, (const i64*)nullptr));
+        TD3_LAUNCHED();
+        if (int rc = td3sort::sort_pairs(w.khi_alt, w.idx_alt, nullptr, w.khi, w.idx, nullptr, n, 0,
+                                         bits_for(n), true, w.scratch, s))
+            return rc;
+    }
     TD3_TIMED(s, "tree_gather", hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, w.idx_alt, pos, n, perm, sorted_pos));
     TD3_LAUNCHED();
     return TD3_OK;
