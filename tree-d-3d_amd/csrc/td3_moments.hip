@@ -103,47 +103,64 @@ __global__ void log_m2m_kernel(const double* __restrict__ center, const int4* __
     if (k == 0) qabs[node] = qa;
 }
 
-// The same for p + 1 <= 16 coefficients, 16 lanes per node: lane k loads the children's
-// coefficient k once and takes the others from its neighbours (shuffles within the 16),
-// instead of every (node, k) thread reloading up to k + 1 of them per child.  Same terms
-// in the same order as log_m2m_kernel.
-__global__ void log_m2m16_kernel(const double* __restrict__ center, const int4* __restrict__ info,
+// The same for p + 1 <= 16 coefficients, a wavefront per node: lane (c, k) = (lane / 16,
+// lane % 16) loads child c's coefficient k once and takes the others from its neighbours
+// (shuffles within the 16), forming child c's term of coefficient k; the four children go
+// at once instead of one after another (a node's latency is one child's chain), and lane
+// (0, k) adds them in child order, skipping empty children: the same terms and the same
+// order of sums as log_m2m_kernel.
+__global__ void log_m2m64_kernel(const double* __restrict__ center, const int4* __restrict__ info,
                                  i64 lo, i64 hi, int p1, const double* __restrict__ binomial,
                                  int bsz, double* __restrict__ qabs, double* __restrict__ mre,
                                  double* __restrict__ mim) {
     const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    const i64 node = lo + t / 16;
-    const int k = (int)(t % 16);
-    if (node >= hi) return;  // uniform over the 16 lanes of a node
+    const i64 node = lo + t / 64;
+    const int lane = (int)(t % 64);
+    const int ci = lane >> 4, k = lane & 15;
+    if (node >= hi) return;  // uniform over the wavefront
     const int4 nf = info[node];
     if (nf.z < 0) return;
     const double cx = center[2 * node], cy = center[2 * node + 1];
-    double are = 0.0, aim = 0.0, qa = 0.0;
-    for (int c = nf.z; c < nf.z + 4; ++c) {
-        qa += qabs[c];
-        const int4 cf = info[c];
-        if (cf.y == cf.x) continue;
-        const double sr = center[2 * c] - cx, si = center[2 * c + 1] - cy;
-        const double my_re = k < p1 ? mre[(i64)c * p1 + k] : 0.0;
-        const double my_im = k < p1 ? mim[(i64)c * p1 + k] : 0.0;
-        // m = k .. 0 with s^(k-m) built upward
-        double pr = 1.0, pi = 0.0, acc_r = 0.0, acc_i = 0.0;
-        for (int j = 0; j < 16; ++j) {
-            const int m = k - j;
-            const double a_re = __shfl(my_re, m < 0 ? 0 : m, 16);
-            const double a_im = __shfl(my_im, m < 0 ? 0 : m, 16);
-            if (m >= 0 && k < p1) {
-                const double b = binomial[k * bsz + m];
-                const double ar = a_re * b, ai = a_im * b;
-                acc_r += ar * pr - ai * pi;
-                acc_i += ar * pi + ai * pr;
-                const double tr = pr * sr - pi * si;
-                pi = pr * si + pi * sr;
-                pr = tr;
-            }
+    const int c = nf.z + ci;
+    const double cqa = qabs[c];
+    const int4 cf = info[c];
+    const bool has = cf.y != cf.x;
+    double acc_r = 0.0, acc_i = 0.0;
+    const double sr = center[2 * c] - cx, si = center[2 * c + 1] - cy;
+    const double my_re = has && k < p1 ? mre[(i64)c * p1 + k] : 0.0;
+    const double my_im = has && k < p1 ? mim[(i64)c * p1 + k] : 0.0;
+    double pr = 1.0, pi = 0.0;
+    for (int j = 0; j < 16; ++j) {
+        const int m = k - j;
+        const double a_re = __shfl(my_re, m < 0 ? 0 : m, 16);
+        const double a_im = __shfl(my_im, m < 0 ? 0 : m, 16);
+        if (has && m >= 0 && k < p1) {
+            const double b = binomial[k * bsz + m];
+            const double ar = a_re * b, ai = a_im * b;
+            acc_r += ar * pr - ai * pi;
+            acc_i += ar * pi + ai * pr;
+            const double tr = pr * sr - pi * si;
+            pi = pr * si + pi * sr;
+            pr = tr;
         }
-        are += acc_r;
-        aim += acc_i;
+    }
+    double cr[4], cim[4], cq[4];
+    int ch[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        cr[q] = __shfl(acc_r, 16 * q + k, 64);
+        cim[q] = __shfl(acc_i, 16 * q + k, 64);
+        cq[q] = __shfl(cqa, 16 * q + k, 64);
+        ch[q] = __shfl(has ? 1 : 0, 16 * q + k, 64);
+    }
+    if (ci != 0) return;
+    double are = 0.0, aim = 0.0, qa = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        qa += cq[q];
+        if (!ch[q]) continue;
+        are += cr[q];
+        aim += cim[q];
     }
     if (k < p1) {
         mre[node * p1 + k] = are;
@@ -269,7 +286,7 @@ int moments_impl(int kernel, int64_t order, const double* sorted_pos, const doub
         for (i64 d = n_levels - 2; d >= 0; --d) {
             const i64 lo = levels_host[d], hi = levels_host[d + 1];
             if (p1 <= 16)
-                TD3_TIMED(s, "moments_m2m", hipLaunchKernelGGL(log_m2m16_kernel, dim3(td3_blocks((hi - lo) * 16, 256)), dim3(256), 0, s,
+                TD3_TIMED(s, "moments_m2m", hipLaunchKernelGGL(log_m2m64_kernel, dim3(td3_blocks((hi - lo) * 64, 256)), dim3(256), 0, s,
                           node_center, info, lo, hi, p1, binomial, (int)bsz, node_qabs, moment_re,
                           moment_im));
             else
