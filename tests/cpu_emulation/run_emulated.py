@@ -307,7 +307,7 @@ def case_forest_resident():
     # too few nodes, or keys too short for the depth: the roots alone, as leaves (direct
     # sums per system), reported; the next resident arrays have room / every level
     starts = np.concatenate(([0], np.cumsum(sizes)))
-    for capacity, key in ((len(sizes) + 3, 48), (4 * n, 2)):
+    for capacity, key in ((len(sizes) + 3, 48), (len(sizes) + 3, 20), (4 * n, 2)):
         small = ensemble.Ensemble.from_arrays(*args, **kw)
         small.compute_fields()
         small._resident = ensemble._Resident(small, capacity, key)

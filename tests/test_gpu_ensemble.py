@@ -149,16 +149,17 @@ def test_graph_replays_equal_steps(td3):
     _same_state(eager, graph)
 
 
-def test_resident_step_overflow_sums_each_system_directly(td3):
+@pytest.mark.parametrize("key", [48, 20])
+def test_resident_step_overflow_sums_each_system_directly(td3, key):
     """A node capacity too small for the forest: the resident step makes every root a leaf
     (exact sums per system), reports it, and the next recording has room."""
     ens_mod = sys.modules["tree_d_3d_amd.ensemble"]
     sizes = [700, 40, 1300]
     ens = td3.Ensemble(gaussian_systems(td3, sizes, seed=3))
     ens.compute_fields()
-    ens._resident = ens_mod._Resident(ens, len(sizes) + 1, 48)
+    ens._resident = ens_mod._Resident(ens, len(sizes) + 1, key)
     ens.graph_capacity = len(sizes) + 1
-    print("resident step with 4 nodes of capacity", flush=True)
+    print(f"resident step with 4 nodes of capacity, {key}-level keys", flush=True)
     ens.step_resident()
     assert ens._resident is None and ens.graph_capacity > 8
     starts = np.concatenate(([0], np.cumsum(sizes)))

@@ -372,10 +372,14 @@ __global__ void leaf_key_kernel(const i64* __restrict__ node_start, const i64* _
 // particles (input indices sorted_idx[s..e), all different) each go to the position of
 // their rank among them.  Writes order[s..e) for every leaf, so all of order[0, n).
 constexpr int kRankLeaf = 32;
+// With sizes (the build without a read-back): the node count is sizes[0], and nothing is
+// done when the nodes overflowed (sizes[2]: then identity_order_kernel writes the order).
 __global__ void leaf_rank_kernel(const i64* __restrict__ node_start, const i64* __restrict__ node_end,
                                  const i64* __restrict__ node_child, i64 count,
-                                 const u32* __restrict__ sorted_idx, u32* __restrict__ order) {
+                                 const u32* __restrict__ sorted_idx, u32* __restrict__ order,
+                                 const i64* __restrict__ sizes) {
     const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sizes != nullptr && (sizes[2] != 0 || node >= sizes[0])) return;
     if (node >= count || node_child[node] >= 0) return;
     const i64 s = node_start[node], e = node_end[node];
     u32 v[kRankLeaf];
@@ -388,6 +392,14 @@ __global__ void leaf_rank_kernel(const i64* __restrict__ node_start, const i64* 
         for (int b = 0; b < k; ++b) rank += v[b] < v[a] ? 1 : 0;
         order[s + rank] = v[a];
     }
+}
+
+// A forest of the roots alone (the nodes overflowed): each root's leaf holds its system's
+// particles, which are contiguous and in input order already, so order[k] = k.
+__global__ void identity_order_kernel(i64 n, const i64* __restrict__ sizes, u32* __restrict__ order) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || sizes[2] == 0) return;
+    order[k] = (u32)k;
 }
 
 __global__ void gather_kernel(const u32* __restrict__ order, const double* __restrict__ pos, i64 n,
@@ -568,13 +580,25 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                            node_end, node_child, node_depth, (int4*)node_info,
                            (const u32*)s_total, (const i64*)sizes_dev));
         TD3_LAUNCHED();
-        TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(capacity, 256)), dim3(256), 0, s, node_start,
-                           node_end, node_child, (i64)capacity, w.idx, w.khi_alt,
-                           (const i64*)sizes_dev));
-        TD3_LAUNCHED();
-        if (int rc = td3sort::sort_pairs(w.khi_alt, w.idx_alt, nullptr, w.khi, w.idx, nullptr, n, 0,
-                                         bits_for(n), true, w.scratch, s))
-            return rc;
+        if (leaf_capacity <= kRankLeaf && nkey < max_depth) {
+            // no node is deeper than the key's levels, so none is at max_depth and every
+            // leaf holds at most leaf_capacity particles (or the forest is the roots alone)
+            TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_rank_kernel, dim3(td3_blocks(capacity, 256)), dim3(256), 0, s,
+                               node_start, node_end, node_child, (i64)capacity, w.idx, w.idx_alt,
+                               (const i64*)sizes_dev));
+            TD3_LAUNCHED();
+            hipLaunchKernelGGL(identity_order_kernel, dim3(nb), dim3(256), 0, s, (i64)n,
+                               (const i64*)sizes_dev, w.idx_alt);
+            TD3_LAUNCHED();
+        } else {
+            TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(capacity, 256)), dim3(256), 0, s, node_start,
+                               node_end, node_child, (i64)capacity, w.idx, w.khi_alt,
+                               (const i64*)sizes_dev));
+            TD3_LAUNCHED();
+            if (int rc = td3sort::sort_pairs(w.khi_alt, w.idx_alt, nullptr, w.khi, w.idx, nullptr, n,
+                                             0, bits_for(n), true, w.scratch, s))
+                return rc;
+        }
         TD3_TIMED(s, "tree_gather", hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, w.idx_alt, pos, n, perm, sorted_pos));
         TD3_LAUNCHED();
         return TD3_OK;
@@ -636,7 +660,8 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     // own; otherwise the input indices are sorted by their leaf's start.
     if (leaf_capacity <= kRankLeaf && n_levels - 1 < max_depth) {
         TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_rank_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s,
-                           node_start, node_end, node_child, count, w.idx, w.idx_alt));
+                           node_start, node_end, node_child, count, w.idx, w.idx_alt,
+                           (const i64*)nullptr));
         TD3_LAUNCHED();
     } else {
         TD3_TIMED(s, "tree_leaf_keys", hipLaunchKernelGGL(leaf_key_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_start,
