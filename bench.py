@@ -139,10 +139,23 @@ def extra_config3(td3, n_src=1 << 22, grid=1024):
     return out
 
 
-def extra_config4(td3, systems=512, steps=100):
+def pmc_copies():
+    """Two copies of a 256 MiB device buffer: the known traffic that calibrates the
+    FETCH_SIZE / WRITE_SIZE counters of a --pmc run."""
+    import torch
+
+    src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def extra_config4(td3, systems=512, steps=100, graph=True):
     """Batched ensemble: 512 Coulomb Barnes-Hut systems, N log-uniform in [1024, 65536]
     (seed 4), Gaussian clouds (sigma 0.1) of unit charges, B_z 1, k_e 1, dt 1e-4,
-    theta 0.5, eps 1e-3; every step one forest build, one walk launch, two push launches."""
+    theta 0.5, eps 1e-3; every step one forest build, one walk launch, two push launches.
+    graph=False leaves out the recorded steps (counter runs take a few direct steps)."""
     rng = np.random.default_rng(4)
     sizes = np.round(np.exp(rng.uniform(np.log(1024), np.log(65536), systems))).astype(int)
     n = int(sizes.sum())
@@ -161,6 +174,12 @@ def extra_config4(td3, systems=512, steps=100):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     drift = np.abs(ens.total_energies() - e0) / np.abs(e0)
+    direct = {"workload": f"{systems} systems, sum N = {n}, {steps} steps, method tree",
+              "ms_per_step": 1e3 * t / steps, "particle_steps_per_s": n * steps / t,
+              "walk_waves_per_block": waves,
+              "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))}}
+    if not graph:
+        return direct
     # the same steps as one recorded HIP graph per step (Ensemble.run): the first step
     # direct, the second recording and replaying, the rest timed replays
     rec = td3.Ensemble.from_arrays(pos, np.zeros((2, n)), np.ones(n), np.ones(n), sizes, roots,
@@ -176,10 +195,7 @@ def extra_config4(td3, systems=512, steps=100):
     t_graph = time.perf_counter() - t0
     same = all(torch.equal(getattr(rec, k), getattr(ens, k))
                for k in ("positions", "velocities", "forces"))
-    return {"workload": f"{systems} systems, sum N = {n}, {steps} steps, method tree",
-            "ms_per_step": 1e3 * t / steps, "particle_steps_per_s": n * steps / t,
-            "walk_waves_per_block": waves,
-            "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))},
+    return {**direct,
             "graph": {"ms_per_step": 1e3 * t_graph / (steps - 2),
                       "particle_steps_per_s": n * (steps - 2) / t_graph,
                       "first_two_steps_ms": 1e3 * t_record,
@@ -264,8 +280,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--pmc", action="store_true",
-                    help="one warm mixed step and a 256 MiB device copy only (for rocprofv3 "
-                         "--pmc runs: the copy calibrates FETCH_SIZE / WRITE_SIZE)")
+                    help="one warm mixed step and a 256 MiB device copy only (with --extra "
+                         "config4: three direct ensemble steps and the copy); for rocprofv3 "
+                         "--pmc runs: the copy calibrates FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other configurations (ensemble, fmm_at, adjoint, walk)")
     ap.add_argument("--extra", choices=["config1", "config3", "config4", "config5"],
@@ -283,6 +300,12 @@ def main() -> int:
     from tree_d_3d_amd import _abi
 
     if args.extra:
+        if args.pmc and args.extra == "config4":
+            # counter runs: three direct ensemble steps, then the copies that calibrate
+            # FETCH_SIZE / WRITE_SIZE
+            print(json.dumps({"config4": extra_config4(td3, steps=3, graph=False)}), flush=True)
+            pmc_copies()
+            return 0
         fn = {"config1": extra_config1, "config3": extra_config3, "config4": extra_config4,
               "config5": extra_config5}[args.extra]
         print(json.dumps({args.extra: fn(td3)}), flush=True)
@@ -314,11 +337,7 @@ def main() -> int:
         step("mixed")
         torch.cuda.synchronize()
         step("mixed")
-        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")  # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(2):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+        pmc_copies()
         print(json.dumps({"pmc": "one mixed step, then two 256 MiB copies"}))
         return 0
     t_mixed, (tree, phi_m, field_m) = timed("mixed", args.steps, args.warmup)
