@@ -57,6 +57,17 @@ static inline unsigned td3_blocks(i64 items, int threads) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// MI355X deals a launch's blocks to its eight XCDs round robin (block b runs on XCD b % 8),
+// and each XCD has its own L2.  A launch of td3_xcd_grid(blocks) blocks that takes its work
+// from block td3_xcd_block() instead of blockIdx.x gives each XCD one contiguous eighth of
+// the blocks, so neighbouring work (the same tree, nearby leaves) shares an L2.  Blocks
+// past the original count come out of td3_xcd_block() too: their work is past the end.
+constexpr unsigned kXcds = 8;
+static inline unsigned td3_xcd_grid(unsigned blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ unsigned td3_xcd_block() {
+    return (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+}
+
 // status[0] keeps the smallest positive report (1 + an index), so the result does not
 // depend on which wave reports first; a negative report (stack or depth overflow)
 // wins over every positive one.

@@ -47,8 +47,6 @@ __device__ __forceinline__ double lane_value(double v, int i) {
 }
 __device__ __forceinline__ i64 lane_value(i64 v, int i) { return (i64)lane_bits((u64)v, i); }
 
-constexpr unsigned kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
-
 template <int KERNEL, int PREC, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
     WalkTree tr, const double* __restrict__ targets, i64 m, const i64* __restrict__ order,
@@ -59,13 +57,9 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
     __shared__ u64 s_mask[WAVES][kMaxStack];
     const int wave = threadIdx.x / TD3_WAVE;
     const int lane = td3_lane();
-    // Blocks are dealt to the eight XCDs round robin (block b runs on XCD b % 8).  The grid
-    // is a multiple of 8 and block b walks the targets of block (b % 8) (grid / 8) + b / 8,
-    // so each XCD walks one contiguous eighth of the target order (in a forest, whole
-    // systems) and its L2 holds those trees only.
-    const u32 per_xcd = gridDim.x / kXcds;
-    const u32 lb = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-    const i64 slot = ((i64)lb * WAVES + wave) * TD3_WAVE + lane;
+    // each XCD walks one contiguous eighth of the target order (in a forest, whole
+    // systems), so its L2 holds those trees only
+    const i64 slot = ((i64)td3_xcd_block() * WAVES + wave) * TD3_WAVE + lane;
     const bool active = slot < m;
     const i64 ti = active ? order[slot] : 0;
     const double tx = active ? targets[ti] : 0.0;
@@ -306,7 +300,7 @@ void launch_walk_w(hipStream_t s, const WalkTree& tr, const double* targets, i64
                    double eps2, int with_potential, int stack_size, double* phi, double* field,
                    i64* status) {
     // a whole number of blocks per XCD (the extra blocks' targets are past m: they exit)
-    const unsigned blocks = (td3_blocks(m, 64 * WAVES) + kXcds - 1) / kXcds * kXcds;
+    const unsigned blocks = td3_xcd_grid(td3_blocks(m, 64 * WAVES));
     TD3_TIMED(s, "walk", hipLaunchKernelGGL((walk_kernel<KERNEL, PREC, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, tr, targets,
                        m, order, exclude, target_root, theta, eps2, with_potential, stack_size, phi,
                        field, status));
