@@ -205,6 +205,34 @@ def test_field_jacobian_matches_autograd_of_direct_sum(td3):
         assert relative(gx[1], gy[0]) < 1e-12  # the Jacobian is symmetric
 
 
+def test_mixed_fmm_with_jacobian_and_dipoles(td3):
+    """A fresh tree's first fmm in mixed precision with jacobian=True, and with dipoles,
+    takes rows that were read back (not the deferred rows of the plain mixed FMM): it
+    returns, records fmm_pairs, and stays within 1e-6 of the FP64 results."""
+    rng = np.random.default_rng(31)
+    n = 5000
+    pos = rng.uniform(-1, 1, (2, n))
+    q = rng.choice([-1.0, 1.0], n)
+    dip = rng.normal(0, 1, (2, n))
+    print("mixed fmm jacobian and dipoles", flush=True)
+    fp64 = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=12)
+    want = fp64.fmm(theta=0.5, softening=1e-3, jacobian=True)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=12, precision="mixed")
+    got = tree.fmm(theta=0.5, softening=1e-3, jacobian=True)
+    assert tree.fmm_pairs == fp64.fmm_pairs
+    for a, b in zip(got, want):
+        assert relative(torch.as_tensor(a), torch.as_tensor(b)) < 1e-6
+    fp64.set_charges(q, dipoles=dip)
+    want = fp64.fmm(theta=0.5, softening=1e-3)
+    dtree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=12,
+                             precision="mixed")
+    dtree.set_charges(q, dipoles=dip)
+    got = dtree.fmm(theta=0.5, softening=1e-3)
+    assert dtree.fmm_pairs == fp64.fmm_pairs
+    for a, b in zip(got, want):
+        assert relative(torch.as_tensor(a), torch.as_tensor(b)) < 1e-6
+
+
 def test_softened_near_field_jacobian_matches_autograd(td3):
     # with one leaf every pair is a near (softened) pair, so the softened direct sum is
     # the FMM's own semantics: the near-field Jacobian 2 q (I / rho - 2 d d^T / rho^2)

@@ -229,3 +229,16 @@ def test_simulation_run_graph_equals_step(td3):
     c = gaussian_systems(td3, [500], seed=1, method="direct")[0]
     with pytest.raises(ValueError):
         c.run(2, graph=True)
+
+
+def test_ensemble_refuses_systems_that_resolve_to_another_method(td3):
+    """A default ('auto') simulation resolves by its own size and theta (simulation.py:
+    567-581): the ensemble, which runs the walk, takes it only when that is 'tree'."""
+    print("ensemble method check", flush=True)
+    small = gaussian_systems(td3, [300], seed=5, method="auto")  # few charges: direct
+    with pytest.raises(ValueError, match="tree"):
+        td3.Ensemble(small)
+    walk = gaussian_systems(td3, [300], seed=5, method="auto", theta=1.2,
+                            direct_threshold=10)  # theta >= 1: the walk
+    assert walk[0]._resolve_method() == "tree"
+    td3.Ensemble(walk)

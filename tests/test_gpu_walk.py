@@ -96,3 +96,21 @@ def test_walk_torch_inputs_stay_on_device(td3):
     tree = td3.FlatQuadtree(tpos, torch.from_numpy(q).cuda(), td3.Quad(0, 0, 1))
     phi, field = tree.evaluate(tpos, exclude=torch.arange(2000, device="cuda"), softening=1e-3)
     assert phi.is_cuda and field.is_cuda and field.shape == (2, 2000)
+
+
+def test_mixed_log_walk_deep_tree_high_order(td3):
+    """Cells 2^-40 wide at order 40: the mixed walk's radius-scaled moments a_k h^-k stay
+    finite (h^-k alone overflows FP64 there), so the fields stay finite and within FP32
+    rounding of the FP64 walk's."""
+    rng = np.random.default_rng(9)
+    tight = 0.3 + 1e-12 * rng.uniform(-1, 1, (2, 64))
+    pos = np.concatenate((tight, rng.uniform(-1, 1, (2, 500))), axis=1)
+    q = rng.choice([-1.0, 1.0], pos.shape[1])
+    targets = rng.uniform(-1, 1, (2, 300))
+    print("deep tree order 40 mixed walk", flush=True)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=40, leaf_capacity=1)
+    assert int(np.max(tree.node_depth)) >= 38
+    phi, field = tree.evaluate(targets, theta=0.5, precision="mixed")
+    rphi, rfield = tree.evaluate(targets, theta=0.5, precision="fp64")
+    assert np.all(np.isfinite(phi)) and np.all(np.isfinite(field))
+    assert relative(phi, rphi) < 1e-5 and relative(field, rfield) < 1e-5

@@ -284,6 +284,66 @@ def test_cartesian_taylor_coefficients_match_sympy(oracle):
         assert local[0, i] == pytest.approx(want, rel=1e-12, abs=1e-14)
 
 
+def test_cartesian_moments_equal_brute_force_sums(oracle):
+    # m_(a, b) = sum over the cell's particles of q (x - cx)^a (y - cy)^b
+    # (cart_moments_upward_pyccel.py:14-82: P2M at the leaves, M2M shifts above them)
+    rng = np.random.default_rng(17)
+    pos, q = cloud(600, seed=17)
+    order = 7
+    t = oracle.OracleTree(pos, q, (0, 0, 1), order=order, leaf_capacity=5)
+    moments = oracle._cart_moments(t)
+    tb = oracle.cart_tables(order)
+    assert int(np.max(t.node_depth)) >= 4
+    internal = np.flatnonzero((t.node_child >= 0) & (t.node_end > t.node_start))
+    leaves = np.flatnonzero((t.node_child < 0) & (t.node_end > t.node_start))
+    nodes = np.concatenate(([0], rng.choice(internal, 25), rng.choice(leaves, 25)))
+    for node in nodes:
+        members = t.perm[t.node_start[node]:t.node_end[node]]
+        dx = pos[0, members] - t.node_center[node, 0]
+        dy = pos[1, members] - t.node_center[node, 1]
+        want = np.array([np.sum(q[members] * dx ** a * dy ** b)
+                         for a, b in zip(tb["ex"], tb["ey"])])
+        scale = np.sum(np.abs(q[members])) * (np.sqrt(2) * t.node_half[node]) ** (tb["ex"] + tb["ey"])
+        assert np.all(np.abs(moments[node] - want) <= 1e-12 * scale), node
+
+
+def test_cartesian_l2l_is_an_exact_recentring(oracle):
+    # cart_downward_pyccel.py:14-55: T_b(child) += sum_g C(g, b) T_g s^(g - b) re-expands
+    # the parent's polynomial sum_g T_g (r - c)^g about the child's centre, exactly (the
+    # degree does not grow); after the pass a cell's polynomial is the sum of its own and
+    # every ancestor's starting polynomials, each about its own centre
+    rng = np.random.default_rng(18)
+    pos = rng.uniform(-1, 1, (2, 300))
+    order = 6
+    t = oracle.OracleTree(pos, np.ones(300), (0, 0, 1), order=order, leaf_capacity=4)
+    tb = oracle.cart_tables(order)
+    nc = oracle.cart_size(order)
+    start = rng.normal(size=(t.node_count, nc))
+    local = start.copy()
+    lev = np.ascontiguousarray(t.levels, dtype=np.int64)
+    oracle.lib().oracle_cart_downward(
+        lev.ctypes.data, len(lev) - 1, t.node_center.ctypes.data, t.node_child.ctypes.data,
+        tb["mono_parent"].ctypes.data, tb["mono_dir"].ctypes.data, tb["shift"].ctypes.data,
+        tb["shift_coef"].ctypes.data, len(tb["shift"]), nc, local.ctypes.data)
+    parent = np.full(t.node_count, -1)
+    for node in range(t.node_count):
+        if t.node_child[node] >= 0:
+            parent[t.node_child[node]:t.node_child[node] + 4] = node
+
+    def poly(coef, centre, r):
+        d = r - centre
+        return np.sum(coef * d[0] ** tb["ex"] * d[1] ** tb["ey"])
+
+    for node in rng.choice(t.node_count, 60):
+        r = t.node_center[node] + t.node_half[node] * rng.uniform(-1, 1, 2)
+        want, a = 0.0, node
+        while a >= 0:
+            want += poly(start[a], t.node_center[a], r)
+            a = parent[a]
+        got = poly(local[node], t.node_center[node], r)
+        assert abs(got - want) < 1e-12 * (t.node_depth[node] + 1) * nc, node
+
+
 @pytest.mark.parametrize("softening", [0.0, 0.02])
 def test_coulomb_fmm_converges_and_conserves_momentum(oracle, softening):
     # test_fmm.py:189-207

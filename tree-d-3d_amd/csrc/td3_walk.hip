@@ -209,10 +209,17 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
                     const float w_re = (float)(rx / d2 * h), w_im = (float)(-ry / d2 * h);
                     float prev_re = 1.0f, prev_im = 0.0f, cur_re = w_re, cur_im = w_im;
                     float f_re = 0.0f, f_im = 0.0f, series = 0.0f;
-                    double inv_h = 1.0;
+                    // a_k h^-k with h = mant 2^e: mant^-k (at most 2^64) times an exact
+                    // power of two applied by ldexp, so no intermediate h^-k overflows
+                    // (deep cells at high order) and the scaled moment stays finite
+                    int h_exp;
+                    const double h_mant = frexp(h, &h_exp);
+                    const double inv_mant = 1.0 / h_mant;
+                    double mant_pow = 1.0;
                     for (int k = 0; k < tr.p1; ++k) {
-                        const float a_re = (float)(are[k] * inv_h), a_im = (float)(aim[k] * inv_h);
-                        inv_h /= h;
+                        const float a_re = (float)ldexp(are[k] * mant_pow, -h_exp * k);
+                        const float a_im = (float)ldexp(aim[k] * mant_pow, -h_exp * k);
+                        mant_pow *= inv_mant;
                         f_re += a_re * cur_re - a_im * cur_im;
                         f_im += a_re * cur_im + a_im * cur_re;
                         if (k > 0) series += (a_re * prev_re - a_im * prev_im) / (float)k;

@@ -229,8 +229,12 @@ class Ensemble:
                     raise ValueError(f"every system of an ensemble must share {name}")
             if sim.kernel != "coulomb":
                 raise ValueError("an ensemble runs the Coulomb kernel")
-            if sim.method not in ("auto", "tree"):
-                raise ValueError("an ensemble runs the Barnes-Hut walk (method 'tree')")
+            # an 'auto' system resolves by its own size and theta (simulation.py:567-581);
+            # the ensemble takes it only when that resolution is the walk
+            if sim._resolve_method() != "tree":
+                raise ValueError(
+                    "an ensemble runs the Barnes-Hut walk: every system's method must be "
+                    f"(or resolve to) 'tree', got {sim.method!r} -> {sim._resolve_method()!r}")
         leaf = first.leaf_capacity if first.leaf_capacity is not None else 8
         if any((s.leaf_capacity if s.leaf_capacity is not None else 8) != leaf for s in sims):
             raise ValueError("every system of an ensemble must share leaf_capacity")

@@ -805,9 +805,10 @@ class FlatQuadtree:
             self._scaled_moments(_abi.FP64)
             if prec != _abi.FP64:
                 self._scaled_moments(prec)
-            if prec != _abi.FP64 and self.order + 1 <= 16:
+            if prec != _abi.FP64 and self.order + 1 <= 16 and not jacobian and sdip is None:
                 # mixed precision needs no row counts before its last read-back: the rows
-                # are built and used at full capacity and checked with the status
+                # are built and used at full capacity and checked with the status (the
+                # adjoint's evaluation below takes rows that were read back)
                 pending = self._rows(theta, deferred=True)
         if pending is not None:
             m2l_rows, p2p_rows, dcounts = pending
@@ -1020,6 +1021,7 @@ class FlatQuadtree:
             ex = _device(exclude, torch.int64).reshape(-1)
         leaf = self.leaf_capacity if leaf_capacity is None else int(leaf_capacity)
         tree, (m2l_rows, (prow, pcol, _), counts) = self._target_plan(t, theta, leaf)
+        self.fmm_at_pairs = counts
         if self.kernel != "log":
             return self._coulomb_fmm_at(t, ex, tree, m2l_rows, prow, pcol, softening, potential,
                                         precision, like_torch)
