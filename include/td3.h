@@ -64,6 +64,22 @@ int td3_direct(int kernel, int precision, const double *targets, int64_t m,
                size_t workspace_bytes, void *stream);
 
 /*
+ * td3_direct for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc); no counterpart in the reference.  Same
+ * arguments and results as td3_direct (bit for bit), without the workspace: the chunk
+ * partials live in a scratch array of the library, one per GPU, of
+ * td3_direct_noalloc_scratch_bytes() bytes (16 MiB), held until the process ends.  A call
+ * whose td3_direct_workspace_bytes(m, n) exceeds it returns TD3_ERR_CAPACITY before any
+ * launch.  Calls no allocator.  Every call on a GPU uses that GPU's one scratch array:
+ * calls that overlap on one GPU (other streams or threads) race.
+ */
+size_t td3_direct_noalloc_scratch_bytes(void);
+int td3_direct_noalloc(int kernel, int precision, const double *targets, int64_t m,
+                       const int64_t *exclude, const double *pos, int64_t n,
+                       const double *charge, double softening, double *phi, double *field,
+                       int64_t *status, void *stream);
+
+/*
  * Flat quadtree build, bit-identical to the reference's (the arrays of
  * kernels/quadtree_build/quadtree_build_pyccel.py:12-137 and of the per-level GPU build
  * flat_tree.py:532-625 via kernels/quadtree_quadrants, kernels/quadtree_scatter).

@@ -60,6 +60,17 @@ typedef void* hipEvent_t;
 #define hipMemcpyDeviceToDevice 0
 #define hipMemcpyHostToDevice 0
 inline const char* hipGetErrorString(hipError_t) { return "emulated"; }
+// one emulated device; a __device__ variable is a host variable, its address its own
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetDevice(int* d) {
+    *d = 0;
+    return 0;
+}
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+    *p = const_cast<void*>(symbol);
+    return 0;
+}
 inline hipError_t hipGetLastError() { return 0; }
 inline hipError_t hipMemsetAsync(void* p, int v, size_t n, hipStream_t) {
     memset(p, v, n);

@@ -37,6 +37,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_last_error": ([], ctypes.c_char_p),
     "td3_direct_workspace_bytes": ([_I, _I], _Z),
     "td3_direct": ([_C, _C, _P, _I, _P, _P, _I, _P, _D, _P, _P, _P, _P, _Z, _P], _C),
+    "td3_direct_noalloc_scratch_bytes": ([], _Z),
+    "td3_direct_noalloc": ([_C, _C, _P, _I, _P, _P, _I, _P, _D, _P, _P, _P, _P], _C),
     "td3_tree_workspace_bytes": ([_I, _I], _Z),
     "td3_tree_build": ([_P, _I, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                         _P, _Z, _P, _P, _P], _C),

@@ -10,7 +10,17 @@
 // the 256 CUs, the sources are split into chunks along grid.y and a second kernel adds
 // the chunk partials in a fixed order (deterministic, no float atomics).
 
+#include <mutex>
+
 #include "td3_common.h"
+
+// Scratch of td3_direct_noalloc: the per-chunk partial sums (phi, E_x, E_y of every target
+// and source chunk) that td3_direct keeps in its caller's workspace.  One copy per GPU,
+// held until the process ends; every call on a GPU shares its copy, so calls that overlap
+// on one GPU race.  Its host-side name is not a device address: kernels get the address
+// that hipGetSymbolAddress returns for the current device.
+constexpr size_t kDirectScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) double scratch[kDirectScratchBytes / sizeof(double)];
 
 namespace {
 
@@ -191,4 +201,49 @@ extern "C" int td3_direct(int kernel, int precision, const double* targets, int6
         TD3_LAUNCHED();
     }
     return TD3_OK;
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+int direct_scratch(void** out) {
+    int dev = 0;
+    TD3_HIP(hipGetDevice(&dev));
+    TD3_ARG(dev >= 0 && dev < kMaxDevices, "device index");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        TD3_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TD3_OK;
+}
+}  // namespace
+
+extern "C" size_t td3_direct_noalloc_scratch_bytes(void) { return kDirectScratchBytes; }
+
+extern "C" int td3_direct_noalloc(int kernel, int precision, const double* targets, int64_t m,
+                                  const int64_t* exclude, const double* pos, int64_t n,
+                                  const double* charge, double softening, double* phi,
+                                  double* field, int64_t* status, void* stream) {
+    TD3_ARG(m >= 0 && n >= 0, "sizes");
+    const size_t need = td3_direct_workspace_bytes(m, n);
+    if (need > kDirectScratchBytes) {
+        td3_set_error("td3_direct_noalloc: %lld targets x %lld sources need %zu bytes of "
+                      "scratch, more than its %zu (call td3_direct with a workspace)",
+                      (long long)m, (long long)n, need, kDirectScratchBytes);
+        return TD3_ERR_CAPACITY;
+    }
+    void* ws = nullptr;
+    if (need > 0) {
+        const int code = direct_scratch(&ws);
+        if (code != TD3_OK) return code;
+    }
+    // the chunk partials are written in full before they are read: nothing to clear
+    return td3_direct(kernel, precision, targets, m, exclude, pos, n, charge, softening, phi,
+                      field, status, ws, need, stream);
 }
