@@ -471,6 +471,19 @@ int td3_system_bounds(const double *positions, int64_t n, int64_t systems,
                       const int64_t *system_start, double *roots, uint32_t *grown, void *stream);
 
 /*
+ * Energies of an ensemble's systems (simulation.py:837-916 per system: kinetic
+ * sum 1/2 m |v|^2 and potential sum 1/2 q phi), in FP64: out[s] the kinetic and
+ * out[systems + s] the potential energy of system s (particles system_start[s] ..
+ * system_start[s + 1] - 1; velocities (2, N); potential (N,) may be NULL: zeros).  One
+ * workgroup per system adds its particles in a fixed order, so results do not change from
+ * run to run.  No counterpart in the reference (its systems are summed one at a time on
+ * the host).
+ */
+int td3_system_energies(const double *masses, const double *velocities, const double *charges,
+                        const double *potential, int64_t n, int64_t systems,
+                        const int64_t *system_start, double *out, void *stream);
+
+/*
  * Kernel timing for measurement (no counterpart in the reference, whose profiling is
  * host-side regions): td3_timing_enable(1) clears the record and brackets every later
  * kernel launch with HIP events on its stream; td3_timing_get(name) synchronises and

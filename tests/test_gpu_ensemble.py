@@ -112,8 +112,29 @@ def _same_state(a, b):
     a.compute_fields()
     b.compute_fields()
     assert torch.equal(a._potential, b._potential) and torch.equal(a._electric, b._electric)
-    # per-system sums (index_add_ adds in no fixed order): to rounding
-    np.testing.assert_allclose(a.total_energies(), b.total_energies(), rtol=1e-12)
+    # per-system sums in a fixed order (td3_system_energies): equal states, equal energies
+    assert np.array_equal(a.total_energies(), b.total_energies())
+
+
+def test_system_energies_equal_fp64_sums(td3):
+    """td3_system_energies (a workgroup per system, fixed order) against NumPy's FP64 sums
+    of 1/2 m |v|^2 and 1/2 q phi per system, and bit-equal from call to call."""
+    sizes = [1, 700, 3000, 255, 257]
+    ens = td3.Ensemble(gaussian_systems(td3, sizes, seed=6))
+    print("system energies", flush=True)
+    ens.run(3, graph=False)
+    kin, pot = ens.kinetic_energies(), ens.potential_energies()
+    m, q = ens.masses.cpu().numpy(), ens.charges.cpu().numpy()
+    v, phi = ens.velocities.cpu().numpy(), ens._potential.cpu().numpy()
+    start = np.concatenate(([0], np.cumsum(sizes)))
+    for s in range(len(sizes)):
+        sl = slice(start[s], start[s + 1])
+        want_k = np.sum(0.5 * m[sl] * (v[0, sl] ** 2 + v[1, sl] ** 2))
+        want_p = np.sum(0.5 * q[sl] * phi[sl])
+        assert abs(kin[s] - want_k) <= 1e-13 * abs(want_k) + 1e-300
+        assert abs(pot[s] - want_p) <= 1e-13 * abs(want_p)
+    assert np.array_equal(kin, ens.kinetic_energies())
+    assert np.array_equal(pot, ens.potential_energies())
 
 
 def test_graph_replays_equal_steps(td3):

@@ -12,6 +12,7 @@ from .quad import Quad
 from .simulation import ChargedBody, ElectrostaticSimulation
 from .adjoint import log_energy, log_fields, log_potentials
 from .ensemble import Ensemble
+from .repeated import RepeatedSums
 
 __all__ = ["K_E", "MAX_ORDER", "ChargedBody", "Ensemble", "ElectrostaticSimulation", "FlatNode",
-           "FlatQuadtree", "Quad", "direct_field", "log_energy", "log_fields", "log_potentials"]
+           "FlatQuadtree", "Quad", "RepeatedSums", "direct_field", "log_energy", "log_fields", "log_potentials"]

@@ -123,3 +123,57 @@ extern "C" int td3_push(int first_half, int64_t n, double* positions, double* ve
     TD3_LAUNCHED();
     return TD3_OK;
 }
+
+// ------------------------------------------------------------------ energies per system
+namespace {
+
+constexpr int kEnergyThreads = 256;
+
+// One workgroup per system sums its contiguous particles: thread t takes particles
+// start + t, start + t + 256, ... in order, then a fixed tree over the 256 partial sums,
+// so every run adds the same numbers in the same order (no float atomics).
+// out[s] = sum 1/2 m |v|^2, out[S + s] = sum 1/2 q phi.
+__global__ __launch_bounds__(kEnergyThreads) void system_energies_kernel(
+    const double* __restrict__ mass, const double* __restrict__ vel,
+    const double* __restrict__ charge, const double* __restrict__ potential, i64 n,
+    i64 systems, const i64* __restrict__ system_start, double* __restrict__ out) {
+    __shared__ double s_kin[kEnergyThreads], s_pot[kEnergyThreads];
+    const i64 s = blockIdx.x;
+    const i64 lo = system_start[s], hi = system_start[s + 1];
+    double kin = 0.0, pot = 0.0;
+    for (i64 i = lo + threadIdx.x; i < hi; i += kEnergyThreads) {
+        const double vx = vel[i], vy = vel[n + i];
+        kin += 0.5 * mass[i] * (vx * vx + vy * vy);
+        if (potential != nullptr) pot += 0.5 * charge[i] * potential[i];
+    }
+    s_kin[threadIdx.x] = kin;
+    s_pot[threadIdx.x] = pot;
+    __syncthreads();
+    for (int half = kEnergyThreads / 2; half > 0; half >>= 1) {
+        if ((int)threadIdx.x < half) {
+            s_kin[threadIdx.x] += s_kin[threadIdx.x + half];
+            s_pot[threadIdx.x] += s_pot[threadIdx.x + half];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[s] = s_kin[0];
+        out[systems + s] = s_pot[0];
+    }
+}
+
+}  // namespace
+
+extern "C" int td3_system_energies(const double* masses, const double* velocities,
+                                   const double* charges, const double* potential, int64_t n,
+                                   int64_t systems, const int64_t* system_start, double* out,
+                                   void* stream) {
+    TD3_ARG(n >= 0 && systems > 0, "sizes");
+    hipStream_t s = (hipStream_t)stream;
+    TD3_TIMED(s, "system_energies",
+              hipLaunchKernelGGL(system_energies_kernel, dim3((unsigned)systems),
+                                 dim3(kEnergyThreads), 0, s, masses, velocities, charges,
+                                 potential, n, systems, system_start, out));
+    TD3_LAUNCHED();
+    return TD3_OK;
+}

@@ -486,17 +486,22 @@ class Ensemble:
         self._after_resident(steps - done)
 
     # ------------------------------------------------------------------ diagnostics
-    def _per_system(self, values: torch.Tensor) -> np.ndarray:
-        out = torch.zeros(self.systems, dtype=torch.float64, device=values.device)
-        return _host(out.index_add_(0, self._index, values))
+    def _energies(self, potential: torch.Tensor | None) -> np.ndarray:
+        """(2, S): kinetic and potential energy of every system (td3_system_energies: one
+        workgroup per system, FP64, a fixed order of additions)."""
+        out = torch.empty((2, self.systems), dtype=torch.float64, device=self.positions.device)
+        _abi.call("td3_system_energies", _abi.ptr(self.masses), _abi.ptr(self.velocities),
+                  _abi.ptr(self.charges), _abi.ptr(potential), self.num_particles, self.systems,
+                  _abi.ptr(self.system_start), _abi.ptr(out), _abi.stream())
+        return _host(out)
 
     def kinetic_energies(self) -> np.ndarray:
-        return self._per_system(0.5 * self.masses * (self.velocities ** 2).sum(dim=0))
+        return self._energies(None)[0]
 
     def potential_energies(self) -> np.ndarray:
         """1/2 sum q phi per system (simulation.py:853-916, from the walk's potential)."""
         self.compute_fields()
-        return self._per_system(0.5 * self.charges * self._potential)
+        return self._energies(self._potential)[1]
 
     def total_energies(self) -> np.ndarray:
         return self.kinetic_energies() + self.potential_energies()

@@ -698,6 +698,9 @@ class FlatQuadtree:
         self._scaled = cached
         return re, im
 
+    #: target trees and dual lists `fmm_at` has built (kept per targets, theta and leaf)
+    plan_builds = 0
+
     #: in mixed precision, M2L into cells at depth <= this stays FP64 (their terms are the
     #: largest: a cell's net charge grows with its size)
     FP64_TOP_DEPTH = 5
@@ -985,6 +988,7 @@ class FlatQuadtree:
             rows.append((row, col, dst))
         lists = (rows[0], rows[1], (int(out[0]), int(out[1])))
         self._traversals[key] = (targets.clone(), tree, lists)
+        self.plan_builds += 1
         return tree, lists
 
     def fmm_at(self, targets, *, exclude=None, theta: float = 0.5, softening: float = 0.0,
