@@ -29,8 +29,8 @@ import __graft_entry__  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # measured (rocprofv3 --pmc, calibrated by a 256 MiB copy) for this workload's M2L launch
 M2L_TRAFFIC_BYTES = 923.2e6
-KERNEL_NAMES = ("m2l_mfma", "m2l_mfma64", "m2l", "m2l_fixup", "eval", "l2l", "rows_count", "rows_scan",
-                "rows_write",
+KERNEL_NAMES = ("m2l_mfma", "m2l_mfma64", "m2l", "m2l_fixup", "eval", "l2l", "rows_level",
+                "rows_place",
                 "traverse",
                 "row_sort", "lists_fill",
                 "lists_degree", "sort_scatter", "sort_hist", "scan", "tree_keys", "tree_count",

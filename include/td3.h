@@ -241,25 +241,26 @@ int td3_traverse(const double *node_center, const double *node_half, const void 
 /*
  * The FMM's interaction rows by node, straight from the traversal rules of
  * kernels/fmm_traverse/fmm_traverse_pyccel.py:15-106 (flat_tree.py:1318-1358) without
- * the pair lists: per level, every node walks the pairs it inherits from its parent
- * (one lane per pair, depth first over the other cell) and writes its own rows.  The
- * rows hold the same pairs as td3_traverse + td3_lists (M2L both ways, mode 0; P2P both
- * ways with a self pair once, mode 1), each row in a fixed walk order (not sorted);
- * m2l_dst (may be NULL) gets the row's node of every M2L entry.  levels_host as
- * td3_moments.  out_counts (HOST int64[4]): M2L entries, P2P entries, non-empty leaves
- * (self pairs), capacity overflow.  TD3_ERR_CAPACITY asks for larger cap_m2l / cap_p2p / cap_inherit
- * (cap_inherit bounds the pairs handed to one level).  Any depth (the walks follow child
- * and parent links, no stack).  Synchronises the stream once, unless device_counts
+ * the pair lists: level by level, every node (a wavefront) walks the pairs it inherits
+ * from its parent (one lane per pair, depth first over the other cell) into slots of
+ * `width` entries of its own (M2L row, P2P row, pairs handed to its children); one scan
+ * per row kind and a packing copy then place the rows.  The rows hold the same pairs as
+ * td3_traverse + td3_lists (M2L both ways, mode 0; P2P both ways with a self pair once,
+ * mode 1), each row in a fixed walk order (not sorted); m2l_dst (may be NULL) gets the
+ * row's node of every M2L entry.  levels_host as td3_moments.  out_counts (HOST
+ * int64[4]): M2L entries, P2P entries, non-empty leaves (self pairs), overflow (0; else
+ * the slot width a node needs, or 1 when only cap_m2l / cap_p2p were too small).
+ * TD3_ERR_CAPACITY asks for larger cap_m2l / cap_p2p / width.  Any depth (the walks follow
+ * child and parent links, no stack).  Synchronises the stream once, unless device_counts
  * (DEVICE int64[4], may be NULL) is given: then the same four counts are written on the
  * device, with no read-back and no synchronisation (out_counts may then be NULL), and
  * on overflow every row is left empty, so the rows can be evaluated safely before the
  * caller reads the counts and builds again with larger capacities.
  */
-size_t td3_rows_workspace_bytes(const int64_t *levels_host, int64_t n_levels,
-                                int64_t cap_inherit);
+size_t td3_rows_workspace_bytes(const int64_t *levels_host, int64_t n_levels, int64_t width);
 int td3_rows(const double *node_center, const double *node_half, const void *node_info,
              const int64_t *levels_host, int64_t n_levels, double theta, int64_t cap_m2l,
-             int64_t cap_p2p, int64_t cap_inherit, uint32_t *m2l_row, int32_t *m2l_col,
+             int64_t cap_p2p, int64_t width, uint32_t *m2l_row, int32_t *m2l_col,
              int32_t *m2l_dst, uint32_t *p2p_row, int32_t *p2p_col, void *workspace,
              size_t workspace_bytes, int64_t *out_counts, int64_t *device_counts,
              void *stream);

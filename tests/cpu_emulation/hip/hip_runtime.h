@@ -407,6 +407,14 @@ template <typename T>
 inline T atomicOr(T* p, T v) {
     return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
 }
+template <class T>
+inline T atomicMax(T* p, T v) {
+    T old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST,
+                                                   __ATOMIC_SEQ_CST)) {
+    }
+    return old;
+}
 inline unsigned long long atomicCAS(unsigned long long* p, unsigned long long cmp,
                                     unsigned long long v) {
     __atomic_compare_exchange_n(p, &cmp, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);

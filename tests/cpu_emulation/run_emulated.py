@@ -198,7 +198,8 @@ def case_rows_overflow():
     # the FMM notices with its status and builds them again
     pos, q = cloud(400, 3)
     t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10, leaf_capacity=4)
-    t._ROW_CAPS = (1, 1, 1)
+    t._ROW_CAPS = (1, 1)
+    t._ROW_WIDTH = 8
     got = t.fmm(theta=0.5, softening=1e-3, precision="mixed")
     r = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10, leaf_capacity=4)
     want = r.fmm(theta=0.5, softening=1e-3, precision="mixed")

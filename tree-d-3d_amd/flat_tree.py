@@ -574,7 +574,8 @@ class FlatQuadtree:
         c = self.node_count
         levels = np.ascontiguousarray(self.levels, dtype=np.int64)
         n_levels = len(levels) - 1
-        caps = [f * c + 64 for f in self._ROW_CAPS]
+        caps = [f * c + 64 for f in self._ROW_CAPS] + [self.__dict__.get("_row_width",
+                                                                          self._ROW_WIDTH)]
         out = np.zeros(4, dtype=np.int64)
         if deferred:
             i32 = dict(dtype=torch.int32, device=dev)
@@ -605,8 +606,11 @@ class FlatQuadtree:
                 _abi.stream(),
             )
             if code == _abi.TD3_ERR_CAPACITY:
-                caps = [max(2 * caps[i], int(out[i]) + 64) if i < 2 else 2 * caps[i]
-                        for i in range(3)]
+                # out[3] > 1: a node's slots overflowed (the width it needs); the rows'
+                # totals grow either way
+                caps = [max(2 * caps[0], int(out[0]) + 64), max(2 * caps[1], int(out[1]) + 64),
+                        max(caps[2], int(out[3]) + 8) if out[3] > 1 else caps[2]]
+                self._row_width = caps[2]
                 continue
             _abi.check(code, "td3_rows")
             break
@@ -705,9 +709,12 @@ class FlatQuadtree:
     #: largest: a cell's net charge grows with its size)
     FP64_TOP_DEPTH = 5
 
-    #: first capacities of the interaction rows per node (M2L, P2P entries, pairs handed
-    #: to one level): exceeded, the rows are built again with twice the room
-    _ROW_CAPS = (96, 40, 24)
+    #: first capacities of the interaction rows per node (M2L and P2P entries in all):
+    #: exceeded, the rows are built again with twice the room
+    _ROW_CAPS = (96, 40)
+    #: entries per node of the row construction's slots (M2L row, P2P row, hand-downs to
+    #: the children); a node that needs more has the rows built again with its width
+    _ROW_WIDTH = 128
 
     def _local_expansions(self, theta: float, precision: int, target=None, m2l_rows=None):
         """Scaled local expansions B_l = b_l r^l of every cell of ``target`` (this tree by
