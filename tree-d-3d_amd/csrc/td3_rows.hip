@@ -62,9 +62,21 @@ __device__ __forceinline__ int decide(const RowsTree& t, double4 ga, int4 fa, in
     const double4 gf = a_second ? gx : ga, gs = a_second ? ga : gx;
     const double dx = __dadd_rn(gf.x, -gs.x);
     const double dy = __dadd_rn(gf.y, -gs.y);
-    const double distance = __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
+    const double d2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
     const double radii = __dmul_rn(1.4142135623730951, __dadd_rn(gf.z, gs.z));
-    if (radii < __dmul_rn(theta, distance)) return 1;
+    // The reference's test is radii < fl(theta fl(sqrt(d2))); its right side lies within a
+    // relative 3e-16 of theta sqrt(d2).  Squares decide it without the square root unless
+    // radii^2 and theta^2 d2 agree to 1e-12 (their own roundings are a few 1e-16); only
+    // then is the reference's expression evaluated.
+    const double lhs = radii * radii, rhs = theta * theta * d2;
+    bool accept;
+    if (lhs < rhs * (1.0 - 1e-12))
+        accept = true;
+    else if (lhs > rhs * (1.0 + 1e-12))
+        accept = false;
+    else
+        accept = radii < __dmul_rn(theta, __dsqrt_rn(d2));
+    if (accept) return 1;
     if (leaf_a && leaf_x) return 2;
     const bool leaf_f = a_second ? leaf_x : leaf_a, leaf_s = a_second ? leaf_a : leaf_x;
     const bool split_first = leaf_s || (!leaf_f && gf.z >= gs.z);

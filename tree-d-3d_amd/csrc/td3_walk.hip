@@ -178,8 +178,15 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
         const double d2 = __dadd_rn(__dmul_rn(rx, rx), __dmul_rn(ry, ry));
         if (KERNEL == TD3_LOG)
             acc = !inside && __dmul_rn(__dmul_rn(4.0, h), h) < __dmul_rn(__dmul_rn(theta, theta), d2);
-        else
-            acc = !inside && __dmul_rn(2.0, h) < __dmul_rn(theta, __dsqrt_rn(d2));
+        else {
+            // 2h < fl(theta fl(sqrt(d2))): decided on squares unless they agree to 1e-12
+            // (the right side lies within 3e-16 of theta sqrt(d2)), where the reference's
+            // expression is evaluated
+            const double lhs = 4.0 * h * h, rhs = theta * theta * d2;
+            const bool near_tie = !(lhs < rhs * (1.0 - 1e-12)) && !(lhs > rhs * (1.0 + 1e-12));
+            acc = !inside && (near_tie ? __dmul_rn(2.0, h) < __dmul_rn(theta, __dsqrt_rn(d2))
+                                       : lhs < rhs);
+        }
         const u64 accept = __ballot(mine && acc);
         const u64 reject = __ballot(mine && !acc);
         if (mine && acc && !bad) {
