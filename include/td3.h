@@ -472,6 +472,27 @@ int td3_system_bounds(const double *positions, int64_t n, int64_t systems,
                       const int64_t *system_start, double *roots, uint32_t *grown, void *stream);
 
 /*
+ * Close-pair softening of line charges (the grounded strip's correction in the physical
+ * plane, strip.py:168-193 and :288-323; replaces the host k-d tree pair search plus
+ * kernels/soften_pairs/soften_pairs_cuda.cu:9-42).  For every target t of targets (2, M),
+ * every source s of the tree (sorted_pos (2, N), sorted_charge, perm, node arrays of
+ * td3_tree_build) with |t - s| <= radius and perm index != exclude[t] adds the softened
+ * minus bare line-charge kernel: phi[t] += -q ln((r^2 + eps^2) / r^2) (when
+ * with_potential) and field[:, t] += 2 q d (1 / (r^2 + eps^2) - 1 / r^2), d = t - s
+ * (accumulated, by target index).  A zero separation reports 1 + t in status.  order (M,
+ * may be NULL) is the order the targets are walked in; exclude (M,) may be NULL.
+ * workspace: td3_soften_near_workspace_bytes.  No read-back.
+ */
+size_t td3_soften_near_workspace_bytes(int64_t node_count);
+int td3_soften_near(const double *targets, int64_t m, const int64_t *order,
+                    const int64_t *exclude, const double *sorted_pos, const double *sorted_charge,
+                    const int64_t *perm, int64_t n, const double *node_center,
+                    const double *node_half, const void *node_info, int64_t node_count,
+                    double radius, double softening, int with_potential, double *phi,
+                    double *field, int64_t *status, void *workspace, size_t workspace_bytes,
+                    void *stream);
+
+/*
  * Energies of an ensemble's systems (simulation.py:837-916 per system: kinetic
  * sum 1/2 m |v|^2 and potential sum 1/2 q phi), in FP64: out[s] the kinetic and
  * out[systems + s] the potential energy of system s (particles system_start[s] ..

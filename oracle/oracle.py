@@ -712,3 +712,120 @@ def push(positions, velocities, forces, electric, masses, charges, fixed, dt, b_
         velocities[:, mobile] = np.stack((c * vx + s * vy, c * vy - s * vx))
     else:
         forces[:] = electric + charges * b_z * np.stack((velocities[1], -velocities[0]))
+
+
+# ------------------------------------------------------------------ the grounded strip
+def soften_pairs(pos, charge, pairs, eps2, with_potential, phi, field):
+    """kernels/soften_pairs/soften_pairs_pyccel.py:14-60, in place: for each listed pair the
+    softened minus bare line-charge kernel added to both partners; raises on a zero
+    separation."""
+    for i, j in pairs:
+        dx = pos[0, i] - pos[0, j]
+        dy = pos[1, i] - pos[1, j]
+        r2 = dx * dx + dy * dy
+        if r2 == 0.0:
+            raise ValueError("Coincident charges")
+        soft = r2 + eps2
+        if with_potential:
+            dphi = -np.log(soft / r2)
+            phi[i] += charge[j] * dphi
+            phi[j] += charge[i] * dphi
+        scale = 2.0 / soft - 2.0 / r2
+        field[0, i] += charge[j] * scale * dx
+        field[1, i] += charge[j] * scale * dy
+        field[0, j] -= charge[i] * scale * dx
+        field[1, j] -= charge[i] * scale * dy
+
+
+def _strip_map(width, positions):
+    """strip.py:136-141: zeta = exp(i pi z / L), charges strictly between the plates."""
+    xy = np.asarray(positions, dtype=float).reshape(2, -1)
+    if np.any((xy[0] <= 0) | (xy[0] >= width)):
+        raise ValueError("Charges must lie strictly between the plates")
+    return np.exp(1j * np.pi * (xy[0] + 1j * xy[1]) / width)
+
+
+def _strip_root(xy):
+    """strip.py:153-158."""
+    lo, hi = xy.min(axis=1), xy.max(axis=1)
+    centre = (lo + hi) / 2
+    half = float(np.max(hi - lo) / 2) * (1 + 1e-9) + 1e-300
+    return (float(centre[0]), float(centre[1]), half)
+
+
+def strip_field_at_charges(width, positions, charges, *, order=14, theta=0.5, leaf_capacity=32,
+                           k_e=8.9875517923e9, method="fmm", softening=0.0, cutoff=6.0,
+                           potential=True):
+    """strip.py:185-233 (GroundedStripField.field_at_charges): the charges and their images in
+    the mapped plane, one FMM (or direct sum), mapped back with each charge's regular
+    self term, then the close pairs softened in the physical plane (k-d tree pairs within
+    cutoff * softening, strip.py:168-180)."""
+    from scipy.spatial import cKDTree
+
+    zeta = _strip_map(width, positions)
+    q0 = np.asarray(charges, dtype=float).ravel()
+    n = len(zeta)
+    if not n:
+        return np.zeros(0), np.zeros((2, 0))
+    points = np.concatenate((zeta, np.conj(zeta)))
+    xy = np.ascontiguousarray(np.stack((points.real, points.imag)))
+    q = np.concatenate((q0, -q0))
+    if method == "fmm":
+        tree = OracleTree(xy, q, _strip_root(xy), kernel="log", order=order,
+                          leaf_capacity=leaf_capacity)
+        phi, raw = tree.fmm(theta=theta, potential=potential)
+    else:
+        phi, raw = direct_field(xy, xy, q, exclude=np.arange(2 * n), kernel="log")
+    mapped = raw[0, :n] - 1j * raw[1, :n]
+    complex_field = mapped * (1j * np.pi / width) * zeta
+    complex_field += 2 * q0 * (1j * np.pi / (2 * width))
+    field = np.stack((complex_field.real, -complex_field.imag), axis=0)
+    if potential:
+        phi_own = phi[:n] - 2 * q0 * np.log(np.pi * np.abs(zeta) / width)
+    else:
+        phi_own = np.zeros(n)
+    if softening > 0:
+        pos = np.ascontiguousarray(positions, dtype=float).reshape(2, -1)
+        pairs = cKDTree(pos.T).query_pairs(cutoff * softening, output_type="ndarray")
+        soften_pairs(pos, q0, pairs, softening ** 2, potential, phi_own, field)
+    return k_e * phi_own, k_e * field
+
+
+def strip_probe(width, positions, charges, points, *, order=14, theta=0.5, leaf_capacity=32,
+                k_e=8.9875517923e9, method="fmm", softening=0.0, cutoff=6.0):
+    """strip.py:241-323 (GroundedStripField.probe): the walk in the mapped plane at the
+    mapped probes, mapped back; sources within cutoff * softening of a probe softened."""
+    from scipy.spatial import cKDTree
+
+    zeta = _strip_map(width, positions)
+    q0 = np.asarray(charges, dtype=float).ravel()
+    points_xy = np.asarray(points, dtype=float).reshape(2, -1)
+    probes = _strip_map(width, points_xy)
+    probe_xy = np.ascontiguousarray(np.stack((probes.real, probes.imag)))
+    src = np.concatenate((zeta, np.conj(zeta)))
+    xy = np.ascontiguousarray(np.stack((src.real, src.imag)))
+    q = np.concatenate((q0, -q0))
+    if method == "fmm":
+        root = _strip_root(np.concatenate((xy, probe_xy), axis=1))
+        tree = OracleTree(xy, q, root, kernel="log", order=order, leaf_capacity=leaf_capacity)
+        phi, raw = tree.evaluate(probe_xy, theta=theta)
+    else:
+        phi, raw = direct_field(probe_xy, xy, q, kernel="log")
+    mapped = raw[0] - 1j * raw[1]
+    complex_field = mapped * (1j * np.pi / width) * probes
+    field = np.stack((complex_field.real, -complex_field.imag), axis=0)
+    if softening > 0:
+        sources = np.asarray(positions, dtype=float).reshape(2, -1).T
+        targets = points_xy.T
+        near = cKDTree(sources).query_ball_point(targets, cutoff * softening,
+                                                 return_sorted=False)
+        eps2 = softening ** 2
+        for t, lst in enumerate(near):
+            for s in lst:
+                d = targets[t] - sources[s]
+                r2 = float(d @ d)
+                if r2 == 0:
+                    raise ValueError("A probe coincides with a charge")
+                phi[t] -= q0[s] * np.log((r2 + eps2) / r2)
+                field[:, t] += q0[s] * (2 / (r2 + eps2) - 2 / r2) * d
+    return k_e * phi, k_e * field

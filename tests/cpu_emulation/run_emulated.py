@@ -107,6 +107,26 @@ def case_direct_noalloc():
                                   None, None, None, None) == _abi.TD3_ERR_CAPACITY
 
 
+def case_strip():
+    # the close-pair softening walk (td3_soften_near) at the charges and at probes
+    rng = np.random.default_rng(8)
+    pos = np.stack((rng.uniform(0.05, 0.95, 300), rng.uniform(-1, 1, 300)))
+    pos[:, 1::10] = pos[:, 0::10] + 0.002
+    q = rng.normal(size=300)
+    probes = np.stack((rng.uniform(0.05, 0.95, 40), rng.uniform(-1, 1, 40)))
+    probes[:, :10] = pos[:, :10] + 0.003
+    for method in ("direct", "fmm"):
+        strip = td3.GroundedStripField(1.0, k_e=1.0, method=method, softening=0.01, order=10)
+        got = strip.field_at_charges(pos, q)
+        want = oracle.strip_field_at_charges(1.0, pos, q, k_e=1.0, method=method,
+                                             softening=0.01, order=10)
+        assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
+        got = strip.probe(pos, q, probes)
+        want = oracle.strip_probe(1.0, pos, q, probes, k_e=1.0, method=method, softening=0.01,
+                                  order=10)
+        assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
+
+
 def case_tree():
     rng = np.random.default_rng(5)
     angle = rng.uniform(0, 2 * np.pi, 300)

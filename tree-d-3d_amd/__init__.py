@@ -13,6 +13,7 @@ from .simulation import ChargedBody, ElectrostaticSimulation
 from .adjoint import log_energy, log_fields, log_potentials
 from .ensemble import Ensemble
 from .repeated import RepeatedSums
+from .strip import GroundedStripField
 
 __all__ = ["K_E", "MAX_ORDER", "ChargedBody", "Ensemble", "ElectrostaticSimulation", "FlatNode",
-           "FlatQuadtree", "Quad", "RepeatedSums", "direct_field", "log_energy", "log_fields", "log_potentials"]
+           "FlatQuadtree", "GroundedStripField", "Quad", "RepeatedSums", "direct_field", "log_energy", "log_fields", "log_potentials"]

@@ -87,6 +87,9 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                  _C),
     "td3_push": ([_C, _I, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P], _C),
     "td3_system_bounds": ([_P, _I, _I, _P, _P, _P, _P], _C),
+    "td3_soften_near_workspace_bytes": ([_I], _Z),
+    "td3_soften_near": ([_P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _I, _D, _D, _C, _P, _P, _P,
+                         _P, _Z, _P], _C),
     "td3_system_energies": ([_P, _P, _P, _P, _I, _I, _P, _P, _P], _C),
     "td3_timing_enable": ([_C], _C),
     "td3_timing_get": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
