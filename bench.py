@@ -347,11 +347,50 @@ def extra_repeated(td3, n=200_000, products=10):
             "tree_builds": sums.tree_builds, "plan_builds": sums.plan_builds}
 
 
+def extra_smooth(td3, n=262_144, steps=5):
+    """The smooth-membership FMM (smooth.py:242-428) at config 5's size: static cells of
+    16 points per leaf, order 12, theta 0.5, compact softening 1e-3, charges +-1; the step
+    is SmoothFMM(...) + .fields() + .energy on the fixed cells, with hard membership and
+    with a band of half the smallest leaf; errors against the FP64 direct sum."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    pos = torch.tensor(np.concatenate((rng.normal(0.0, 0.25, (2, n // 2)).clip(-0.99, 0.99),
+                                       rng.uniform(-0.99, 0.99, (2, n - n // 2))), axis=1),
+                       device="cuda")
+    q = torch.tensor(rng.choice([-1.0, 1.0], n), device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cells = td3.smooth.BoxTree(pos, td3.Quad(0.0, 0.0, 1.0), leaf_capacity=16)
+    torch.cuda.synchronize()
+    build = time.perf_counter() - t0
+    ref_phi, ref_field = td3.smooth.direct_fields(pos, q, 1e-3)
+    out = {"workload": f"SmoothFMM N={n}, leaf 16, order 12, theta 0.5, softening 1e-3 "
+                       "(memberships + moments + rows + M2L + L2L + eval + combine)",
+           "cells_ms": 1e3 * build, "cells": cells.node_count}
+    for label, frac in (("hard", 0.0), ("band_half_leaf", 0.5)):
+        kw = dict(order=12, theta=0.5, band=frac * cells.min_leaf_half, softening=1e-3)
+        f = td3.smooth.SmoothFMM(cells, pos, q, **kw)
+        f.fields()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f = td3.smooth.SmoothFMM(cells, pos, q, **kw)
+            _, field = f.fields()
+            f.energy
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        err = float((field - ref_field).abs().max() / ref_field.abs().max())
+        out[label] = {"ms": ms, "targets_per_s": n / (ms / 1e3),
+                      "memberships": f.membership_count, "field_err_vs_direct": err}
+    return out
+
+
 def extras(td3):
     out = {}
     for name, fn in (("config1", extra_config1), ("config3", extra_config3),
                      ("config4", extra_config4), ("config5", extra_config5),
-                     ("repeated", extra_repeated)):
+                     ("repeated", extra_repeated), ("smooth", extra_smooth)):
         try:
             out[name] = fn(td3)
         except Exception as exc:  # reported, not raised: the metric above stands alone
@@ -444,7 +483,8 @@ def main() -> int:
                          "--pmc runs: the copy calibrates FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other configurations (ensemble, fmm_at, adjoint, walk)")
-    ap.add_argument("--extra", choices=["config1", "config3", "config4", "config5", "repeated"],
+    ap.add_argument("--extra", choices=["config1", "config3", "config4", "config5", "repeated",
+                                        "smooth"],
                     help="run only this other configuration and print its JSON (for "
                          "profiling it); the metric line is not printed")
     args = ap.parse_args()
@@ -470,7 +510,8 @@ def main() -> int:
             pmc_copies()
             return 0
         fn = {"config1": extra_config1, "config3": extra_config3, "config4": extra_config4,
-              "config5": extra_config5, "repeated": extra_repeated}[args.extra]
+              "config5": extra_config5, "repeated": extra_repeated,
+              "smooth": extra_smooth}[args.extra]
         print(json.dumps({args.extra: fn(td3)}), flush=True)
         return 0
 

@@ -506,6 +506,105 @@ int td3_system_energies(const double *masses, const double *velocities, const do
                         const int64_t *system_start, double *out, void *stream);
 
 /*
+ * Static square cells of the smooth-membership FMM (smooth.py:83-121, BoxTree): the cells
+ * td3_tree_build makes (same split rule, children, breadth-first numbering; the reference's
+ * BoxTree passes max_depth 40), except that a point with y == cy goes to the lower
+ * children (smooth.py:109-110; td3_tree_build sends it to the upper ones,
+ * quadtree_quadrants_pyccel.py:38-41).  Same arguments, outputs and workspace as
+ * td3_tree_build.
+ */
+int td3_box_tree_build(const double *pos, int64_t n, double xmid, double ymid, double half,
+                       int64_t leaf_capacity, int64_t max_depth, int64_t capacity,
+                       double *node_center, double *node_half, int64_t *node_start,
+                       int64_t *node_end, int64_t *node_child, int64_t *node_depth,
+                       int64_t *levels, int64_t *perm, double *sorted_pos, void *node_info,
+                       void *workspace, size_t workspace_bytes, int64_t *out_sizes,
+                       int64_t *levels_host, void *stream);
+
+/*
+ * Depth-first order of the leaves of a cell set (node_info of td3_tree_build, levels_host
+ * its level starts): leaf_span[c] (C,) the leaves below cell c (1 for a leaf), leaf_rank[c]
+ * (C,) the depth-first rank of its first leaf, so the leaves below c are the ranks
+ * leaf_rank[c] .. leaf_rank[c] + leaf_span[c] - 1.  For td3_smooth_members.  No
+ * counterpart in the reference (its memberships are grouped by leaf with a dictionary,
+ * smooth.py:356-358).
+ */
+int td3_box_leaf_order(const void *node_info, int64_t node_count, const int64_t *levels_host,
+                       int64_t n_levels, uint32_t *leaf_rank, uint32_t *leaf_span, void *stream);
+
+/*
+ * Memberships of N particles in the leaves of a cell set (smooth.py:122-193), and their
+ * layout for the FMM (smooth.py:283-303).  band <= 0: the one leaf of the descent right
+ * when x > cx, down when y <= cy, weight 1.  band > 0 (at most the smallest leaf
+ * half-width; the caller checks): every leaf whose box grown by band strictly holds the
+ * particle, weight w = product of the quintic steps across its four edges (edges on the
+ * root (root_x, root_y, root_half) keep 1), kept when w > 0 - the reference's operations
+ * in its order, so sets and weights agree bit for bit.  Outputs, with S = 4 slots per
+ * particle: slot_leaf (N, S) int32 (-1: empty), slot_weight (3, N, S) (w, dw/dx, dw/dy),
+ * slot_member (N, S) int32 (the slot's sorted membership, -1: empty); and per sorted
+ * membership (capacity 4N, sorted by the depth-first rank of its leaf): member_pos (2, 4N)
+ * (stride 4N), member_charge (4N,) q w, member_src (4N, 4) (x, y, q w, particle index as a
+ * double), member_leaf (4N,) int32; member_info (C, 4) int32 {start, end, child, depth}:
+ * each cell's range of sorted memberships (a cell is occupied when it is not empty), for
+ * td3_moments (kernel TD3_LOG, n = 4N) and td3_rows; member_count (DEVICE int64[1]) the
+ * number of memberships.  leaf_rank / leaf_span from td3_box_leaf_order, leaf_count the
+ * number of leaves.  status (DEVICE int64[1], zeroed by the caller) gets 1 + a particle
+ * that met more than four grown cells on one level.  No read-back.
+ */
+size_t td3_smooth_members_workspace_bytes(int64_t n, int64_t leaf_count);
+int td3_smooth_members(const double *pos, const double *charge, int64_t n,
+                       const double *node_center, const double *node_half, const void *node_info,
+                       int64_t node_count, const uint32_t *leaf_rank, const uint32_t *leaf_span,
+                       int64_t leaf_count, double root_x, double root_y, double root_half,
+                       double band, int32_t *slot_leaf, double *slot_weight, int32_t *slot_member,
+                       double *member_pos, double *member_charge, double *member_src,
+                       int32_t *member_leaf, void *member_info, int64_t *member_count,
+                       int64_t *status, void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * Per-membership potential Phi_m and gradient G_m (smooth.py:344-379): the L2P of the
+ * leaf's scaled local expansion (td3_l2l over td3_m2l of the td3_moments of the
+ * memberships) plus the compact-softened kernel over the memberships of the leaf's P2P row
+ * (td3_rows on member_info) that belong to other particles.  phi_m (4N,), g_m (2, 4N)
+ * (stride cap = 4N), by sorted membership.  status: 1 + a particle that met another at
+ * distance 0 with softening 0.  No read-back.
+ */
+int td3_smooth_eval(int64_t order, const double *member_src, const int32_t *member_leaf,
+                    int64_t cap, const int64_t *member_count, const double *node_center,
+                    const double *node_half, const void *member_info, const double *local_re,
+                    const double *local_im, const uint32_t *p2p_row, const int32_t *p2p_col,
+                    double softening, double *phi_m, double *g_m, int64_t *status,
+                    void *stream);
+
+/*
+ * Per-particle potential phi = -2 sum w Phi_m, field E = 2 sum (w G_m + Phi_m grad w) over
+ * its slots (smooth.py:415-422) in input order (either may be NULL), and the energy
+ * U = -sum q w Phi_m (smooth.py:410-413) into energy (DEVICE double[1], may be NULL),
+ * summed in a fixed order (workspace td3_smooth_combine_workspace_bytes).  No read-back.
+ */
+size_t td3_smooth_combine_workspace_bytes(int64_t n);
+int td3_smooth_combine(int64_t n, int64_t cap, const double *charge, const int32_t *slot_member,
+                       const double *slot_weight, const double *phi_m, const double *g_m,
+                       double *phi, double *field, double *energy, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
+/*
+ * Direct sums of the compact-softened line-charge kernel (smooth.py:196-201 direct_energy,
+ * :424-428 potential_at, :431-440 direct_fields): phi[t] = -2 sum q Phi(t - s) and
+ * field[:, t] = 2 sum q grad Phi(t - s) (either may be NULL), Phi = ln r outside the
+ * cloud of radius softening, ln a - 11/12 + (3u - 3u^2/2 + u^3/3)/2 inside (u = r^2/a^2).
+ * self_pairs: targets are the sources and source t is skipped for target t; then energy
+ * (DEVICE double[1], may be NULL) gets sum q phi / 2 in a fixed order (workspace
+ * td3_compact_direct_workspace_bytes(m)).  A zero separation without softening reports
+ * 1 + t in status.  No read-back.
+ */
+size_t td3_compact_direct_workspace_bytes(int64_t m);
+int td3_compact_direct(const double *targets, int64_t m, int self_pairs, const double *sources,
+                       const double *charge, int64_t n, double softening, double *phi,
+                       double *field, double *energy, int64_t *status, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
+/*
  * Kernel timing for measurement (no counterpart in the reference, whose profiling is
  * host-side regions): td3_timing_enable(1) clears the record and brackets every later
  * kernel launch with HIP events on its stream; td3_timing_get(name) synchronises and

@@ -127,6 +127,51 @@ def case_strip():
         assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
 
 
+def case_smooth():
+    # the smooth-membership FMM: cells with points on split lines, memberships and their
+    # layout, the FMM stages on it (both M2L paths: order 8 on the matrix-core path, 20 on
+    # the lane-per-pair path), the combine and the compact direct sums
+    from collections import namedtuple
+
+    from oracle import smooth_oracle as so
+
+    box = namedtuple("Quad", "xmid ymid half")(0.0, 0.0, 1.0)
+    rng = np.random.default_rng(11)
+    pos = rng.uniform(-0.9, 0.9, (2, 260))
+    pos[0, :15] = np.round(pos[0, :15] * 8) / 8  # on vertical cell edges
+    pos[1, 15:30] = np.round(pos[1, 15:30] * 8) / 8  # on horizontal ones
+    q = rng.choice([-1.0, 1.0], 260)
+    tree = td3.smooth.BoxTree(pos, box, leaf_capacity=8)
+    ref_tree = so.BoxTree(pos, box, leaf_capacity=8)
+    assert np.array_equal(tree.child, ref_tree.child)
+    assert np.array_equal(tree.center, ref_tree.center) and np.array_equal(tree.half, ref_tree.half)
+    assert np.array_equal(tree.parent, ref_tree.parent)
+    for band_frac, order, soft, sym in ((0.0, 8, 0.0, True), (0.4, 8, 0.02, False),
+                                        (0.9, 20, 0.05, True)):
+        band = band_frac * tree.min_leaf_half
+        f = td3.smooth.SmoothFMM(tree, pos, q, order=order, theta=0.6, band=band,
+                                 softening=soft, symmetric=sym)
+        r = so.SmoothFMM(ref_tree, pos, q, order=order, theta=0.6, band=band, softening=soft,
+                         symmetric=sym)
+        got = sorted(zip(f.part.tolist(), f.leaf.tolist(), f.w.tolist()))
+        want = sorted(zip(r.part.tolist(), r.leaf.tolist(), r.w.tolist()))
+        assert got == want  # same sets, same weights bit for bit
+        phi, field = f.fields()
+        rphi, rfield = r.fields()
+        assert relative(phi, rphi) < 1e-12 and relative(field, rfield) < 1e-12
+        assert abs(f.energy - r.energy) < 1e-12 * np.abs(q).sum() ** 2
+    probes = rng.uniform(-1, 1, (2, 25))
+    for soft in (0.0, 0.05):
+        phi, field = td3.smooth.direct_fields(pos, q, soft)
+        rphi, rfield = so.direct_fields(pos, q, soft)
+        assert relative(phi, rphi) < 1e-13 and relative(field, rfield) < 1e-13
+        u = td3.smooth.direct_energy(pos, q, soft)
+        assert abs(u - so.direct_energy(pos, q, soft)) < 1e-12 * abs(u)
+        f = td3.smooth.SmoothFMM(tree, pos, q, order=6, softening=soft)
+        assert relative(f.potential_at(probes),
+                        so.SmoothFMM(ref_tree, pos, q, order=6, softening=soft).potential_at(probes)) < 1e-13
+
+
 def case_tree():
     rng = np.random.default_rng(5)
     angle = rng.uniform(0, 2 * np.pi, 300)

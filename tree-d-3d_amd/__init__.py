@@ -2,7 +2,9 @@
 quadtree, Barnes-Hut walk and FMM) as hand-written gfx950 HIP kernels behind a C ABI.
 
 The Python surface mirrors the reference's ``tree_d_3d`` (flat_tree.py, quad.py,
-constants.py): ``direct_field``, ``FlatQuadtree``, ``FlatNode``, ``Quad``, ``K_E``.
+constants.py, smooth.py): ``direct_field``, ``FlatQuadtree``, ``FlatNode``, ``Quad``,
+``K_E``, and the module ``smooth`` (``BoxTree``, ``SmoothFMM``, ``direct_energy``,
+``direct_fields``, ``comoving_fields``).
 Import it as ``tree_d_3d_amd`` (see ``__graft_entry__.load_package``).
 """
 
@@ -14,6 +16,8 @@ from .adjoint import log_energy, log_fields, log_potentials
 from .ensemble import Ensemble
 from .repeated import RepeatedSums
 from .strip import GroundedStripField
+from . import smooth
 
 __all__ = ["K_E", "MAX_ORDER", "ChargedBody", "Ensemble", "ElectrostaticSimulation", "FlatNode",
-           "FlatQuadtree", "GroundedStripField", "Quad", "RepeatedSums", "direct_field", "log_energy", "log_fields", "log_potentials"]
+           "FlatQuadtree", "GroundedStripField", "Quad", "RepeatedSums", "direct_field", "log_energy", "log_fields", "log_potentials",
+           "smooth"]

@@ -91,6 +91,17 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_soften_near": ([_P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _I, _D, _D, _C, _P, _P, _P,
                          _P, _Z, _P], _C),
     "td3_system_energies": ([_P, _P, _P, _P, _I, _I, _P, _P, _P], _C),
+    "td3_box_tree_build": ([_P, _I, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                            _P, _P, _Z, _P, _P, _P], _C),
+    "td3_box_leaf_order": ([_P, _I, _P, _I, _P, _P, _P], _C),
+    "td3_smooth_members_workspace_bytes": ([_I, _I], _Z),
+    "td3_smooth_members": ([_P, _P, _I, _P, _P, _P, _I, _P, _P, _I, _D, _D, _D, _D, _P, _P, _P,
+                            _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P], _C),
+    "td3_smooth_eval": ([_I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P], _C),
+    "td3_smooth_combine_workspace_bytes": ([_I], _Z),
+    "td3_smooth_combine": ([_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P], _C),
+    "td3_compact_direct_workspace_bytes": ([_I], _Z),
+    "td3_compact_direct": ([_P, _I, _C, _P, _P, _I, _D, _P, _P, _P, _P, _P, _Z, _P], _C),
     "td3_timing_enable": ([_C], _C),
     "td3_timing_get": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                         ctypes.POINTER(ctypes.c_int64)], _C),

@@ -65,9 +65,12 @@ __device__ __forceinline__ int system_of(u64 hi, Keys kl) {
 }
 
 // Root (centre, half) of a system: roots (S, 3) when given, else the single root.
+// y_down: a point with y == cy goes to the lower children (the static cells of
+// smooth.py:109-110) instead of the upper ones (quadtree_quadrants_pyccel.py:38-41).
 struct Root {
     const double* roots;
     double cx, cy, h;
+    int y_down;
 };
 
 __device__ __forceinline__ void root_of(Root rt, int sys, double* cx, double* cy, double* h) {
@@ -94,7 +97,7 @@ __global__ void keys_kernel(const double* __restrict__ pos, i64 n, Root rt,
     root_of(rt, sys, &cx, &cy, &h);
     u64 hi = kl.sbits ? (u64)sys << (64 - kl.sbits) : 0ull, lo = 0;
     for (int level = 1; level <= nlev; ++level) {
-        const int q = (x > cx ? 1 : 0) + (y < cy ? 2 : 0);
+        const int q = (x > cx ? 1 : 0) + ((y < cy || (rt.y_down && y == cy)) ? 2 : 0);
         h = 0.5 * h;
         cx = (q & 1) ? cx + h : cx - h;
         cy = (q & 2) ? cy - h : cy + h;
@@ -690,6 +693,7 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     rt.cx = xmid;
     rt.cy = ymid;
     rt.h = half;
+    rt.y_down = 0;
     return build_impl(pos, n, 1, rt, nullptr, nullptr, leaf_capacity, max_depth, capacity,
                       node_center, node_half, node_start, node_end, node_child, node_depth, levels,
                       perm, sorted_pos, node_info, workspace, workspace_bytes, out_sizes,
@@ -711,6 +715,7 @@ extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
     Root rt;
     rt.roots = roots;
     rt.cx = rt.cy = rt.h = 0.0;
+    rt.y_down = 0;
     return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
                       capacity, node_center, node_half, node_start, node_end, node_child,
                       node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
@@ -734,8 +739,29 @@ extern "C" int td3_forest_build_nosync(const double* pos, int64_t n, int64_t sys
     Root rt;
     rt.roots = roots;
     rt.cx = rt.cy = rt.h = 0.0;
+    rt.y_down = 0;
     return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
                       capacity, node_center, node_half, node_start, node_end, node_child,
                       node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
                       nullptr, nullptr, sizes, key_levels, (hipStream_t)stream);
+}
+
+extern "C" int td3_box_tree_build(const double* pos, int64_t n, double xmid, double ymid,
+                                  double half, int64_t leaf_capacity, int64_t max_depth,
+                                  int64_t capacity, double* node_center, double* node_half,
+                                  int64_t* node_start, int64_t* node_end, int64_t* node_child,
+                                  int64_t* node_depth, int64_t* levels, int64_t* perm,
+                                  double* sorted_pos, void* node_info, void* workspace,
+                                  size_t workspace_bytes, int64_t* out_sizes,
+                                  int64_t* levels_host, void* stream) {
+    Root rt;
+    rt.roots = nullptr;
+    rt.cx = xmid;
+    rt.cy = ymid;
+    rt.h = half;
+    rt.y_down = 1;
+    return build_impl(pos, n, 1, rt, nullptr, nullptr, leaf_capacity, max_depth, capacity,
+                      node_center, node_half, node_start, node_end, node_child, node_depth, levels,
+                      perm, sorted_pos, node_info, workspace, workspace_bytes, out_sizes,
+                      levels_host, nullptr, 0, (hipStream_t)stream);
 }
