@@ -59,6 +59,35 @@ def test_lists_equal_reference_traversal(td3, oracle, leaf):
     np.testing.assert_array_equal(directed(prow, pcol), as_directed(p2p, True))
 
 
+def test_lists_of_big_leaves_beside_a_dense_cluster(td3, oracle):
+    """Big leaves beside a dense cluster: a leaf's walk reaches thousands of the cluster's
+    cells, more than the breadth-first rows kernel queues, so the rows are built again
+    depth first (more rows_level launches than levels); the pair sets stay the
+    reference's, and so do the fields."""
+    from tree_d_3d_amd import _abi
+
+    rng = np.random.default_rng(3)
+    sparse = rng.uniform(-1, 1, (2, 60))
+    cluster = np.array([[0.26], [0.26]]) + rng.normal(0, 0.004, (2, 6000))
+    pos = np.concatenate((sparse, cluster), axis=1).clip(-1, 1)
+    q = rng.normal(size=pos.shape[1])
+    print("lists: big leaves beside a cluster", flush=True)
+    _abi.timing_enable(True)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", leaf_capacity=8)
+    (mrow, mcol, _), (prow, pcol, _), counts = tree._lists(0.5)
+    _, launches = _abi.timing_get("rows_level")
+    _abi.timing_enable(False)
+    assert launches > len(tree.levels) - 1
+    ref = oracle.OracleTree(pos, q, (0, 0, 1), kernel="log", leaf_capacity=8)
+    m2l, p2p = ref.traverse(0.5)
+    assert counts == (len(m2l), len(p2p))
+    np.testing.assert_array_equal(directed(mrow, mcol), as_directed(m2l, False))
+    np.testing.assert_array_equal(directed(prow, pcol), as_directed(p2p, True))
+    phi, field = tree.fmm(theta=0.5, precision="fp64")
+    rphi, rfield = ref.fmm(theta=0.5)
+    assert relative(phi, rphi) < 1e-12 and relative(field, rfield) < 1e-12
+
+
 @pytest.mark.parametrize("order", [2, 6, 12, 18])
 def test_fmm_fp64_matches_oracle(td3, oracle, order):
     pos, q = cloud(3000)

@@ -39,6 +39,7 @@ constexpr u32 kSecond = 0x80000000u;
 constexpr int kWaves = 4;       // nodes (wavefronts) per block of the level kernel
 constexpr int kBuffered = 8;    // events per lane kept in registers during the walk
 constexpr int kTileNodes = 256; // nodes per tile of the row placement
+constexpr int kRing = 512;      // live queued cells per wavefront of the breadth-first kernel
 constexpr u32 kEvM2L = 1u << 29, kEvP2P = 2u << 29, kEvHand = 3u << 29, kEvKind = 3u << 29;
 
 struct RowsTree {
@@ -281,6 +282,116 @@ __global__ __launch_bounds__(64 * kWaves) void rows_level_kernel(RowsTree t, i64
     }
 }
 
+// The nodes [lo, hi) of one level, a wavefront each, breadth first: every round the 64
+// lanes take the next 64 items (the node's entries in order, then the cells queued by
+// earlier rounds), decide them together (one memory latency per round instead of one per
+// visited cell of a lane's depth-first walk), write their events at offsets from ballots,
+// and queue the four children of every split cell behind the rest (a ring in LDS).  Rows
+// come in round order, the same on every run.  A queue that would outgrow the ring sets
+// bit 2 of flag[0]; td3_rows then builds the rows again with rows_level_kernel.
+__global__ __launch_bounds__(64 * kWaves) void rows_level_bfs_kernel(RowsTree t, i64 lo, i64 hi,
+                                                                      double theta, Slots sl,
+                                                                      u32* __restrict__ flag) {
+    __shared__ u32 s_ring[kWaves * kRing];
+    const int lane = td3_lane();
+    const int wave = threadIdx.x / 64;
+    const i64 node = lo + (i64)blockIdx.x * kWaves + wave;
+    if (node >= hi) return;  // the whole wavefront (no block barrier below)
+    u32* ring = s_ring + wave * kRing;
+    const int a = (int)node;
+    const size_t slot = (size_t)a * sl.width;
+    const int4 fa = t.info[a];
+    const double4 ga = t.geo[a];
+    const u64 below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    u32 total_m = 0, total_p = 0, total_h = 0;
+    bool full = false;
+    if (fa.y > fa.x) {
+        u32 entries = 1, inh_n = 0;
+        int first = 0, k = 0;
+        const u32* hand = nullptr;
+        if (a != 0) {
+            const int p = t.parent[a];
+            k = (a - t.sib0) & 3;
+            first = a - k;
+            inh_n = sl.n_inh[p];
+            inh_n = inh_n < sl.width ? inh_n : sl.width;
+            hand = sl.inh + (size_t)p * sl.width;
+            entries = inh_n + 4;
+        }
+        u32 taken = 0, head = 0, tail = 0;  // entries taken; the ring's live items
+        while (taken < entries || head < tail) {
+            u32 n, item = 0u;
+            if (taken < entries) {
+                n = entries - taken < 64u ? entries - taken : 64u;
+                if ((u32)lane < n) {
+                    const u32 i = taken + lane;
+                    const int j = (int)(i - inh_n);
+                    item = a == 0 ? 0u : (i < inh_n ? hand[i] : entry_with(first + j, j < k));
+                }
+                taken += n;
+            } else {
+                n = tail - head < 64u ? tail - head : 64u;
+                if ((u32)lane < n) item = ring[(head + lane) & (kRing - 1)];
+                head += n;
+            }
+            const bool valid = (u32)lane < n;
+            const int x = (int)(item & ~kSecond);
+            const bool second = (item & kSecond) != 0;
+            int kind = 0, child = -1;
+            if (valid) {
+                if (x == a)
+                    kind = fa.z < 0 ? 2 : 0;  // a leaf's own P2P entry
+                else
+                    kind = decide(t, ga, fa, x, second, theta, &child);
+            }
+            const u64 bm = __ballot(kind == 1), bp = __ballot(kind == 2),
+                      bh = __ballot(kind == 3), bs = __ballot(kind == 4);
+            if (kind == 1) {
+                const u32 pos = total_m + (u32)__popcll(bm & below);
+                if (pos < sl.width) sl.m2l[slot + pos] = (u32)x;
+            } else if (kind == 2) {
+                const u32 pos = total_p + (u32)__popcll(bp & below);
+                if (pos < sl.width) sl.p2p[slot + pos] = (u32)x;
+            } else if (kind == 3) {
+                const u32 pos = total_h + (u32)__popcll(bh & below);
+                if (pos < sl.width) sl.inh[slot + pos] = entry_with(x, second);
+            }
+            total_m += (u32)__popcll(bm);
+            total_p += (u32)__popcll(bp);
+            total_h += (u32)__popcll(bh);
+            const u32 pushed = 4u * (u32)__popcll(bs);
+            if (pushed) {
+                if (tail - head + pushed > (u32)kRing) {
+                    full = true;
+                    break;
+                }
+                // every lane has read its item from the ring before any lane writes to it
+                __builtin_amdgcn_wave_barrier();
+                if (kind == 4) {
+                    const u32 pos = tail + 4u * (u32)__popcll(bs & below);
+                    for (int c = 0; c < 4; ++c)
+                        ring[(pos + c) & (kRing - 1)] = entry_with(child + c, second);
+                }
+                tail += pushed;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    if (lane == 0) {
+        sl.n_m2l[a] = total_m;
+        sl.n_p2p[a] = total_p;
+        sl.n_inh[a] = total_h;
+        u32 most = total_m > total_p ? total_m : total_p;
+        most = most > total_h ? most : total_h;
+        if (full) atomicOr(flag, 4u);
+        if (most > sl.width) {
+            atomicOr(flag, 1u);
+            atomicMax(flag + 1, most);
+        }
+    }
+}
+
 // Sum of a u32 over the block (kTileNodes threads); every thread gets it.
 __device__ __forceinline__ u32 block_sum_u32(u32 v, u32* s_wave) {
     u32 total;
@@ -390,17 +501,23 @@ __global__ __launch_bounds__(kTileNodes) void rows_place_kernel(
     pack_rows(s_start_p, tile_p, carry_p, tile0, count, sl.p2p, sl.width, cap_p2p, p2p_col, nullptr);
 }
 
-// Parent links, and the count of non-empty leaves (each holds one self pair in P2P).
+// Parent links, and the count of non-empty leaves (each holds one self pair in P2P),
+// counted by a ballot per wavefront and one add per wavefront (adds to one address
+// serialise).
 __global__ void rows_parent_kernel(const int4* __restrict__ info, i64 count, int* __restrict__ parent,
                                    u32* __restrict__ leaves) {
     const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= count) return;
-    if (node == 0) parent[0] = -1;
-    const int4 f = info[node];
-    if (f.z >= 0)
-        for (int k = 0; k < 4; ++k) parent[f.z + k] = (int)node;
-    else if (f.y > f.x)
-        atomicAdd(leaves, 1u);
+    bool full_leaf = false;
+    if (node < count) {
+        if (node == 0) parent[0] = -1;
+        const int4 f = info[node];
+        if (f.z >= 0)
+            for (int k = 0; k < 4; ++k) parent[f.z + k] = (int)node;
+        else
+            full_leaf = f.y > f.x;
+    }
+    const u64 m = __ballot(full_leaf);
+    if (td3_lane() == 0 && m) atomicAdd(leaves, (u32)__popcll(m));
 }
 
 __global__ void rows_geo_kernel(const double* __restrict__ center, const double* __restrict__ half,
@@ -497,6 +614,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     hipLaunchKernelGGL(rows_geo_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, node_center,
                        node_half, count, w.geo);
     TD3_LAUNCHED();
+  for (int depth_first = 0; depth_first < 2; ++depth_first) {
     TD3_HIP(hipMemsetAsync(w.flag, 0, sizeof(u32) * (4 + 2 * (size_t)w.tiles), s));
     hipLaunchKernelGGL(rows_parent_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s,
                        (const int4*)node_info, count, w.parent, w.flag + 2);
@@ -504,9 +622,16 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     for (int d = 0; d < n_levels; ++d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
         if (hi <= lo) continue;
-        TD3_TIMED(s, "rows_level",
-                  hipLaunchKernelGGL(rows_level_kernel, dim3(td3_blocks(hi - lo, kWaves)),
-                                     dim3(64 * kWaves), 0, s, t, lo, hi, theta, w.sl, w.flag));
+        if (depth_first)
+            TD3_TIMED(s, "rows_level",
+                      hipLaunchKernelGGL(rows_level_kernel, dim3(td3_blocks(hi - lo, kWaves)),
+                                         dim3(64 * kWaves), 0, s, t, lo, hi, theta, w.sl,
+                                         w.flag));
+        else
+            TD3_TIMED(s, "rows_level",
+                      hipLaunchKernelGGL(rows_level_bfs_kernel, dim3(td3_blocks(hi - lo, kWaves)),
+                                         dim3(64 * kWaves), 0, s, t, lo, hi, theta, w.sl,
+                                         w.flag));
         TD3_LAUNCHED();
     }
     TD3_TIMED(s, "rows_place",
@@ -534,6 +659,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     TD3_HIP(hipMemcpyAsync(host, w.flag, sizeof(u32) * 4, hipMemcpyDeviceToHost, s));
     TD3_HIP(hipMemcpyAsync(host + 4, w.totals, sizeof(u32) * 2, hipMemcpyDeviceToHost, s));
     TD3_HIP(hipStreamSynchronize(s));
+    if ((host[0] & 4u) && !depth_first) continue;  // a queue outgrew its ring: depth first
     out_counts[0] = host[4];
     out_counts[1] = host[5];
     out_counts[2] = host[2];
@@ -544,4 +670,6 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
         return TD3_ERR_CAPACITY;
     }
     return TD3_OK;
+  }
+  return TD3_OK;
 }
