@@ -91,8 +91,11 @@ int td3_direct_noalloc(int kernel, int precision, const double *targets, int64_t
  * {node_count, n_levels, needed capacity}; levels_host (HOST int64[67], may be NULL) the
  * same level starts as levels, so that the caller needs no read-back of its own.  Returns
  * TD3_ERR_CAPACITY when capacity <
- * node_count (call again with out_sizes[2]).  Synchronises the stream once (one
- * read-back of the split counts).  Supports N < 2^31 and trees of depth <= 64.
+ * node_count (call again with out_sizes[2]), and TD3_ERR_ARG with the reference's
+ * messages ("Particle position must be finite", "Particle lies outside tree bounds",
+ * flat_tree.py:785-794) for a position that is not finite or lies outside the closed root
+ * square.  Synchronises the stream once (one read-back of the split counts and of the
+ * positions' check).  Supports N < 2^31 and trees of depth <= 64.
  */
 size_t td3_tree_workspace_bytes(int64_t n, int64_t capacity);
 
@@ -292,8 +295,8 @@ int td3_scale_moments(int precision, const double *moment_re, const double *mome
  * row_start are td3_lists rows of the M2L pairs; coeff (order+1)^2 device table
  * (flat_tree.py:234-245, either truncation); scaled_re/im from td3_scale_moments of the
  * same precision; dst_* are the destination tree's cells (the rows), src_* the source
- * tree's (the columns; the same arrays for fmm, the target tree for fmm_at).  Rows of the
- * range with pairs are WRITTEN (not accumulated); other rows are untouched.  FP32 up to
+ * tree's (the columns; the same arrays for fmm, the target tree for fmm_at).  Every row of
+ * the range is WRITTEN (not accumulated): zeros for a row without pairs.  FP32 up to
  * order 15 runs on the matrix cores, a wavefront per row, and so does FP64 up to order 15
  * when workspace is NULL; otherwise a lane per pair, with the workspace
  * (td3_m2l_workspace_bytes of at least the range's pair count) holding the partial sums

@@ -130,8 +130,22 @@ def test_moments_match_oracle(td3, oracle, kernel):
 def test_validation_and_empty(td3):
     with pytest.raises(ValueError, match=r"\(2, N\)"):
         td3.FlatQuadtree(np.zeros((3, 2)), np.zeros(3), td3.Quad(0, 0, 1))
-    with pytest.raises(ValueError, match="outside"):
+    with pytest.raises(ValueError, match="^Particle lies outside tree bounds$"):
         td3.FlatQuadtree([[2.0], [0.0]], [1.0], td3.Quad(0, 0, 1))
+    # flat_tree.py:785-794: finite first, then the bounds; the closed square is inside
+    many = np.random.default_rng(0).uniform(-1, 1, (2, 5000))
+    for bad, message in ((np.nan, "finite"), (np.inf, "finite"), (1.0 + 1e-15, "outside"),
+                         (-1.5, "outside")):
+        pos = many.copy()
+        pos[1, 4321] = bad
+        with pytest.raises(ValueError, match=message):
+            td3.FlatQuadtree(pos, np.ones(5000), td3.Quad(0, 0, 1))
+    pos = many.copy()
+    pos[:, :4] = [[-1.0, 1.0, -1.0, 1.0], [-1.0, -1.0, 1.0, 1.0]]  # the corners
+    assert td3.FlatQuadtree(pos, np.ones(5000), td3.Quad(0, 0, 1)).node_count > 1
+    pos[0, 7], pos[1, 9] = np.nan, 3.0
+    with pytest.raises(ValueError, match="must be finite"):
+        td3.FlatQuadtree(pos, np.ones(5000), td3.Quad(0, 0, 1))
     with pytest.raises(ValueError, match="kernel"):
         td3.FlatQuadtree([[0.0], [0.0]], [1.0], td3.Quad(0, 0, 1), kernel="yukawa")
     with pytest.raises(ValueError, match="order"):

@@ -235,11 +235,20 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
     const i64 d = node_lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
     if (d >= node_hi) return;  // uniform over the wavefront
     const u32 r0 = row_start[d], r1 = row_start[d + 1];
-    if (r0 == r1) return;
     const int j = lane & 15, q = lane >> 4;
     // Element r of lane (j, q) of the product holds D[row_of(r)][j]:
     // v_mfma_f32_16x16x4f32 gives row 4 q + r, v_mfma_f64_16x16x4f64 row q + 4 r.
     auto row_of = [q](int r) { return kFP64 ? q + 4 * r : 4 * q + r; };
+    if (r0 == r1) {
+        // a row without pairs is written too (zeros), so the caller need not clear it
+        if (j == 0)
+            for (int r = 0; r < 4; ++r)
+                if (row_of(r) < p1) {
+                    lre[d * p1 + row_of(r)] = 0.0;
+                    lim[d * p1 + row_of(r)] = 0.0;
+                }
+        return;
+    }
     T ca[4];
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
@@ -369,12 +378,16 @@ __global__ void m2l_fixup_kernel(const u32* __restrict__ row_start, i64 node_lo,
     if (node >= node_hi) return;
     const u32 e_lo = row_start[node_lo];
     const u32 rs0 = row_start[node], rs1 = row_start[node + 1];
-    if (rs1 == rs0) return;
+    const int l = j / 2;
+    if (rs1 == rs0) {
+        // a row without pairs is written too (zeros), so the caller need not clear it
+        (j & 1 ? lim : lre)[node * p1 + l] = 0.0;
+        return;
+    }
     const u32 wa = (rs0 - e_lo) / 64, wb = (rs1 - 1 - e_lo) / 64;
     if (wa == wb) return;
     double v = (double)side[((i64)wa * 2 + 1) * 2 * p1 + j];
     for (u32 w = wa + 1; w <= wb; ++w) v += (double)side[((i64)w * 2) * 2 * p1 + j];
-    const int l = j / 2;
     if (j & 1)
         lim[node * p1 + l] = v;
     else

@@ -101,6 +101,9 @@ struct Slots {
     u32* n_inh;
     u32* tile_m2l;  // per kTileNodes nodes
     u32* tile_p2p;
+    u32* tile_leaf;
+    u32* carry_m2l;  // per tile: its first row entry
+    u32* carry_p2p;
     u32 width;
 };
 
@@ -405,17 +408,6 @@ __device__ __forceinline__ u32 block_sum_u32(u32 v, u32* s_wave) {
 }
 
 // The M2L and P2P entries of every tile of kTileNodes nodes.
-__global__ __launch_bounds__(kTileNodes) void rows_tile_kernel(i64 count, Slots sl) {
-    __shared__ u32 s_wave[kTileNodes / 64];
-    const i64 node = (i64)blockIdx.x * kTileNodes + threadIdx.x;
-    const u32 m = block_sum_u32(node < count ? sl.n_m2l[node] : 0u, s_wave);
-    const u32 p = block_sum_u32(node < count ? sl.n_p2p[node] : 0u, s_wave);
-    if (threadIdx.x == 0) {
-        sl.tile_m2l[blockIdx.x] = m;
-        sl.tile_p2p[blockIdx.x] = p;
-    }
-}
-
 // Exclusive prefix over the block in thread order; *total gets the block's sum.
 __device__ __forceinline__ u32 block_excl_u32(u32 v, u32* s_wave, u32* total) {
     u32 wave_total;
@@ -431,6 +423,50 @@ __device__ __forceinline__ u32 block_excl_u32(u32 v, u32* s_wave, u32* total) {
     }
     *total = all;
     return before + in_wave;
+}
+
+// The M2L entries, P2P entries and non-empty leaves (each holds one self pair in P2P) of
+// every tile of kTileNodes nodes.
+__global__ __launch_bounds__(kTileNodes) void rows_tile_kernel(i64 count, Slots sl,
+                                                               const int4* __restrict__ info) {
+    __shared__ u32 s_wave[kTileNodes / 64];
+    const i64 node = (i64)blockIdx.x * kTileNodes + threadIdx.x;
+    bool full_leaf = false;
+    if (node < count) {
+        const int4 f = info[node];
+        full_leaf = f.z < 0 && f.y > f.x;
+    }
+    const u32 m = block_sum_u32(node < count ? sl.n_m2l[node] : 0u, s_wave);
+    const u32 p = block_sum_u32(node < count ? sl.n_p2p[node] : 0u, s_wave);
+    const u32 l = block_sum_u32(full_leaf ? 1u : 0u, s_wave);
+    if (threadIdx.x == 0) {
+        sl.tile_m2l[blockIdx.x] = m;
+        sl.tile_p2p[blockIdx.x] = p;
+        sl.tile_leaf[blockIdx.x] = l;
+    }
+}
+
+// The tiles' row starts (an exclusive scan of their counts, one block) and the non-empty
+// leaves of the tree (flag[2]).
+__global__ __launch_bounds__(kTileNodes) void rows_carry_kernel(i64 tiles, Slots sl,
+                                                                u32* __restrict__ flag) {
+    __shared__ u32 s_wave[kTileNodes / 64];
+    u32 run_m = 0, run_p = 0, leaves = 0;
+    for (i64 base = 0; base < tiles; base += kTileNodes) {
+        const i64 t = base + threadIdx.x;
+        const bool in = t < tiles;
+        u32 sum_m, sum_p;
+        const u32 em = block_excl_u32(in ? sl.tile_m2l[t] : 0u, s_wave, &sum_m);
+        const u32 ep = block_excl_u32(in ? sl.tile_p2p[t] : 0u, s_wave, &sum_p);
+        leaves += block_sum_u32(in ? sl.tile_leaf[t] : 0u, s_wave);
+        if (in) {
+            sl.carry_m2l[t] = run_m + em;
+            sl.carry_p2p[t] = run_p + ep;
+        }
+        run_m += sum_m;
+        run_p += sum_p;
+    }
+    if (threadIdx.x == 0) flag[2] = leaves;
 }
 
 // Packs the slots of the tile's nodes into their rows, which follow each other: output
@@ -469,13 +505,7 @@ __global__ __launch_bounds__(kTileNodes) void rows_place_kernel(
     __shared__ u32 s_start_m[kTileNodes], s_start_p[kTileNodes];
     const u32 tile = blockIdx.x;
     const i64 tile0 = (i64)tile * kTileNodes;
-    u32 cm = 0, cp = 0;
-    for (u32 t = threadIdx.x; t < tile; t += kTileNodes) {
-        cm += sl.tile_m2l[t];
-        cp += sl.tile_p2p[t];
-    }
-    const u32 carry_m = block_sum_u32(cm, s_wave);
-    const u32 carry_p = block_sum_u32(cp, s_wave);
+    const u32 carry_m = sl.carry_m2l[tile], carry_p = sl.carry_p2p[tile];
     const i64 node = tile0 + threadIdx.x;
     const u32 nm = node < count ? sl.n_m2l[node] : 0u;
     const u32 np = node < count ? sl.n_p2p[node] : 0u;
@@ -501,23 +531,14 @@ __global__ __launch_bounds__(kTileNodes) void rows_place_kernel(
     pack_rows(s_start_p, tile_p, carry_p, tile0, count, sl.p2p, sl.width, cap_p2p, p2p_col, nullptr);
 }
 
-// Parent links, and the count of non-empty leaves (each holds one self pair in P2P),
-// counted by a ballot per wavefront and one add per wavefront (adds to one address
-// serialise).
-__global__ void rows_parent_kernel(const int4* __restrict__ info, i64 count, int* __restrict__ parent,
-                                   u32* __restrict__ leaves) {
+// Parent links.
+__global__ void rows_parent_kernel(const int4* __restrict__ info, i64 count, int* __restrict__ parent) {
     const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    bool full_leaf = false;
-    if (node < count) {
-        if (node == 0) parent[0] = -1;
-        const int4 f = info[node];
-        if (f.z >= 0)
-            for (int k = 0; k < 4; ++k) parent[f.z + k] = (int)node;
-        else
-            full_leaf = f.y > f.x;
-    }
-    const u64 m = __ballot(full_leaf);
-    if (td3_lane() == 0 && m) atomicAdd(leaves, (u32)__popcll(m));
+    if (node >= count) return;
+    if (node == 0) parent[0] = -1;
+    const int4 f = info[node];
+    if (f.z >= 0)
+        for (int k = 0; k < 4; ++k) parent[f.z + k] = (int)node;
 }
 
 __global__ void rows_geo_kernel(const double* __restrict__ center, const double* __restrict__ half,
@@ -575,6 +596,9 @@ size_t rows_carve(RowsWs* w, void* p, i64 count, i64 width) {
     w->sl.tile_m2l = w->flag ? w->flag + 4 : nullptr;
     w->sl.tile_p2p = w->flag ? w->flag + 4 + w->tiles : nullptr;
     w->totals = (u32*)take(sizeof(u32) * 2);
+    w->sl.tile_leaf = (u32*)take(sizeof(u32) * (size_t)w->tiles);
+    w->sl.carry_m2l = (u32*)take(sizeof(u32) * (size_t)w->tiles);
+    w->sl.carry_p2p = (u32*)take(sizeof(u32) * (size_t)w->tiles);
     w->parent = (int*)take(sizeof(int) * (size_t)count);
     w->geo = (double4*)take(sizeof(double4) * (size_t)count);
     return off;
@@ -617,7 +641,7 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
   for (int depth_first = 0; depth_first < 2; ++depth_first) {
     TD3_HIP(hipMemsetAsync(w.flag, 0, sizeof(u32) * (4 + 2 * (size_t)w.tiles), s));
     hipLaunchKernelGGL(rows_parent_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s,
-                       (const int4*)node_info, count, w.parent, w.flag + 2);
+                       (const int4*)node_info, count, w.parent);
     TD3_LAUNCHED();
     for (int d = 0; d < n_levels; ++d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
@@ -636,7 +660,11 @@ extern "C" int td3_rows(const double* node_center, const double* node_half, cons
     }
     TD3_TIMED(s, "rows_place",
               hipLaunchKernelGGL(rows_tile_kernel, dim3((unsigned)w.tiles), dim3(kTileNodes), 0,
-                                 s, count, w.sl));
+                                 s, count, w.sl, (const int4*)node_info));
+    TD3_LAUNCHED();
+    TD3_TIMED(s, "rows_place",
+              hipLaunchKernelGGL(rows_carry_kernel, dim3(1), dim3(kTileNodes), 0, s, w.tiles,
+                                 w.sl, w.flag));
     TD3_LAUNCHED();
     TD3_TIMED(s, "rows_place",
               hipLaunchKernelGGL(rows_place_kernel, dim3((unsigned)w.tiles), dim3(kTileNodes), 0,

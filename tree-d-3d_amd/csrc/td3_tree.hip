@@ -67,10 +67,13 @@ __device__ __forceinline__ int system_of(u64 hi, Keys kl) {
 // Root (centre, half) of a system: roots (S, 3) when given, else the single root.
 // y_down: a point with y == cy goes to the lower children (the static cells of
 // smooth.py:109-110) instead of the upper ones (quadtree_quadrants_pyccel.py:38-41).
+// check: the keys kernel reports positions that are not finite (bit 0) or outside the
+// closed root square (bit 1), the reference's checks (flat_tree.py:785-794).
 struct Root {
     const double* roots;
     double cx, cy, h;
     int y_down;
+    int check;
 };
 
 __device__ __forceinline__ void root_of(Root rt, int sys, double* cx, double* cy, double* h) {
@@ -87,7 +90,8 @@ __device__ __forceinline__ void root_of(Root rt, int sys, double* cx, double* cy
 
 __global__ void keys_kernel(const double* __restrict__ pos, i64 n, Root rt,
                             const int* __restrict__ psys, Keys kl, int nlev,
-                            u64* __restrict__ key_hi, u64* __restrict__ key_lo) {
+                            u64* __restrict__ key_hi, u64* __restrict__ key_lo,
+                            u32* __restrict__ report) {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double x = pos[i];
@@ -95,6 +99,12 @@ __global__ void keys_kernel(const double* __restrict__ pos, i64 n, Root rt,
     const int sys = psys ? psys[i] : 0;
     double cx, cy, h;
     root_of(rt, sys, &cx, &cy, &h);
+    if (rt.check) {
+        if (!(x - x == 0.0) || !(y - y == 0.0))
+            atomicOr(report, 1u);
+        else if (x < cx - h || x > cx + h || y < cy - h || y > cy + h)
+            atomicOr(report, 2u);
+    }
     u64 hi = kl.sbits ? (u64)sys << (64 - kl.sbits) : 0ull, lo = 0;
     for (int level = 1; level <= nlev; ++level) {
         const int q = (x > cx ? 1 : 0) + ((y < cy || (rt.y_down && y == cy)) ? 2 : 0);
@@ -497,6 +507,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     const int max_split = (int)(max_depth - 1 < max_levels ? max_depth - 1 : max_levels);
     u32* depth_count = w.small;          // [65]
     u32* flag = w.small + 80;            // [1]
+    u32* report = w.small + 81;          // [1] the positions' check
     u32* bstart = w.small + 96;          // [66]
     u32* s_total = w.small + 176;        // [1]
     const unsigned nb = td3_blocks(n, 256);
@@ -512,20 +523,27 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
     }
     int nkey = 0;
     bool deep = false;
-    u32 host_small[81];
+    u32 host_small[82];
     for (int stage = 0; stage < nstage; ++stage) {
         nkey = stages[stage];
         deep = nkey > kl.l1;
+        if (nosync) {
+            // a fill kernel, not a memset: this build may be recorded into a HIP graph
+            hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(128), 0, s, w.small, 96);
+            TD3_LAUNCHED();
+        } else {
+            TD3_HIP(hipMemsetAsync(w.small, 0, sizeof(u32) * 96, s));
+        }
         if (!deep) {
             TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, rt, psys, kl, nkey,
-                               w.khi, (u64*)nullptr));
+                               w.khi, (u64*)nullptr, report));
             TD3_LAUNCHED();
             if (int rc = td3sort::sort_pairs(w.khi, w.idx, nullptr, w.khi_alt, w.idx_alt, nullptr, n,
                                              64 - kl.sbits - 2 * nkey, 64, true, w.scratch, s))
                 return rc;
         } else {
             TD3_TIMED(s, "tree_keys", hipLaunchKernelGGL(keys_kernel, dim3(nb), dim3(256), 0, s, pos, n, rt, psys, kl, nkey,
-                               w.khi, w.klo));
+                               w.khi, w.klo, report));
             TD3_LAUNCHED();
             if (int rc = td3sort::sort_pairs(w.klo, w.idx, w.khi, w.klo_alt, w.idx_alt, w.khi_alt, n,
                                              64 - 2 * (nkey - kl.l1), 64, true, w.scratch, s))
@@ -535,19 +553,20 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                 return rc;
         }
         // 3. splitting-node starts
-        if (nosync) {
-            // a fill kernel, not a memset: this build may be recorded into a HIP graph
-            hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(128), 0, s, w.small, 96);
-            TD3_LAUNCHED();
-        } else {
-            TD3_HIP(hipMemsetAsync(w.small, 0, sizeof(u32) * 96, s));
-        }
         TD3_TIMED(s, "tree_count", hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
                            (i64)leaf_capacity, max_split, nkey, kl, w.cnt, depth_count, flag));
         TD3_LAUNCHED();
         if (nosync) break;  // a set flag is read on the device (levels_nosync_kernel)
-        TD3_HIP(hipMemcpyAsync(host_small, w.small, sizeof(u32) * 81, hipMemcpyDeviceToHost, s));
+        TD3_HIP(hipMemcpyAsync(host_small, w.small, sizeof(u32) * 82, hipMemcpyDeviceToHost, s));
         TD3_HIP(hipStreamSynchronize(s));
+        if (host_small[81] & 1u) {
+            td3_set_error("Particle position must be finite");
+            return TD3_ERR_ARG;
+        }
+        if (host_small[81] & 2u) {
+            td3_set_error("Particle lies outside tree bounds");
+            return TD3_ERR_ARG;
+        }
         if (!host_small[80]) break;
         if (stage == nstage - 1) {
             td3_set_error("the tree needs more than %d levels (max_depth %lld)", max_levels,
@@ -694,6 +713,7 @@ extern "C" int td3_tree_build(const double* pos, int64_t n, double xmid, double 
     rt.cy = ymid;
     rt.h = half;
     rt.y_down = 0;
+    rt.check = 1;
     return build_impl(pos, n, 1, rt, nullptr, nullptr, leaf_capacity, max_depth, capacity,
                       node_center, node_half, node_start, node_end, node_child, node_depth, levels,
                       perm, sorted_pos, node_info, workspace, workspace_bytes, out_sizes,
@@ -716,6 +736,7 @@ extern "C" int td3_forest_build(const double* pos, int64_t n, int64_t systems,
     rt.roots = roots;
     rt.cx = rt.cy = rt.h = 0.0;
     rt.y_down = 0;
+    rt.check = 0;
     return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
                       capacity, node_center, node_half, node_start, node_end, node_child,
                       node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
@@ -740,6 +761,7 @@ extern "C" int td3_forest_build_nosync(const double* pos, int64_t n, int64_t sys
     rt.roots = roots;
     rt.cx = rt.cy = rt.h = 0.0;
     rt.y_down = 0;
+    rt.check = 0;
     return build_impl(pos, n, systems, rt, particle_system, system_start, leaf_capacity, max_depth,
                       capacity, node_center, node_half, node_start, node_end, node_child,
                       node_depth, levels, perm, sorted_pos, node_info, workspace, workspace_bytes,
@@ -760,6 +782,7 @@ extern "C" int td3_box_tree_build(const double* pos, int64_t n, double xmid, dou
     rt.cy = ymid;
     rt.h = half;
     rt.y_down = 1;
+    rt.check = 0;
     return build_impl(pos, n, 1, rt, nullptr, nullptr, leaf_capacity, max_depth, capacity,
                       node_center, node_half, node_start, node_end, node_child, node_depth, levels,
                       perm, sorted_pos, node_info, workspace, workspace_bytes, out_sizes,

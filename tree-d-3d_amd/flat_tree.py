@@ -153,6 +153,7 @@ class FlatNode:
 
     @property
     def total_charge(self) -> float:
+        self.tree._log_node_arrays()
         return float(self.tree._d["node_q"][self.index])
 
     @property
@@ -161,10 +162,12 @@ class FlatNode:
 
     @property
     def dipole(self) -> np.ndarray:
+        self.tree._log_node_arrays()
         return self.tree._d["node_p"][self.index].cpu().numpy().copy()
 
     @property
     def second_moment(self) -> np.ndarray:
+        self.tree._log_node_arrays()
         sxx, sxy, syy = (float(v) for v in self.tree._d["node_s"][self.index].tolist())
         return np.array([[sxx, sxy], [sxy, syy]])
 
@@ -244,18 +247,8 @@ class FlatQuadtree:
             raise ValueError("positions and charges must have the same length")
         self._d["pos"] = pos
         self._d["charge"] = charge
-        lo_x, hi_x = root_quad.xmid - root_quad.half, root_quad.xmid + root_quad.half
-        lo_y, hi_y = root_quad.ymid - root_quad.half, root_quad.ymid + root_quad.half
-        if n:
-            # both checks in one read-back
-            bad = torch.stack((
-                (~torch.isfinite(pos)).any(),
-                ((pos[0] < lo_x) | (pos[0] > hi_x) | (pos[1] < lo_y) | (pos[1] > hi_y)).any(),
-            )).cpu()
-            if bool(bad[0]):
-                raise ValueError("Particle position must be finite")
-            if bool(bad[1]):
-                raise ValueError("Particle lies outside tree bounds")
+        # the positions are checked (finite, inside the closed root square: flat_tree.py:
+        # 785-794) by the build's first kernel, read back with the split counts
         self.leaf_capacity = int(leaf_capacity)
         self.max_depth = int(max_depth)
         self._root = (float(root_quad.xmid), float(root_quad.ymid), float(root_quad.half))
@@ -271,6 +264,8 @@ class FlatQuadtree:
     # ---------------------------------------------------------------- arrays
     def __getattr__(self, name):
         if name in _PUBLIC:
+            if name in ("node_q", "node_p", "node_s") and "_d" in self.__dict__:
+                self._log_node_arrays()
             d = self.__dict__.get("_d", {})
             if name not in d:
                 raise AttributeError(name)
@@ -344,6 +339,12 @@ class FlatQuadtree:
             if code == _abi.TD3_ERR_CAPACITY:
                 capacity = max(2 * capacity, int(out[2]))
                 continue
+            if code == _abi.TD3_ERR_ARG:
+                message = _abi.last_error()
+                for known in ("Particle position must be finite",
+                              "Particle lies outside tree bounds"):
+                    if known in message:
+                        raise ValueError(known)
             _abi.check(code, "td3_tree_build")
             break
         count, n_levels = int(out[0]), int(out[1])
@@ -375,10 +376,14 @@ class FlatQuadtree:
         c, n = self.node_count, self._d["pos"].shape[1]
         f64 = dict(dtype=torch.float64, device=dev)
         # written in full by the kernels (log: q and p from the moments; s stays zero)
-        node_q = torch.empty(c, **f64)
+        # log: q and p are read off the moments only when asked for (_log_node_arrays)
         node_qabs = torch.empty(c, **f64)
-        node_p = torch.empty((c, 2), **f64)
-        node_s = (torch.zeros if self.kernel == "log" else torch.empty)((c, 3), **f64)
+        if self.kernel == "log":
+            node_q = node_p = node_s = None
+        else:
+            node_q = torch.empty(c, **f64)
+            node_p = torch.empty((c, 2), **f64)
+            node_s = torch.empty((c, 3), **f64)
         p1 = self.order + 1
         if self.kernel == "log":
             mre = torch.empty((c, p1), **f64)
@@ -406,14 +411,25 @@ class FlatQuadtree:
                 len(levels) - 1, _abi.ptr(binom), bsz, _abi.ptr(node_q), _abi.ptr(node_qabs),
                 _abi.ptr(node_p), _abi.ptr(node_s), _abi.ptr(mre), _abi.ptr(mim), _abi.stream(),
             )
-        if self.kernel == "log":
-            node_q.copy_(mre[:, 0])
-            node_p[:, 0] = mre[:, 1]
-            node_p[:, 1] = mim[:, 1]
-        self._d.update(node_q=node_q, node_qabs=node_qabs, node_p=node_p, node_s=node_s,
-                       moment_re=mre, moment_im=mim)
+        for name in ("node_q", "node_p", "node_s"):
+            self._d.pop(name, None)
+        if self.kernel != "log":
+            self._d.update(node_q=node_q, node_p=node_p, node_s=node_s)
+        self._d.update(node_qabs=node_qabs, moment_re=mre, moment_im=mim)
         self._invalidate_host("node_q", "node_qabs", "node_p", "node_s", "moment_re",
                               "moment_im")
+
+    def _log_node_arrays(self) -> None:
+        """The log tree's node_q, node_p (a_0 and a_1 of the moments) and node_s (zero),
+        made when first asked for: the FMM itself reads the moments."""
+        d = self._d
+        if self.kernel != "log" or "node_q" in d:
+            return
+        mre, mim = d["moment_re"], d["moment_im"]
+        d["node_q"] = mre[:, 0].clone()
+        d["node_p"] = torch.stack((mre[:, 1], mim[:, 1]), dim=1).contiguous()
+        d["node_s"] = torch.zeros((mre.shape[0], 3), dtype=torch.float64, device=mre.device)
+        self._invalidate_host("node_q", "node_p", "node_s")
 
     def set_charges(self, charges, dipoles=None) -> None:
         """flat_tree.py:853-868: new charges for the same particles; keeps cells and lists.
@@ -502,6 +518,7 @@ class FlatQuadtree:
         if n and m:
             status = torch.zeros(1, dtype=torch.int64, device=t.device)
             order = self._target_order(t)
+            self._log_node_arrays()
             d = self._d
             _abi.call(
                 "td3_walk", _abi.LOG if self.kernel == "log" else _abi.COULOMB,
@@ -725,8 +742,9 @@ class FlatQuadtree:
         mrow, mcol, mdst = m2l_rows
         c, p1 = target.node_count, self.order + 1
         dev = self._d["pos"].device
-        lre = torch.zeros((c, p1), dtype=torch.float64, device=dev)
-        lim = torch.zeros((c, p1), dtype=torch.float64, device=dev)
+        # every row is written by the M2L of its range (zeros without pairs)
+        lre = torch.empty((c, p1), dtype=torch.float64, device=dev)
+        lim = torch.empty((c, p1), dtype=torch.float64, device=dev)
         coeff = self._m2l_coefficients()
         levels = np.ascontiguousarray(target.levels, dtype=np.int64)
         if precision == _abi.FP64:
