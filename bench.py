@@ -29,7 +29,7 @@ import __graft_entry__  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # measured (rocprofv3 --pmc, calibrated by a 256 MiB copy) for this workload's M2L launch
 M2L_TRAFFIC_BYTES = 923.2e6
-KERNEL_NAMES = ("m2l_mfma", "m2l_mfma64", "m2l", "m2l_fixup", "eval", "l2l", "rows_level",
+KERNEL_NAMES = ("m2l_f32", "m2l_records", "m2l_mfma64", "m2l", "m2l_fixup", "eval", "l2l", "rows_level",
                 "rows_place",
                 "traverse",
                 "row_sort", "lists_fill",
@@ -643,17 +643,17 @@ def main() -> int:
     mrow_h = mrow.cpu().numpy().astype(np.int64)
     first_fp32 = int(tree.levels[min(fp64_top + 1, len(tree.levels) - 1)])
     mfma_pairs = int(mrow_h[-1] - mrow_h[first_fp32])
-    ms = kernels["m2l_mfma"]["ms"] / kernels["m2l_mfma"]["launches"]
+    ms = kernels["m2l_f32"]["ms"] / kernels["m2l_f32"]["launches"]
     achieved = 910.0 * mfma_pairs / (ms * 1e-3) / 1e12
     result["roofline"] = {
-        "kernel": "m2l_mfma_kernel (FP32 M2L on v_mfma_f32_16x16x4_f32, destinations below "
+        "kernel": "m2l_f32_kernel (FP32 M2L on v_mfma_f32_16x16x4_f32, destinations below "
                   "the FP64 top levels)",
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
-        # HBM bytes of one m2l_mfma_kernel launch, rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE
+        # HBM bytes of one m2l_f32_kernel launch, rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE
         # x 1.000 (factors from a 256 MiB copy in the same run)
         "traffic": M2L_TRAFFIC_BYTES,
         "algorithmic": {"m2l_directed_pairs": m2l_directed, "mfma_directed_pairs": mfma_pairs,

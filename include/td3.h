@@ -296,11 +296,11 @@ int td3_scale_moments(int precision, const double *moment_re, const double *mome
  * (flat_tree.py:234-245, either truncation); scaled_re/im from td3_scale_moments of the
  * same precision; dst_* are the destination tree's cells (the rows), src_* the source
  * tree's (the columns; the same arrays for fmm, the target tree for fmm_at).  Every row of
- * the range is WRITTEN (not accumulated): zeros for a row without pairs.  FP32 up to
- * order 15 runs on the matrix cores, a wavefront per row, and so does FP64 up to order 15
- * when workspace is NULL; otherwise a lane per pair, with the workspace
- * (td3_m2l_workspace_bytes of at least the range's pair count) holding the partial sums
- * of rows that span several wavefronts.  No host read-back.
+ * the range is WRITTEN (not accumulated): zeros for a row without pairs.  FP64 up to
+ * order 15 with workspace NULL runs on the matrix cores, a wavefront per row; otherwise a
+ * lane per pair, with the workspace (td3_m2l_workspace_bytes of at least the range's pair
+ * count) holding the partial sums of rows that span several wavefronts.  The fast FP32
+ * path up to order 15 is td3_m2l_f32.  No host read-back.
  */
 size_t td3_m2l_workspace_bytes(int64_t pair_count, int64_t order, int precision);
 int td3_m2l(int precision, int64_t order, const int32_t *col, const int32_t *dst,
@@ -309,6 +309,36 @@ int td3_m2l(int precision, int64_t order, const int32_t *col, const int32_t *dst
             const double *src_half, const void *scaled_re, const void *scaled_im,
             const double *coeff, double *local_re, double *local_im, void *workspace,
             size_t workspace_bytes, void *stream);
+
+/*
+ * Source records of the FP32 M2L (order 1..15), one per cell of td3_m2l_record_bytes(order)
+ * bytes (a multiple of 32): the cell's centre (x, y) as FP64, then A_k = a_k / r^k
+ * (r = sqrt(2) h) as FP32 (re, im) for k <= order, except that the slot of Im A_0 holds r
+ * (a_0 is a net charge: real), then zeros.  A lane of the M2L loads its four coefficients
+ * in two 16-byte reads and the cell's geometry from the same record.  No counterpart in the
+ * reference (its M2L reads the FP64 moments directly, fmm_m2l_cuda.cu:12-80); records is
+ * DEVICE memory of node_count * td3_m2l_record_bytes(order) bytes.  No read-back.
+ */
+size_t td3_m2l_record_bytes(int64_t order);
+int td3_m2l_records(int64_t order, const double *moment_re, const double *moment_im,
+                    const double *node_center, const double *node_half, int64_t node_count,
+                    void *records, void *stream);
+
+/*
+ * td3_m2l in FP32 (order 1..15) on the matrix cores from td3_m2l_records of the source tree:
+ * the same rows (col, row_start), coefficients, destination cells and outputs (scaled FP64
+ * locals, every row of [node_lo, node_hi) WRITTEN, zeros without pairs).  Replaces
+ * kernels/fmm_m2l (fmm_m2l_cuda.cu:12-80) / kernels/fmm_m2l_host (fmm_m2l_host_pyccel.py:
+ * 16-105) below the FP64 top levels of the mixed-precision FMM.  A wavefront takes eight
+ * consecutive rows and runs their pairs through the matrix product 16 at a time; every row
+ * is summed in one fixed order.  The imaginary part of a_0 ln t is dropped (it reaches
+ * neither the potential -2 Re W, nor the field W', nor, through L2L, any other
+ * coefficient).  No workspace, no read-back.
+ */
+int td3_m2l_f32(int64_t order, const int32_t *col, const uint32_t *row_start,
+                int64_t node_lo, int64_t node_hi, const double *dst_center,
+                const double *dst_half, const void *records, const double *coeff,
+                double *local_re, double *local_im, void *stream);
 
 /*
  * L2L down the tree on scaled locals, level by level from the root (flat_tree.py:1540-1552,

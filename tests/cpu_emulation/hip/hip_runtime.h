@@ -40,6 +40,10 @@ struct alignas(16) float4 {
     float x, y, z, w;
 };
 inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }
+struct alignas(16) double2 {
+    double x, y;
+};
+inline double2 make_double2(double x, double y) { return {x, y}; }
 struct alignas(32) double4 {
     double x, y, z, w;
 };
@@ -359,6 +363,32 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     auto all = td3emu::exchange(td3emu_bits(v));
     return td3emu_from<T>(all[td3emu::lane_id() ^ (unsigned)mask]);
 }
+// DPP moves within rows of 16 lanes: quad_perm (controls 0x00-0xff), row_mirror (0x140) and
+// row_half_mirror (0x141), every lane enabled.
+inline int __builtin_amdgcn_mov_dpp(int v, int ctrl, int, int, bool) {
+    auto all = td3emu::exchange(td3emu_bits(v));
+    const unsigned lane = td3emu::lane_id();
+    unsigned src;
+    if (ctrl >= 0 && ctrl <= 0xff)
+        src = (lane & ~3u) | (((unsigned)ctrl >> (2 * (lane & 3))) & 3u);
+    else if (ctrl == 0x140)
+        src = (lane & ~15u) | (15u - (lane & 15u));
+    else if (ctrl == 0x141)
+        src = (lane & ~7u) | (7u - (lane & 7u));
+    else
+        abort();
+    return td3emu_from<int>(all[src]);
+}
+inline int __float_as_int(float f) {
+    int i;
+    memcpy(&i, &f, 4);
+    return i;
+}
+inline float __int_as_float(int i) {
+    float f;
+    memcpy(&f, &i, 4);
+    return f;
+}
 
 // Matrix cores: v_mfma_{f32,f64}_16x16x4 on one wavefront.  Lane l holds A[l % 16][l / 16]
 // and B[l / 16][l % 16]; element r of lane l of the result is D[4 (l / 16) + r][l % 16] for
@@ -437,6 +467,7 @@ inline float __frsqrt_rn(float a) { return 1.0f / std::sqrt(a); }
 // the hardware's approximate transcendentals (v_log_f32 is log2)
 inline float __builtin_amdgcn_logf(float a) { return std::log2(a); }
 inline float __builtin_amdgcn_rcpf(float a) { return 1.0f / a; }
+inline double __builtin_amdgcn_rcp(double a) { return 1.0 / a; }
 inline float __builtin_amdgcn_rsqf(float a) { return 1.0f / std::sqrt(a); }
 inline double rsqrt(double a) { return 1.0 / std::sqrt(a); }
 using std::atan2;

@@ -258,6 +258,47 @@ def case_fmm():
         assert relative(got[1], want[1]) < 1e-12
 
 
+def case_m2l_f32():
+    # the FP32 M2L from source records (td3_m2l_f32) against the FP64 M2L (td3_m2l), row by
+    # row over every row of a deep tree (rows from empty to long, several rows ending inside
+    # one tile, wavefronts whose rows run past the last node), at orders that fill one to
+    # four of the product's k groups.  Errors relative to each row's largest coefficient:
+    # b_0 sums FP32 terms a_0 ln|t| (up to ~10 each), the others are small FP32 products
+    pos, q = cloud(1500, 21)
+    for order in (3, 5, 12, 15):
+        t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=order,
+                             leaf_capacity=2)
+        mrow, mcol, mdst = t._lists(0.5)[0]
+        c, p1 = t.node_count, order + 1
+        coeff = t._m2l_coefficients()
+        are, aim = t._scaled_moments(_abi.FP64)
+        nbytes = _abi.lib().td3_m2l_workspace_bytes(int(mcol.shape[0]), order, _abi.FP64)
+        ws = _abi.workspace(nbytes)
+        want = [torch.full((c, p1), np.nan, dtype=torch.float64) for _ in range(2)]
+        got = [torch.full((c, p1), np.nan, dtype=torch.float64) for _ in range(2)]
+        geo = (_abi.ptr(t._d["node_center"]), _abi.ptr(t._d["node_half"]))
+        _abi.call("td3_m2l", _abi.FP64, order, _abi.ptr(mcol), _abi.ptr(mdst), _abi.ptr(mrow),
+                  0, c, *geo, *geo, _abi.ptr(are), _abi.ptr(aim), _abi.ptr(coeff),
+                  _abi.ptr(want[0]), _abi.ptr(want[1]), _abi.ptr(ws), nbytes, 0)
+        for lo, hi in ((0, c), (5, c - 3)):
+            _abi.call("td3_m2l_f32", order, _abi.ptr(mcol), _abi.ptr(mrow), lo, hi, *geo,
+                      _abi.ptr(t._m2l_records()), _abi.ptr(coeff), _abi.ptr(got[0]),
+                      _abi.ptr(got[1]), 0)
+        g = [x.numpy() for x in got]
+        w = [x.numpy() for x in want]
+        assert not np.isnan(g[0]).any() and not np.isnan(g[1]).any()  # every row written
+        scale = np.abs(w[0]).max(axis=1) + np.abs(w[1]).max(axis=1) + 1e-300
+        err = np.abs(g[0] - w[0]) / scale[:, None]
+        erri = np.abs(g[1] - w[1]) / scale[:, None]
+        assert err[:, 0].max() < 5e-6, (order, err[:, 0].max())
+        # FP32 drops Im b_0 (the imaginary part of a_0 ln t): it reaches neither the
+        # potential, nor the field, nor any other coefficient through L2L
+        assert max(err[:, 1:].max(), erri[:, 1:].max()) < 5e-7, (order, err[:, 1:].max(),
+                                                                 erri[:, 1:].max())
+        empty = np.diff(np.asarray(mrow.cpu() if hasattr(mrow, "cpu") else mrow)) == 0
+        assert empty.any() and not g[0][empty].any() and not g[1][empty].any()
+
+
 def case_rows_overflow():
     # rows built without a read-back overflow their capacities: every row is left empty,
     # the FMM notices with its status and builds them again

@@ -14,7 +14,7 @@ import pytest
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_emulation")
 QUICK = ["direct", "moments"]
-DEFAULT = ["direct", "direct_noalloc", "strip", "smooth", "tree", "forest", "forest_resident", "moments", "walk", "fmm", "rows_overflow", "fmm_at",
+DEFAULT = ["direct", "direct_noalloc", "strip", "smooth", "tree", "forest", "forest_resident", "moments", "walk", "fmm", "m2l_f32", "rows_overflow", "fmm_at",
            "coulomb", "push", "adjoint"]
 FULL = DEFAULT + ["big_tree", "rows"]
 

@@ -62,6 +62,9 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_scale_moments": ([_C, _P, _P, _P, _I, _I, _P, _P, _P], _C),
     "td3_m2l_workspace_bytes": ([_I, _I, _C], _Z),
     "td3_m2l": ([_C, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P], _C),
+    "td3_m2l_record_bytes": ([_I], _Z),
+    "td3_m2l_records": ([_I, _P, _P, _P, _P, _I, _P, _P], _C),
+    "td3_m2l_f32": ([_I, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P], _C),
     "td3_l2l": ([_I, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P], _C),
     "td3_unscale_locals": ([_P, _I, _I, _P, _P, _P], _C),
     "td3_fmm_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _D, _C,

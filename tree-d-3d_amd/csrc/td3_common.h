@@ -99,3 +99,18 @@ __device__ __forceinline__ float td3_wave_sumf(float v) {
 
 __device__ __forceinline__ int td3_lane() { return (int)(threadIdx.x & (TD3_WAVE - 1)); }
 
+// Sum over each row of 16 lanes by DPP moves (quad_perm [1,0,3,2], [2,3,0,1], then the
+// half-row and row mirrors): four adds, no LDS traffic.  Every lane of the row gets the
+// row's sum, the same terms in an order fixed per lane.
+template <int CTRL>
+__device__ __forceinline__ float td3_dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float td3_row16_sum(float v) {
+    v += td3_dpp_f<0xb1>(v);
+    v += td3_dpp_f<0x4e>(v);
+    v += td3_dpp_f<0x141>(v);
+    v += td3_dpp_f<0x140>(v);
+    return v;
+}
+

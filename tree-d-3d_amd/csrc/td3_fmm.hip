@@ -177,15 +177,16 @@ __global__ __launch_bounds__(256) void m2l_kernel(
 }
 
 // ------------------------------------------------------------------------------ M2L on MFMA
-// The same translation with S = coeff^T X as a matrix product on the matrix cores
-// (v_mfma_f32_16x16x4_f32), for orders up to 15 (p + 1 <= 16 coefficients, zero padded).
-// One wavefront per destination row; the row's pairs go 16 at a time.  Lane (j, q) =
-// (lane % 16, lane / 16) takes pair j of the tile: it forms X_k = A_k u^k for
-// k = q, q + 4, q + 8, q + 12 (the B operand of the four K = 4 products) and holds
-// coeff[4 kb + q, j] (the A operand); the result D[l][j] = S_l of pair j comes back in lane
-// (j, l / 4), element l % 4.  The lane scales it by (-1)^(l+1) v^l (l >= 1) or forms
-// A_0 ln t - S_0 (l = 0) and adds it to its running sum; the 16 lanes of each q are summed
-// at the end in a fixed butterfly, so the result is deterministic.
+// The FP64 translation with S = coeff^T X as a matrix product on the matrix cores
+// (v_mfma_f64_16x16x4f64), for orders up to 15 (p + 1 <= 16 coefficients, zero padded):
+// the top levels of the mixed-precision FMM.  One wavefront per destination row; the row's
+// pairs go 16 at a time.  Lane (j, q) = (lane % 16, lane / 16) takes pair j of the tile: it
+// forms X_k = A_k u^k for k = q, q + 4, q + 8, q + 12 (the B operand of the four K = 4
+// products) and holds coeff[4 kb + q, j] (the A operand); the result D[l][j] = S_l of pair j
+// comes back in lane (j, l % 4), element l / 4 (v_mfma_f64_16x16x4f64 gives row q + 4 r).
+// The lane scales it by (-1)^(l+1) v^l (l >= 1) or forms A_0 ln t - S_0 (l = 0) and adds it
+// to its running sum; the 16 lanes of each q are summed at the end in a fixed butterfly, so
+// the result is deterministic.
 typedef float td3_f32x4 __attribute__((ext_vector_type(4)));
 typedef double td3_f64x4 __attribute__((ext_vector_type(4)));
 
@@ -198,47 +199,29 @@ __device__ __forceinline__ td3_f64x4 mfma16(double a, double b, td3_f64x4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<float> {
-    typedef td3_f32x4 type;
-};
-template <>
-struct Vec4<double> {
-    typedef td3_f64x4 type;
-};
-
-template <typename T>
-__device__ __forceinline__ Cx<T> cpow_small(Cx<T> u, Cx<T> u2, Cx<T> u3, int q) {
-    // u^q for q in 0..3, selected without divergence
-    Cx<T> r = {(T)1, (T)0};
+// u^q for q in 0..3, selected without divergence
+__device__ __forceinline__ Cx<double> cpow_small(Cx<double> u, Cx<double> u2, Cx<double> u3,
+                                                 int q) {
+    Cx<double> r = {1.0, 0.0};
     r = q == 1 ? u : r;
     r = q == 2 ? u2 : r;
     r = q == 3 ? u3 : r;
     return r;
 }
 
-__device__ __forceinline__ float rcp_of(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double rcp_of(double x) { return 1.0 / x; }
-
-template <typename T>
 __global__ __launch_bounds__(256) void m2l_mfma_kernel(
     const int* __restrict__ col, const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
     const double* __restrict__ dcenter, const double* __restrict__ dhalf,
     const double* __restrict__ scenter, const double* __restrict__ shalf,
-    const T* __restrict__ mre, const T* __restrict__ mim, int p1,
+    const double* __restrict__ mre, const double* __restrict__ mim, int p1,
     const double* __restrict__ coeff, double* __restrict__ lre, double* __restrict__ lim) {
-    typedef typename Vec4<T>::type V4;
-    constexpr bool kFP64 = sizeof(T) == 8;
     const int lane = td3_lane();
     const i64 d = node_lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
     if (d >= node_hi) return;  // uniform over the wavefront
     const u32 r0 = row_start[d], r1 = row_start[d + 1];
     const int j = lane & 15, q = lane >> 4;
-    // Element r of lane (j, q) of the product holds D[row_of(r)][j]:
-    // v_mfma_f32_16x16x4f32 gives row 4 q + r, v_mfma_f64_16x16x4f64 row q + 4 r.
-    auto row_of = [q](int r) { return kFP64 ? q + 4 * r : 4 * q + r; };
+    // element r of lane (j, q) of the product holds D[q + 4 r][j]
+    auto row_of = [q](int r) { return q + 4 * r; };
     if (r0 == r1) {
         // a row without pairs is written too (zeros), so the caller need not clear it
         if (j == 0)
@@ -249,95 +232,73 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
                 }
         return;
     }
-    T ca[4];
+    double ca[4];
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
         const int k = 4 * kb + q;
-        ca[kb] = (k < p1 && j < p1) ? (T)coeff[k * p1 + j] : (T)0;
+        ca[kb] = (k < p1 && j < p1) ? coeff[k * p1 + j] : 0.0;
     }
     const double cx = dcenter[2 * d], cy = dcenter[2 * d + 1];
-    const T rt = (T)(1.4142135623730951 * dhalf[d]);
-    T acc_re[4] = {(T)0, (T)0, (T)0, (T)0}, acc_im[4] = {(T)0, (T)0, (T)0, (T)0};
+    const double rt = 1.4142135623730951 * dhalf[d];
+    double acc_re[4] = {0.0, 0.0, 0.0, 0.0}, acc_im[4] = {0.0, 0.0, 0.0, 0.0};
     for (u32 e0 = r0; e0 < r1; e0 += 16) {
         const u32 e = e0 + (u32)j;
         const bool valid = e < r1;
-        T x_re[4] = {(T)0, (T)0, (T)0, (T)0}, x_im[4] = {(T)0, (T)0, (T)0, (T)0};
-        Cx<T> v = {(T)0, (T)0};
-        T lnr = (T)0, lni = (T)0;
+        double x_re[4] = {0.0, 0.0, 0.0, 0.0}, x_im[4] = {0.0, 0.0, 0.0, 0.0};
+        Cx<double> v = {0.0, 0.0};
+        double lnr = 0.0, lni = 0.0;
         if (valid) {
             const u32 src = (u32)col[e];
-            // t = c_dst - c_src in FP64 (exact for nearby centres)
-            const double tr64 = cx - scenter[2 * src], ti64 = cy - scenter[2 * src + 1];
-            const T tr = (T)tr64, ti = (T)ti64;
-            const T t2 = tr * tr + ti * ti;
-            const T inv = rcp_of(t2);
-            const T wr = tr * inv, wi = -ti * inv;
-            const T rs = (T)(1.4142135623730951 * shalf[src]);
-            const Cx<T> u = {rs * wr, rs * wi};
+            const double tr = cx - scenter[2 * src], ti = cy - scenter[2 * src + 1];
+            const double t2 = tr * tr + ti * ti;
+            const double inv = 1.0 / t2;
+            const double wr = tr * inv, wi = -ti * inv;
+            const double rs = 1.4142135623730951 * shalf[src];
+            const Cx<double> u = {rs * wr, rs * wi};
             v = {rt * wr, rt * wi};
-            const Cx<T> u2 = cmul(u, u);
-            const Cx<T> u3 = cmul(u2, u);
-            const Cx<T> u4 = cmul(u2, u2);
-            Cx<T> up = cpow_small(u, u2, u3, q);
+            const Cx<double> u2 = cmul(u, u);
+            const Cx<double> u3 = cmul(u2, u);
+            const Cx<double> u4 = cmul(u2, u2);
+            Cx<double> up = cpow_small(u, u2, u3, q);
             // rows of this lane's k = q + 4 kb: one address, immediate offsets
-            const T* ar = mre + (src * (u32)p1 + (u32)q);
-            const T* ai = mim + (src * (u32)p1 + (u32)q);
+            const double* ar = mre + (src * (u32)p1 + (u32)q);
+            const double* ai = mim + (src * (u32)p1 + (u32)q);
 #pragma unroll
             for (int kb = 0; kb < 4; ++kb) {
                 const int k = 4 * kb + q;
                 if (k < p1) {
-                    const Cx<T> x = cmul(Cx<T>{ar[4 * kb], ai[4 * kb]}, up);
+                    const Cx<double> x = cmul(Cx<double>{ar[4 * kb], ai[4 * kb]}, up);
                     x_re[kb] = x.re;
                     x_im[kb] = x.im;
                 }
                 up = cmul(up, u4);
             }
-            if (kFP64) {
-                if (q == 0) {
-                    lnr = (T)(0.5 * log(tr64 * tr64 + ti64 * ti64));
-                    lni = (T)atan2(ti64, tr64);
-                }
-            } else {
-                // ln |t| (v_log_f32 is log2)
-                lnr = (T)(0.34657359027997264f * __builtin_amdgcn_logf((float)t2));
+            if (q == 0) {
+                lnr = 0.5 * log(t2);
+                lni = atan2(ti, tr);
             }
         }
-        V4 dre = {(T)0, (T)0, (T)0, (T)0}, dim = {(T)0, (T)0, (T)0, (T)0};
+        td3_f64x4 dre = {0.0, 0.0, 0.0, 0.0}, dim = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
             dre = mfma16(ca[kb], x_re[kb], dre);
             dim = mfma16(ca[kb], x_im[kb], dim);
         }
         if (valid) {
-            // v^l for the first element's row l, then the step to the next element's
-            const Cx<T> v2 = cmul(v, v);
-            const Cx<T> v4 = cmul(v2, v2);
-            Cx<T> vp, step;
-            if (kFP64) {
-                vp = cpow_small(v, v2, cmul(v2, v), q);
-                step = v4;
-            } else {
-                const Cx<T> v8 = cmul(v4, v4);
-                vp = {(T)1, (T)0};
-                vp = q == 1 ? v4 : vp;
-                vp = q == 2 ? v8 : vp;
-                vp = q == 3 ? cmul(v8, v4) : vp;
-                step = v;
-            }
+            // v^l for the first element's row l = q, then the step v^4 to the next
+            const Cx<double> v2 = cmul(v, v);
+            Cx<double> vp = cpow_small(v, v2, cmul(v2, v), q);
+            const Cx<double> step = cmul(v2, v2);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int l = row_of(r);
-                const Cx<T> sl = {dre[r], dim[r]};
+                const Cx<double> sl = {dre[r], dim[r]};
                 if (l == 0) {
-                    // A_0 = X_0 (u^0 = 1 in lane q = 0) is real: the cell's charge.  The
-                    // FP32 path keeps only the real part A_0 ln|t| of A_0 ln t: the
-                    // imaginary part of the constant term reaches neither the potential
-                    // (-2 Re W), nor the field (W'), nor, through L2L, any other coefficient.
                     acc_re[0] += x_re[0] * lnr - x_im[0] * lni - sl.re;
                     acc_im[0] += x_re[0] * lni + x_im[0] * lnr - sl.im;
                 } else if (l < p1) {
-                    const Cx<T> b = cmul(sl, vp);
-                    const T sign = (l & 1) ? (T)1 : (T)-1;
+                    const Cx<double> b = cmul(sl, vp);
+                    const double sign = (l & 1) ? 1.0 : -1.0;
                     acc_re[r] += sign * b.re;
                     acc_im[r] += sign * b.im;
                 }
@@ -359,8 +320,8 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(
         for (int r = 0; r < 4; ++r) {
             const int l = row_of(r);
             if (l < p1) {
-                lre[d * p1 + l] = (double)acc_re[r];
-                lim[d * p1 + l] = (double)acc_im[r];
+                lre[d * p1 + l] = acc_re[r];
+                lim[d * p1 + l] = acc_im[r];
             }
         }
     }
@@ -392,6 +353,219 @@ __global__ void m2l_fixup_kernel(const u32* __restrict__ row_start, i64 node_lo,
         lim[node * p1 + l] = v;
     else
         lre[node * p1 + l] = v;
+}
+
+// ------------------------------------------------------------------------------ FP32 M2L
+// Source records (td3_m2l_records), one per cell, m2l_record_floats(p1) floats (a multiple
+// of 32 bytes):
+//   floats [0, 4)            the centre x, y as FP64
+//   floats [4, 4 + 2 p1)     A_k = a_k / r^k (re, im) as FP32, r = sqrt(2) h, except that
+//                            the slot of Im A_0 holds r (a_0 is a net charge: real)
+//   the rest                 zeros
+// so the four coefficients A_{4q} .. A_{4q+3} a lane needs are the two 16-byte words at
+// floats 4 + 8 q and 8 + 8 q: inside the record whenever 4 q < p1 (4 q + 2 < p1), zeros
+// past the order.
+__host__ __device__ __forceinline__ int m2l_record_floats(int p1) {
+    return (16 + 8 * p1 + 31) / 32 * 8;
+}
+
+// One thread per 16-byte word of a record: scaled as scale_moments_kernel scales.
+__global__ void m2l_records_kernel(const double* __restrict__ mre, const double* __restrict__ mim,
+                                   const double* __restrict__ center,
+                                   const double* __restrict__ half, i64 count, int p1, int rf,
+                                   float* __restrict__ rec) {
+    const int words = rf / 4;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * words) return;
+    const i64 node = t / words;
+    const int w = (int)(t % words);
+    float* out = rec + node * rf + 4 * w;
+    if (w == 0) {
+        *reinterpret_cast<double2*>(out) = make_double2(center[2 * node], center[2 * node + 1]);
+        return;
+    }
+    const double r = 1.4142135623730951 * half[node];
+    const double inv = 1.0 / r;
+    float v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int f = 4 * w + i - 4;  // float index in the coefficient area
+        const int k = f >> 1;
+        float x = 0.0f;
+        if (f == 1) {
+            x = (float)r;
+        } else if (k < p1) {
+            double s = 1.0;
+            for (int m = 0; m < k; ++m) s *= inv;
+            x = (float)(((f & 1) ? mim : mre)[node * p1 + k] * s);
+        }
+        v[i] = x;
+    }
+    *reinterpret_cast<float4*>(out) = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// FP32 M2L on the matrix cores from source records, for orders up to 15.  A wavefront sums
+// kM2lRows consecutive destination rows, whose pairs are one contiguous run of the rows'
+// columns: the run goes through the matrix product 16 pairs at a time regardless of where
+// rows begin and end, so only the run's last tile is padded.
+//
+// Lane (j, q) = (lane % 16, lane / 16) takes pair j of the tile and the coefficients
+// k = 4 q + kb (kb = 0..3): it forms X_k = A_k u^k (u = r_src / t, t = c_dst - c_src) as the
+// B operand of the four K = 4 products and holds coeff'[4 q + kb, j] as the A operand, where
+// coeff' is the reference's coeff (flat_tree.py:234-245) with the signs of the translation
+// folded in (-1 for l = 0, (-1)^(l+1) otherwise).  The product S'_l of pair j comes back in
+// lane (j, q), element r, for l = 4 q + r: the same four exponents as its inputs, so the
+// lane forms u^{4q+r} and v^{4q+r} (v = r_dst / t) alike, from the fourth powers.  The lane
+// adds v^l S'_l, and A_0 ln|t| for l = 0 (the imaginary part of
+// A_0 ln t reaches neither the potential (-2 Re W), nor the field (W'), nor any other
+// coefficient through L2L), to its running sums; when a row ends inside a tile, the lanes
+// whose pair belongs to it add theirs, the 16 lanes of each q are summed in a fixed
+// butterfly and the row is written, so every row is summed in one fixed order.
+constexpr int kM2lRows = 8;
+
+__global__ __launch_bounds__(256) void m2l_f32_kernel(
+    const int* __restrict__ col, const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
+    const double* __restrict__ dcenter, const double* __restrict__ dhalf,
+    const float* __restrict__ rec, int rf, int p1, const double* __restrict__ coeff,
+    double* __restrict__ lre, double* __restrict__ lim) {
+    const int lane = td3_lane();
+    const i64 d0 = node_lo + (((i64)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kM2lRows;
+    if (d0 >= node_hi) return;  // uniform over the wavefront
+    const int nrows = (int)(node_hi - d0 < kM2lRows ? node_hi - d0 : kM2lRows);
+    // lane i <= nrows holds the start of row d0 + i (row nrows - 1 ends at lane nrows's)
+    const int lane_rs = lane <= nrows ? (int)row_start[d0 + lane] : 0;
+    const int j = lane & 15, q = lane >> 4;
+    float ca[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        const int k = 4 * q + kb;
+        const float sign = j == 0 ? -1.0f : ((j & 1) ? 1.0f : -1.0f);
+        ca[kb] = (k < p1 && j < p1) ? sign * (float)coeff[k * p1 + j] : 0.0f;
+    }
+    auto row_at = [&](int i) { return (u32)__builtin_amdgcn_readlane(lane_rs, i); };
+    const u32 e_end = row_at(nrows);
+    float acc_re[4] = {0.0f, 0.0f, 0.0f, 0.0f}, acc_im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // writes row r from the running sums (every lane takes part in the butterfly)
+    auto flush = [&](int r) {
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            acc_re[rr] = td3_row16_sum(acc_re[rr]);
+            acc_im[rr] = td3_row16_sum(acc_im[rr]);
+        }
+        if (j == 0) {
+            const i64 d = d0 + r;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int l = 4 * q + rr;
+                if (l < p1) {
+                    lre[d * p1 + l] = (double)acc_re[rr];
+                    lim[d * p1 + l] = (double)acc_im[rr];
+                }
+            }
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) acc_re[rr] = acc_im[rr] = 0.0f;
+    };
+    int r = 0;                 // the first row not yet written (uniform)
+    u32 r_hi = row_at(1);      // its end
+    for (u32 e0 = row_at(0); e0 < e_end; e0 += 16) {
+        const bool valid = e0 + (u32)j < e_end;
+        // lanes past the run take its last pair and add nothing
+        const u32 e = valid ? e0 + (u32)j : e_end - 1;
+        // the row of this lane's pair: r plus the later rows that start at or before e
+        int rl = r;
+        if (r_hi < e0 + 16)
+            for (int t = r + 1; t < nrows && row_at(t) < e0 + 16; ++t) rl += e >= row_at(t) ? 1 : 0;
+        float x_re[4], x_im[4];
+        Cx<float> pw[4], vw[4];
+        float a0_ln = 0.0f;
+        {
+            const i64 d = d0 + rl;
+            const float* rp = rec + (i64)col[e] * rf;
+            const double2 g = *reinterpret_cast<const double2*>(rp);
+            const float rs = rp[5];
+            float4 a1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a2 = a1;
+            if (4 * q < p1) a1 = *reinterpret_cast<const float4*>(rp + 4 + 8 * q);
+            if (4 * q + 2 < p1) a2 = *reinterpret_cast<const float4*>(rp + 8 + 8 * q);
+            if (q == 0) a1.y = 0.0f;  // the slot of Im A_0 holds r
+            // t = c_dst - c_src and w = 1 / t in FP64 (on gfx950 at the rate of FP32 without
+            // packing), u = r_src w and v = r_dst w rounded once to FP32
+            const double tr = dcenter[2 * d] - g.x, ti = dcenter[2 * d + 1] - g.y;
+            const double t2 = tr * tr + ti * ti;
+            const double inv = __builtin_amdgcn_rcp(t2);
+            const double wr = tr * inv, wi = -ti * inv;
+            const Cx<float> u = {(float)(rs * wr), (float)(rs * wi)};
+            const double rt = 1.4142135623730951 * dhalf[d];
+            const Cx<float> v = {(float)(rt * wr), (float)(rt * wi)};
+            const Cx<float> u2 = cmul(u, u);
+            const Cx<float> u4 = cmul(u2, u2);
+            const Cx<float> u8 = cmul(u4, u4);
+            const Cx<float> u12 = cmul(u8, u4);
+            Cx<float> up = {1.0f, 0.0f};
+            up = q == 1 ? u4 : up;
+            up = q == 2 ? u8 : up;
+            up = q == 3 ? u12 : up;
+            pw[0] = up;
+            pw[1] = cmul(pw[0], u);
+            pw[2] = cmul(pw[1], u);
+            pw[3] = cmul(pw[2], u);
+            const Cx<float> a[4] = {{a1.x, a1.y}, {a1.z, a1.w}, {a2.x, a2.y}, {a2.z, a2.w}};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                const Cx<float> x = cmul(a[kb], pw[kb]);
+                x_re[kb] = x.re;
+                x_im[kb] = x.im;
+            }
+            const Cx<float> v2 = cmul(v, v);
+            const Cx<float> v4 = cmul(v2, v2);
+            const Cx<float> v8 = cmul(v4, v4);
+            Cx<float> vp = {1.0f, 0.0f};
+            vp = q == 1 ? v4 : vp;
+            vp = q == 2 ? v8 : vp;
+            vp = q == 3 ? cmul(v8, v4) : vp;
+            vw[0] = vp;
+            vw[1] = cmul(vw[0], v);
+            vw[2] = cmul(vw[1], v);
+            vw[3] = cmul(vw[2], v);
+            // A_0 ln|t| (v_log_f32 is log2)
+            if (q == 0) a0_ln = a1.x * (0.34657359027997264f * __builtin_amdgcn_logf((float)t2));
+        }
+        td3_f32x4 dre = {0.0f, 0.0f, 0.0f, 0.0f}, dim = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            dre = mfma16(ca[kb], x_re[kb], dre);
+            dim = mfma16(ca[kb], x_im[kb], dim);
+        }
+        float c_re[4], c_im[4];
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const float vr = vw[rr].re, vi = vw[rr].im;
+            c_re[rr] = valid ? dre[rr] * vr - dim[rr] * vi : 0.0f;
+            c_im[rr] = valid ? dre[rr] * vi + dim[rr] * vr : 0.0f;
+        }
+        c_re[0] += a0_ln;
+        // rows that end inside this tile: their lanes' terms in, summed, written
+        while (r < nrows && r_hi <= e0 + 16) {
+            const bool mine = valid && rl == r;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                acc_re[rr] += mine ? c_re[rr] : 0.0f;
+                acc_im[rr] += mine ? c_im[rr] : 0.0f;
+                c_re[rr] = mine ? 0.0f : c_re[rr];
+                c_im[rr] = mine ? 0.0f : c_im[rr];
+            }
+            flush(r);
+            ++r;
+            r_hi = r < nrows ? row_at(r + 1) : e_end;
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            acc_re[rr] += c_re[rr];
+            acc_im[rr] += c_im[rr];
+        }
+    }
+    // rows past the last pair (or every row, without pairs) are empty: zeros
+    for (; r < nrows; ++r) flush(r);
 }
 
 // ------------------------------------------------------------------------------ scaling
@@ -897,23 +1071,16 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
     TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
     hipStream_t s = (hipStream_t)stream;
     if (node_hi == node_lo) return TD3_OK;
-    if (order + 1 <= 16 && (precision != TD3_FP64 || workspace == nullptr)) {
-        // a wavefront per destination row on the matrix cores: FP32, or FP64 when no
-        // workspace is given (a grid by rows, so the pair count need not be known; for a
-        // whole tree the lane-per-pair kernel is the faster FP64 one: on gfx950
-        // v_mfma_f64_16x16x4f64 runs at the vector FP64 rate, and that kernel does less
-        // work around the product)
+    if (order + 1 <= 16 && precision == TD3_FP64 && workspace == nullptr) {
+        // FP64 without a workspace: a wavefront per destination row on the matrix cores (a
+        // grid by rows, so the pair count need not be known; for a whole tree the
+        // lane-per-pair kernel is the faster FP64 one: on gfx950 v_mfma_f64_16x16x4f64 runs
+        // at the vector FP64 rate, and that kernel does less work around the product)
         const dim3 grid(td3_blocks((node_hi - node_lo) * 64, 256));
-        if (precision == TD3_FP64)
-            TD3_TIMED(s, "m2l_mfma64", hipLaunchKernelGGL((m2l_mfma_kernel<double>), grid, dim3(256), 0, s, col, row_start,
-                      node_lo, node_hi, dst_center, dst_half, src_center, src_half,
-                      (const double*)scaled_re, (const double*)scaled_im, (int)order + 1,
-                      coeff, local_re, local_im));
-        else
-            TD3_TIMED(s, "m2l_mfma", hipLaunchKernelGGL((m2l_mfma_kernel<float>), grid, dim3(256), 0, s, col, row_start,
-                      node_lo, node_hi, dst_center, dst_half, src_center, src_half,
-                      (const float*)scaled_re, (const float*)scaled_im, (int)order + 1,
-                      coeff, local_re, local_im));
+        TD3_TIMED(s, "m2l_mfma64", hipLaunchKernelGGL(m2l_mfma_kernel, grid, dim3(256), 0, s, col,
+                  row_start, node_lo, node_hi, dst_center, dst_half, src_center, src_half,
+                  (const double*)scaled_re, (const double*)scaled_im, (int)order + 1, coeff,
+                  local_re, local_im));
         TD3_LAUNCHED();
         return TD3_OK;
     }
@@ -933,6 +1100,43 @@ extern "C" int td3_m2l(int precision, int64_t order, const int32_t* col, const i
                                dst_center, dst_half, src_center, src_half,
                                (const float*)scaled_re, (const float*)scaled_im, coeff, local_re,
                                local_im, (float*)workspace);
+}
+
+extern "C" size_t td3_m2l_record_bytes(int64_t order) {
+    return order >= 1 && order <= 15 ? 4 * (size_t)m2l_record_floats((int)order + 1) : 0;
+}
+
+extern "C" int td3_m2l_records(int64_t order, const double* moment_re, const double* moment_im,
+                               const double* node_center, const double* node_half,
+                               int64_t node_count, void* records, void* stream) {
+    TD3_ARG(order >= 1 && order <= 15, "order (1..15)");
+    TD3_ARG(node_count >= 0, "node count");
+    if (node_count == 0) return TD3_OK;
+    const int p1 = (int)order + 1, rf = m2l_record_floats(p1);
+    hipStream_t s = (hipStream_t)stream;
+    const i64 items = node_count * (rf / 4);
+    TD3_TIMED(s, "m2l_records", hipLaunchKernelGGL(m2l_records_kernel, dim3(td3_blocks(items, 256)),
+              dim3(256), 0, s, moment_re, moment_im, node_center, node_half, node_count, p1, rf,
+              (float*)records));
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+extern "C" int td3_m2l_f32(int64_t order, const int32_t* col, const uint32_t* row_start,
+                           int64_t node_lo, int64_t node_hi, const double* dst_center,
+                           const double* dst_half, const void* records, const double* coeff,
+                           double* local_re, double* local_im, void* stream) {
+    TD3_ARG(order >= 1 && order <= 15, "order (1..15)");
+    TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
+    if (node_hi == node_lo) return TD3_OK;
+    const int p1 = (int)order + 1;
+    hipStream_t s = (hipStream_t)stream;
+    const i64 waves = (node_hi - node_lo + kM2lRows - 1) / kM2lRows;
+    TD3_TIMED(s, "m2l_f32", hipLaunchKernelGGL(m2l_f32_kernel, dim3(td3_blocks(waves * 64, 256)),
+              dim3(256), 0, s, col, row_start, node_lo, node_hi, dst_center, dst_half,
+              (const float*)records, m2l_record_floats(p1), p1, coeff, local_re, local_im));
+    TD3_LAUNCHED();
+    return TD3_OK;
 }
 
 extern "C" int td3_l2l(int64_t order, const double* node_center, const double* node_half,

@@ -719,6 +719,22 @@ class FlatQuadtree:
         self._scaled = cached
         return re, im
 
+    def _m2l_records(self) -> torch.Tensor:
+        """The FP32 M2L's source records of every cell (td3_m2l_records; order <= 15),
+        kept with the scaled moments until `set_charges`."""
+        cached = self.__dict__.get("_scaled", {})
+        if "records" in cached:
+            return cached["records"]
+        nbytes = _abi.lib().td3_m2l_record_bytes(self.order)
+        rec = torch.empty(max(self.node_count * nbytes // 4, 1), dtype=torch.float32,
+                          device=self._d["pos"].device)
+        _abi.call("td3_m2l_records", self.order, _abi.ptr(self._d["moment_re"]),
+                  _abi.ptr(self._d["moment_im"]), _abi.ptr(self._d["node_center"]),
+                  _abi.ptr(self._d["node_half"]), self.node_count, _abi.ptr(rec), _abi.stream())
+        cached["records"] = rec
+        self._scaled = cached
+        return rec
+
     #: target trees and dual lists `fmm_at` has built (kept per targets, theta and leaf)
     plan_builds = 0
 
@@ -756,10 +772,18 @@ class FlatQuadtree:
         for prec, lo, hi in ranges:
             if hi <= lo:
                 continue
+            if prec != _abi.FP64 and self.order + 1 <= 16:
+                # FP32 on the matrix cores from the source records: no workspace, no pair
+                # count needed
+                _abi.call("td3_m2l_f32", self.order, _abi.ptr(mcol), _abi.ptr(mrow), lo, hi,
+                          _abi.ptr(td["node_center"]), _abi.ptr(td["node_half"]),
+                          _abi.ptr(self._m2l_records()), _abi.ptr(coeff), _abi.ptr(lre),
+                          _abi.ptr(lim), _abi.stream())
+                continue
             are, aim = self._scaled_moments(prec)
             if precision != _abi.FP64 and self.order + 1 <= 16:
-                # mixed: both ranges on the matrix cores, gridded by rows (no workspace,
-                # no pair count needed)
+                # the FP64 top levels of the mixed run on the matrix cores, gridded by rows
+                # (no workspace, no pair count needed)
                 nbytes, ws = 0, None
             else:
                 nbytes = _abi.lib().td3_m2l_workspace_bytes(int(mcol.shape[0]), self.order,
@@ -832,7 +856,10 @@ class FlatQuadtree:
             self._sources4()
             self._scaled_moments(_abi.FP64)
             if prec != _abi.FP64:
-                self._scaled_moments(prec)
+                if self.order + 1 <= 16:
+                    self._m2l_records()
+                else:
+                    self._scaled_moments(prec)
             if prec != _abi.FP64 and self.order + 1 <= 16 and not jacobian and sdip is None:
                 # mixed precision needs no row counts before its last read-back: the rows
                 # are built and used at full capacity and checked with the status (the
