@@ -419,9 +419,25 @@ __global__ void m2l_records_kernel(const double* __restrict__ mre, const double*
 // adds v^l S'_l, and A_0 ln|t| for l = 0 (the imaginary part of
 // A_0 ln t reaches neither the potential (-2 Re W), nor the field (W'), nor any other
 // coefficient through L2L), to its running sums; when a row ends inside a tile, the lanes
-// whose pair belongs to it add theirs, the 16 lanes of each q are summed in a fixed
-// butterfly and the row is written, so every row is summed in one fixed order.
+// whose pair belongs to it add theirs, the 16 lanes of each q are summed in a fixed order
+// (m2l_row_sums) and the row is written, so every row is summed in one fixed order.
 constexpr int kM2lRows = 8;
+
+// The eight running sums of a lane (S_l re, im for its four l) summed over each row of 16
+// lanes by recursive halving over DPP mirrors: lane j ends with value m(j) = 4 [j >= 8] +
+// 2 [j % 8 >= 4] + [j % 4 >= 2] summed over the row's 16 lanes (lanes j and j ^ 1 alike).
+__device__ __forceinline__ float m2l_row_sums(const float (&f)[8], int j) {
+    const bool h1 = j >= 8, h2 = (j & 7) >= 4, h3 = (j & 2) != 0;
+    float w[4], x[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        w[i] = (h1 ? f[4 + i] : f[i]) + td3_dpp_f<0x140>(h1 ? f[i] : f[4 + i]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        x[i] = (h2 ? w[2 + i] : w[i]) + td3_dpp_f<0x141>(h2 ? w[i] : w[2 + i]);
+    const float y = (h3 ? x[1] : x[0]) + td3_dpp_f<0x4e>(h3 ? x[0] : x[1]);
+    return y + td3_dpp_f<0xb1>(y);
+}
 
 __global__ __launch_bounds__(256) void m2l_f32_kernel(
     const int* __restrict__ col, const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
@@ -442,80 +458,114 @@ __global__ __launch_bounds__(256) void m2l_f32_kernel(
         const float sign = j == 0 ? -1.0f : ((j & 1) ? 1.0f : -1.0f);
         ca[kb] = (k < p1 && j < p1) ? sign * (float)coeff[k * p1 + j] : 0.0f;
     }
+    // the value this lane writes after a row's sums: l = 4 q + m / 2, re or im
+    const int m_out = 4 * (j >= 8) + 2 * ((j & 7) >= 4) + ((j & 2) != 0);
+    const int l_out = 4 * q + (m_out >> 1);
+    double* const out = (m_out & 1) ? lim : lre;
+    const bool writer = (j & 1) == 0 && l_out < p1;
     auto row_at = [&](int i) { return (u32)__builtin_amdgcn_readlane(lane_rs, i); };
     const u32 e_end = row_at(nrows);
-    float acc_re[4] = {0.0f, 0.0f, 0.0f, 0.0f}, acc_im[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    // writes row r from the running sums (every lane takes part in the butterfly)
-    auto flush = [&](int r) {
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            acc_re[rr] = td3_row16_sum(acc_re[rr]);
-            acc_im[rr] = td3_row16_sum(acc_im[rr]);
-        }
-        if (j == 0) {
-            const i64 d = d0 + r;
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-                const int l = 4 * q + rr;
-                if (l < p1) {
-                    lre[d * p1 + l] = (double)acc_re[rr];
-                    lim[d * p1 + l] = (double)acc_im[rr];
-                }
-            }
-        }
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) acc_re[rr] = acc_im[rr] = 0.0f;
-    };
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int r = 0;                 // the first row not yet written (uniform)
     u32 r_hi = row_at(1);      // its end
-    for (u32 e0 = row_at(0); e0 < e_end; e0 += 16) {
+    // A tile's pair for this lane (lanes past the run take its last pair, and add nothing).
+    // Each tile works on the source record loaded during the tile before, and loads the
+    // column index of the tile after next.
+    auto pair_at = [&](u32 base) { return base + (u32)j < e_end ? base + (u32)j : e_end - 1; };
+    struct SourceRecord {
+        double2 g;
+        float rs;
+        float4 a1, a2;
+    };
+    auto load_record = [&](int src) {
+        const float* rp = rec + (i64)src * rf;
+        SourceRecord s;
+        s.g = *reinterpret_cast<const double2*>(rp);
+        s.rs = rp[5];
+        s.a1 = s.a2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (4 * q < p1) s.a1 = *reinterpret_cast<const float4*>(rp + 4 + 8 * q);
+        if (4 * q + 2 < p1) s.a2 = *reinterpret_cast<const float4*>(rp + 8 + 8 * q);
+        return s;
+    };
+    u32 e0 = row_at(0);
+    SourceRecord cur = {};
+    int src_next = 0;
+    if (e0 < e_end) {
+        cur = load_record(col[pair_at(e0)]);
+        src_next = col[pair_at(e0 + 16)];
+    }
+    for (; e0 < e_end; e0 += 16) {
         const bool valid = e0 + (u32)j < e_end;
-        // lanes past the run take its last pair and add nothing
-        const u32 e = valid ? e0 + (u32)j : e_end - 1;
+        const u32 e = pair_at(e0);
         // the row of this lane's pair: r plus the later rows that start at or before e
         int rl = r;
         if (r_hi < e0 + 16)
             for (int t = r + 1; t < nrows && row_at(t) < e0 + 16; ++t) rl += e >= row_at(t) ? 1 : 0;
+        const i64 d = d0 + rl;
+        const double dcx = dcenter[2 * d], dcy = dcenter[2 * d + 1], dh = dhalf[d];
+        const SourceRecord next = load_record(src_next);
+        src_next = col[pair_at(e0 + 32)];
+        const double2 g = cur.g;
+        const float rs = cur.rs;
+        float4 a1 = cur.a1;
+        const float4 a2 = cur.a2;
+        cur = next;
+        if (q == 0) a1.y = 0.0f;  // the slot of Im A_0 holds r
+        // t = c_dst - c_src and w = 1 / t in FP64 (on gfx950 at the rate of FP32 without
+        // packing), u = r_src w and v = r_dst w rounded once to FP32
+        const double tr = dcx - g.x, ti = dcy - g.y;
+        const double t2 = tr * tr + ti * ti;
+        const double inv = __builtin_amdgcn_rcp(t2);
+        const double wr = tr * inv, wi = -ti * inv;
+        const Cx<float> u = {(float)(rs * wr), (float)(rs * wi)};
+        const double rt = 1.4142135623730951 * dh;
+        const float rtf = (float)rt;
+        const Cx<float> u2 = cmul(u, u);
+        const Cx<float> u4 = cmul(u2, u2);
+        const Cx<float> u8 = cmul(u4, u4);
+        Cx<float> up = {valid ? 1.0f : 0.0f, 0.0f};
+        up = q == 1 && valid ? u4 : up;
+        up = q == 2 && valid ? u8 : up;
+        up = q == 3 && valid ? cmul(u8, u4) : up;
+        Cx<float> pw[4];
+        pw[0] = up;
+        pw[1] = cmul(pw[0], u);
+        pw[2] = cmul(pw[1], u);
+        pw[3] = cmul(pw[2], u);
+        const Cx<float> a[4] = {{a1.x, a1.y}, {a1.z, a1.w}, {a2.x, a2.y}, {a2.z, a2.w}};
         float x_re[4], x_im[4];
-        Cx<float> pw[4], vw[4];
-        float a0_ln = 0.0f;
-        {
-            const i64 d = d0 + rl;
-            const float* rp = rec + (i64)col[e] * rf;
-            const double2 g = *reinterpret_cast<const double2*>(rp);
-            const float rs = rp[5];
-            float4 a1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a2 = a1;
-            if (4 * q < p1) a1 = *reinterpret_cast<const float4*>(rp + 4 + 8 * q);
-            if (4 * q + 2 < p1) a2 = *reinterpret_cast<const float4*>(rp + 8 + 8 * q);
-            if (q == 0) a1.y = 0.0f;  // the slot of Im A_0 holds r
-            // t = c_dst - c_src and w = 1 / t in FP64 (on gfx950 at the rate of FP32 without
-            // packing), u = r_src w and v = r_dst w rounded once to FP32
-            const double tr = dcenter[2 * d] - g.x, ti = dcenter[2 * d + 1] - g.y;
-            const double t2 = tr * tr + ti * ti;
-            const double inv = __builtin_amdgcn_rcp(t2);
-            const double wr = tr * inv, wi = -ti * inv;
-            const Cx<float> u = {(float)(rs * wr), (float)(rs * wi)};
-            const double rt = 1.4142135623730951 * dhalf[d];
-            const Cx<float> v = {(float)(rt * wr), (float)(rt * wi)};
-            const Cx<float> u2 = cmul(u, u);
-            const Cx<float> u4 = cmul(u2, u2);
-            const Cx<float> u8 = cmul(u4, u4);
-            const Cx<float> u12 = cmul(u8, u4);
-            Cx<float> up = {1.0f, 0.0f};
-            up = q == 1 ? u4 : up;
-            up = q == 2 ? u8 : up;
-            up = q == 3 ? u12 : up;
-            pw[0] = up;
-            pw[1] = cmul(pw[0], u);
-            pw[2] = cmul(pw[1], u);
-            pw[3] = cmul(pw[2], u);
-            const Cx<float> a[4] = {{a1.x, a1.y}, {a1.z, a1.w}, {a2.x, a2.y}, {a2.z, a2.w}};
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) {
-                const Cx<float> x = cmul(a[kb], pw[kb]);
-                x_re[kb] = x.re;
-                x_im[kb] = x.im;
+        for (int kb = 0; kb < 4; ++kb) {
+            const Cx<float> x = cmul(a[kb], pw[kb]);
+            x_re[kb] = x.re;
+            x_im[kb] = x.im;
+        }
+        // A_0 ln|t| (v_log_f32 is log2)
+        const float a0_ln = q == 0 && valid
+                                ? a1.x * (0.34657359027997264f * __builtin_amdgcn_logf((float)t2))
+                                : 0.0f;
+        td3_f32x4 dre = {0.0f, 0.0f, 0.0f, 0.0f}, dim = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            dre = mfma16(ca[kb], x_re[kb], dre);
+            dim = mfma16(ca[kb], x_im[kb], dim);
+        }
+        // v^l S'_l.  Cells of one tree have radii r_root 2^-depth, so r_dst / r_src is a
+        // power of two (equal FP32 significands): v = 2^delta u exactly and v^l S'_l is
+        // u^l S'_l scaled by 2^(l delta).  Otherwise (cells of two trees) v's own powers.
+        float c[8];
+        const int sig_mask = 0x7fffff;
+        const bool dyadic = (__float_as_int(rtf) & sig_mask) == (__float_as_int(rs) & sig_mask);
+        if (__ballot(!dyadic) == 0) {
+            const int delta = ((__float_as_int(rtf) >> 23) & 0xff) - ((__float_as_int(rs) >> 23) & 0xff);
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int sh = (4 * q + rr) * delta;
+                c[2 * rr] = ldexpf(dre[rr] * pw[rr].re - dim[rr] * pw[rr].im, sh);
+                c[2 * rr + 1] = ldexpf(dre[rr] * pw[rr].im + dim[rr] * pw[rr].re, sh);
             }
+        } else {
+            const Cx<float> v = {(float)(rt * wr), (float)(rt * wi)};
             const Cx<float> v2 = cmul(v, v);
             const Cx<float> v4 = cmul(v2, v2);
             const Cx<float> v8 = cmul(v4, v4);
@@ -523,49 +573,36 @@ __global__ __launch_bounds__(256) void m2l_f32_kernel(
             vp = q == 1 ? v4 : vp;
             vp = q == 2 ? v8 : vp;
             vp = q == 3 ? cmul(v8, v4) : vp;
-            vw[0] = vp;
-            vw[1] = cmul(vw[0], v);
-            vw[2] = cmul(vw[1], v);
-            vw[3] = cmul(vw[2], v);
-            // A_0 ln|t| (v_log_f32 is log2)
-            if (q == 0) a0_ln = a1.x * (0.34657359027997264f * __builtin_amdgcn_logf((float)t2));
-        }
-        td3_f32x4 dre = {0.0f, 0.0f, 0.0f, 0.0f}, dim = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            dre = mfma16(ca[kb], x_re[kb], dre);
-            dim = mfma16(ca[kb], x_im[kb], dim);
-        }
-        float c_re[4], c_im[4];
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            const float vr = vw[rr].re, vi = vw[rr].im;
-            c_re[rr] = valid ? dre[rr] * vr - dim[rr] * vi : 0.0f;
-            c_im[rr] = valid ? dre[rr] * vi + dim[rr] * vr : 0.0f;
-        }
-        c_re[0] += a0_ln;
-        // rows that end inside this tile: their lanes' terms in, summed, written
-        while (r < nrows && r_hi <= e0 + 16) {
-            const bool mine = valid && rl == r;
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
-                acc_re[rr] += mine ? c_re[rr] : 0.0f;
-                acc_im[rr] += mine ? c_im[rr] : 0.0f;
-                c_re[rr] = mine ? 0.0f : c_re[rr];
-                c_im[rr] = mine ? 0.0f : c_im[rr];
+                c[2 * rr] = dre[rr] * vp.re - dim[rr] * vp.im;
+                c[2 * rr + 1] = dre[rr] * vp.im + dim[rr] * vp.re;
+                vp = cmul(vp, v);
             }
-            flush(r);
+        }
+        c[0] += a0_ln;
+        // rows that end inside this tile: their lanes' terms in, summed, written
+        while (r < nrows && r_hi <= e0 + 16) {
+            const bool mine = rl == r;
+            float f[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float t = mine ? c[i] : 0.0f;
+                f[i] = acc[i] + t;
+                c[i] -= t;
+                acc[i] = 0.0f;
+            }
+            const float sum = m2l_row_sums(f, j);
+            if (writer) out[(d0 + r) * p1 + l_out] = (double)sum;
             ++r;
             r_hi = r < nrows ? row_at(r + 1) : e_end;
         }
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            acc_re[rr] += c_re[rr];
-            acc_im[rr] += c_im[rr];
-        }
+        for (int i = 0; i < 8; ++i) acc[i] += c[i];
     }
     // rows past the last pair (or every row, without pairs) are empty: zeros
-    for (; r < nrows; ++r) flush(r);
+    for (; r < nrows; ++r)
+        if (writer) out[(d0 + r) * p1 + l_out] = 0.0;
 }
 
 // ------------------------------------------------------------------------------ scaling
