@@ -22,7 +22,16 @@ namespace {
 
 __host__ __device__ constexpr int cidx(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
 __host__ __device__ constexpr int csize(int p) { return (p + 1) * (p + 2) / 2; }
-__host__ __device__ constexpr double cfact(int n) { return n <= 1 ? 1.0 : n * cfact(n - 1); }
+// n! for n < 32 as a table the compiler folds where n is known
+struct CartFactorials {
+    double v[32];
+    constexpr CartFactorials() : v() {
+        double f = 1.0;
+        v[0] = 1.0;
+        for (int i = 1; i < 32; ++i) v[i] = f *= i;
+    }
+};
+__device__ constexpr CartFactorials kCartFact{};
 
 // -------------------------------------------------------------------------- moments
 // m_g = sum_j q_j dx^gx dy^gy over a leaf's particles; one thread per (leaf, g).
@@ -146,6 +155,12 @@ __global__ void cart_prime_kernel(const double* __restrict__ mom, i64 count, int
 }
 
 // -------------------------------------------------------------------------- M2L
+// One lane per directed pair over 64 consecutive pairs of the destination-sorted CSR: the
+// lane's T_b stay in registers, then go through LDS 32 at a time, where the lanes of each
+// destination are summed in lane order (rows within the 64 written divided by b!, the
+// partial sums of rows that cross a wave boundary to side, primed, for the fixup).
+constexpr int kCartGroup = 32;
+
 template <int P>
 __global__ __launch_bounds__(256) void cart_m2l_kernel(
     const int* __restrict__ col, const int* __restrict__ dst, const u32* __restrict__ row_start,
@@ -153,13 +168,15 @@ __global__ __launch_bounds__(256) void cart_m2l_kernel(
     const double* __restrict__ mprime, double eps2, double* __restrict__ local,
     double* __restrict__ side) {
     constexpr int NC = csize(P);
-    const int lane = td3_lane();
+    __shared__ double s_t[4][kCartGroup][64];
+    __shared__ int s_seg[4][65];
+    const int lane = td3_lane(), w = (int)(threadIdx.x / 64);
     const u32 wave = (u32)((blockIdx.x * blockDim.x + threadIdx.x) / 64);
     const u32 w0 = e_lo + wave * 64;
-    if (w0 >= e_hi) return;
+    if (w0 >= e_hi) return;  // uniform over the wavefront
     const u32 e = w0 + lane;
     const bool valid = e < e_hi;
-    const int d = valid ? dst[e] : -1 - lane;
+    const int d = valid ? dst[e] : -1;
     double T[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) T[i] = 0.0;
@@ -202,39 +219,40 @@ __global__ __launch_bounds__(256) void cart_m2l_kernel(
             }
         }
     }
-    // segmented inclusive scan over lanes of equal destination (as the log-kernel M2L)
+    // segments of equal destination among the valid lanes, by their first lane
     const int prev_d = __shfl_up(d, 1, 64);
-    const bool head = lane == 0 || prev_d != d;
+    const bool head = valid && (lane == 0 || prev_d != d);
     const u64 heads = __ballot(head);
-    const u64 le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const int seg_start = 63 - __clzll(heads & le);
+    const int nseg = __popcll(heads);
+    if (head) s_seg[w][__popcll(heads & ((1ull << lane) - 1ull))] = lane;
+    if (lane == 0) s_seg[w][nseg] = (int)(e_hi - w0 < 64 ? e_hi - w0 : 64);
 #pragma unroll
-    for (int i = 0; i < NC; ++i) {
+    for (int g0 = 0; g0 < NC; g0 += kCartGroup) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const double r = __shfl_up(T[i], off, 64);
-            if (lane - off >= seg_start) T[i] += r;
-        }
-    }
-    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-    if (!valid || !tail) return;
-    const u32 rs0 = row_start[d], rs1 = row_start[d + 1];
-    const bool before = rs0 < w0;
-    const bool after = rs1 > w0 + 64;
-    double* outp;
-    if (!before && !after)
-        outp = local + (i64)d * NC;
-    else
-        outp = side + ((i64)((w0 - e_lo) / 64) * 2 + (before ? 0 : 1)) * NC;
-    // divide by b! here for rows finished in this wave; partials stay primed
-    const bool finish = !before && !after;
-#pragma unroll
-    for (int nb = 0; nb <= P; ++nb) {
-#pragma unroll
-        for (int by = 0; by <= nb; ++by) {
-            const int bx = nb - by;
-            const double inv = 1.0 / (cfact(bx) * cfact(by));
-            outp[cidx(bx, by)] = finish ? T[cidx(bx, by)] * inv : T[cidx(bx, by)];
+        for (int i = 0; i < kCartGroup; ++i)
+            if (g0 + i < NC) s_t[w][i][lane] = T[g0 + i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int width = NC - g0 < kCartGroup ? NC - g0 : kCartGroup;
+        for (int it = lane; it < nseg * width; it += 64) {
+            const int sg = it / width, i = it % width, b = g0 + i;
+            const int l0 = s_seg[w][sg], l1 = s_seg[w][sg + 1];
+            double v = 0.0;
+            for (int l = l0; l < l1; ++l) v += s_t[w][i][l];
+            const int dd = dst[w0 + l0];
+            const u32 rs0 = row_start[dd], rs1 = row_start[dd + 1];
+            const bool before = rs0 < w0, after = rs1 > w0 + 64;
+            if (!before && !after) {
+                int deg = 0;
+                while (csize(deg) <= b) ++deg;
+                const int by = b - deg * (deg + 1) / 2, bx = deg - by;
+                local[(i64)dd * NC + b] = v / (kCartFact.v[bx] * kCartFact.v[by]);
+            } else {
+                // partial sums stay primed: the fixup divides by b!
+                side[((i64)((w0 - e_lo) / 64) * 2 + (before ? 0 : 1)) * NC + b] = v;
+            }
         }
     }
 }
@@ -255,10 +273,7 @@ __global__ void cart_m2l_fixup_kernel(const u32* __restrict__ row_start, i64 nod
     int deg = 0;
     while (csize(deg) <= i) ++deg;
     const int by = i - deg * (deg + 1) / 2, bx = deg - by;
-    double f = 1.0;
-    for (int k = 2; k <= bx; ++k) f *= k;
-    for (int k = 2; k <= by; ++k) f *= k;
-    local[node * nc + i] = v / f;
+    local[node * nc + i] = v / (kCartFact.v[bx] * kCartFact.v[by]);
 }
 
 // Orders above 12: the same correlation with runtime loops (slower, any p); one wavefront
