@@ -384,6 +384,16 @@ inline int __float_as_int(float f) {
     memcpy(&i, &f, 4);
     return i;
 }
+inline long long __double_as_longlong(double d) {
+    long long i;
+    memcpy(&i, &d, 8);
+    return i;
+}
+inline double __longlong_as_double(long long i) {
+    double d;
+    memcpy(&d, &i, 8);
+    return d;
+}
 inline float __int_as_float(int i) {
     float f;
     memcpy(&f, &i, 4);

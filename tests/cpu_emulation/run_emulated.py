@@ -325,6 +325,10 @@ def case_fmm_at():
     assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
     mixed = t.fmm_at(targets, exclude=exclude, theta=0.5, precision="mixed")
     assert relative(mixed[0], want[0]) < 1e-5 and relative(mixed[1], want[1]) < 1e-5
+    # target leaves of up to 30 points: 16 targets per pass of the row evaluation
+    got = t.fmm_at(targets, exclude=exclude, theta=0.5, leaf_capacity=30)
+    want = r.fmm_at(targets, exclude=exclude, theta=0.5, leaf_capacity=30)
+    assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
 
 
 def case_coulomb():

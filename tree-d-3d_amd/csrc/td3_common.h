@@ -106,6 +106,24 @@ template <int CTRL>
 __device__ __forceinline__ float td3_dpp_f(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
 }
+template <int CTRL>
+__device__ __forceinline__ double td3_dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum of a double over each group of `lanes` (4, 8 or 16, uniform) consecutive lanes by DPP
+// moves: every lane of the group gets the group's sum.
+__device__ __forceinline__ double td3_group_sum(double v, int lanes) {
+    v += td3_dpp_d<0xb1>(v);
+    v += td3_dpp_d<0x4e>(v);
+    if (lanes >= 8) v += td3_dpp_d<0x141>(v);
+    if (lanes >= 16) v += td3_dpp_d<0x140>(v);
+    return v;
+}
+
 __device__ __forceinline__ float td3_row16_sum(float v) {
     v += td3_dpp_f<0xb1>(v);
     v += td3_dpp_f<0x4e>(v);

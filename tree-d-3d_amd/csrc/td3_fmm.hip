@@ -734,6 +734,156 @@ __global__ void l2l16_kernel(const double* __restrict__ center, const double* __
 // TD3_COULOMB: the Cartesian locals T_g of degree <= order in lre (reference layout),
 // softened near field q / sqrt(r^2 + eps^2) (cart_evaluate_host_pyccel.py:16-110,
 // cart_targets_host_pyccel.py:16-98).
+// The exact sum over one partner leaf's sources (sf: the leaf's {start, end}) at a target
+// (tx, ty), into the FP64 sums.  MODE 0: the sources themselves (k the target's own index,
+// skipped; tq its charge); MODE 1: other points (sources without charge and the source
+// tskip skipped).  FP64: every term in FP64.  Mixed: differences in FP64 (exact for nearby
+// particles, whatever cells hold them), the kernel in FP32, FP32 sums over the leaf, then
+// FP64.  A pair at distance 0 (softening 0) sets bad instead of adding a term.
+template <int KERNEL, int PREC, int MODE>
+__device__ __forceinline__ void p2p_leaf(double tx, double ty, double tq, i64 k, i64 tskip,
+                                         int4 sf, const double4* __restrict__ src4,
+                                         const i64* __restrict__ sperm, double eps2,
+                                         int with_potential, double& acc_p, double& acc_x,
+                                         double& acc_y, bool& bad) {
+    if (PREC == TD3_FP64) {
+        for (int ks = sf.x; ks < sf.y; ++ks) {
+            const double4 sv = src4[ks];
+            const double sq = sv.z;
+            if (MODE == 0 && ks == k) continue;
+            if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
+            const double dx = tx - sv.x, dy = ty - sv.y;
+            const double r2 = dx * dx + dy * dy + eps2;
+            if (r2 == 0.0) {
+                if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
+                continue;
+            }
+            if (KERNEL == TD3_COULOMB) {
+                const double inv_r = 1.0 / sqrt(r2);
+                const double cr3 = sq * inv_r * inv_r * inv_r;
+                acc_p += sq * inv_r;
+                acc_x += cr3 * dx;
+                acc_y += cr3 * dy;
+            } else {
+                if (with_potential) acc_p -= sq * log(r2);
+                const double inv = 2.0 * sq / r2;
+                acc_x += inv * dx;
+                acc_y += inv * dy;
+            }
+        }
+    } else {
+        const float eps2f = (float)eps2;
+        float fp = 0.0f, fx = 0.0f, fy = 0.0f;
+        for (int ks = sf.x; ks < sf.y; ++ks) {
+            const double4 sv = src4[ks];
+            const double sq = sv.z;
+            if (MODE == 0 && ks == k) continue;
+            if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
+            const double ddx = tx - sv.x, ddy = ty - sv.y;
+            const float dx = (float)ddx, dy = (float)ddy, qf = (float)sq;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
+            if (r2 == 0.0f) {
+                // zero or below FP32 range: decide on the FP64 pair
+                const double dr2 = ddx * ddx + ddy * ddy + eps2;
+                if (dr2 == 0.0) {
+                    if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
+                    continue;
+                }
+                if (KERNEL == TD3_COULOMB) {
+                    const double inv_r = 1.0 / sqrt(dr2);
+                    acc_p += sq * inv_r;
+                    acc_x += sq * inv_r * inv_r * inv_r * ddx;
+                    acc_y += sq * inv_r * inv_r * inv_r * ddy;
+                } else {
+                    if (with_potential) acc_p -= sq * log(dr2);
+                    acc_x += 2.0 * sq / dr2 * ddx;
+                    acc_y += 2.0 * sq / dr2 * ddy;
+                }
+                continue;
+            }
+            if (KERNEL == TD3_COULOMB) {
+                const float inv_r = __builtin_amdgcn_rsqf(r2);
+                const float c = qf * inv_r;
+                const float cr3 = c * inv_r * inv_r;
+                fp += c;
+                fx = fmaf(cr3, dx, fx);
+                fy = fmaf(cr3, dy, fy);
+            } else {
+                // -q ln r2 = -ln 2 q log2 r2 and 2 q d / r2; the constants at the end
+                if (with_potential) fp = fmaf(qf, __builtin_amdgcn_logf(r2), fp);
+                const float inv = qf * __builtin_amdgcn_rcpf(r2);
+                fx = fmaf(inv, dx, fx);
+                fy = fmaf(inv, dy, fy);
+            }
+        }
+        if (KERNEL == TD3_COULOMB) {
+            acc_p += (double)fp;
+            acc_x += (double)fx;
+            acc_y += (double)fy;
+        } else {
+            acc_p -= 0.6931471805599453 * (double)fp;
+            acc_x += 2.0 * (double)fx;
+            acc_y += 2.0 * (double)fy;
+        }
+    }
+}
+
+// The far field at a target of leaf `leaf` from its local expansion, added to the exact
+// sums, written at input index i.  Coulomb: phi += sum_g T_g u^g, E -= grad (u = z - c,
+// unscaled Cartesian locals); log: Horner on the scaled local expansion at (z - c) / r.
+template <int KERNEL>
+__device__ __forceinline__ void l2p_write(double tx, double ty, i64 leaf,
+                                          const double* __restrict__ tcenter,
+                                          const double* __restrict__ thalf,
+                                          const double* __restrict__ lre,
+                                          const double* __restrict__ lim, int p1, int order,
+                                          double acc_p, double acc_x, double acc_y,
+                                          int with_potential, i64 i, i64 m,
+                                          double* __restrict__ phi, double* __restrict__ field) {
+    const double cx = tcenter[2 * leaf], cy = tcenter[2 * leaf + 1];
+    if (KERNEL == TD3_COULOMB) {
+        const double ux = tx - cx, uy = ty - cy;
+        const double* T = lre + leaf * (i64)p1;
+        double w = 0.0, gx = 0.0, gy = 0.0, pyb = 1.0, pyb1 = 0.0;
+        for (int by = 0; by <= order; ++by) {
+            double px = 1.0, px1 = 0.0;
+            for (int bx = 0; bx + by <= order; ++bx) {
+                const double t = T[(bx + by) * (bx + by + 1) / 2 + by];
+                w += t * px * pyb;
+                gx += bx * t * px1 * pyb;
+                gy += by * t * px * pyb1;
+                px1 = px;
+                px *= ux;
+            }
+            pyb1 = pyb;
+            pyb *= uy;
+        }
+        phi[i] = with_potential ? acc_p + w : 0.0;
+        field[i] = acc_x - gx;
+        field[m + i] = acc_y - gy;
+    } else {
+        const double inv_r = 1.0 / (1.4142135623730951 * thalf[leaf]);
+        const double ur = (tx - cx) * inv_r, ui = (ty - cy) * inv_r;
+        const double* bre = lre + leaf * (i64)p1;
+        const double* bim = lim + leaf * (i64)p1;
+        double w_re = bre[p1 - 1], w_im = bim[p1 - 1];
+        double d_re = 0.0, d_im = 0.0;
+        for (int l = p1 - 2; l >= 0; --l) {
+            const double nd = d_re * ur - d_im * ui + w_re;
+            d_im = d_re * ui + d_im * ur + w_im;
+            d_re = nd;
+            const double nw = w_re * ur - w_im * ui + bre[l];
+            w_im = w_re * ui + w_im * ur + bim[l];
+            w_re = nw;
+        }
+        phi[i] = with_potential ? acc_p - 2.0 * w_re : 0.0;
+        field[i] = acc_x + 2.0 * d_re * inv_r;
+        field[m + i] = acc_y - 2.0 * d_im * inv_r;
+    }
+}
+
+// L2P + P2P with a lane per target (the FMM at its own sources, MODE 0): the lane sums its
+// leaf's P2P row (the next partner leaf's range loaded while this one's sources are summed).
 template <int KERNEL, int PREC, int MODE>
 __global__ __launch_bounds__(256) void eval_kernel(
     const double* __restrict__ tpos, const i64* __restrict__ tperm, const i64* __restrict__ exclude,
@@ -747,151 +897,79 @@ __global__ __launch_bounds__(256) void eval_kernel(
     double* __restrict__ phi, double* __restrict__ field, i64* __restrict__ status) {
     const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
-    {
-        const i64 leaf = leaf_of[k];
+    const i64 leaf = leaf_of[k];
+    const double tx = tpos[k], ty = tpos[m + k];
+    const double tq = MODE == 0 ? scharge[k] : 0.0;
+    const i64 tskip = MODE == 1 ? exclude[tperm[k]] : -1;
+    double acc_p = 0.0, acc_x = 0.0, acc_y = 0.0;
+    bool bad = false;
+    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
+    int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
+    for (u32 r = r0; r < r1; ++r) {
+        const int4 sf = sf_next;
+        if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
+        p2p_leaf<KERNEL, PREC, MODE>(tx, ty, tq, k, tskip, sf, src4, sperm, eps2, with_potential,
+                                     acc_p, acc_x, acc_y, bad);
+    }
+    const i64 i = tperm[k];
+    if (bad) td3_report(status, i + 1);
+    l2p_write<KERNEL>(tx, ty, leaf, tcenter, thalf, lre, lim, p1, order, acc_p, acc_x, acc_y,
+                      with_potential, i, m, phi, field);
+}
+
+// L2P + P2P at other points (fmm_at, MODE 1), a wavefront per target leaf: its targets go
+// tl = 4, 8 or 16 at a time (by the leaf's size), each with 64 / tl lanes that split the
+// leaf's P2P row between them (lane s of a target takes partner leaves s, s + 64 / tl, ...),
+// then sum their partial sums over DPP in a fixed order.  A leaf whose row holds thousands
+// of sources (a grid over a dense cluster) is spread over the whole wavefront instead of
+// one lane per target.
+template <int KERNEL, int PREC>
+__global__ __launch_bounds__(256) void eval_rows_kernel(
+    const double* __restrict__ tpos, const i64* __restrict__ tperm,
+    const i64* __restrict__ exclude, i64 m, const double* __restrict__ tcenter,
+    const double* __restrict__ thalf, const int4* __restrict__ tinfo, i64 t_count,
+    const double* __restrict__ lre, const double* __restrict__ lim, int p1, int order,
+    const u32* __restrict__ prow, const int* __restrict__ pcol, const i64* __restrict__ sperm,
+    const int4* __restrict__ sinfo, const double4* __restrict__ src4, double eps2,
+    int with_potential, double* __restrict__ phi, double* __restrict__ field,
+    i64* __restrict__ status) {
+    const int lane = td3_lane();
+    const i64 leaf = ((i64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (leaf >= t_count) return;  // uniform over the wavefront
+    const int4 tf = tinfo[leaf];
+    if (tf.z >= 0 || tf.y <= tf.x) return;  // an internal or empty cell
+    const int nt = tf.y - tf.x;
+    const int tl = nt <= 4 ? 4 : (nt <= 8 ? 8 : 16);
+    const int lanes = 64 / tl, sub = lane & (lanes - 1);
+    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
+    for (int c0 = 0; c0 < nt; c0 += tl) {
+        const int tloc = c0 + lane / lanes;
+        const bool tvalid = tloc < nt;
+        const i64 k = tf.x + (tvalid ? tloc : 0);
         const double tx = tpos[k], ty = tpos[m + k];
-        const double tq = MODE == 0 ? scharge[k] : 0.0;
-        const i64 tskip = MODE == 1 ? exclude[tperm[k]] : -1;
+        const i64 tskip = exclude[tperm[k]];
         double acc_p = 0.0, acc_x = 0.0, acc_y = 0.0;
         bool bad = false;
-        const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
-        if (PREC == TD3_FP64) {
-            // the next partner leaf's range is loaded while this one's sources are summed
-            int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
-            for (u32 r = r0; r < r1; ++r) {
+        if (tvalid) {
+            u32 r = r0 + (u32)sub;
+            int4 sf_next = r < r1 ? sinfo[pcol[r]] : make_int4(0, 0, 0, 0);
+            for (; r < r1; r += (u32)lanes) {
                 const int4 sf = sf_next;
-                if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
-                for (int ks = sf.x; ks < sf.y; ++ks) {
-                    const double4 sv = src4[ks];
-                    const double sq = sv.z;
-                    if (MODE == 0 && ks == k) continue;
-                    if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
-                    const double dx = tx - sv.x, dy = ty - sv.y;
-                    const double r2 = dx * dx + dy * dy + eps2;
-                    if (r2 == 0.0) {
-                        if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
-                        continue;
-                    }
-                    if (KERNEL == TD3_COULOMB) {
-                        const double inv_r = 1.0 / sqrt(r2);
-                        const double cr3 = sq * inv_r * inv_r * inv_r;
-                        acc_p += sq * inv_r;
-                        acc_x += cr3 * dx;
-                        acc_y += cr3 * dy;
-                    } else {
-                        if (with_potential) acc_p -= sq * log(r2);
-                        const double inv = 2.0 * sq / r2;
-                        acc_x += inv * dx;
-                        acc_y += inv * dy;
-                    }
-                }
-            }
-        } else {
-            // Differences in FP64 (exact for nearby particles, whatever cells hold them),
-            // kernel in FP32; FP32 sums over one partner leaf, FP64 sums over the leaves.
-            const float eps2f = (float)eps2;
-            // the next partner leaf's range is loaded while this one's sources are summed
-            int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
-            for (u32 r = r0; r < r1; ++r) {
-                const int4 sf = sf_next;
-                if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
-                float fp = 0.0f, fx = 0.0f, fy = 0.0f;
-                for (int ks = sf.x; ks < sf.y; ++ks) {
-                    const double4 sv = src4[ks];
-                    const double sq = sv.z;
-                    if (MODE == 0 && ks == k) continue;
-                    if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
-                    const double ddx = tx - sv.x, ddy = ty - sv.y;
-                    const float dx = (float)ddx, dy = (float)ddy, qf = (float)sq;
-                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
-                    if (r2 == 0.0f) {
-                        // zero or below FP32 range: decide on the FP64 pair
-                        const double dr2 = ddx * ddx + ddy * ddy + eps2;
-                        if (dr2 == 0.0) {
-                            if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
-                            continue;
-                        }
-                        if (KERNEL == TD3_COULOMB) {
-                            const double inv_r = 1.0 / sqrt(dr2);
-                            acc_p += sq * inv_r;
-                            acc_x += sq * inv_r * inv_r * inv_r * ddx;
-                            acc_y += sq * inv_r * inv_r * inv_r * ddy;
-                        } else {
-                            if (with_potential) acc_p -= sq * log(dr2);
-                            acc_x += 2.0 * sq / dr2 * ddx;
-                            acc_y += 2.0 * sq / dr2 * ddy;
-                        }
-                        continue;
-                    }
-                    if (KERNEL == TD3_COULOMB) {
-                        const float inv_r = __builtin_amdgcn_rsqf(r2);
-                        const float c = qf * inv_r;
-                        const float cr3 = c * inv_r * inv_r;
-                        fp += c;
-                        fx = fmaf(cr3, dx, fx);
-                        fy = fmaf(cr3, dy, fy);
-                    } else {
-                        // -q ln r2 = -ln 2 q log2 r2 and 2 q d / r2; the constants at the end
-                        if (with_potential) fp = fmaf(qf, __builtin_amdgcn_logf(r2), fp);
-                        const float inv = qf * __builtin_amdgcn_rcpf(r2);
-                        fx = fmaf(inv, dx, fx);
-                        fy = fmaf(inv, dy, fy);
-                    }
-                }
-                if (KERNEL == TD3_COULOMB) {
-                    acc_p += (double)fp;
-                    acc_x += (double)fx;
-                    acc_y += (double)fy;
-                } else {
-                    acc_p -= 0.6931471805599453 * (double)fp;
-                    acc_x += 2.0 * (double)fx;
-                    acc_y += 2.0 * (double)fy;
-                }
+                if (r + lanes < r1) sf_next = sinfo[pcol[r + lanes]];
+                p2p_leaf<KERNEL, PREC, 1>(tx, ty, 0.0, k, tskip, sf, src4, sperm, eps2,
+                                          with_potential, acc_p, acc_x, acc_y, bad);
             }
         }
-        const i64 i = tperm[k];
-        if (bad) td3_report(status, i + 1);
-        const double cx = tcenter[2 * leaf], cy = tcenter[2 * leaf + 1];
-        if (KERNEL == TD3_COULOMB) {
-            // phi += sum_g T_g u^g, E -= grad (u = z - c, unscaled Cartesian locals)
-            const double ux = tx - cx, uy = ty - cy;
-            const double* T = lre + leaf * (i64)p1;
-            double w = 0.0, gx = 0.0, gy = 0.0, pyb = 1.0, pyb1 = 0.0;
-            for (int by = 0; by <= order; ++by) {
-                double px = 1.0, px1 = 0.0;
-                for (int bx = 0; bx + by <= order; ++bx) {
-                    const double t = T[(bx + by) * (bx + by + 1) / 2 + by];
-                    w += t * px * pyb;
-                    gx += bx * t * px1 * pyb;
-                    gy += by * t * px * pyb1;
-                    px1 = px;
-                    px *= ux;
-                }
-                pyb1 = pyb;
-                pyb *= uy;
-            }
-            phi[i] = with_potential ? acc_p + w : 0.0;
-            field[i] = acc_x - gx;
-            field[m + i] = acc_y - gy;
-        } else {
-            // Horner on the scaled local expansion at u' = (z - c) / r
-            const double inv_r = 1.0 / (1.4142135623730951 * thalf[leaf]);
-            const double ur = (tx - cx) * inv_r, ui = (ty - cy) * inv_r;
-            const double* bre = lre + leaf * (i64)p1;
-            const double* bim = lim + leaf * (i64)p1;
-            double w_re = bre[p1 - 1], w_im = bim[p1 - 1];
-            double d_re = 0.0, d_im = 0.0;
-            for (int l = p1 - 2; l >= 0; --l) {
-                const double nd = d_re * ur - d_im * ui + w_re;
-                d_im = d_re * ui + d_im * ur + w_im;
-                d_re = nd;
-                const double nw = w_re * ur - w_im * ui + bre[l];
-                w_im = w_re * ui + w_im * ur + bim[l];
-                w_re = nw;
-            }
-            phi[i] = with_potential ? acc_p - 2.0 * w_re : 0.0;
-            field[i] = acc_x + 2.0 * d_re * inv_r;
-            field[m + i] = acc_y - 2.0 * d_im * inv_r;
+        acc_p = td3_group_sum(acc_p, lanes);
+        acc_x = td3_group_sum(acc_x, lanes);
+        acc_y = td3_group_sum(acc_y, lanes);
+        const u64 group = (lanes == 64 ? ~0ull : ((1ull << lanes) - 1ull)) << (lane & ~(lanes - 1));
+        const bool group_bad = (__ballot(bad) & group) != 0;
+        if (tvalid && sub == 0) {
+            const i64 i = tperm[k];
+            if (group_bad) td3_report(status, i + 1);
+            l2p_write<KERNEL>(tx, ty, leaf, tcenter, thalf, lre, lim, p1, order, acc_p, acc_x,
+                              acc_y, with_potential, i, m, phi, field);
         }
     }
 }
@@ -1028,6 +1106,30 @@ int launch_eval(int precision, hipStream_t s, const double* tpos, const i64* tpe
                            tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre, lim, p1, order,
                            prow, pcol, spos, scharge, sperm, n, sinfo, leaf_of, src4, eps2,
                            with_potential, phi, field, status));
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+template <int KERNEL>
+int launch_eval_rows(int precision, hipStream_t s, const double* tpos, const i64* tperm,
+                     const i64* exclude, i64 m, const double* tcenter, const double* thalf,
+                     const int4* tinfo, i64 t_count, const double* lre, const double* lim,
+                     int order, const u32* prow, const int* pcol, const i64* sperm,
+                     const int4* sinfo, const double4* src4, double eps2, int with_potential,
+                     double* phi, double* field, i64* status) {
+    TD3_ARG(src4 != nullptr, "sources4 (td3_pack_sources of the source tree)");
+    const unsigned blocks = td3_blocks(t_count * 64, 256);
+    const int p1 = KERNEL == TD3_LOG ? order + 1 : (order + 1) * (order + 2) / 2;
+    if (precision == TD3_FP64)
+        TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_rows_kernel<KERNEL, TD3_FP64>), dim3(blocks),
+                  dim3(256), 0, s, tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre,
+                  lim, p1, order, prow, pcol, sperm, sinfo, src4, eps2, with_potential, phi,
+                  field, status));
+    else
+        TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_rows_kernel<KERNEL, TD3_MIXED>), dim3(blocks),
+                  dim3(256), 0, s, tpos, tperm, exclude, m, tcenter, thalf, tinfo, t_count, lre,
+                  lim, p1, order, prow, pcol, sperm, sinfo, src4, eps2, with_potential, phi,
+                  field, status));
     TD3_LAUNCHED();
     return TD3_OK;
 }
@@ -1240,12 +1342,11 @@ extern "C" int td3_fmm_at_eval(int kernel, int precision, int64_t order,
     TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(order >= 1 && order <= 64, "order");
     TD3_ARG(m >= 1 && n >= 1 && t_count >= 1, "sizes");
-    auto go = kernel == TD3_LOG ? launch_eval<TD3_LOG, 1> : launch_eval<TD3_COULOMB, 1>;
+    auto go = kernel == TD3_LOG ? launch_eval_rows<TD3_LOG> : launch_eval_rows<TD3_COULOMB>;
     return go(precision, (hipStream_t)stream, sorted_targets, target_perm, exclude, m, t_center,
               t_half, (const int4*)t_info, t_count, local_re, local_im, (int)order, p2p_row,
-              p2p_col, sorted_pos, sorted_charge, perm, n, (const int4*)s_info, t_leaf_of,
-              (const double4*)s_sources4, softening * softening, with_potential, phi, field,
-              status);
+              p2p_col, perm, (const int4*)s_info, (const double4*)s_sources4,
+              softening * softening, with_potential, phi, field, status);
 }
 
 namespace {
