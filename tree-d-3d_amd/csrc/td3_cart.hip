@@ -95,46 +95,82 @@ __global__ void cart_m2m_kernel(const double* __restrict__ center, const int4* _
 }
 
 // The same with a wavefront per node (orders up to 13: nc <= kCartStage): the four
-// children's moments are staged in LDS once, instead of every (node, i) thread
-// reloading up to (g_x + 1)(g_y + 1) of them per child.  Same terms in the same order.
+// children's moments, the powers of their offsets and the binomials are staged in LDS once,
+// and the shift is done one axis at a time (sum_{b <= g} C(g, b) s^(g - b) m_b factors into
+// a shift in x followed by one in y): at most 4 (p + 1) terms per output and pass instead of
+// 4 (gx + 1)(gy + 1).  Lane l forms output l, and output 64 + l past 64 (at order 10 and 11
+// the cheapest low-degree lanes).
 constexpr int kCartStage = 105;  // csize(13)
+constexpr int kCartDeg = 14;     // 13 + 1
+
+// binomials C(n, k), n, k <= p, from the caller's table into LDS (every thread of the block)
+__device__ __forceinline__ void cart_stage_binomials(double (*s_bin)[kCartDeg],
+                                                     const double* __restrict__ binom, int bsz,
+                                                     int p) {
+    for (int i = threadIdx.x; i < (p + 1) * (p + 1); i += blockDim.x)
+        s_bin[i / (p + 1)][i % (p + 1)] = binom[(i / (p + 1)) * bsz + i % (p + 1)];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void cart_degree_split(int i, int& gx, int& gy) {
+    int deg = 0;
+    while (csize(deg) <= i) ++deg;
+    gy = i - deg * (deg + 1) / 2;
+    gx = deg - gy;
+}
+
 __global__ __launch_bounds__(256) void cart_m2m_lds_kernel(
     const double* __restrict__ center, const int4* __restrict__ info, i64 lo, i64 hi, int p,
     int nc, const double* __restrict__ binom, int bsz, double* __restrict__ mom) {
+    __shared__ double s_bin[kCartDeg][kCartDeg];
     __shared__ double s_m[4][4][kCartStage];
+    __shared__ double s_pw[4][4][2][kCartDeg];
+    cart_stage_binomials(s_bin, binom, bsz, p);
     const int w = (int)(threadIdx.x / 64), lane = td3_lane();
     const i64 node = lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
     if (node >= hi) return;  // uniform over the wavefront
     const int4 nf = info[node];
     if (nf.z < 0) return;
+    const double cx = center[2 * node], cy = center[2 * node + 1];
     for (int c = 0; c < 4; ++c) {
         const int4 cf = info[nf.z + c];
         for (int i = lane; i < nc; i += 64)
             s_m[w][c][i] = cf.y == cf.x ? 0.0 : mom[(i64)(nf.z + c) * nc + i];
+        // s^k of the child's offset, k <= p, by repeated products
+        if (lane < 2) {
+            const double sv = center[2 * (nf.z + c) + lane] - (lane ? cy : cx);
+            double pw = 1.0;
+            for (int k = 0; k <= p; ++k) {
+                s_pw[w][c][lane][k] = pw;
+                pw *= sv;
+            }
+        }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const double cx = center[2 * node], cy = center[2 * node + 1];
-    for (int i = lane; i < nc; i += 64) {
-        int deg = 0;
-        while (csize(deg) <= i) ++deg;
-        const int gy = i - deg * (deg + 1) / 2, gx = deg - gy;
-        double acc = 0.0;
-        for (int c = 0; c < 4; ++c) {
-            const int4 cf = info[nf.z + c];
-            if (cf.y == cf.x) continue;
-            const double sx = center[2 * (nf.z + c)] - cx, sy = center[2 * (nf.z + c) + 1] - cy;
-            double px = 1.0;
-            for (int bx = gx; bx >= 0; --bx) {
-                const double fx = binom[gx * bsz + bx] * px;
-                double py = 1.0;
-                for (int by = gy; by >= 0; --by) {
-                    acc += fx * binom[gy * bsz + by] * s_m[w][c][cidx(bx, by)] * py;
-                    py *= sy;
-                }
-                px *= sx;
-            }
+    // the shift in x: Y_c(gx, by) = sum_{bx <= gx} C(gx, bx) s_x^(gx - bx) m_c(bx, by), in
+    // place of each child's moments (entries of the same degree-ordered layout; entry
+    // (gx, by) reads only (bx <= gx, by), so each child's entries are formed from the
+    // highest gx down, one gx at a time)
+    for (int gx = p; gx >= 0; --gx) {
+        for (int it = lane; it < 4 * (p - gx + 1); it += 64) {
+            const int c = it / (p - gx + 1), by = it % (p - gx + 1);
+            double acc = 0.0;
+            for (int bx = gx; bx >= 0; --bx)
+                acc += s_bin[gx][bx] * s_pw[w][c][0][gx - bx] * s_m[w][c][cidx(bx, by)];
+            s_m[w][c][cidx(gx, by)] = acc;  // read by lower gx only after this row is done
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    // the shift in y and the sum over the children
+    for (int i = lane; i < nc; i += 64) {
+        int gx, gy;
+        cart_degree_split(i, gx, gy);
+        double acc = 0.0;
+        for (int c = 0; c < 4; ++c)
+            for (int by = gy; by >= 0; --by)
+                acc += s_bin[gy][by] * s_pw[w][c][1][gy - by] * s_m[w][c][cidx(gx, by)];
         mom[node * nc + i] = acc;
     }
 }
@@ -369,40 +405,44 @@ __global__ void cart_l2l_kernel(const double* __restrict__ center, const int4* _
     local[child * nc + i] += acc;
 }
 
-// The same with a wavefront per parent: its coefficients staged in LDS once for the four
-// children's 4 nc outputs.
+// The same with a wavefront per parent: its coefficients, the powers of its children's
+// offsets and the binomials staged in LDS once for the four children's 4 nc outputs.  Same
+// terms in the same order as cart_l2l_kernel.
 __global__ __launch_bounds__(256) void cart_l2l_lds_kernel(
     const double* __restrict__ center, const int4* __restrict__ info, i64 lo, i64 hi, int p,
     int nc, const double* __restrict__ binom, int bsz, double* __restrict__ local) {
+    __shared__ double s_bin[kCartDeg][kCartDeg];
     __shared__ double s_l[4][kCartStage];
+    __shared__ double s_pw[4][4][2][kCartDeg];
+    cart_stage_binomials(s_bin, binom, bsz, p);
     const int w = (int)(threadIdx.x / 64), lane = td3_lane();
     const i64 parent = lo + ((i64)blockIdx.x * blockDim.x + threadIdx.x) / 64;
     if (parent >= hi) return;  // uniform over the wavefront
     const int4 pf = info[parent];
     if (pf.z < 0) return;
     for (int i = lane; i < nc; i += 64) s_l[w][i] = local[parent * nc + i];
+    if (lane < 8) {
+        const int c = lane >> 1, axis = lane & 1;
+        const double sv = center[2 * (pf.z + c) + axis] - center[2 * parent + axis];
+        double pw = 1.0;
+        for (int k = 0; k <= p; ++k) {
+            s_pw[w][c][axis][k] = pw;
+            pw *= sv;
+        }
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     for (int o = lane; o < 4 * nc; o += 64) {
-        const i64 child = pf.z + o / nc;
-        const int i = o % nc;
-        int deg = 0;
-        while (csize(deg) <= i) ++deg;
-        const int by = i - deg * (deg + 1) / 2, bx = deg - by;
-        const double sx = center[2 * child] - center[2 * parent];
-        const double sy = center[2 * child + 1] - center[2 * parent + 1];
+        const int c = o / nc, i = o % nc;
+        int bx, by;
+        cart_degree_split(i, bx, by);
         double acc = 0.0;
-        double px = 1.0;
         for (int gx = bx; gx <= p; ++gx) {
-            const double fx = binom[gx * bsz + bx] * px;
-            double py = 1.0;
-            for (int gy = by; gx + gy <= p; ++gy) {
-                acc += fx * binom[gy * bsz + by] * s_l[w][cidx(gx, gy)] * py;
-                py *= sy;
-            }
-            px *= sx;
+            const double fx = s_bin[gx][bx] * s_pw[w][c][0][gx - bx];
+            for (int gy = by; gx + gy <= p; ++gy)
+                acc += fx * s_bin[gy][by] * s_l[w][cidx(gx, gy)] * s_pw[w][c][1][gy - by];
         }
-        local[child * nc + i] += acc;
+        local[(pf.z + c) * (i64)nc + i] += acc;
     }
 }
 
