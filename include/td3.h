@@ -273,11 +273,15 @@ int td3_rows(const double *node_center, const double *node_half, const void *nod
  * b under a and a under b; mode 1 the same with a self pair once (the P2P leaves); mode 2
  * one way, b under a (fmm_at's target rows).  row_start (node_count + 1) uint32, col
  * int32 sorted ascending within each row, dst (may be NULL) the row of each entry.
+ * col_count: the number of nodes the columns index (node_count for modes 0 and 1, the
+ * source tree's for mode 2).  The directed entries are radix-sorted as (row, column)
+ * keys, so the rows and their order come with no atomics; the workspace
+ * (td3_lists_workspace_bytes) holds the keys.  No read-back.
  */
-size_t td3_lists_workspace_bytes(int64_t node_count);
+size_t td3_lists_workspace_bytes(int64_t node_count, int64_t count, int mode);
 int td3_lists(const int32_t *pairs, int64_t count, int mode, int64_t node_count,
-              uint32_t *row_start, int32_t *col, int32_t *dst, void *workspace,
-              size_t workspace_bytes, void *stream);
+              int64_t col_count, uint32_t *row_start, int32_t *col, int32_t *dst,
+              void *workspace, size_t workspace_bytes, void *stream);
 
 /*
  * Radius-scaled moments A_k = a_k / r^k, r = sqrt(2) node_half (FP64 -> double or float

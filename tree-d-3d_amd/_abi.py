@@ -57,8 +57,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                   _I, _P, _D, _D, _C, _P, _P, _P, _P], _C),
     "td3_traverse_workspace_bytes": ([_I], _Z),
     "td3_traverse": ([_P, _P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P, _Z, _P, _P], _C),
-    "td3_lists_workspace_bytes": ([_I], _Z),
-    "td3_lists": ([_P, _I, _C, _I, _P, _P, _P, _P, _Z, _P], _C),
+    "td3_lists_workspace_bytes": ([_I, _I, _C], _Z),
+    "td3_lists": ([_P, _I, _C, _I, _I, _P, _P, _P, _P, _Z, _P], _C),
     "td3_scale_moments": ([_C, _P, _P, _P, _I, _I, _P, _P, _P], _C),
     "td3_m2l_workspace_bytes": ([_I, _I, _C], _Z),
     "td3_m2l": ([_C, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P], _C),

@@ -174,100 +174,56 @@ __global__ void traverse_init_kernel(int2* frontier, u32* ctr) {
     ctr[3] = 1u;
 }
 
-// Row lengths of the directed lists.  mode 0: both ways; 1: both ways, a self pair once;
-// 2: one way (b under a only).
-__device__ __forceinline__ bool both_ways(int mode, int2 p) {
-    return mode == 0 || (mode == 1 && p.x != p.y);
-}
-
-__global__ void degree_kernel(const int2* __restrict__ pairs, u32 count, int mode,
-                              u32* __restrict__ deg) {
+// The directed entries of a pair list as sort keys (row node << sb) | column.  mode 0:
+// both ways; 1: both ways, a self pair once (its second slot a padding key, all ones, which
+// sorts after every entry); 2: one way (b under a only).
+__global__ void list_keys_kernel(const int2* __restrict__ pairs, u32 count, int mode, int sb,
+                                 u64* __restrict__ keys) {
     const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     const int2 p = pairs[k];
-    atomicAdd(&deg[p.x], 1u);
-    if (both_ways(mode, p)) atomicAdd(&deg[p.y], 1u);
-}
-
-__global__ void fill_kernel(const int2* __restrict__ pairs, u32 count, int mode,
-                            u32* __restrict__ cursor, int* __restrict__ col,
-                            int* __restrict__ dst) {
-    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    const int2 p = pairs[k];
-    u32 o = atomicAdd(&cursor[p.x], 1u);
-    col[o] = p.y;
-    if (dst) dst[o] = p.x;
-    if (both_ways(mode, p)) {
-        o = atomicAdd(&cursor[p.y], 1u);
-        col[o] = p.x;
-        if (dst) dst[o] = p.y;
+    const u64 ab = ((u64)(u32)p.x << sb) | (u32)p.y;
+    if (mode == 2) {
+        keys[k] = ab;
+        return;
     }
+    keys[2 * (u64)k] = ab;
+    keys[2 * (u64)k + 1] = (mode == 0 || p.x != p.y) ? (((u64)(u32)p.y << sb) | (u32)p.x) : ~0ull;
 }
 
-// Sorts every row of col ascending: one wavefront per row, bitonic in LDS.
-constexpr int kSortMax = 4096;
-__global__ __launch_bounds__(256) void row_sort_kernel(const u32* __restrict__ row_start, i64 rows,
-                                                       int* __restrict__ col, u32* __restrict__ err) {
-    __shared__ int s_buf[4][kSortMax];
-    const int wave = threadIdx.x / 64, lane = td3_lane();
-    int* buf = s_buf[wave];
-    for (i64 r = (i64)blockIdx.x * 4 + wave; r < rows; r += (i64)gridDim.x * 4) {
-        const u32 s = row_start[r], e = row_start[r + 1];
-        const int len = (int)(e - s);
-        if (len <= 1) continue;
-        if (len <= 64) {
-            // bitonic network across the lanes, one entry per lane
-            int v = lane < len ? col[s + lane] : 0x7fffffff;
-#pragma unroll
-            for (int size = 2; size <= 64; size <<= 1) {
-#pragma unroll
-                for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                    const int o = __shfl_xor(v, stride, 64);
-                    const bool up = (lane & size) == 0;
-                    const bool lower = (lane & stride) == 0;
-                    v = (lower == up) ? (v < o ? v : o) : (v > o ? v : o);
-                }
-            }
-            if (lane < len) col[s + lane] = v;
-            continue;
-        }
-        if (len > kSortMax) {
-            if (lane == 0) atomicOr(err, 1u);
-            continue;
-        }
-        int pw = 2;
-        while (pw < len) pw <<= 1;
-        for (int i = lane; i < pw; i += 64) buf[i] = i < len ? col[s + i] : 0x7fffffff;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        for (int size = 2; size <= pw; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int i = lane; i < pw / 2; i += 64) {
-                    const int lo = 2 * i - (i & (stride - 1));
-                    const int hi = lo + stride;
-                    const bool up = (lo & size) == 0;
-                    const int x = buf[lo], y = buf[hi];
-                    if ((x > y) == up) {
-                        buf[lo] = y;
-                        buf[hi] = x;
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            }
-        }
-        for (int i = lane; i < len; i += 64) col[s + i] = buf[i];
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+// Rows from the sorted keys: entry k's column (and row node), and the starts of every row
+// from the one after entry k - 1's up to entry k's (a padding key counts as row node_count,
+// so the first one closes every row).
+__global__ void list_rows_kernel(const u64* __restrict__ keys, i64 n, int sb, i64 node_count,
+                                 u32* __restrict__ row_start, int* __restrict__ col,
+                                 int* __restrict__ dst) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const u64 key = keys[k];
+    const i64 d = (i64)(key >> sb) < node_count ? (i64)(key >> sb) : node_count;
+    if (d < node_count) {
+        col[k] = (int)(key & ((1ull << sb) - 1ull));
+        if (dst) dst[k] = (int)d;
     }
+    i64 prev = -1;
+    if (k > 0) {
+        const i64 pd = (i64)(keys[k - 1] >> sb);
+        prev = pd < node_count ? pd : node_count;
+    }
+    for (i64 r = prev + 1; r <= d; ++r) row_start[r] = (u32)k;
+    if (k == n - 1)
+        for (i64 r = d + 1; r <= node_count; ++r) row_start[r] = (u32)n;
 }
 
-// dst of each directed pair after the row sort (rows keep their node).
-__global__ void row_dst_kernel(const u32* __restrict__ row_start, i64 rows, int* __restrict__ dst) {
+__global__ void list_empty_kernel(i64 node_count, u32* __restrict__ row_start) {
     const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    for (u32 k = row_start[r]; k < row_start[r + 1]; ++k) dst[k] = (int)r;
+    if (r <= node_count) row_start[r] = 0u;
+}
+
+static inline int bits_for(i64 v) {  // bits of v (at least 1)
+    int b = 1;
+    while (b < 63 && (v >> b) != 0) ++b;
+    return b;
 }
 
 }  // namespace
@@ -374,50 +330,50 @@ extern "C" int td3_dual_traverse(const double* t_center, const double* t_half, c
     return TD3_OK;
 }
 
-extern "C" size_t td3_lists_workspace_bytes(int64_t node_count) {
-    return (size_t)(node_count + 1) * sizeof(u32) * 2 + td3sort::scratch_bytes(node_count + 1) + 1024;
+static inline i64 list_entries(i64 count, int mode) { return mode == 2 ? count : 2 * count; }
+
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+extern "C" size_t td3_lists_workspace_bytes(int64_t node_count, int64_t count, int mode) {
+    const i64 n = list_entries(count < 0 ? 0 : count, mode) + 1;
+    return 2 * align256((size_t)n * sizeof(u64)) + 2 * align256((size_t)n * sizeof(u32)) +
+           align256(td3sort::scratch_bytes(n)) + 256;
 }
 
 extern "C" int td3_lists(const int32_t* pairs, int64_t count, int mode, int64_t node_count,
-                         uint32_t* row_start, int32_t* col, int32_t* dst, void* workspace,
-                         size_t workspace_bytes, void* stream) {
-    TD3_ARG(count >= 0 && node_count >= 1, "sizes");
+                         int64_t col_count, uint32_t* row_start, int32_t* col, int32_t* dst,
+                         void* workspace, size_t workspace_bytes, void* stream) {
+    TD3_ARG(count >= 0 && node_count >= 1 && col_count >= 1, "sizes");
     TD3_ARG(mode >= 0 && mode <= 2, "mode");
-    TD3_ARG(workspace_bytes >= td3_lists_workspace_bytes(node_count), "workspace too small");
+    TD3_ARG(mode == 2 || col_count == node_count, "col_count (the same tree for modes 0, 1)");
+    TD3_ARG(workspace_bytes >= td3_lists_workspace_bytes(node_count, count, mode),
+            "workspace too small");
     hipStream_t s = (hipStream_t)stream;
-    u32* deg = (u32*)workspace;
-    u32* cursor = deg + (node_count + 1);
-    u32* scratch = (u32*)(((uintptr_t)(cursor + node_count + 1) + 255) & ~(uintptr_t)255);
-    TD3_HIP(hipMemsetAsync(deg, 0, sizeof(u32) * (size_t)(node_count + 1), s));
-    if (count > 0) {
-        TD3_TIMED(s, "lists_degree", hipLaunchKernelGGL(degree_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, (const int2*)pairs,
-                           (u32)count, mode, deg));
+    const i64 n = list_entries(count, mode);
+    if (n == 0) {
+        hipLaunchKernelGGL(list_empty_kernel, dim3(td3_blocks(node_count + 1, 256)), dim3(256), 0,
+                           s, node_count, row_start);
         TD3_LAUNCHED();
+        return TD3_OK;
     }
-    if (int rc = td3sort::exclusive_scan_u32(deg, row_start, node_count + 1, nullptr, scratch, s)) return rc;
-    if (count > 0) {
-        TD3_HIP(hipMemcpyAsync(cursor, row_start, sizeof(u32) * (size_t)node_count, hipMemcpyDeviceToDevice, s));
-        TD3_TIMED(s, "lists_fill", hipLaunchKernelGGL(fill_kernel, dim3(td3_blocks(count, 256)), dim3(256), 0, s, (const int2*)pairs,
-                           (u32)count, mode, cursor, col, (int*)nullptr));
-        TD3_LAUNCHED();
-        u32* err = deg;  // reuse: zero it first
-        TD3_HIP(hipMemsetAsync(err, 0, sizeof(u32), s));
-        unsigned blocks = td3_blocks(node_count, 4);
-        if (blocks > 8192) blocks = 8192;
-        TD3_TIMED(s, "row_sort", hipLaunchKernelGGL(row_sort_kernel, dim3(blocks), dim3(256), 0, s, row_start, node_count, col, err));
-        TD3_LAUNCHED();
-        if (dst) {
-            hipLaunchKernelGGL(row_dst_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0, s, row_start,
-                               node_count, dst);
-            TD3_LAUNCHED();
-        }
-        u32 herr = 0;
-        TD3_HIP(hipMemcpyAsync(&herr, err, sizeof(u32), hipMemcpyDeviceToHost, s));
-        TD3_HIP(hipStreamSynchronize(s));
-        if (herr) {
-            td3_set_error("an interaction row holds more than %d cells", kSortMax);
-            return TD3_ERR_ARG;
-        }
-    }
+    // keys: row node above column, node_count < 2^db so that a padding key (all ones) sorts
+    // after every entry
+    const int sb = bits_for(col_count - 1), db = bits_for(node_count);
+    TD3_ARG(sb + db <= 64, "sizes (keys of more than 64 bits)");
+    char* base = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+    u64* keys = (u64*)base;
+    u64* keys_alt = (u64*)(base + align256((size_t)n * sizeof(u64)));
+    u32* vals = (u32*)(base + 2 * align256((size_t)n * sizeof(u64)));
+    u32* vals_alt = (u32*)((char*)vals + align256((size_t)n * sizeof(u32)));
+    u32* scratch = (u32*)((char*)vals_alt + align256((size_t)n * sizeof(u32)));
+    TD3_TIMED(s, "lists_keys", hipLaunchKernelGGL(list_keys_kernel, dim3(td3_blocks(count, 256)),
+              dim3(256), 0, s, (const int2*)pairs, (u32)count, mode, sb, keys));
+    TD3_LAUNCHED();
+    if (int rc = td3sort::sort_pairs(keys, vals, nullptr, keys_alt, vals_alt, nullptr, n, 0,
+                                     sb + db, true, scratch, s))
+        return rc;
+    TD3_TIMED(s, "lists_rows", hipLaunchKernelGGL(list_rows_kernel, dim3(td3_blocks(n, 256)),
+              dim3(256), 0, s, keys, n, sb, node_count, row_start, col, dst));
+    TD3_LAUNCHED();
     return TD3_OK;
 }

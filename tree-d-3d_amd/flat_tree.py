@@ -669,9 +669,9 @@ class FlatQuadtree:
             row = torch.empty(c + 1, dtype=torch.int32, device=dev)
             col = torch.empty(max(2 * count, 1), dtype=torch.int32, device=dev)
             dst = torch.empty(max(2 * count, 1), dtype=torch.int32, device=dev)
-            nbytes = _abi.lib().td3_lists_workspace_bytes(c)
+            nbytes = _abi.lib().td3_lists_workspace_bytes(c, count, self_once)
             ws = _abi.workspace(nbytes)
-            _abi.call("td3_lists", _abi.ptr(pairs), count, self_once, c, _abi.ptr(row),
+            _abi.call("td3_lists", _abi.ptr(pairs), count, self_once, c, c, _abi.ptr(row),
                       _abi.ptr(col), _abi.ptr(dst), _abi.ptr(ws), nbytes, _abi.stream())
             rows.append((row, col, dst))
         return (rows[0], rows[1], (n_m2l, n_p2p))
@@ -1033,10 +1033,10 @@ class FlatQuadtree:
             row = torch.empty(ct + 1, dtype=torch.int32, device=dev)
             col = torch.empty(max(count, 1), dtype=torch.int32, device=dev)
             dst = torch.empty(max(count, 1), dtype=torch.int32, device=dev)
-            nbytes = _abi.lib().td3_lists_workspace_bytes(ct)
+            nbytes = _abi.lib().td3_lists_workspace_bytes(ct, count, 2)
             ws = _abi.workspace(nbytes)
-            _abi.call("td3_lists", _abi.ptr(pairs), count, 2, ct, _abi.ptr(row), _abi.ptr(col),
-                      _abi.ptr(dst), _abi.ptr(ws), nbytes, _abi.stream())
+            _abi.call("td3_lists", _abi.ptr(pairs), count, 2, ct, cs, _abi.ptr(row),
+                      _abi.ptr(col), _abi.ptr(dst), _abi.ptr(ws), nbytes, _abi.stream())
             rows.append((row, col, dst))
         lists = (rows[0], rows[1], (int(out[0]), int(out[1])))
         self._traversals[key] = (targets.clone(), tree, lists)
