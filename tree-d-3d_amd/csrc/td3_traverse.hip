@@ -191,28 +191,32 @@ __global__ void list_keys_kernel(const int2* __restrict__ pairs, u32 count, int 
     keys[2 * (u64)k + 1] = (mode == 0 || p.x != p.y) ? (((u64)(u32)p.y << sb) | (u32)p.x) : ~0ull;
 }
 
-// Rows from the sorted keys: entry k's column (and row node), and the starts of every row
-// from the one after entry k - 1's up to entry k's (a padding key counts as row node_count,
-// so the first one closes every row).
-__global__ void list_rows_kernel(const u64* __restrict__ keys, i64 n, int sb, i64 node_count,
-                                 u32* __restrict__ row_start, int* __restrict__ col,
-                                 int* __restrict__ dst) {
+// Entry k's column (and row node) from the sorted keys (padding keys, past the entries, are
+// skipped).
+__global__ void list_cols_kernel(const u64* __restrict__ keys, i64 n, int sb, i64 node_count,
+                                 int* __restrict__ col, int* __restrict__ dst) {
     const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const u64 key = keys[k];
-    const i64 d = (i64)(key >> sb) < node_count ? (i64)(key >> sb) : node_count;
-    if (d < node_count) {
-        col[k] = (int)(key & ((1ull << sb) - 1ull));
-        if (dst) dst[k] = (int)d;
+    if ((key >> sb) >= (u64)node_count) return;
+    col[k] = (int)(key & ((1ull << sb) - 1ull));
+    if (dst) dst[k] = (int)(key >> sb);
+}
+
+// row_start[r], r <= node_count: the first sorted key whose row node is r or more (a
+// padding key counts as row node_count), by binary search.
+__global__ void list_starts_kernel(const u64* __restrict__ keys, i64 n, int sb, i64 node_count,
+                                   u32* __restrict__ row_start) {
+    const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > node_count) return;
+    i64 lo = 0, hi = n;
+    while (lo < hi) {
+        const i64 mid = (lo + hi) >> 1;
+        const u64 d = keys[mid] >> sb;
+        if (d < (u64)r) lo = mid + 1;
+        else hi = mid;
     }
-    i64 prev = -1;
-    if (k > 0) {
-        const i64 pd = (i64)(keys[k - 1] >> sb);
-        prev = pd < node_count ? pd : node_count;
-    }
-    for (i64 r = prev + 1; r <= d; ++r) row_start[r] = (u32)k;
-    if (k == n - 1)
-        for (i64 r = d + 1; r <= node_count; ++r) row_start[r] = (u32)n;
+    row_start[r] = (u32)lo;
 }
 
 __global__ void list_empty_kernel(i64 node_count, u32* __restrict__ row_start) {
@@ -372,8 +376,12 @@ extern "C" int td3_lists(const int32_t* pairs, int64_t count, int mode, int64_t 
     if (int rc = td3sort::sort_pairs(keys, vals, nullptr, keys_alt, vals_alt, nullptr, n, 0,
                                      sb + db, true, scratch, s))
         return rc;
-    TD3_TIMED(s, "lists_rows", hipLaunchKernelGGL(list_rows_kernel, dim3(td3_blocks(n, 256)),
-              dim3(256), 0, s, keys, n, sb, node_count, row_start, col, dst));
+    TD3_TIMED(s, "lists_rows", hipLaunchKernelGGL(list_cols_kernel, dim3(td3_blocks(n, 256)),
+              dim3(256), 0, s, keys, n, sb, node_count, col, dst));
+    TD3_LAUNCHED();
+    TD3_TIMED(s, "lists_rows", hipLaunchKernelGGL(list_starts_kernel,
+              dim3(td3_blocks(node_count + 1, 256)), dim3(256), 0, s, keys, n, sb, node_count,
+              row_start));
     TD3_LAUNCHED();
     return TD3_OK;
 }
