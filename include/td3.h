@@ -406,17 +406,19 @@ int td3_log_fmm_eval_ext(int64_t order, const double *sorted_pos, const int64_t 
 
 /*
  * td3_fmm_at_eval for the log kernel with td3_log_fmm_eval_ext's dipoles and Jacobian
- * (jacobian (3, M) in target order); no counterpart in the reference.
+ * (jacobian (3, M) in target order); no counterpart in the reference.  t_info (t_count,)
+ * int4 {start, end, child, depth} are the target tree's cells: a wavefront per target
+ * leaf, whose targets split the leaf's P2P row between lanes.
  */
 int td3_log_fmm_at_eval_ext(int64_t order, const double *sorted_targets,
                             const int64_t *target_perm, const int64_t *exclude, int64_t m,
-                            const double *t_center, const double *t_half,
-                            const int32_t *t_leaf_of, const double *local_re,
-                            const double *local_im, const uint32_t *p2p_row,
-                            const int32_t *p2p_col, const int64_t *perm, int64_t n,
-                            const void *s_info, const void *s_sources4,
-                            const double *sorted_dipole, double softening, double *phi,
-                            double *field, double *jacobian, int64_t *status, void *stream);
+                            const double *t_center, const double *t_half, const void *t_info,
+                            int64_t t_count, const double *local_re, const double *local_im,
+                            const uint32_t *p2p_row, const int32_t *p2p_col,
+                            const int64_t *perm, int64_t n, const void *s_info,
+                            const void *s_sources4, const double *sorted_dipole,
+                            double softening, double *phi, double *field, double *jacobian,
+                            int64_t *status, void *stream);
 
 /*
  * (x, y, q, 0) float64 per sorted source (N x 32 bytes): the near field's source records

@@ -74,8 +74,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                                _C),
     "td3_log_fmm_eval_ext": ([_I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P,
                               _P, _P, _P], _C),
-    "td3_log_fmm_at_eval_ext": ([_I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P,
-                                 _P, _D, _P, _P, _P, _P, _P], _C),
+    "td3_log_fmm_at_eval_ext": ([_I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _I, _P,
+                                 _P, _P, _D, _P, _P, _P, _P, _P], _C),
     "td3_leaf_index": ([_P, _I, _P, _P], _C),
     "td3_dual_traverse": ([_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P, _Z,
                            _P, _P], _C),

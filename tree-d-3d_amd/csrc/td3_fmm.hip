@@ -982,63 +982,61 @@ __global__ __launch_bounds__(256) void eval_rows_kernel(
 // field's is 2 W'' (E_x - i E_y = 2 W' is analytic), the near field's
 // 2 q (I / (r^2 + eps^2) - 2 d d^T / (r^2 + eps^2)^2).  Targets, exclusions and status as
 // in eval_kernel; a dipole counts as a source as a charge does.
+// The near sum of log_ext_eval_kernel over one partner leaf (sf: its {start, end}), into
+// the FP64 sums of the potential, the field and (EXT 2) the Jacobian.
 template <int MODE, int EXT>
-__global__ __launch_bounds__(256) void log_ext_eval_kernel(
-    const double* __restrict__ tpos, const i64* __restrict__ tperm, const i64* __restrict__ exclude,
-    i64 m, const double* __restrict__ tcenter, const double* __restrict__ thalf,
-    const double* __restrict__ lre, const double* __restrict__ lim, int p1,
-    const u32* __restrict__ prow, const int* __restrict__ pcol, const i64* __restrict__ sperm,
-    i64 n, const int4* __restrict__ sinfo, const int* __restrict__ leaf_of,
-    const double4* __restrict__ src4, const double* __restrict__ sdip, double eps2,
-    double* __restrict__ phi, double* __restrict__ field, double* __restrict__ jac,
-    i64* __restrict__ status) {
-    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
-    const i64 leaf = leaf_of[k];
-    const double tx = tpos[k], ty = tpos[m + k];
-    const double tq = MODE == 0 ? src4[k].z : 0.0;
-    const i64 tskip = MODE == 1 ? exclude[tperm[k]] : -1;
-    double ap = 0.0, ax = 0.0, ay = 0.0, jxx = 0.0, jxy = 0.0, jyy = 0.0;
-    bool bad = false;
-    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
-    for (u32 r = r0; r < r1; ++r) {
-        const int4 sf = sinfo[pcol[r]];
-        for (int ks = sf.x; ks < sf.y; ++ks) {
-            if (MODE == 0 && ks == k) continue;
-            const double4 sv = src4[ks];
-            const double sq = sv.z;
-            const double px = EXT == 1 ? sdip[ks] : 0.0, py = EXT == 1 ? sdip[n + ks] : 0.0;
-            const bool live = sq != 0.0 || px != 0.0 || py != 0.0;
-            if (MODE == 1 && (!live || (tskip >= 0 && sperm[ks] == tskip))) continue;
-            const double dx = tx - sv.x, dy = ty - sv.y;
-            const double r2 = dx * dx + dy * dy + eps2;
-            if (r2 == 0.0) {
-                if (MODE == 1 || (tq != 0.0 && live)) bad = true;
-                continue;
-            }
-            ap -= sq * log(r2);
-            const double c = 2.0 * sq / r2;
-            ax += c * dx;
-            ay += c * dy;
-            if (EXT == 1) {
-                const double inv = 1.0 / r2;
-                const double pd = px * dx + py * dy;
-                const double c4 = 4.0 * pd * inv * inv;
-                ap -= 2.0 * pd * inv;
-                ax += 2.0 * px * inv - c4 * dx;
-                ay += 2.0 * py * inv - c4 * dy;
-            }
-            if (EXT == 2) {
-                const double c2 = 2.0 * c / r2;
-                jxx += c - c2 * dx * dx;
-                jxy -= c2 * dx * dy;
-                jyy += c - c2 * dy * dy;
-            }
+__device__ __forceinline__ void ext_p2p_leaf(double tx, double ty, double tq, i64 k, i64 tskip,
+                                             int4 sf, const double4* __restrict__ src4,
+                                             const double* __restrict__ sdip,
+                                             const i64* __restrict__ sperm, i64 n, double eps2,
+                                             double (&acc)[6], bool& bad) {
+    for (int ks = sf.x; ks < sf.y; ++ks) {
+        if (MODE == 0 && ks == k) continue;
+        const double4 sv = src4[ks];
+        const double sq = sv.z;
+        const double px = EXT == 1 ? sdip[ks] : 0.0, py = EXT == 1 ? sdip[n + ks] : 0.0;
+        const bool live = sq != 0.0 || px != 0.0 || py != 0.0;
+        if (MODE == 1 && (!live || (tskip >= 0 && sperm[ks] == tskip))) continue;
+        const double dx = tx - sv.x, dy = ty - sv.y;
+        const double r2 = dx * dx + dy * dy + eps2;
+        if (r2 == 0.0) {
+            if (MODE == 1 || (tq != 0.0 && live)) bad = true;
+            continue;
+        }
+        acc[0] -= sq * log(r2);
+        const double c = 2.0 * sq / r2;
+        acc[1] += c * dx;
+        acc[2] += c * dy;
+        if (EXT == 1) {
+            const double inv = 1.0 / r2;
+            const double pd = px * dx + py * dy;
+            const double c4 = 4.0 * pd * inv * inv;
+            acc[0] -= 2.0 * pd * inv;
+            acc[1] += 2.0 * px * inv - c4 * dx;
+            acc[2] += 2.0 * py * inv - c4 * dy;
+        }
+        if (EXT == 2) {
+            const double c2 = 2.0 * c / r2;
+            acc[3] += c - c2 * dx * dx;
+            acc[4] -= c2 * dx * dy;
+            acc[5] += c - c2 * dy * dy;
         }
     }
-    const i64 i = tperm[k];
-    if (bad) td3_report(status, i + 1);
-    // Horner on the scaled local expansion at u' = (z - c) / r: W, W' and W'' / 2
+}
+
+// The far field of log_ext_eval_kernel at a target of leaf `leaf` (Horner on the scaled
+// local expansion at u' = (z - c) / r: W, W' and W'' / 2), added to the near sums, written
+// at input index i.
+template <int EXT>
+__device__ __forceinline__ void ext_l2p_write(double tx, double ty, i64 leaf,
+                                              const double* __restrict__ tcenter,
+                                              const double* __restrict__ thalf,
+                                              const double* __restrict__ lre,
+                                              const double* __restrict__ lim, int p1,
+                                              const double (&acc)[6], i64 i, i64 m,
+                                              double* __restrict__ phi,
+                                              double* __restrict__ field,
+                                              double* __restrict__ jac) {
     const double inv_r = 1.0 / (1.4142135623730951 * thalf[leaf]);
     const double ur = (tx - tcenter[2 * leaf]) * inv_r, ui = (ty - tcenter[2 * leaf + 1]) * inv_r;
     const double* bre = lre + leaf * (i64)p1;
@@ -1056,14 +1054,88 @@ __global__ __launch_bounds__(256) void log_ext_eval_kernel(
         w_im = w_re * ui + w_im * ur + bim[l];
         w_re = nw;
     }
-    phi[i] = ap - 2.0 * w_re;
-    field[i] = ax + 2.0 * d_re * inv_r;
-    field[m + i] = ay - 2.0 * d_im * inv_r;
+    phi[i] = acc[0] - 2.0 * w_re;
+    field[i] = acc[1] + 2.0 * d_re * inv_r;
+    field[m + i] = acc[2] - 2.0 * d_im * inv_r;
     if (EXT == 2) {
         const double s = 4.0 * inv_r * inv_r;
-        jac[i] = jxx + s * e_re;
-        jac[m + i] = jxy - s * e_im;
-        jac[2 * m + i] = jyy - s * e_re;
+        jac[i] = acc[3] + s * e_re;
+        jac[m + i] = acc[4] - s * e_im;
+        jac[2 * m + i] = acc[5] - s * e_re;
+    }
+}
+
+template <int MODE, int EXT>
+__global__ __launch_bounds__(256) void log_ext_eval_kernel(
+    const double* __restrict__ tpos, const i64* __restrict__ tperm, const i64* __restrict__ exclude,
+    i64 m, const double* __restrict__ tcenter, const double* __restrict__ thalf,
+    const double* __restrict__ lre, const double* __restrict__ lim, int p1,
+    const u32* __restrict__ prow, const int* __restrict__ pcol, const i64* __restrict__ sperm,
+    i64 n, const int4* __restrict__ sinfo, const int* __restrict__ leaf_of,
+    const double4* __restrict__ src4, const double* __restrict__ sdip, double eps2,
+    double* __restrict__ phi, double* __restrict__ field, double* __restrict__ jac,
+    i64* __restrict__ status) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const i64 leaf = leaf_of[k];
+    const double tx = tpos[k], ty = tpos[m + k];
+    const double tq = MODE == 0 ? src4[k].z : 0.0;
+    const i64 tskip = MODE == 1 ? exclude[tperm[k]] : -1;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    bool bad = false;
+    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
+    for (u32 r = r0; r < r1; ++r)
+        ext_p2p_leaf<MODE, EXT>(tx, ty, tq, k, tskip, sinfo[pcol[r]], src4, sdip, sperm, n, eps2,
+                                acc, bad);
+    const i64 i = tperm[k];
+    if (bad) td3_report(status, i + 1);
+    ext_l2p_write<EXT>(tx, ty, leaf, tcenter, thalf, lre, lim, p1, acc, i, m, phi, field, jac);
+}
+
+// The same at other points (fmm_at, MODE 1) with a wavefront per target leaf, as
+// eval_rows_kernel: 64 / tl lanes per target split the leaf's P2P row, DPP sums.
+template <int EXT>
+__global__ __launch_bounds__(256) void log_ext_eval_rows_kernel(
+    const double* __restrict__ tpos, const i64* __restrict__ tperm, const i64* __restrict__ exclude,
+    i64 m, const double* __restrict__ tcenter, const double* __restrict__ thalf,
+    const int4* __restrict__ tinfo, i64 t_count, const double* __restrict__ lre,
+    const double* __restrict__ lim, int p1, const u32* __restrict__ prow,
+    const int* __restrict__ pcol, const i64* __restrict__ sperm, i64 n,
+    const int4* __restrict__ sinfo, const double4* __restrict__ src4,
+    const double* __restrict__ sdip, double eps2, double* __restrict__ phi,
+    double* __restrict__ field, double* __restrict__ jac, i64* __restrict__ status) {
+    const int lane = td3_lane();
+    const i64 leaf = ((i64)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (leaf >= t_count) return;  // uniform over the wavefront
+    const int4 tf = tinfo[leaf];
+    if (tf.z >= 0 || tf.y <= tf.x) return;  // an internal or empty cell
+    const int nt = tf.y - tf.x;
+    const int tl = nt <= 4 ? 4 : (nt <= 8 ? 8 : 16);
+    const int lanes = 64 / tl, sub = lane & (lanes - 1);
+    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
+    constexpr int kAcc = EXT == 2 ? 6 : 3;
+    for (int c0 = 0; c0 < nt; c0 += tl) {
+        const int tloc = c0 + lane / lanes;
+        const bool tvalid = tloc < nt;
+        const i64 k = tf.x + (tvalid ? tloc : 0);
+        const double tx = tpos[k], ty = tpos[m + k];
+        const i64 tskip = exclude[tperm[k]];
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        bool bad = false;
+        if (tvalid)
+            for (u32 r = r0 + (u32)sub; r < r1; r += (u32)lanes)
+                ext_p2p_leaf<1, EXT>(tx, ty, 0.0, k, tskip, sinfo[pcol[r]], src4, sdip, sperm, n,
+                                     eps2, acc, bad);
+#pragma unroll
+        for (int a = 0; a < kAcc; ++a) acc[a] = td3_group_sum(acc[a], lanes);
+        const u64 group = (lanes == 64 ? ~0ull : ((1ull << lanes) - 1ull)) << (lane & ~(lanes - 1));
+        const bool group_bad = (__ballot(bad) & group) != 0;
+        if (tvalid && sub == 0) {
+            const i64 i = tperm[k];
+            if (group_bad) td3_report(status, i + 1);
+            ext_l2p_write<EXT>(tx, ty, leaf, tcenter, thalf, lre, lim, p1, acc, i, m, phi, field,
+                               jac);
+        }
     }
 }
 
@@ -1398,19 +1470,39 @@ extern "C" int td3_log_fmm_eval_ext(int64_t order, const double* sorted_pos, con
 extern "C" int td3_log_fmm_at_eval_ext(int64_t order, const double* sorted_targets,
                                        const int64_t* target_perm, const int64_t* exclude,
                                        int64_t m, const double* t_center, const double* t_half,
-                                       const int32_t* t_leaf_of, const double* local_re,
-                                       const double* local_im, const uint32_t* p2p_row,
-                                       const int32_t* p2p_col, const int64_t* perm, int64_t n,
-                                       const void* s_info, const void* s_sources4,
-                                       const double* sorted_dipole, double softening,
-                                       double* phi, double* field, double* jacobian,
-                                       int64_t* status, void* stream) {
-    TD3_ARG(m >= 1 && n >= 1, "sizes");
-    TD3_ARG(exclude != nullptr, "exclude");
-    return launch_ext<1>((hipStream_t)stream, sorted_targets, target_perm, exclude, m, t_center,
-                         t_half, local_re, local_im, (int)order, p2p_row, p2p_col, perm, n,
-                         (const int4*)s_info, t_leaf_of, (const double4*)s_sources4,
-                         sorted_dipole, softening * softening, phi, field, jacobian, status);
+                                       const void* t_info, int64_t t_count,
+                                       const double* local_re, const double* local_im,
+                                       const uint32_t* p2p_row, const int32_t* p2p_col,
+                                       const int64_t* perm, int64_t n, const void* s_info,
+                                       const void* s_sources4, const double* sorted_dipole,
+                                       double softening, double* phi, double* field,
+                                       double* jacobian, int64_t* status, void* stream) {
+    TD3_ARG(m >= 1 && n >= 1 && t_count >= 1, "sizes");
+    TD3_ARG(order >= 1 && order <= 64, "order");
+    TD3_ARG(exclude != nullptr && t_info != nullptr, "exclude, t_info");
+    TD3_ARG(s_sources4 != nullptr, "sources4 (td3_pack_sources of the source tree)");
+    TD3_ARG(sorted_dipole == nullptr || jacobian == nullptr, "dipole sources with a Jacobian");
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned blocks = td3_blocks(t_count * 64, 256);
+    const int p1 = (int)order + 1;
+    const double eps2 = softening * softening;
+    const int4* ti = (const int4*)t_info;
+    const int4* si = (const int4*)s_info;
+    const double4* src4 = (const double4*)s_sources4;
+#define TD3_EXT_ROWS(EXT)                                                                       \
+    TD3_TIMED(s, "eval_ext", hipLaunchKernelGGL((log_ext_eval_rows_kernel<EXT>), dim3(blocks),  \
+              dim3(256), 0, s, sorted_targets, target_perm, exclude, m, t_center, t_half, ti,    \
+              t_count, local_re, local_im, p1, p2p_row, p2p_col, perm, n, si, src4,              \
+              sorted_dipole, eps2, phi, field, jacobian, status))
+    if (sorted_dipole != nullptr)
+        TD3_EXT_ROWS(1);
+    else if (jacobian != nullptr)
+        TD3_EXT_ROWS(2);
+    else
+        TD3_EXT_ROWS(0);
+#undef TD3_EXT_ROWS
+    TD3_LAUNCHED();
+    return TD3_OK;
 }
 
 extern "C" int td3_leaf_index(const void* node_info, int64_t node_count, int32_t* leaf_of,

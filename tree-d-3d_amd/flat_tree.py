@@ -1090,7 +1090,8 @@ class FlatQuadtree:
             _abi.call(
                 "td3_log_fmm_at_eval_ext", self.order, _abi.ptr(tree._sorted_pos),
                 _abi.ptr(td["perm"]), _abi.ptr(ex), m, _abi.ptr(td["node_center"]),
-                _abi.ptr(td["node_half"]), _abi.ptr(tree._leaf_index()), _abi.ptr(lre),
+                _abi.ptr(td["node_half"]), _abi.ptr(tree._node_info), tree.node_count,
+                _abi.ptr(lre),
                 _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._d["perm"]), n,
                 _abi.ptr(self._node_info), _abi.ptr(self._sources4()), _abi.ptr(sdip),
                 softening, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(jac), _abi.ptr(status),
