@@ -484,6 +484,26 @@ int td3_cart_m2l(int64_t order, const int32_t *col, const int32_t *dst, const ui
                  int64_t node_lo, int64_t node_hi, const double *dst_center,
                  const double *src_center, const double *mprime, double softening, double *local,
                  void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * The mixed-precision Cartesian M2L (order 1..12), no counterpart in the reference (its
+ * Coulomb FMM is FP64 on the CPU): the translation of td3_cart_m2l on radius-scaled
+ * quantities in FP32.  The softened kernel is homogeneous, A'_g(t; eps) = h^-(|g|+1)
+ * A'_g(t/h; eps/h), so with h the destination's half size (dst_half) and mscaled_a =
+ * m'_a / h_src^|a| (td3_cart_scale_prime of the source tree, FP32):
+ * T_b = (1 / b!) h^-(|b|+1) sum_a mscaled_a (h_src / h)^|a| A'_{a+b}(t/h; eps/h), every
+ * term bounded by the pair's separation.  The recurrence, products and per-pair sums in
+ * FP32; sums over a destination's pairs, the scaling and local (written as td3_cart_m2l
+ * writes it) in FP64.  Same rows, workspace and read-back as td3_cart_m2l.
+ */
+int td3_cart_scale_prime(int64_t order, const double *mprime, const double *node_half,
+                         int64_t node_count, float *out, void *stream);
+int td3_cart_m2l_f32(int64_t order, const int32_t *col, const int32_t *dst,
+                     const uint32_t *row_start, int64_t node_lo, int64_t node_hi,
+                     const double *dst_center, const double *dst_half,
+                     const double *src_center, const double *src_half, const float *mscaled,
+                     double softening, double *local, void *workspace,
+                     size_t workspace_bytes, void *stream);
 int td3_cart_l2l(int64_t order, const double *node_center, const void *node_info,
                  const int64_t *levels_host, int64_t n_levels, const double *binomial,
                  int64_t bsz, double *local, void *stream);

@@ -84,6 +84,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_cart_moments": ([_I, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P], _C),
     "td3_cart_m2l_workspace_bytes": ([_I, _I, _I], _Z),
     "td3_cart_m2l": ([_I, _P, _P, _P, _I, _I, _P, _P, _P, _D, _P, _P, _Z, _P], _C),
+    "td3_cart_scale_prime": ([_I, _P, _P, _I, _P, _P], _C),
+    "td3_cart_m2l_f32": ([_I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _D, _P, _P, _Z, _P], _C),
     "td3_cart_l2l": ([_I, _P, _P, _P, _I, _P, _I, _P, _P], _C),
     "td3_rows_workspace_bytes": ([_P, _I, _I], _Z),
     "td3_rows": ([_P, _P, _P, _P, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _Z, _P, _P, _P],

@@ -195,15 +195,26 @@ __global__ void cart_prime_kernel(const double* __restrict__ mom, i64 count, int
 // lane's T_b stay in registers, then go through LDS 32 at a time, where the lanes of each
 // destination are summed in lane order (rows within the 64 written divided by b!, the
 // partial sums of rows that cross a wave boundary to side, primed, for the fixup).
+//
+// T = double: the reference's arithmetic on m'_a.  T = float (mixed precision): the same
+// translation on radius-scaled quantities, so that every term stays in FP32 range at any
+// depth.  The softened kernel is homogeneous, A'_g(t; eps) = h^-(|g|+1) A'_g(t/h; eps/h),
+// so with h the destination's half size and mscaled_a = m'_a / h_src^|a| (per cell, once):
+//   T_b = (1 / b!) h^-(|b|+1) sum_a (mscaled_a kappa^|a|) A'_{a+b}(t/h; eps/h),
+// kappa = h_src / h.  A'(t/h) is bounded by the separation of the pair, mscaled by the
+// cell's charge: the recurrence, the products and the lane's sums run in FP32, the sums
+// over a destination's pairs and the final scaling in FP64.
 constexpr int kCartGroup = 32;
 
-template <int P>
+template <int P, typename T>
 __global__ __launch_bounds__(256) void cart_m2l_kernel(
     const int* __restrict__ col, const int* __restrict__ dst, const u32* __restrict__ row_start,
     u32 e_lo, u32 e_hi, const double* __restrict__ dcenter, const double* __restrict__ scenter,
-    const double* __restrict__ mprime, double eps2, double* __restrict__ local,
+    const double* __restrict__ dhalf, const double* __restrict__ shalf,
+    const T* __restrict__ mprime, double eps2, double* __restrict__ local,
     double* __restrict__ side) {
     constexpr int NC = csize(P);
+    constexpr bool kScaled = sizeof(T) == 4;
     __shared__ double s_t[4][kCartGroup][64];
     __shared__ int s_seg[4][65];
     const int lane = td3_lane(), w = (int)(threadIdx.x / 64);
@@ -213,46 +224,58 @@ __global__ __launch_bounds__(256) void cart_m2l_kernel(
     const u32 e = w0 + lane;
     const bool valid = e < e_hi;
     const int d = valid ? dst[e] : -1;
-    double T[NC];
+    T Tb[NC];
 #pragma unroll
-    for (int i = 0; i < NC; ++i) T[i] = 0.0;
+    for (int i = 0; i < NC; ++i) Tb[i] = (T)0;
     if (valid) {
         const int src = col[e];
-        const double tx = dcenter[2 * d] - scenter[2 * src];
-        const double ty = dcenter[2 * d + 1] - scenter[2 * src + 1];
-        const double r2 = tx * tx + ty * ty + eps2;
-        const double inv_r2 = 1.0 / r2;
-        double A[NC];
-        A[0] = 1.0 / sqrt(r2);
+        double tx = dcenter[2 * d] - scenter[2 * src];
+        double ty = dcenter[2 * d + 1] - scenter[2 * src + 1];
+        double e2 = eps2;
+        T kappa = (T)1;
+        if (kScaled) {
+            const double inv_h = 1.0 / dhalf[d];
+            tx *= inv_h;
+            ty *= inv_h;
+            e2 *= inv_h * inv_h;
+            kappa = (T)(shalf[src] * inv_h);
+        }
+        const double r2 = tx * tx + ty * ty + e2;
+        const T txt = (T)tx, tyt = (T)ty;
+        const T inv_r2 = (T)(1.0 / r2);
+        T A[NC];
+        A[0] = (T)(1.0 / sqrt(r2));
 #pragma unroll
         for (int n = 1; n <= P; ++n) {
 #pragma unroll
             for (int gy = 0; gy <= n; ++gy) {
                 const int gx = n - gy;
-                double s = 0.0;
-                if (gx >= 1) s += (double)((2 * n - 1) * gx) * tx * A[cidx(gx - 1, gy)];
-                if (gy >= 1) s += (double)((2 * n - 1) * gy) * ty * A[cidx(gx, gy - 1)];
-                if (gx >= 2) s += (double)((n - 1) * gx * (gx - 1)) * A[cidx(gx - 2, gy)];
-                if (gy >= 2) s += (double)((n - 1) * gy * (gy - 1)) * A[cidx(gx, gy - 2)];
-                A[cidx(gx, gy)] = -s * inv_r2 * (1.0 / n);
+                T s = (T)0;
+                if (gx >= 1) s += (T)((2 * n - 1) * gx) * txt * A[cidx(gx - 1, gy)];
+                if (gy >= 1) s += (T)((2 * n - 1) * gy) * tyt * A[cidx(gx, gy - 1)];
+                if (gx >= 2) s += (T)((n - 1) * gx * (gx - 1)) * A[cidx(gx - 2, gy)];
+                if (gy >= 2) s += (T)((n - 1) * gy * (gy - 1)) * A[cidx(gx, gy - 2)];
+                A[cidx(gx, gy)] = -s * inv_r2 * (T)(1.0 / n);
             }
         }
-        const double* m = mprime + (i64)src * NC;
+        const T* m = mprime + (i64)src * NC;
+        T kpow = (T)1;
 #pragma unroll
         for (int na = 0; na <= P; ++na) {
 #pragma unroll
             for (int ay = 0; ay <= na; ++ay) {
                 const int ax = na - ay;
-                const double ma = m[cidx(ax, ay)];
+                const T ma = kScaled ? m[cidx(ax, ay)] * kpow : m[cidx(ax, ay)];
 #pragma unroll
                 for (int nb = 0; nb + na <= P; ++nb) {
 #pragma unroll
                     for (int by = 0; by <= nb; ++by) {
                         const int bx = nb - by;
-                        T[cidx(bx, by)] += ma * A[cidx(ax + bx, ay + by)];
+                        Tb[cidx(bx, by)] += ma * A[cidx(ax + bx, ay + by)];
                     }
                 }
             }
+            kpow *= kappa;
         }
     }
     // segments of equal destination among the valid lanes, by their first lane
@@ -268,7 +291,7 @@ __global__ __launch_bounds__(256) void cart_m2l_kernel(
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
         for (int i = 0; i < kCartGroup; ++i)
-            if (g0 + i < NC) s_t[w][i][lane] = T[g0 + i];
+            if (g0 + i < NC) s_t[w][i][lane] = (double)Tb[g0 + i];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const int width = NC - g0 < kCartGroup ? NC - g0 : kCartGroup;
@@ -278,11 +301,18 @@ __global__ __launch_bounds__(256) void cart_m2l_kernel(
             double v = 0.0;
             for (int l = l0; l < l1; ++l) v += s_t[w][i][l];
             const int dd = dst[w0 + l0];
+            int deg = 0;
+            while (csize(deg) <= b) ++deg;
+            if (kScaled) {
+                // h^-(|b| + 1)
+                const double inv_h = 1.0 / dhalf[dd];
+                double sc = inv_h;
+                for (int k = 0; k < deg; ++k) sc *= inv_h;
+                v *= sc;
+            }
             const u32 rs0 = row_start[dd], rs1 = row_start[dd + 1];
             const bool before = rs0 < w0, after = rs1 > w0 + 64;
             if (!before && !after) {
-                int deg = 0;
-                while (csize(deg) <= b) ++deg;
                 const int by = b - deg * (deg + 1) / 2, bx = deg - by;
                 local[(i64)dd * NC + b] = v / (kCartFact.v[bx] * kCartFact.v[by]);
             } else {
@@ -291,6 +321,22 @@ __global__ __launch_bounds__(256) void cart_m2l_kernel(
             }
         }
     }
+}
+
+// mscaled_a = m'_a / h^|a| of every cell, in FP32 (the mixed-precision M2L's moments).
+__global__ void cart_scale_prime_kernel(const double* __restrict__ mprime,
+                                        const double* __restrict__ half, i64 count, int nc,
+                                        float* __restrict__ out) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * nc) return;
+    const i64 node = t / nc;
+    const int i = (int)(t % nc);
+    int deg = 0;
+    while (csize(deg) <= i) ++deg;
+    const double inv_h = 1.0 / half[node];
+    double sc = 1.0;
+    for (int k = 0; k < deg; ++k) sc *= inv_h;
+    out[t] = (float)(mprime[t] * sc);
 }
 
 __global__ void cart_m2l_fixup_kernel(const u32* __restrict__ row_start, i64 node_lo, i64 node_hi,
@@ -446,15 +492,16 @@ __global__ __launch_bounds__(256) void cart_l2l_lds_kernel(
     }
 }
 
-template <int P>
+template <int P, typename T>
 int launch_cart_m2l(hipStream_t s, const int* col, const int* dst, const u32* row_start, u32 e_lo,
                     u32 e_hi, i64 node_lo, i64 node_hi, const double* dcenter,
-                    const double* scenter, const double* mprime, double eps2, double* local,
-                    double* side) {
+                    const double* scenter, const double* dhalf, const double* shalf,
+                    const T* mprime, double eps2, double* local, double* side) {
     constexpr int NC = csize(P);
     const i64 waves = ((i64)(e_hi - e_lo) + 63) / 64;
-    TD3_TIMED(s, "cart_m2l", hipLaunchKernelGGL((cart_m2l_kernel<P>), dim3(td3_blocks(waves * 64, 256)), dim3(256), 0, s,
-                       col, dst, row_start, e_lo, e_hi, dcenter, scenter, mprime, eps2, local, side));
+    TD3_TIMED(s, "cart_m2l", hipLaunchKernelGGL((cart_m2l_kernel<P, T>), dim3(td3_blocks(waves * 64, 256)), dim3(256), 0, s,
+                       col, dst, row_start, e_lo, e_hi, dcenter, scenter, dhalf, shalf, mprime, eps2,
+                       local, side));
     TD3_LAUNCHED();
     const i64 items = (node_hi - node_lo) * NC;
     hipLaunchKernelGGL(cart_m2l_fixup_kernel, dim3(td3_blocks(items, 256)), dim3(256), 0, s, row_start,
@@ -526,8 +573,9 @@ extern "C" int td3_cart_m2l(int64_t order, const int32_t* col, const int32_t* ds
     switch ((int)order) {
 #define TD3_CART_CASE(PP) \
     case PP:              \
-        return launch_cart_m2l<PP>(s, col, dst, row_start, bounds[0], bounds[1], node_lo, node_hi, \
-                                   dst_center, src_center, mprime, eps2, local, side);
+        return launch_cart_m2l<PP, double>(s, col, dst, row_start, bounds[0], bounds[1], node_lo,    \
+                                           node_hi, dst_center, src_center, nullptr, nullptr,      \
+                                           mprime, eps2, local, side);
         TD3_CART_CASE(1)
         TD3_CART_CASE(2)
         TD3_CART_CASE(3)
@@ -551,6 +599,63 @@ extern "C" int td3_cart_m2l(int64_t order, const int32_t* col, const int32_t* ds
                        side, local));
     TD3_LAUNCHED();
     return TD3_OK;
+}
+
+extern "C" int td3_cart_scale_prime(int64_t order, const double* mprime, const double* node_half,
+                                    int64_t node_count, float* out, void* stream) {
+    TD3_ARG(order >= 1 && order <= 12 && node_count >= 0, "order (1..12), sizes");
+    if (node_count == 0) return TD3_OK;
+    const int nc = csize((int)order);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(cart_scale_prime_kernel, dim3(td3_blocks(node_count * nc, 256)), dim3(256),
+                       0, s, mprime, node_half, node_count, nc, out);
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+extern "C" int td3_cart_m2l_f32(int64_t order, const int32_t* col, const int32_t* dst,
+                                const uint32_t* row_start, int64_t node_lo, int64_t node_hi,
+                                const double* dst_center, const double* dst_half,
+                                const double* src_center, const double* src_half,
+                                const float* mscaled, double softening, double* local,
+                                void* workspace, size_t workspace_bytes, void* stream) {
+    TD3_ARG(order >= 1 && order <= 12, "order (1..12)");
+    TD3_ARG(node_lo >= 0 && node_hi >= node_lo, "node range");
+    hipStream_t s = (hipStream_t)stream;
+    if (node_hi == node_lo) return TD3_OK;
+    u32 bounds[2];
+    TD3_HIP(hipMemcpyAsync(&bounds[0], row_start + node_lo, sizeof(u32), hipMemcpyDeviceToHost, s));
+    TD3_HIP(hipMemcpyAsync(&bounds[1], row_start + node_hi, sizeof(u32), hipMemcpyDeviceToHost, s));
+    TD3_HIP(hipStreamSynchronize(s));
+    const i64 pairs = (i64)bounds[1] - (i64)bounds[0];
+    if (pairs <= 0) return TD3_OK;
+    TD3_ARG(workspace_bytes >= td3_cart_m2l_workspace_bytes(pairs, order, node_hi - node_lo),
+            "workspace too small");
+    const double eps2 = softening * softening;
+    double* side = (double*)workspace;
+    switch ((int)order) {
+#define TD3_CART_CASE(PP) \
+    case PP:              \
+        return launch_cart_m2l<PP, float>(s, col, dst, row_start, bounds[0], bounds[1], node_lo,    \
+                                          node_hi, dst_center, src_center, dst_half, src_half,     \
+                                          mscaled, eps2, local, side);
+        TD3_CART_CASE(1)
+        TD3_CART_CASE(2)
+        TD3_CART_CASE(3)
+        TD3_CART_CASE(4)
+        TD3_CART_CASE(5)
+        TD3_CART_CASE(6)
+        TD3_CART_CASE(7)
+        TD3_CART_CASE(8)
+        TD3_CART_CASE(9)
+        TD3_CART_CASE(10)
+        TD3_CART_CASE(11)
+        TD3_CART_CASE(12)
+#undef TD3_CART_CASE
+        default:
+            break;
+    }
+    return TD3_ERR_ARG;
 }
 
 extern "C" int td3_cart_l2l(int64_t order, const double* node_center, const void* node_info,

@@ -456,6 +456,7 @@ class FlatQuadtree:
         self._d["charge"].copy_(charges)
         self._invalidate_host("charge")
         self._cart = None
+        self._cart_f32 = None
         self._scaled = {}
         self._src4 = None
         self._sdip = None
@@ -484,8 +485,8 @@ class FlatQuadtree:
         if targets.shape == pos.shape and bool(torch.equal(targets, pos)):
             return self._d["perm"]
         m = targets.shape[1]
-        lo = targets.min(dim=1).values
-        span = float((targets.max(dim=1).values - lo).max())
+        lo, hi = torch.aminmax(targets, dim=1)
+        span = float((hi - lo).max())
         scale = (65535.0 / span) if span > 0 else 0.0
         order = torch.empty(m, dtype=torch.int64, device=targets.device)
         nbytes = _abi.lib().td3_target_order_workspace_bytes(m)
@@ -937,8 +938,22 @@ class FlatQuadtree:
         self._cart = (mom, prime)
         return self._cart
 
-    def _cart_locals(self, target, m2l_rows, softening):
-        """Cartesian local expansions of every cell of ``target``: M2L then L2L."""
+    def _cart_scaled(self) -> torch.Tensor:
+        """FP32 m'_a / h^|a| of every cell, the mixed-precision Cartesian M2L's moments
+        (kept with the moments until `set_charges`)."""
+        cached = self.__dict__.get("_cart_f32")
+        if cached is not None:
+            return cached
+        _, prime = self._cart_moments()
+        out = torch.empty_like(prime, dtype=torch.float32)
+        _abi.call("td3_cart_scale_prime", self.order, _abi.ptr(prime),
+                  _abi.ptr(self._d["node_half"]), self.node_count, _abi.ptr(out), _abi.stream())
+        self._cart_f32 = out
+        return out
+
+    def _cart_locals(self, target, m2l_rows, softening, precision=_abi.FP64):
+        """Cartesian local expansions of every cell of ``target``: M2L (FP64, or FP32 on
+        radius-scaled terms in mixed precision up to order 12) then L2L."""
         mrow, mcol, mdst = m2l_rows
         nc = (self.order + 1) * (self.order + 2) // 2
         dev = self._d["pos"].device
@@ -947,10 +962,17 @@ class FlatQuadtree:
         nbytes = _abi.lib().td3_cart_m2l_workspace_bytes(int(mcol.shape[0]), self.order,
                                                          target.node_count)
         ws = _abi.workspace(nbytes)
-        _abi.call("td3_cart_m2l", self.order, _abi.ptr(mcol), _abi.ptr(mdst), _abi.ptr(mrow), 0,
-                  target.node_count, _abi.ptr(target._d["node_center"]),
-                  _abi.ptr(self._d["node_center"]), _abi.ptr(prime), float(softening),
-                  _abi.ptr(local), _abi.ptr(ws), nbytes, _abi.stream())
+        if precision != _abi.FP64 and self.order <= 12:
+            _abi.call("td3_cart_m2l_f32", self.order, _abi.ptr(mcol), _abi.ptr(mdst),
+                      _abi.ptr(mrow), 0, target.node_count, _abi.ptr(target._d["node_center"]),
+                      _abi.ptr(target._d["node_half"]), _abi.ptr(self._d["node_center"]),
+                      _abi.ptr(self._d["node_half"]), _abi.ptr(self._cart_scaled()),
+                      float(softening), _abi.ptr(local), _abi.ptr(ws), nbytes, _abi.stream())
+        else:
+            _abi.call("td3_cart_m2l", self.order, _abi.ptr(mcol), _abi.ptr(mdst),
+                      _abi.ptr(mrow), 0, target.node_count, _abi.ptr(target._d["node_center"]),
+                      _abi.ptr(self._d["node_center"]), _abi.ptr(prime), float(softening),
+                      _abi.ptr(local), _abi.ptr(ws), nbytes, _abi.stream())
         bsz = 2 * self.order + 1
         binom = _device_table("binomial", _binomial_table, bsz)
         levels = np.ascontiguousarray(target.levels, dtype=np.int64)
@@ -969,7 +991,7 @@ class FlatQuadtree:
         kernel (softened in the far field too)."""
         prec = _precision(precision or self.precision)
         m2l_rows, (prow, pcol, _), counts = self._lists(theta)
-        local = self._cart_locals(self, m2l_rows, softening)
+        local = self._cart_locals(self, m2l_rows, softening, prec)
         n = self._d["pos"].shape[1]
         dev = self._d["pos"].device
         phi = torch.zeros(n, dtype=torch.float64, device=dev)
@@ -1000,8 +1022,7 @@ class FlatQuadtree:
         from .quad import Quad
 
         m = targets.shape[1]
-        lo = targets.min(dim=1).values.cpu().numpy()
-        hi = targets.max(dim=1).values.cpu().numpy()
+        lo, hi = (v.cpu().numpy() for v in torch.aminmax(targets, dim=1))
         centre = 0.5 * (lo + hi)
         half = 0.5 * float(np.max(hi - lo)) * (1 + 1e-12) + 1e-300
         tree = FlatQuadtree(targets, torch.zeros(m, dtype=torch.float64, device=targets.device),
@@ -1122,7 +1143,7 @@ class FlatQuadtree:
                         precision, like_torch):
         """fmm_at's Coulomb branch (flat_tree.py:1247-1295)."""
         prec = _precision(precision or self.precision)
-        local = self._cart_locals(tree, m2l_rows, softening)
+        local = self._cart_locals(tree, m2l_rows, softening, prec)
         m, n = t.shape[1], self._d["pos"].shape[1]
         phi = torch.zeros(m, dtype=torch.float64, device=t.device)
         field = torch.zeros((2, m), dtype=torch.float64, device=t.device)
