@@ -250,6 +250,9 @@ def case_fmm():
         want = r.fmm(theta=0.5, softening=1e-3)
         assert t.fmm_pairs == r.fmm_pairs, (t.fmm_pairs, r.fmm_pairs)
         assert relative(got[0], want[0]) < 1e-12 and relative(got[1], want[1]) < 1e-12
+        field_only = t.fmm(theta=0.5, softening=1e-3, potential=False)
+        assert not np.asarray(field_only[0]).any()
+        assert relative(field_only[1], want[1]) < 1e-12
         mixed = t.fmm(theta=0.5, softening=1e-3, precision="mixed")
         assert relative(mixed[1], want[1]) < 1e-5
         t.set_charges(q[::-1].copy())

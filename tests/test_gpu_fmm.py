@@ -180,6 +180,60 @@ def test_fmm_is_deterministic(td3):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("potential", [True, False])
+def test_fmm_on_two_streams_equals_one_launch(td3, precision, potential):
+    """fmm() runs the near field (td3_fmm_near) on a second stream beside the far field
+    and adds it in the L2P (td3_fmm_far): bit for bit the one-launch td3_fmm_eval on the
+    same local expansions and rows, on a fresh tree (rows built beside the moments) and
+    on one whose rows are kept."""
+    import torch
+    from tree_d_3d_amd import _abi
+
+    pos, q = config2(60000, seed=11)
+    tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", precision=precision)
+    prec = _abi.FP64 if precision == "fp64" else _abi.MIXED
+    n = pos.shape[1]
+    for _ in range(2):
+        phi, field = tree.fmm(softening=1e-4, potential=potential)
+        lre, lim = tree._local_expansions(0.5, prec)
+        _, (prow, pcol, _), _ = tree._lists(0.5)
+        want_phi = torch.empty(n, dtype=torch.float64, device="cuda")
+        want_field = torch.empty((2, n), dtype=torch.float64, device="cuda")
+        status = torch.zeros(1, dtype=torch.int64, device="cuda")
+        _abi.call("td3_fmm_eval", _abi.LOG, prec, tree.order, _abi.ptr(tree._sorted_pos),
+                  _abi.ptr(tree._scharge), _abi.ptr(tree._d["perm"]), n,
+                  _abi.ptr(tree._d["node_center"]), _abi.ptr(tree._d["node_half"]),
+                  _abi.ptr(tree._node_info), tree.node_count, _abi.ptr(lre), _abi.ptr(lim),
+                  _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(tree._leaf_index()),
+                  _abi.ptr(tree._sources4()), 1e-4, 1 if potential else 0,
+                  _abi.ptr(want_phi), _abi.ptr(want_field), _abi.ptr(status), _abi.stream())
+        assert int(status.item()) == 0
+        np.testing.assert_array_equal(np.asarray(phi), want_phi.cpu().numpy())
+        np.testing.assert_array_equal(np.asarray(field), want_field.cpu().numpy())
+        if not potential:
+            assert not np.asarray(phi).any()
+
+
+def test_moments_made_on_first_use(td3, oracle):
+    """The cells' moments wait for their first reader (fmm, the arrays): the same values
+    whichever comes first, and again after set_charges."""
+    pos, q = cloud(2000, seed=4)
+    ref = oracle.OracleTree(pos, q, (0, 0, 1), kernel="log", order=10)
+    scale = np.abs(ref.moment_re).max()
+    for first in ("array", "fmm"):
+        tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10)
+        if first == "fmm":
+            tree.fmm()
+        np.testing.assert_allclose(np.asarray(tree.moment_re), ref.moment_re, rtol=0,
+                                   atol=1e-12 * scale)
+        np.testing.assert_allclose(np.asarray(tree.node_q), ref.node_q, rtol=0,
+                                   atol=1e-12 * scale)
+    tree.set_charges(-q)
+    np.testing.assert_allclose(np.asarray(tree.moment_im), -ref.moment_im, rtol=0,
+                               atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_fmm_at_matches_oracle(td3, oracle, precision):
     rng = np.random.default_rng(13)
     pos, q = cloud(4000, seed=13)

@@ -917,6 +917,52 @@ __global__ __launch_bounds__(256) void eval_kernel(
                       with_potential, i, m, phi, field);
 }
 
+// The two halves of eval_kernel (MODE 0) as separate launches, so that the near field can
+// run while the far field is still being translated: eval_near_kernel writes each
+// particle's exact sums (potential, field x, field y) in tree order to near (3, N);
+// eval_far_kernel adds the local expansion's terms and writes the outputs in input order.
+// The sums and their order are eval_kernel's.
+template <int KERNEL, int PREC>
+__global__ __launch_bounds__(256) void eval_near_kernel(
+    const double* __restrict__ tpos, const i64* __restrict__ tperm, i64 m,
+    const u32* __restrict__ prow, const int* __restrict__ pcol, const int4* __restrict__ sinfo,
+    const int* __restrict__ leaf_of, const double4* __restrict__ src4, double eps2,
+    int with_potential, double* __restrict__ near, i64* __restrict__ status) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const i64 leaf = leaf_of[k];
+    const double tx = tpos[k], ty = tpos[m + k];
+    const double tq = src4[k].z;
+    double acc_p = 0.0, acc_x = 0.0, acc_y = 0.0;
+    bool bad = false;
+    const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
+    int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
+    for (u32 r = r0; r < r1; ++r) {
+        const int4 sf = sf_next;
+        if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
+        p2p_leaf<KERNEL, PREC, 0>(tx, ty, tq, k, -1, sf, src4, nullptr, eps2, with_potential,
+                                  acc_p, acc_x, acc_y, bad);
+    }
+    if (bad) td3_report(status, tperm[k] + 1);
+    near[k] = acc_p;
+    near[m + k] = acc_x;
+    near[2 * m + k] = acc_y;
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(256) void eval_far_kernel(
+    const double* __restrict__ tpos, const i64* __restrict__ tperm, i64 m,
+    const double* __restrict__ tcenter, const double* __restrict__ thalf,
+    const double* __restrict__ lre, const double* __restrict__ lim, int p1, int order,
+    const int* __restrict__ leaf_of, const double* __restrict__ near, int with_potential,
+    double* __restrict__ phi, double* __restrict__ field) {
+    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    l2p_write<KERNEL>(tpos[k], tpos[m + k], leaf_of[k], tcenter, thalf, lre, lim, p1, order,
+                      near[k], near[m + k], near[2 * m + k], with_potential, tperm[k], m, phi,
+                      field);
+}
+
 // L2P + P2P at other points (fmm_at, MODE 1), a wavefront per target leaf: its targets go
 // tl = 4, 8 or 16 at a time (by the leaf's size), each with 64 / tl lanes that split the
 // leaf's P2P row between them (lane s of a target takes partner leaves s, s + 64 / tl, ...),
@@ -1398,6 +1444,57 @@ extern "C" int td3_fmm_eval(int kernel, int precision, int64_t order, const doub
               sorted_pos, sorted_charge, perm, n, (const int4*)node_info, leaf_of,
               (const double4*)sources4, softening * softening, with_potential, phi, field,
               status);
+}
+
+extern "C" int td3_fmm_near(int kernel, int precision, const double* sorted_pos,
+                            const int64_t* perm, int64_t n, const void* node_info,
+                            const uint32_t* p2p_row, const int32_t* p2p_col,
+                            const int32_t* leaf_of, const void* sources4, double softening,
+                            int with_potential, double* near, int64_t* status, void* stream) {
+    TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
+    TD3_ARG(n >= 1, "sizes");
+    TD3_ARG(leaf_of != nullptr && sources4 != nullptr, "leaf_of, sources4");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid(td3_blocks(n, 256));
+    const double eps2 = softening * softening;
+    const int4* info = (const int4*)node_info;
+    const double4* src4 = (const double4*)sources4;
+#define TD3_NEAR(K, P)                                                                           \
+    TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_near_kernel<K, P>), grid, dim3(256), 0, s,     \
+              sorted_pos, perm, n, p2p_row, p2p_col, info, leaf_of, src4, eps2, with_potential,  \
+              near, status))
+    if (kernel == TD3_LOG) {
+        if (precision == TD3_FP64) TD3_NEAR(TD3_LOG, TD3_FP64);
+        else TD3_NEAR(TD3_LOG, TD3_MIXED);
+    } else {
+        if (precision == TD3_FP64) TD3_NEAR(TD3_COULOMB, TD3_FP64);
+        else TD3_NEAR(TD3_COULOMB, TD3_MIXED);
+    }
+#undef TD3_NEAR
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+extern "C" int td3_fmm_far(int kernel, int64_t order, const double* sorted_pos,
+                           const int64_t* perm, int64_t n, const double* node_center,
+                           const double* node_half, const double* local_re,
+                           const double* local_im, const int32_t* leaf_of, const double* near,
+                           int with_potential, double* phi, double* field, void* stream) {
+    TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
+    TD3_ARG(order >= 1 && order <= 64 && n >= 1, "order, sizes");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid(td3_blocks(n, 256));
+    const int p1 = kernel == TD3_LOG ? (int)order + 1 : (int)((order + 1) * (order + 2) / 2);
+    if (kernel == TD3_LOG)
+        TD3_TIMED(s, "eval_far", hipLaunchKernelGGL(eval_far_kernel<TD3_LOG>, grid, dim3(256), 0, s,
+                  sorted_pos, perm, n, node_center, node_half, local_re, local_im, p1,
+                  (int)order, leaf_of, near, with_potential, phi, field));
+    else
+        TD3_TIMED(s, "eval_far", hipLaunchKernelGGL(eval_far_kernel<TD3_COULOMB>, grid, dim3(256), 0,
+                  s, sorted_pos, perm, n, node_center, node_half, local_re, local_im, p1,
+                  (int)order, leaf_of, near, with_potential, phi, field));
+    TD3_LAUNCHED();
+    return TD3_OK;
 }
 
 extern "C" int td3_fmm_at_eval(int kernel, int precision, int64_t order,

@@ -37,7 +37,7 @@ import torch
 
 from . import _abi
 from ._utils import _check, _finite, _host
-from .flat_tree import COULOMB_ORDER, FlatQuadtree
+from .flat_tree import COULOMB_ORDER, FlatQuadtree, _Arrays
 from .simulation import ElectrostaticSimulation
 
 
@@ -51,7 +51,7 @@ class _Forest(FlatQuadtree):
         self.precision = "fp64"
         self._like_torch = True
         self._host_cache: dict = {}
-        self._d = {"pos": positions, "charge": charges}
+        self._d = _Arrays(pos=positions, charge=charges)
         self.leaf_capacity, self.max_depth = int(leaf_capacity), int(max_depth)
         self._traversals: dict = {}
         self.fmm_pairs = (0, 0)

@@ -188,6 +188,97 @@ class FlatNode:
         return tuple(FlatNode(self.tree, first + k) for k in range(4))
 
 
+class _Arrays(dict):
+    """A tree's device arrays by name.  The cells' moments (and the cell charges that
+    come with them) are made on first use, by the call `defer` was given: work queued
+    after the tree that needs the cells only (the interaction rows) starts before them."""
+
+    MOMENTS = ("node_q", "node_p", "node_s", "node_qabs", "moment_re", "moment_im")
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        self.pending = None
+
+    def defer(self, make) -> None:
+        for name in self.MOMENTS:
+            self.pop(name, None)
+        self.pending = make
+
+    def __missing__(self, key):
+        make = self.pending
+        if make is None or key not in self.MOMENTS:
+            raise KeyError(key)
+        self.pending = None
+        make()
+        return dict.__getitem__(self, key)
+
+
+class _Streams:
+    """A second stream (default priority) beside the caller's.  `side()` makes it
+    current; from the start it runs after ``ready`` (an event of the caller's stream) or
+    else after everything the caller has queued.  `mark()` and `mark_caller()` record an
+    event on the second stream and on the caller's; `wait(e)` makes the caller's stream
+    wait for e, `side_wait(e)` the second stream.  `used_by_caller` tells torch's allocator
+    that tensors made on the second stream are read by the caller's.  On a CPU device
+    (the kernels' emulation) everything stays on the one stream."""
+
+    _side_streams: dict = {}
+
+    def __init__(self, dev: torch.device, ready=None):
+        self.on_gpu = dev.type == "cuda"
+        if self.on_gpu:
+            self.main = torch.cuda.current_stream(dev)
+            st = _Streams._side_streams.get(dev.index)
+            if st is None:
+                st = _Streams._side_streams[dev.index] = torch.cuda.Stream(dev)
+            self.stream = st
+            if ready is not None:
+                st.wait_event(ready)
+            else:
+                st.wait_stream(self.main)
+
+    @staticmethod
+    def record(dev: torch.device):
+        if dev.type != "cuda":
+            return None
+        event = torch.cuda.Event()
+        event.record()
+        return event
+
+    def side(self):
+        import contextlib
+
+        return torch.cuda.stream(self.stream) if self.on_gpu else contextlib.nullcontext()
+
+    def mark(self):
+        if not self.on_gpu:
+            return None
+        event = torch.cuda.Event()
+        event.record(self.stream)
+        return event
+
+    def mark_caller(self):
+        if not self.on_gpu:
+            return None
+        event = torch.cuda.Event()
+        event.record(self.main)
+        return event
+
+    def wait(self, event) -> None:
+        if event is not None:
+            self.main.wait_event(event)
+
+    def side_wait(self, event) -> None:
+        if event is not None:
+            self.stream.wait_event(event)
+
+    def used_by_caller(self, *tensors) -> None:
+        if self.on_gpu:
+            for t in tensors:
+                if t is not None:
+                    t.record_stream(self.main)
+
+
 class FlatQuadtree:
     """Array quadtree for point charges in a plane or for line charges, on the GPU.
 
@@ -239,7 +330,7 @@ class FlatQuadtree:
                 raise ValueError(f"{name} must be a positive integer")
         self._like_torch = _is_torch(positions)
         self._host_cache: dict = {}
-        self._d: dict = {}
+        self._d = _Arrays()
         pos = _points(positions, "positions").clone()
         charge = _device(charges).reshape(-1).clone()
         n = pos.shape[1]
@@ -259,17 +350,17 @@ class FlatQuadtree:
             return
         self._build()
         self._sorted_charge()
-        self._moments()
+        self._d.defer(self._moments)
 
     # ---------------------------------------------------------------- arrays
     def __getattr__(self, name):
         if name in _PUBLIC:
             if name in ("node_q", "node_p", "node_s") and "_d" in self.__dict__:
                 self._log_node_arrays()
-            d = self.__dict__.get("_d", {})
-            if name not in d:
-                raise AttributeError(name)
-            t = d[name]
+            try:
+                t = self.__dict__["_d"][name]
+            except KeyError:
+                raise AttributeError(name) from None
             if self.__dict__.get("_like_torch", True):
                 return t
             cache = self.__dict__["_host_cache"]
@@ -369,6 +460,8 @@ class FlatQuadtree:
         _abi.call("td3_permute", _abi.ptr(self._d["charge"]), 1, n, _abi.ptr(self._d["perm"]), n,
                   _abi.ptr(out), _abi.stream())
         self._scharge = out
+        # work on a second stream that needs the cells and the sorted charges starts here
+        self._ready = _Streams.record(self._d["pos"].device)
 
     def _moments(self) -> None:
         """flat_tree.py:663-744: the cells' charges and multipole moments."""
@@ -467,7 +560,7 @@ class FlatQuadtree:
                 _abi.call("td3_permute", _abi.ptr(dip.contiguous()), 2, n,
                           _abi.ptr(self._d["perm"]), n, _abi.ptr(sdip), _abi.stream())
                 self._sdip = sdip
-            self._moments()
+            self._d.defer(self._moments)
 
     @property
     def root(self) -> FlatNode:
@@ -756,6 +849,13 @@ class FlatQuadtree:
         target = self if target is None else target
         if m2l_rows is None:
             m2l_rows = self._lists(theta)[0]
+        lre, lim = self._m2l(precision, target, m2l_rows)
+        self._l2l(target, lre, lim)
+        return lre, lim
+
+    def _m2l(self, precision: int, target, m2l_rows):
+        """Every cell's scaled local expansion from the M2L along its row (zeros without
+        pairs), before L2L."""
         mrow, mcol, mdst = m2l_rows
         c, p1 = target.node_count, self.order + 1
         dev = self._d["pos"].device
@@ -796,12 +896,17 @@ class FlatQuadtree:
                       _abi.ptr(self._d["node_half"]), _abi.ptr(are), _abi.ptr(aim),
                       _abi.ptr(coeff), _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(ws), nbytes,
                       _abi.stream())
+        return lre, lim
+
+    def _l2l(self, target, lre, lim) -> None:
+        """L2L down ``target``, level by level, in place."""
+        levels = np.ascontiguousarray(target.levels, dtype=np.int64)
         bsz = 2 * self.order + 1
         binom = _device_table("binomial", _binomial_table, bsz)
-        _abi.call("td3_l2l", self.order, _abi.ptr(td["node_center"]), _abi.ptr(td["node_half"]),
-                  _abi.ptr(target._node_info), levels.ctypes.data, len(levels) - 1,
-                  _abi.ptr(binom), bsz, _abi.ptr(lre), _abi.ptr(lim), _abi.stream())
-        return lre, lim
+        _abi.call("td3_l2l", self.order, _abi.ptr(target._d["node_center"]),
+                  _abi.ptr(target._d["node_half"]), _abi.ptr(target._node_info),
+                  levels.ctypes.data, len(levels) - 1, _abi.ptr(binom), bsz, _abi.ptr(lre),
+                  _abi.ptr(lim), _abi.stream())
 
     def local_expansions(self, theta: float = 0.5, precision=None):
         """The reference's (unscaled) local expansions ``local_re``, ``local_im`` after L2L."""
@@ -849,66 +954,95 @@ class FlatQuadtree:
         if self.kernel != "log":
             return self._coulomb_fmm(theta, softening, potential, precision)
         prec = _precision(precision or self.precision)
-        pending = None
-        if ("lists", theta) not in self._traversals:
-            # queue the work that needs no interaction rows ahead of them: the GPU runs it
-            # while the host waits for the rows' sizes
-            self._leaf_index()
-            self._sources4()
-            self._scaled_moments(_abi.FP64)
-            if prec != _abi.FP64:
-                if self.order + 1 <= 16:
-                    self._m2l_records()
-                else:
-                    self._scaled_moments(prec)
-            if prec != _abi.FP64 and self.order + 1 <= 16 and not jacobian and sdip is None:
-                # mixed precision needs no row counts before its last read-back: the rows
-                # are built and used at full capacity and checked with the status (the
-                # adjoint's evaluation below takes rows that were read back)
-                pending = self._rows(theta, deferred=True)
-        if pending is not None:
-            m2l_rows, p2p_rows, dcounts = pending
-            lre, lim = self._local_expansions(theta, prec, m2l_rows=m2l_rows)
-            prow, pcol = p2p_rows[0], p2p_rows[1]
-        else:
-            lre, lim = self._local_expansions(theta, prec)
-            _, (prow, pcol, _), counts = self._lists(theta)
+        if not jacobian and sdip is None:
+            return self._fmm_overlapped(theta, softening, potential, prec, phi, field)
+        # the adjoint's evaluation (dipole sources, the field's Jacobian) takes rows that were
+        # read back
+        lre, lim = self._local_expansions(theta, prec)
+        _, (prow, pcol, _), counts = self._lists(theta)
         status = torch.zeros(1, dtype=torch.int64, device=dev)
-        if jacobian or sdip is not None:
-            _abi.call(
-                "td3_log_fmm_eval_ext", self.order, _abi.ptr(self._sorted_pos),
-                _abi.ptr(self._d["perm"]), n, _abi.ptr(self._d["node_center"]),
-                _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), _abi.ptr(lre),
-                _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._leaf_index()),
-                _abi.ptr(self._sources4()), _abi.ptr(sdip), softening, _abi.ptr(phi),
-                _abi.ptr(field), _abi.ptr(jac), _abi.ptr(status), _abi.stream(),
-            )
-            _check(status)
-            if not potential:
-                phi.zero_()
-            self.fmm_pairs = counts
-            return self._fmm_out(phi, field, jac, self._like_torch)
         _abi.call(
-            "td3_fmm_eval", _abi.LOG, prec, self.order, _abi.ptr(self._sorted_pos),
-            _abi.ptr(self._scharge),
+            "td3_log_fmm_eval_ext", self.order, _abi.ptr(self._sorted_pos),
             _abi.ptr(self._d["perm"]), n, _abi.ptr(self._d["node_center"]),
-            _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), self.node_count,
-            _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol),
-            _abi.ptr(self._leaf_index()), _abi.ptr(self._sources4()), softening,
-            1 if potential else 0, _abi.ptr(phi), _abi.ptr(field), _abi.ptr(status),
-            _abi.stream(),
+            _abi.ptr(self._d["node_half"]), _abi.ptr(self._node_info), _abi.ptr(lre),
+            _abi.ptr(lim), _abi.ptr(prow), _abi.ptr(pcol), _abi.ptr(self._leaf_index()),
+            _abi.ptr(self._sources4()), _abi.ptr(sdip), softening, _abi.ptr(phi),
+            _abi.ptr(field), _abi.ptr(jac), _abi.ptr(status), _abi.stream(),
         )
-        if pending is not None:
+        _check(status)
+        if not potential:
+            phi.zero_()
+        self.fmm_pairs = counts
+        return self._fmm_out(phi, field, jac, self._like_torch)
+
+    def _fmm_overlapped(self, theta, softening, potential, prec, phi, field):
+        """The log FMM on two streams.  A second stream, from the moment the tree and the
+        sorted charges exist: the leaf map, the packed sources and the interaction rows,
+        then (once the M2L is done) the near field (td3_fmm_near).  The caller's stream:
+        the cells' moments and their scaled forms (beside the rows: the rows depend on the
+        cells only), the M2L once the rows are there, then the L2L (beside the near field:
+        its small levels leave most of the GPU to it) and the L2P with the near sums added
+        (td3_fmm_far).  Every kernel computes what it computes in sequence; the results
+        are the same bit for bit.
+
+        In mixed precision (orders up to 15) the rows are built at full capacity with their
+        counts left on the device and checked with the status at the end (an overflow
+        leaves every row empty: the rows are built again with room to spare and the FMM
+        runs again on them)."""
+        n = self._d["pos"].shape[1]
+        dev = self._d["pos"].device
+        key = ("lists", theta)
+        cached = self._traversals.get(key)
+        deferred = cached is None and prec != _abi.FP64 and self.order + 1 <= 16
+        streams = _Streams(dev, self.__dict__.get("_ready"))
+        with streams.side():
+            leaf_of, src4 = self._leaf_index(), self._sources4()
+            dcounts = None
+            if cached is not None:
+                m2l_rows, p2p_rows, counts = cached
+            elif deferred:
+                m2l_rows, p2p_rows, dcounts = self._rows(theta, deferred=True)
+            else:
+                m2l_rows, p2p_rows, counts = self._lists(theta)
+            rows_done = streams.mark()
+            near = torch.empty((3, n), dtype=torch.float64, device=dev)
+            status = torch.zeros(1, dtype=torch.int64, device=dev)
+        streams.used_by_caller(leaf_of, src4, *m2l_rows, *p2p_rows, dcounts, near, status)
+        # the M2L's inputs that need the moments only
+        self._scaled_moments(_abi.FP64)
+        if prec != _abi.FP64:
+            if self.order + 1 <= 16:
+                self._m2l_records()
+            else:
+                self._scaled_moments(prec)
+        streams.wait(rows_done)
+        lre, lim = self._m2l(prec, self, m2l_rows)
+        m2l_done = streams.mark_caller()
+        with streams.side():
+            streams.side_wait(m2l_done)
+            _abi.call("td3_fmm_near", _abi.LOG, prec, _abi.ptr(self._sorted_pos),
+                      _abi.ptr(self._d["perm"]), n, _abi.ptr(self._node_info),
+                      _abi.ptr(p2p_rows[0]), _abi.ptr(p2p_rows[1]), _abi.ptr(leaf_of),
+                      _abi.ptr(src4), softening, 1 if potential else 0, _abi.ptr(near),
+                      _abi.ptr(status), _abi.stream())
+            near_done = streams.mark()
+        self._l2l(self, lre, lim)
+        streams.wait(near_done)
+        _abi.call("td3_fmm_far", _abi.LOG, self.order, _abi.ptr(self._sorted_pos),
+                  _abi.ptr(self._d["perm"]), n, _abi.ptr(self._d["node_center"]),
+                  _abi.ptr(self._d["node_half"]), _abi.ptr(lre), _abi.ptr(lim),
+                  _abi.ptr(leaf_of), _abi.ptr(near), 1 if potential else 0, _abi.ptr(phi),
+                  _abi.ptr(field), _abi.stream())
+        if deferred:
             host = torch.cat((status, dcounts)).cpu().numpy()
             if host[4]:
                 # the rows overflowed (every row was left empty): build them again with
                 # room to spare (read back, capacities doubled until they fit), then
                 # evaluate again on them
                 self._lists(theta)
-                return self.fmm(theta=theta, softening=softening, potential=potential,
-                                precision=precision)
+                return self._fmm_overlapped(theta, softening, potential, prec, phi, field)
             lists = self._sliced_rows(m2l_rows, p2p_rows, host[1:])
-            self._traversals[("lists", theta)] = lists
+            self._traversals[key] = lists
             counts = lists[2]
             _check(host[:1])
         else:
