@@ -226,6 +226,39 @@ def pmc_copies():
     torch.cuda.synchronize()
 
 
+def walk_roofline(ens, steps=3):
+    """The Barnes-Hut walk of an ensemble step against its roofline: flops 30 per accepted
+    cell (Coulomb quadrupole) and 12 per source term, each with one FP64 rsqrt besides; bytes
+    bounded below by 7 words per cell some lane of a wavefront accepts (read once for the
+    wavefront) and 4 words per source a wavefront reads (x, y, q, perm).  The counts come
+    from one counted launch (td3_walk_counts) on the state after the timed steps; the
+    launch time is the walk's own (HIP events on its stream) over a few more steps."""
+    import torch
+    from tree_d_3d_amd import _abi
+
+    _abi.timing_enable(True)
+    for _ in range(steps):
+        forest = ens.step()
+    torch.cuda.synchronize()
+    ms, launches = _abi.timing_get("walk")
+    _abi.timing_enable(False)
+    ms /= max(launches, 1)
+    c = forest.walk_counts(ens.theta, ens.softening)
+    flops = 30 * c["accepted_per_target"] + 12 * c["terms_per_target"]
+    rsqrt = c["accepted_per_target"] + c["terms_per_target"]
+    nbytes = 56 * c["accepted_per_wave"] + 32 * c["sources_per_wave"]
+    tflops = flops / (ms * 1e-3) / 1e12
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    f_flop, f_hbm = tflops / FP64_PEAK_TFLOPS, gbs / HBM_PEAK_GBS
+    return {"kernel": "walk_kernel<coulomb, fp64> (a wavefront per 64 targets)",
+            "launch_ms": ms, "counts": c, "flops": flops, "rsqrt": rsqrt,
+            "bytes_lower_bound": nbytes,
+            "achieved": {"TFLOP/s": tflops, "GB/s": gbs},
+            "frac": {"fp64": f_flop, "hbm": f_hbm},
+            "bound": "fp64" if f_flop >= f_hbm else "hbm",
+            "peak": {"fp64_TFLOPs": FP64_PEAK_TFLOPS, "hbm_GBs": HBM_PEAK_GBS}}
+
+
 def extra_config4(td3, systems=512, steps=100, graph=True):
     """Batched ensemble: 512 Coulomb Barnes-Hut systems, N log-uniform in [1024, 65536]
     (seed 4), Gaussian clouds (sigma 0.1) of unit charges, B_z 1, k_e 1, dt 1e-4,
@@ -270,6 +303,8 @@ def extra_config4(td3, systems=512, steps=100, graph=True):
     t_graph = time.perf_counter() - t0
     same = all(torch.equal(getattr(rec, k), getattr(ens, k))
                for k in ("positions", "velocities", "forces"))
+    # after the comparison: the roofline's steps move the ensemble on
+    direct["walk_roofline"] = walk_roofline(ens)
     return {**direct,
             "graph": {"ms_per_step": 1e3 * t_graph / (steps - 2),
                       "particle_steps_per_s": n * (steps - 2) / t_graph,

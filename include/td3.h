@@ -219,6 +219,24 @@ int td3_walk(int kernel, int precision, const double *targets, int64_t m, const 
              double *phi, double *field, int64_t *status, void *stream);
 
 /*
+ * td3_walk (same arguments, same outputs, the same walk) that also adds up its work into
+ * counts[0..3] (uint64, added to: the caller zeroes them): cells accepted per target, source
+ * terms per target, cells accepted per wavefront (a cell some lane of the wavefront accepts,
+ * whose data the wavefront reads once) and sources read per wavefront (every source of every
+ * leaf a wavefront visits).  A separate launch for measurement (the walk's flops and its
+ * bytes' lower bound); the reference has no counterpart.
+ */
+int td3_walk_counts(int kernel, int precision, const double *targets, int64_t m,
+                    const int64_t *order, const int64_t *exclude, const double *sorted_pos,
+                    const double *sorted_charge, const int64_t *perm, int64_t n,
+                    const double *node_center, const double *node_half, const void *node_info,
+                    const double *node_qabs, const double *node_q, const double *node_p,
+                    const double *node_s, const double *moment_re, const double *moment_im,
+                    int64_t order_p, int64_t tree_depth, const int32_t *target_root,
+                    double theta, double softening, int with_potential, double *phi,
+                    double *field, uint64_t *counts, int64_t *status, void *stream);
+
+/*
  * The walk's launch shape: wavefronts (64 targets each) per block, 1, 2, 4 (default) or 8,
  * for every later td3_walk of the process; 0 only asks.  Returns the previous value.  No
  * counterpart in the reference: the knob of this build's tuner (Ensemble.autotune times

@@ -413,6 +413,12 @@ def case_forest():
         assert np.array_equal(phi[sl].numpy(), want[0].numpy())
         assert np.array_equal(field[:, sl].numpy(), want[1].numpy())
     assert forest.node_count == total, (forest.node_count, total)
+    # the counted walk: at theta 0 nothing is accepted and every pair of a system is summed
+    exact = forest.walk_counts(0.0, 1e-3)
+    assert exact["accepted_per_target"] == 0, exact
+    assert exact["terms_per_target"] == sum(k * (k - 1) for k in sizes), exact
+    far = forest.walk_counts(0.5, 1e-3)
+    assert 0 < far["accepted_per_wave"] <= far["accepted_per_target"], far
 
 
 def case_forest_resident():

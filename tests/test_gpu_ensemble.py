@@ -49,6 +49,52 @@ def test_forest_walk_equals_each_systems_own_tree(td3):
     assert forest.node_count == total
 
 
+def test_walk_counts_closed_forms(td3):
+    """td3_walk_counts: the counted walk's fields equal the plain walk's bit for bit; at
+    theta 0 no cell is accepted and every target meets every other particle of its system
+    (sum of n_s (n_s - 1) terms, and each wavefront reads all of its systems' sources at
+    least once); at theta 0.5 cells are accepted and fewer terms are summed."""
+    ens_mod = sys.modules["tree_d_3d_amd.ensemble"]
+    rng = np.random.default_rng(12)
+    sizes = [1, 70, 640, 2000]
+    pos = torch.tensor(np.concatenate([rng.normal(0, 0.12, (2, k)).clip(-0.95, 0.95)
+                                       for k in sizes], axis=1), device="cuda")
+    q = torch.tensor(rng.choice([-1.0, 1.0], pos.shape[1]), device="cuda")
+    start = torch.tensor(np.concatenate(([0], np.cumsum(sizes))), device="cuda")
+    system = torch.repeat_interleave(torch.arange(len(sizes), dtype=torch.int32, device="cuda"),
+                                     torch.tensor(sizes, device="cuda"))
+    roots = torch.tensor([(0.0, 0.0, 1.0)] * len(sizes), dtype=torch.float64, device="cuda")
+    forest = ens_mod._Forest(pos, q, system, start, roots, 8, 48)
+    exact = forest.walk_counts(0.0, 1e-3)
+    print("theta 0 counts", exact, flush=True)
+    assert exact["accepted_per_target"] == 0 and exact["accepted_per_wave"] == 0
+    assert exact["terms_per_target"] == sum(k * (k - 1) for k in sizes)
+    assert exact["sources_per_wave"] >= sum(sizes)
+    far = forest.walk_counts(0.5, 1e-3)
+    assert far["accepted_per_target"] > 0
+    assert far["accepted_per_wave"] <= far["accepted_per_target"]
+    assert far["terms_per_target"] < exact["terms_per_target"]
+    phi, field = forest.walk(0.5, 1e-3)
+    # the counted launch writes the same fields
+    from tree_d_3d_amd import _abi
+    d = forest._d
+    n = pos.shape[1]
+    phi2 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    field2 = torch.zeros((2, n), dtype=torch.float64, device="cuda")
+    counts = torch.zeros(4, dtype=torch.int64, device="cuda")
+    status = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _abi.call("td3_walk_counts", _abi.COULOMB, _abi.FP64, _abi.ptr(pos), n, _abi.ptr(d["perm"]),
+              _abi.ptr(torch.arange(n, device="cuda")), _abi.ptr(forest._sorted_pos),
+              _abi.ptr(forest._scharge), _abi.ptr(d["perm"]), n, _abi.ptr(d["node_center"]),
+              _abi.ptr(d["node_half"]), _abi.ptr(forest._node_info), _abi.ptr(d["node_qabs"]),
+              _abi.ptr(d["node_q"]), _abi.ptr(d["node_p"]), _abi.ptr(d["node_s"]),
+              _abi.ptr(d["moment_re"]), _abi.ptr(d["moment_im"]), forest.order, forest._depth,
+              _abi.ptr(forest._system), 0.5, 1e-3, 1, _abi.ptr(phi2), _abi.ptr(field2),
+              _abi.ptr(counts), _abi.ptr(status), _abi.stream())
+    assert int(status.item()) == 0
+    assert torch.equal(phi, phi2) and torch.equal(field, field2)
+
+
 def test_ensemble_steps_equal_each_simulation(td3):
     sizes = [1500, 3000, 700, 5000]
     alone = gaussian_systems(td3, sizes, seed=4)

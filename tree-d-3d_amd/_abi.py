@@ -55,6 +55,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_target_order": ([_P, _I, _D, _D, _D, _P, _P, _Z, _P], _C),
     "td3_walk": ([_C, _C, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I,
                   _I, _P, _D, _D, _C, _P, _P, _P, _P], _C),
+    "td3_walk_counts": ([_C, _C, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
+                         _P, _I, _I, _P, _D, _D, _C, _P, _P, _P, _P, _P], _C),
     "td3_traverse_workspace_bytes": ([_I], _Z),
     "td3_traverse": ([_P, _P, _P, _I, _I, _D, _I, _I, _I, _P, _P, _P, _Z, _P, _P], _C),
     "td3_lists_workspace_bytes": ([_I, _I, _C], _Z),

@@ -47,12 +47,17 @@ __device__ __forceinline__ double lane_value(double v, int i) {
 }
 __device__ __forceinline__ i64 lane_value(i64 v, int i) { return (i64)lane_bits((u64)v, i); }
 
-template <int KERNEL, int PREC, int WAVES>
+// COUNT: the same walk also adds up its work into counts[0..3] (td3_walk_counts): cells
+// accepted per target, source terms per target, cells accepted per wavefront (a cell some
+// lane of the wave accepts: its data is read once for the wave) and sources read per
+// wavefront (every source of every leaf the wave visits).
+template <int KERNEL, int PREC, int WAVES, bool COUNT = false>
 __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
     WalkTree tr, const double* __restrict__ targets, i64 m, const i64* __restrict__ order,
     const i64* __restrict__ exclude, const int* __restrict__ target_root, double theta,
     double eps2, int with_potential, int stack_size, double* __restrict__ phi,
-    double* __restrict__ field, i64* __restrict__ status) {
+    double* __restrict__ field, i64* __restrict__ status,
+    unsigned long long* __restrict__ counts = nullptr) {
     __shared__ int s_node[WAVES][kMaxStack];
     __shared__ u64 s_mask[WAVES][kMaxStack];
     const int wave = threadIdx.x / TD3_WAVE;
@@ -70,6 +75,8 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
     if (full == 0) return;
     double pot = 0.0, ex = 0.0, ey = 0.0;
     bool bad = false;
+    u32 n_accept = 0, n_terms = 0;       // COUNT: this lane's
+    u32 w_accept = 0, w_sources = 0;     // COUNT: the wave's (uniform)
     int* stn = s_node[wave];
     u64* stm = s_mask[wave];
     // one stack entry per root among the wave's targets (node 0 for a single tree; the
@@ -111,6 +118,7 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
             // loads per source.  Each target adds the same terms in the same order.
             const int k0 = __builtin_amdgcn_readfirstlane(nf.x);
             const int k1 = __builtin_amdgcn_readfirstlane(nf.y);
+            if (COUNT) w_sources += (u32)(k1 - k0);
             for (int base = k0; base < k1; base += TD3_WAVE) {
                 const int kl = base + lane;
                 double lc = 0.0, lx = 0.0, ly = 0.0;
@@ -135,6 +143,7 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
                         bad = true;
                         continue;
                     }
+                    if (COUNT) ++n_terms;
                     if (PREC == TD3_FP64) {
                         if (KERNEL == TD3_LOG) {
                             if (with_potential) pot -= c * log(r2);
@@ -189,6 +198,10 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
         }
         const u64 accept = __ballot(mine && acc);
         const u64 reject = __ballot(mine && !acc);
+        if (COUNT) {
+            w_accept += accept != 0 ? 1u : 0u;
+            n_accept += (mine && acc && !bad) ? 1u : 0u;
+        }
         if (mine && acc && !bad) {
             if (KERNEL == TD3_LOG) {
                 const double* are = tr.mre + (i64)node * tr.p1;
@@ -277,6 +290,19 @@ __global__ __launch_bounds__(64 * WAVES) void walk_kernel(
     if (overflow) {
         if (lane == 0) td3_report(status, -1);
         return;
+    }
+    if (COUNT) {
+        u32 a = n_accept, t = n_terms;
+        for (int off = TD3_WAVE / 2; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, TD3_WAVE);
+            t += __shfl_xor(t, off, TD3_WAVE);
+        }
+        if (lane == 0) {
+            atomicAdd(&counts[0], (unsigned long long)a);
+            atomicAdd(&counts[1], (unsigned long long)t);
+            atomicAdd(&counts[2], (unsigned long long)w_accept);
+            atomicAdd(&counts[3], (unsigned long long)w_sources);
+        }
     }
     if (!active) return;
     if (bad) td3_report(status, ti + 1);
@@ -432,6 +458,65 @@ extern "C" int td3_walk(int kernel, int precision, const double* targets, int64_
             launch_walk<TD3_COULOMB, TD3_MIXED>(s, tr, targets, m, order, exclude, target_root, theta,
                                                 eps2, with_potential, stack_size, phi, field, status);
     }
+    TD3_LAUNCHED();
+    return TD3_OK;
+}
+
+extern "C" int td3_walk_counts(int kernel, int precision, const double* targets, int64_t m,
+                               const int64_t* order, const int64_t* exclude,
+                               const double* sorted_pos, const double* sorted_charge,
+                               const int64_t* perm, int64_t n, const double* node_center,
+                               const double* node_half, const void* node_info,
+                               const double* node_qabs, const double* node_q,
+                               const double* node_p, const double* node_s,
+                               const double* moment_re, const double* moment_im, int64_t order_p,
+                               int64_t tree_depth, const int32_t* target_root, double theta,
+                               double softening, int with_potential, double* phi, double* field,
+                               uint64_t* counts, int64_t* status, void* stream) {
+    TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
+    TD3_ARG(precision == TD3_FP64 || precision == TD3_MIXED, "precision");
+    TD3_ARG(m >= 0 && n >= 1, "sizes");
+    TD3_ARG(tree_depth >= 0 && tree_depth <= 64, "tree depth");
+    TD3_ARG(counts != nullptr, "counts");
+    if (m == 0) return TD3_OK;
+    WalkTree tr;
+    tr.spos = sorted_pos;
+    tr.scharge = sorted_charge;
+    tr.perm = perm;
+    tr.n = n;
+    tr.center = node_center;
+    tr.half = node_half;
+    tr.info = (const int4*)node_info;
+    tr.qabs = node_qabs;
+    tr.q = node_q;
+    tr.p = node_p;
+    tr.s = node_s;
+    tr.mre = moment_re;
+    tr.mim = moment_im;
+    tr.p1 = (int)order_p + 1;
+    const int stack_size = target_root != nullptr ? kMaxStack
+                           : (int)(3 * tree_depth + 8) < kMaxStack ? (int)(3 * tree_depth + 8)
+                                                                   : kMaxStack;
+    const double eps2 = softening * softening;
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned blocks = td3_xcd_grid(td3_blocks(m, 64 * kWaves));
+    unsigned long long* c = (unsigned long long*)counts;
+#define TD3_WALK_COUNTED(K, P)                                                                  \
+    hipLaunchKernelGGL((walk_kernel<K, P, kWaves, true>), dim3(blocks), dim3(64 * kWaves), 0, s, \
+                       tr, targets, m, order, exclude, target_root, theta, eps2, with_potential,  \
+                       stack_size, phi, field, status, c)
+    if (kernel == TD3_LOG) {
+        if (precision == TD3_FP64)
+            TD3_WALK_COUNTED(TD3_LOG, TD3_FP64);
+        else
+            TD3_WALK_COUNTED(TD3_LOG, TD3_MIXED);
+    } else {
+        if (precision == TD3_FP64)
+            TD3_WALK_COUNTED(TD3_COULOMB, TD3_FP64);
+        else
+            TD3_WALK_COUNTED(TD3_COULOMB, TD3_MIXED);
+    }
+#undef TD3_WALK_COUNTED
     TD3_LAUNCHED();
     return TD3_OK;
 }

@@ -130,6 +130,31 @@ class _Forest(FlatQuadtree):
         _check(status)
         return phi, field
 
+    def walk_counts(self, theta: float, softening: float) -> dict:
+        """The work of `walk` (td3_walk_counts, one extra launch of the same walk): cells
+        accepted and source terms, per target and per wavefront."""
+        pos, d = self._d["pos"], self._d
+        n = pos.shape[1]
+        phi = torch.zeros(n, dtype=torch.float64, device=pos.device)
+        field = torch.zeros((2, n), dtype=torch.float64, device=pos.device)
+        status = torch.zeros(1, dtype=torch.int64, device=pos.device)
+        counts = torch.zeros(4, dtype=torch.int64, device=pos.device)
+        everyone = torch.arange(n, device=pos.device)
+        _abi.call(
+            "td3_walk_counts", _abi.COULOMB, _abi.FP64, _abi.ptr(pos), n, _abi.ptr(d["perm"]),
+            _abi.ptr(everyone), _abi.ptr(self._sorted_pos), _abi.ptr(self._scharge),
+            _abi.ptr(d["perm"]), n, _abi.ptr(d["node_center"]), _abi.ptr(d["node_half"]),
+            _abi.ptr(self._node_info), _abi.ptr(d["node_qabs"]), _abi.ptr(d["node_q"]),
+            _abi.ptr(d["node_p"]), _abi.ptr(d["node_s"]), _abi.ptr(d["moment_re"]),
+            _abi.ptr(d["moment_im"]), self.order, self._depth, _abi.ptr(self._system),
+            float(theta), float(softening), 1, _abi.ptr(phi), _abi.ptr(field),
+            _abi.ptr(counts), _abi.ptr(status), _abi.stream(),
+        )
+        _check(status)
+        c = [int(v) for v in counts.cpu()]
+        return {"accepted_per_target": c[0], "terms_per_target": c[1],
+                "accepted_per_wave": c[2], "sources_per_wave": c[3]}
+
 
 class _Resident:
     """Every array one step of an ensemble touches, allocated once at a fixed node
