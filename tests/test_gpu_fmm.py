@@ -214,6 +214,20 @@ def test_fmm_on_two_streams_equals_one_launch(td3, precision, potential):
             assert not np.asarray(phi).any()
 
 
+def test_abi_stream_is_torchs_current_stream(td3):
+    """_abi.stream() (torch's raw current stream) names the stream torch would use, on the
+    default stream and inside a stream context."""
+    import torch
+    from tree_d_3d_amd import _abi
+
+    _abi.device()
+    assert _abi.stream() == torch.cuda.current_stream().cuda_stream
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        assert _abi.stream() == side.cuda_stream == torch.cuda.current_stream().cuda_stream
+    assert _abi.stream() == torch.cuda.current_stream().cuda_stream
+
+
 def test_moments_made_on_first_use(td3, oracle):
     """The cells' moments wait for their first reader (fmm, the arrays): the same values
     whichever comes first, and again after set_charges."""

@@ -166,15 +166,27 @@ def call(name: str, *args) -> None:
     check(getattr(lib(), name)(*args), name)
 
 
+_GPU_SEEN = False
+# torch's current raw stream of a device, without building a Stream object (the public
+# torch.cuda.current_stream() costs ~10 us of Python per call; a step makes dozens)
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def device() -> torch.device:
     """The GPU the kernels run on (torch's current device); raises without one."""
-    if not torch.cuda.is_available():
-        raise RuntimeError("tree-d-3d_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    global _GPU_SEEN
+    if not _GPU_SEEN:
+        if not torch.cuda.is_available():
+            raise RuntimeError(
+                "tree-d-3d_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        _GPU_SEEN = True
     return torch.device("cuda", torch.cuda.current_device())
 
 
 def stream() -> int:
     """torch's current stream on the current device, as a hipStream_t handle."""
+    if _RAW_STREAM is not None and _GPU_SEEN:
+        return _RAW_STREAM(torch._C._cuda_getDevice())
     return torch.cuda.current_stream().cuda_stream
 
 
