@@ -14,8 +14,9 @@ import pytest
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_emulation")
 QUICK = ["direct", "moments"]
-DEFAULT = ["direct", "direct_noalloc", "strip", "smooth", "tree", "forest", "forest_resident", "moments", "walk", "fmm", "m2l_f32", "rows_overflow", "fmm_at",
-           "coulomb", "push", "adjoint"]
+DEFAULT = ["forest_resident", "m2l_f32", "tree", "adjoint", "smooth", "fmm", "push", "forest",
+           "strip", "rows_overflow", "walk", "coulomb", "fmm_at", "direct_noalloc", "direct",
+           "moments"]
 FULL = DEFAULT + ["big_tree", "rows"]
 
 
@@ -27,12 +28,31 @@ def emulated_library():
     return os.path.join(HERE, "libtd3_emu.so")
 
 
+# cases by measured cost, longest first, dealt to the worker processes in turn
+WORKERS = 4
+
+
+def _batches(cases):
+    batches = [[] for _ in range(min(WORKERS, len(cases)))]
+    for i, case in enumerate(cases):
+        batches[i % len(batches)].append(case)
+    return batches
+
+
 def test_kernels_under_address_sanitizer(emulated_library):
     asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True,
                           text=True).stdout.strip()
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
     mode = os.environ.get("TD3_EMULATION")
     cases = QUICK if mode == "quick" else FULL if mode == "full" else DEFAULT
-    r = subprocess.run([sys.executable, os.path.join(HERE, "run_emulated.py"), *cases],
-                       capture_output=True, text=True, env=env, timeout=3000)
-    assert r.returncode == 0 and "EMULATION OK" in r.stdout, (r.stdout[-3000:] + r.stderr[-6000:])
+    # independent cases in a few processes at once (each process runs its cases in order)
+    procs = [subprocess.Popen([sys.executable, os.path.join(HERE, "run_emulated.py"), *batch],
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                              env=env)
+             for batch in _batches(cases)]
+    failures = []
+    for batch, proc in zip(_batches(cases), procs):
+        out, err = proc.communicate(timeout=3000)
+        if proc.returncode != 0 or "EMULATION OK" not in out:
+            failures.append(f"cases {batch}:\n{out[-3000:]}{err[-6000:]}")
+    assert not failures, "\n\n".join(failures)
