@@ -34,6 +34,7 @@
 namespace {
 
 constexpr int kMaxLevels = 64;
+constexpr unsigned kCountBlocks = 512;  // count_kernel's grid at most
 
 // Key layout: the system index in the top sbits bits of hi, then levels 1..l1 (2 bits
 // each) in hi, levels l1+1.. in lo from its top.
@@ -128,10 +129,15 @@ __global__ __launch_bounds__(256) void count_kernel(const u64* __restrict__ khi,
                                                     u32* __restrict__ depth_count,
                                                     u32* __restrict__ flag) {
     __shared__ u32 s_depth[kMaxLevels + 1];
+    __shared__ u32 s_deeper;
     if (threadIdx.x <= kMaxLevels) s_depth[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_deeper = 0;
     __syncthreads();
-    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    bool deeper = false;
+    // a grid of at most kCountBlocks blocks strides over the particles: the blocks' totals
+    // go to a few depth counters with one atomic each, and same-address atomics serialise
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (i64)gridDim.x * blockDim.x) {
         const u64 ahi = khi[i], alo = klo ? klo[i] : 0ull;
         int prev = -1;
         if (i > 0) prev = lcp_levels(khi[i - 1], klo ? klo[i - 1] : 0ull, ahi, alo, nkey, kl);
@@ -143,11 +149,13 @@ __global__ __launch_bounds__(256) void count_kernel(const u64* __restrict__ khi,
         for (int d = prev + 1; d <= top; ++d) atomicAdd(&s_depth[d], 1u);
         // a common prefix as long as the key while cells that deep may still split:
         // the key must resolve more levels
-        if (m >= nkey && nkey <= max_split_depth) atomicOr(flag, 1u);
+        if (m >= nkey && nkey <= max_split_depth) deeper = true;
     }
+    if (deeper) s_deeper = 1u;
     __syncthreads();
     if (threadIdx.x <= kMaxLevels && s_depth[threadIdx.x])
         atomicAdd(&depth_count[threadIdx.x], s_depth[threadIdx.x]);
+    if (threadIdx.x == 0 && s_deeper) atomicOr(flag, 1u);
 }
 
 // One (depth, particle) entry per splitting node, in particle order.
@@ -553,7 +561,7 @@ int build_impl(const double* pos, int64_t n, i64 roots_n, Root rt, const int* ps
                 return rc;
         }
         // 3. splitting-node starts
-        TD3_TIMED(s, "tree_count", hipLaunchKernelGGL(count_kernel, dim3(nb), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
+        TD3_TIMED(s, "tree_count", hipLaunchKernelGGL(count_kernel, dim3(nb < kCountBlocks ? nb : kCountBlocks), dim3(256), 0, s, w.khi, deep ? w.klo : nullptr, n,
                            (i64)leaf_capacity, max_split, nkey, kl, w.cnt, depth_count, flag));
         TD3_LAUNCHED();
         if (nosync) break;  // a set flag is read on the device (levels_nosync_kernel)
