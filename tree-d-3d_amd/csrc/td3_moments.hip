@@ -66,6 +66,63 @@ __global__ void log_p2m_kernel(const double* __restrict__ spos, const double* __
     if (k == 0) qabs[node] = qa;
 }
 
+// log_p2m_kernel for orders up to 15 with a thread per leaf: the powers (z_j - c)^n run
+// upward by one multiplication each (log_moments_upward_pyccel.py:14-93 forms them the
+// same way) and the p + 1 sums stay in registers, so a leaf's particles are read once and
+// no lane raises z to a power of its own.  Each a_n sums its terms in particle order.
+constexpr int kP2mMax = 16;
+__global__ __launch_bounds__(256) void log_p2m_leaf_kernel(
+    const double* __restrict__ spos, const double* __restrict__ scharge,
+    const double* __restrict__ sdip, i64 n, const double* __restrict__ center,
+    const int4* __restrict__ info, i64 count, int p1, double* __restrict__ qabs,
+    double* __restrict__ mre, double* __restrict__ mim) {
+    const i64 node = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= count) return;
+    const int4 nf = info[node];
+    if (nf.z >= 0) return;
+    const double cx = center[2 * node], cy = center[2 * node + 1];
+    double ar[kP2mMax], ai[kP2mMax];
+#pragma unroll
+    for (int k = 0; k < kP2mMax; ++k) ar[k] = ai[k] = 0.0;
+    double qa = 0.0;
+    for (int j = nf.x; j < nf.y; ++j) {
+        const double q = scharge[j];
+        const double dx = spos[j] - cx, dy = spos[n + j] - cy;
+        qa += fabs(q);
+        double pr = q, pi = 0.0;  // q z^k
+        double er = 0.0, ei = 0.0;  // D z^(k-1), k >= 1
+        if (sdip != nullptr) {
+            er = sdip[j];
+            ei = sdip[n + j];
+        }
+#pragma unroll
+        for (int k = 0; k < kP2mMax; ++k) {
+            if (k < p1) {
+                ar[k] += pr;
+                ai[k] += pi;
+                if (sdip != nullptr && k > 0) {
+                    ar[k] += -k * er;
+                    ai[k] += -k * ei;
+                    const double t = er * dx - ei * dy;
+                    ei = er * dy + ei * dx;
+                    er = t;
+                }
+                const double t = pr * dx - pi * dy;
+                pi = pr * dy + pi * dx;
+                pr = t;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kP2mMax; ++k) {
+        if (k < p1) {
+            mre[node * p1 + k] = ar[k];
+            mim[node * p1 + k] = ai[k];
+        }
+    }
+    qabs[node] = qa;
+}
+
 // a_n += sum_m C(n, m) a_m(child) s^(n-m), s = c_child - c, for the nodes [lo, hi).
 __global__ void log_m2m_kernel(const double* __restrict__ center, const int4* __restrict__ info,
                                i64 lo, i64 hi, int p1, const double* __restrict__ binomial,
@@ -279,9 +336,14 @@ int moments_impl(int kernel, int64_t order, const double* sorted_pos, const doub
     if (kernel == TD3_LOG) {
         TD3_ARG(order >= 1 && order <= 64 && bsz >= order + 1, "order");
         const int p1 = (int)order + 1;
-        TD3_TIMED(s, "moments_p2m", hipLaunchKernelGGL(log_p2m_kernel, dim3(td3_blocks(node_count * p1, 256)), dim3(256), 0, s,
-                           sorted_pos, sorted_charge, sorted_dipole, n, node_center, info, node_count,
-                           p1, node_qabs, moment_re, moment_im));
+        if (p1 <= kP2mMax)
+            TD3_TIMED(s, "moments_p2m", hipLaunchKernelGGL(log_p2m_leaf_kernel, dim3(td3_blocks(node_count, 256)), dim3(256), 0, s,
+                               sorted_pos, sorted_charge, sorted_dipole, n, node_center, info,
+                               node_count, p1, node_qabs, moment_re, moment_im));
+        else
+            TD3_TIMED(s, "moments_p2m", hipLaunchKernelGGL(log_p2m_kernel, dim3(td3_blocks(node_count * p1, 256)), dim3(256), 0, s,
+                               sorted_pos, sorted_charge, sorted_dipole, n, node_center, info, node_count,
+                               p1, node_qabs, moment_re, moment_im));
         TD3_LAUNCHED();
         for (i64 d = n_levels - 2; d >= 0; --d) {
             const i64 lo = levels_host[d], hi = levels_host[d + 1];
