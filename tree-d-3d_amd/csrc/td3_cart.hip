@@ -60,6 +60,47 @@ __global__ void cart_p2m_kernel(const double* __restrict__ spos, const double* _
     mom[node * nc + i] = acc;
 }
 
+// cart_p2m_kernel with a thread per (leaf, a) for orders up to 15: the thread forms
+// q dx^a once per particle and runs up the powers of dy, adding q dx^a dy^b into each
+// m_(a,b) of its column.  Every term is the same product in the same order as there
+// (q, then a factors dx, then b factors dy), summed over the particles in the same order:
+// the moments are bit for bit cart_p2m_kernel's.
+constexpr int kCartP2mCol = 16;
+__global__ __launch_bounds__(256) void cart_p2m_col_kernel(
+    const double* __restrict__ spos, const double* __restrict__ scharge, i64 n,
+    const double* __restrict__ center, const int4* __restrict__ info, i64 count, int p, int nc,
+    double* __restrict__ mom) {
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * (p + 1)) return;
+    const i64 node = t / (p + 1);
+    const int a = (int)(t - node * (p + 1));
+    const int4 nf = info[node];
+    if (nf.z >= 0) return;
+    const double cx = center[2 * node], cy = center[2 * node + 1];
+    double acc[kCartP2mCol];
+#pragma unroll
+    for (int b = 0; b < kCartP2mCol; ++b) acc[b] = 0.0;
+    for (int j = nf.x; j < nf.y; ++j) {
+        const double dx = spos[j] - cx, dy = spos[n + j] - cy;
+        double v = scharge[j];
+        for (int k = 0; k < a; ++k) v *= dx;
+#pragma unroll
+        for (int b = 0; b < kCartP2mCol; ++b) {
+            if (b <= p - a) {
+                acc[b] += v;
+                v *= dy;
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < kCartP2mCol; ++b) {
+        if (b <= p - a) {
+            const int deg = a + b;
+            mom[node * nc + deg * (deg + 1) / 2 + b] = acc[b];
+        }
+    }
+}
+
 // M2M for the nodes [lo, hi): m_g += sum_children sum_{b <= g} C(g,b) m_b(child) s^(g-b).
 __global__ void cart_m2m_kernel(const double* __restrict__ center, const int4* __restrict__ info,
                                 i64 lo, i64 hi, int p, int nc, const double* __restrict__ binom,
@@ -532,8 +573,12 @@ extern "C" int td3_cart_moments(int64_t order, const double* sorted_pos, const d
     hipStream_t s = (hipStream_t)stream;
     const int p = (int)order, nc = csize(p);
     const int4* info = (const int4*)node_info;
-    TD3_TIMED(s, "cart_p2m", hipLaunchKernelGGL(cart_p2m_kernel, dim3(td3_blocks(node_count * nc, 256)), dim3(256), 0, s,
-                       sorted_pos, sorted_charge, n, node_center, info, node_count, p, nc, moments));
+    if (p + 1 <= kCartP2mCol)
+        TD3_TIMED(s, "cart_p2m", hipLaunchKernelGGL(cart_p2m_col_kernel, dim3(td3_blocks(node_count * (p + 1), 256)), dim3(256), 0, s,
+                           sorted_pos, sorted_charge, n, node_center, info, node_count, p, nc, moments));
+    else
+        TD3_TIMED(s, "cart_p2m", hipLaunchKernelGGL(cart_p2m_kernel, dim3(td3_blocks(node_count * nc, 256)), dim3(256), 0, s,
+                           sorted_pos, sorted_charge, n, node_center, info, node_count, p, nc, moments));
     TD3_LAUNCHED();
     for (i64 d = n_levels - 2; d >= 0; --d) {
         const i64 lo = levels_host[d], hi = levels_host[d + 1];
