@@ -41,3 +41,27 @@ def test_no_cpu_fallback_without_gpu(td3):
         pytest.skip("a GPU is present")
     with pytest.raises(RuntimeError, match="GPU"):
         td3.direct_field([[0.0], [0.0]], [[1.0], [1.0]], [1.0])
+
+
+def test_moments_made_on_first_use_and_again_after_a_failure(td3):
+    """A tree's array dict makes the deferred cell moments on the first read of one of
+    them, once; a maker that fails stays deferred for the next read; other names stay
+    missing."""
+    import sys
+
+    arrays = sys.modules["tree_d_3d_amd.flat_tree"]._Arrays(pos=1)
+    calls = []
+
+    def make():
+        calls.append(1)
+        if len(calls) == 1:
+            raise RuntimeError("first attempt")
+        arrays.update(moment_re=5, moment_im=6, node_qabs=7)
+
+    arrays.defer(make)
+    assert "moment_re" not in arrays
+    with pytest.raises(RuntimeError):
+        arrays["moment_re"]
+    assert arrays["moment_im"] == 6 and arrays["moment_re"] == 5 and len(calls) == 2
+    with pytest.raises(KeyError):
+        arrays["node_center"]

@@ -209,7 +209,11 @@ class _Arrays(dict):
         if make is None or key not in self.MOMENTS:
             raise KeyError(key)
         self.pending = None
-        make()
+        try:
+            make()
+        except BaseException:
+            self.pending = make  # made on the next use instead
+            raise
         return dict.__getitem__(self, key)
 
 
