@@ -394,16 +394,17 @@ int td3_fmm_eval(int kernel, int precision, int64_t order, const double *sorted_
 
 /*
  * td3_fmm_eval in two launches, so that the near field can run on a second stream while
- * the far field is translated: td3_fmm_near sums the P2P row of every source (the same
- * terms, order and status as td3_fmm_eval) into near (3, N) float64 in tree order
- * (potential, field x, field y); td3_fmm_far then adds each particle's L2P and writes phi
- * (N,) and field (2, N) in input order, bit for bit td3_fmm_eval's.
+ * the far field is translated: td3_fmm_near sums the P2P row of the sources [first, first
+ * + count) of tree order (the same terms, order and status as td3_fmm_eval) into their
+ * columns of near (3, N) float64 in tree order (potential, field x, field y), so that the
+ * near field can be split into launches; td3_fmm_far then adds each particle's L2P and
+ * writes phi (N,) and field (2, N) in input order, bit for bit td3_fmm_eval's.
  */
 int td3_fmm_near(int kernel, int precision, const double *sorted_pos, const int64_t *perm,
-                 int64_t n, const void *node_info, const uint32_t *p2p_row,
-                 const int32_t *p2p_col, const int32_t *leaf_of, const void *sources4,
-                 double softening, int with_potential, double *near, int64_t *status,
-                 void *stream);
+                 int64_t n, int64_t first, int64_t count, const void *node_info,
+                 const uint32_t *p2p_row, const int32_t *p2p_col, const int32_t *leaf_of,
+                 const void *sources4, double softening, int with_potential, double *near,
+                 int64_t *status, void *stream);
 int td3_fmm_far(int kernel, int64_t order, const double *sorted_pos, const int64_t *perm,
                 int64_t n, const double *node_center, const double *node_half,
                 const double *local_re, const double *local_im, const int32_t *leaf_of,

@@ -193,7 +193,11 @@ def test_fmm_on_two_streams_equals_one_launch(td3, precision, potential):
     tree = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", precision=precision)
     prec = _abi.FP64 if precision == "fp64" else _abi.MIXED
     n = pos.shape[1]
-    for _ in range(2):
+    # fresh rows with the default split of the near field, then kept rows with none of it
+    # early and all of it early
+    for early in (None, 0.0, 1.0):
+        if early is not None:
+            tree.NEAR_EARLY = early
         phi, field = tree.fmm(softening=1e-4, potential=potential)
         lre, lim = tree._local_expansions(0.5, prec)
         _, (prow, pcol, _), _ = tree._lists(0.5)

@@ -71,7 +71,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "td3_unscale_locals": ([_P, _I, _I, _P, _P, _P], _C),
     "td3_fmm_eval": ([_C, _C, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _D, _C,
                       _P, _P, _P, _P], _C),
-    "td3_fmm_near": ([_C, _C, _P, _P, _I, _P, _P, _P, _P, _P, _D, _C, _P, _P, _P], _C),
+    "td3_fmm_near": ([_C, _C, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _D, _C, _P, _P, _P], _C),
     "td3_fmm_far": ([_C, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _C, _P, _P, _P], _C),
     "td3_pack_sources": ([_P, _P, _I, _P, _P], _C),
     "td3_log_moments_dipole": ([_I, _P, _P, _P, _I, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P],

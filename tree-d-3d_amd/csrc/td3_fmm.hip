@@ -924,12 +924,12 @@ __global__ __launch_bounds__(256) void eval_kernel(
 // The sums and their order are eval_kernel's.
 template <int KERNEL, int PREC>
 __global__ __launch_bounds__(256) void eval_near_kernel(
-    const double* __restrict__ tpos, const i64* __restrict__ tperm, i64 m,
+    const double* __restrict__ tpos, const i64* __restrict__ tperm, i64 m, i64 k0, i64 k1,
     const u32* __restrict__ prow, const int* __restrict__ pcol, const int4* __restrict__ sinfo,
     const int* __restrict__ leaf_of, const double4* __restrict__ src4, double eps2,
     int with_potential, double* __restrict__ near, i64* __restrict__ status) {
-    const i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
+    const i64 k = k0 + (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= k1) return;
     const i64 leaf = leaf_of[k];
     const double tx = tpos[k], ty = tpos[m + k];
     const double tq = src4[k].z;
@@ -1447,22 +1447,25 @@ extern "C" int td3_fmm_eval(int kernel, int precision, int64_t order, const doub
 }
 
 extern "C" int td3_fmm_near(int kernel, int precision, const double* sorted_pos,
-                            const int64_t* perm, int64_t n, const void* node_info,
-                            const uint32_t* p2p_row, const int32_t* p2p_col,
-                            const int32_t* leaf_of, const void* sources4, double softening,
-                            int with_potential, double* near, int64_t* status, void* stream) {
+                            const int64_t* perm, int64_t n, int64_t first, int64_t count,
+                            const void* node_info, const uint32_t* p2p_row,
+                            const int32_t* p2p_col, const int32_t* leaf_of, const void* sources4,
+                            double softening, int with_potential, double* near, int64_t* status,
+                            void* stream) {
     TD3_ARG(kernel == TD3_COULOMB || kernel == TD3_LOG, "kernel");
     TD3_ARG(n >= 1, "sizes");
+    TD3_ARG(first >= 0 && count >= 0 && first + count <= n, "target range");
     TD3_ARG(leaf_of != nullptr && sources4 != nullptr, "leaf_of, sources4");
+    if (count == 0) return TD3_OK;
     hipStream_t s = (hipStream_t)stream;
-    const dim3 grid(td3_blocks(n, 256));
+    const dim3 grid(td3_blocks(count, 256));
     const double eps2 = softening * softening;
     const int4* info = (const int4*)node_info;
     const double4* src4 = (const double4*)sources4;
 #define TD3_NEAR(K, P)                                                                           \
     TD3_TIMED(s, "eval", hipLaunchKernelGGL((eval_near_kernel<K, P>), grid, dim3(256), 0, s,     \
-              sorted_pos, perm, n, p2p_row, p2p_col, info, leaf_of, src4, eps2, with_potential,  \
-              near, status))
+              sorted_pos, perm, n, first, first + count, p2p_row, p2p_col, info, leaf_of, src4,    \
+              eps2, with_potential, near, status))
     if (kernel == TD3_LOG) {
         if (precision == TD3_FP64) TD3_NEAR(TD3_LOG, TD3_FP64);
         else TD3_NEAR(TD3_LOG, TD3_MIXED);

@@ -840,6 +840,11 @@ class FlatQuadtree:
     #: largest: a cell's net charge grows with its size)
     FP64_TOP_DEPTH = 5
 
+    #: share of the targets (the first in tree order) whose near field the FMM sums as soon
+    #: as the rows exist, beside the moments' last levels; the rest wait for the M2L and run
+    #: beside the L2L (measured at N = 1M: 0.2 gives 2.30 ms a step against 2.32 ms at 0)
+    NEAR_EARLY = 0.2
+
     #: first capacities of the interaction rows per node (M2L and P2P entries in all):
     #: exceeded, the rows are built again with twice the room
     _ROW_CAPS = (96, 40)
@@ -981,8 +986,9 @@ class FlatQuadtree:
 
     def _fmm_overlapped(self, theta, softening, potential, prec, phi, field):
         """The log FMM on two streams.  A second stream, from the moment the tree and the
-        sorted charges exist: the leaf map, the packed sources and the interaction rows,
-        then (once the M2L is done) the near field (td3_fmm_near).  The caller's stream:
+        sorted charges exist: the leaf map, the packed sources, the interaction rows and the
+        near field (td3_fmm_near) of the first NEAR_EARLY of the targets, then (once the M2L
+        is done) that of the rest.  The caller's stream:
         the cells' moments and their scaled forms (beside the rows: the rows depend on the
         cells only), the M2L once the rows are there, then the L2L (beside the near field:
         its small levels leave most of the GPU to it) and the L2P with the near sums added
@@ -1011,6 +1017,18 @@ class FlatQuadtree:
             rows_done = streams.mark()
             near = torch.empty((3, n), dtype=torch.float64, device=dev)
             status = torch.zeros(1, dtype=torch.int64, device=dev)
+
+            def near_field(first, count):
+                _abi.call("td3_fmm_near", _abi.LOG, prec, _abi.ptr(self._sorted_pos),
+                          _abi.ptr(self._d["perm"]), n, first, count,
+                          _abi.ptr(self._node_info), _abi.ptr(p2p_rows[0]),
+                          _abi.ptr(p2p_rows[1]), _abi.ptr(leaf_of), _abi.ptr(src4), softening,
+                          1 if potential else 0, _abi.ptr(near), _abi.ptr(status),
+                          _abi.stream())
+
+            # the first targets' near field at once, beside the moments' last small levels
+            early = int(n * self.NEAR_EARLY)
+            near_field(0, early)
         streams.used_by_caller(leaf_of, src4, *m2l_rows, *p2p_rows, dcounts, near, status)
         # the M2L's inputs that need the moments only
         self._scaled_moments(_abi.FP64)
@@ -1024,11 +1042,7 @@ class FlatQuadtree:
         m2l_done = streams.mark_caller()
         with streams.side():
             streams.side_wait(m2l_done)
-            _abi.call("td3_fmm_near", _abi.LOG, prec, _abi.ptr(self._sorted_pos),
-                      _abi.ptr(self._d["perm"]), n, _abi.ptr(self._node_info),
-                      _abi.ptr(p2p_rows[0]), _abi.ptr(p2p_rows[1]), _abi.ptr(leaf_of),
-                      _abi.ptr(src4), softening, 1 if potential else 0, _abi.ptr(near),
-                      _abi.ptr(status), _abi.stream())
+            near_field(early, n - early)
             near_done = streams.mark()
         self._l2l(self, lre, lim)
         streams.wait(near_done)
