@@ -11,7 +11,7 @@ def relative(a, b):
 
 
 def config1(seed=1, n=5000):
-    """uniform [-1, 1]^2, charges +-1, Coulomb, theta 0.5, eps 1e-3."""
+    """The Barnes-Hut workload: uniform [-1, 1]^2, charges +-1, Coulomb, theta 0.5, eps 1e-3."""
     rng = np.random.default_rng(seed)
     return rng.uniform(-1, 1, (2, n)), rng.choice([-1.0, 1.0], n)
 
