@@ -29,7 +29,7 @@ import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # measured (rocprofv3 --pmc, calibrated by a 256 MiB copy) for this workload's M2L launch
-M2L_TRAFFIC_BYTES = 669.5e6
+M2L_TRAFFIC_BYTES = 669.8e6
 KERNEL_NAMES = ("m2l_f32", "m2l_records", "m2l_mfma64", "m2l", "m2l_fixup", "eval", "eval_far", "l2l", "rows_level",
                 "rows_place",
                 "traverse",
