@@ -306,14 +306,28 @@ def case_rows_overflow():
     # rows built without a read-back overflow their capacities: every row is left empty,
     # the FMM notices with its status and builds them again
     pos, q = cloud(400, 3)
+    td3.FlatQuadtree._row_width_seen = 0  # no width learned from earlier trees
     t = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10, leaf_capacity=4)
     t._ROW_CAPS = (1, 1)
     t._ROW_WIDTH = 8
     got = t.fmm(theta=0.5, softening=1e-3, precision="mixed")
+    # the width this tree needed becomes the starting width of later trees
+    learned = td3.FlatQuadtree._row_width_seen
+    assert learned > 8, learned
+    later = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10, leaf_capacity=4)
+    later._ROW_WIDTH = 8
+    later.fmm(theta=0.5, softening=1e-3, precision="mixed")
+    assert "_row_width" not in later.__dict__  # built at the learned width, no second build
+    td3.FlatQuadtree._row_width_seen = 0
     r = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log", order=10, leaf_capacity=4)
     want = r.fmm(theta=0.5, softening=1e-3, precision="mixed")
     assert t.fmm_pairs == r.fmm_pairs
     assert relative(got[0], want[0]) < 1e-14 and relative(got[1], want[1]) < 1e-14
+    # the mixed rows carry no per-entry destinations; an FP64 FMM on them makes those
+    want64 = oracle.OracleTree(pos, q, (0, 0, 1), kernel="log", order=10,
+                               leaf_capacity=4).fmm(theta=0.5, softening=1e-3)
+    got64 = r.fmm(theta=0.5, softening=1e-3, precision="fp64")
+    assert relative(got64[0], want64[0]) < 1e-12 and relative(got64[1], want64[1]) < 1e-12
 
 
 def case_fmm_at():

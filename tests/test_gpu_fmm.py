@@ -218,6 +218,22 @@ def test_fmm_on_two_streams_equals_one_launch(td3, precision, potential):
             assert not np.asarray(phi).any()
 
 
+def test_fp64_fmm_on_rows_of_a_mixed_fmm(td3):
+    """The mixed FMM builds its rows without each M2L entry's destination (its M2L walks
+    rows by their starts); an FP64 FMM on those kept rows makes the destinations first and
+    gives, bit for bit, what a tree whose rows were built for FP64 gives."""
+    rng = np.random.default_rng(5)
+    pos, q = rng.uniform(-1, 1, (2, 80000)), rng.choice([-1.0, 1.0], 80000)
+    mixed_first = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log")
+    mixed_first.fmm(softening=1e-4, precision="mixed")
+    assert mixed_first._lists(0.5)[0][2] is None
+    got = mixed_first.fmm(softening=1e-4, precision="fp64")
+    want = td3.FlatQuadtree(pos, q, td3.Quad(0, 0, 1), kernel="log").fmm(softening=1e-4,
+                                                                        precision="fp64")
+    np.testing.assert_array_equal(np.asarray(got[0]), np.asarray(want[0]))
+    np.testing.assert_array_equal(np.asarray(got[1]), np.asarray(want[1]))
+
+
 def test_abi_stream_is_torchs_current_stream(td3):
     """_abi.stream() (torch's raw current stream) names the stream torch would use, on the
     default stream and inside a stream context."""

@@ -689,13 +689,16 @@ class FlatQuadtree:
         c = self.node_count
         levels = np.ascontiguousarray(self.levels, dtype=np.int64)
         n_levels = len(levels) - 1
-        caps = [f * c + 64 for f in self._ROW_CAPS] + [self.__dict__.get("_row_width",
-                                                                          self._ROW_WIDTH)]
+        width = self.__dict__.get("_row_width",
+                                  max(self._ROW_WIDTH, FlatQuadtree._row_width_seen))
+        caps = [f * c + 64 for f in self._ROW_CAPS] + [width]
         out = np.zeros(4, dtype=np.int64)
         if deferred:
             i32 = dict(dtype=torch.int32, device=dev)
             mrow, prow = torch.empty(c + 1, **i32), torch.empty(c + 1, **i32)
-            mcol, mdst = torch.empty(caps[0], **i32), torch.empty(caps[0], **i32)
+            # no destination per entry: the M2L walks the rows by their starts (td3_rows
+            # leaves m2l_dst alone when it is NULL; _row_dst makes it if needed later)
+            mcol, mdst = torch.empty(caps[0], **i32), None
             pcol = torch.empty(caps[1], **i32)
             dcounts = torch.zeros(4, dtype=torch.int64, device=dev)
             nbytes = _abi.lib().td3_rows_workspace_bytes(levels.ctypes.data, n_levels, caps[2])
@@ -726,6 +729,10 @@ class FlatQuadtree:
                 caps = [max(2 * caps[0], int(out[0]) + 64), max(2 * caps[1], int(out[1]) + 64),
                         max(caps[2], int(out[3]) + 8) if out[3] > 1 else caps[2]]
                 self._row_width = caps[2]
+                # later trees of the process start at the width this one needed: a time
+                # step's next tree is much like this one, and a too-narrow first build
+                # costs a second build (and, without a read-back, a second FMM)
+                FlatQuadtree._row_width_seen = max(FlatQuadtree._row_width_seen, caps[2])
                 continue
             _abi.check(code, "td3_rows")
             break
@@ -736,7 +743,9 @@ class FlatQuadtree:
         n_m2l, n_p2p, leaves = int(out[0]), int(out[1]), int(out[2])
         counts = (n_m2l // 2, (n_p2p + leaves) // 2)
         (mrow, mcol, mdst), (prow, pcol, _) = m2l_rows, p2p_rows
-        return (mrow, mcol[: max(n_m2l, 1)], mdst[: max(n_m2l, 1)]), \
+        if mdst is not None:
+            mdst = mdst[: max(n_m2l, 1)]
+        return (mrow, mcol[: max(n_m2l, 1)], mdst), \
             (prow, pcol[: max(n_p2p, 1)], None), counts
 
     def _traverse_lists(self, theta: float):
@@ -851,6 +860,8 @@ class FlatQuadtree:
     #: entries per node of the row construction's slots (M2L row, P2P row, hand-downs to
     #: the children); a node that needs more has the rows built again with its width
     _ROW_WIDTH = 128
+    #: the widest slots a tree of this process has needed (0: none wider than _ROW_WIDTH)
+    _row_width_seen = 0
 
     def _local_expansions(self, theta: float, precision: int, target=None, m2l_rows=None):
         """Scaled local expansions B_l = b_l r^l of every cell of ``target`` (this tree by
@@ -899,6 +910,8 @@ class FlatQuadtree:
                 nbytes = _abi.lib().td3_m2l_workspace_bytes(int(mcol.shape[0]), self.order,
                                                             prec)
                 ws = _abi.workspace(nbytes)
+                if mdst is None:
+                    mdst = self._row_dst(mrow, mcol)
             _abi.call("td3_m2l", prec, self.order, _abi.ptr(mcol), _abi.ptr(mdst),
                       _abi.ptr(mrow), lo, hi, _abi.ptr(td["node_center"]),
                       _abi.ptr(td["node_half"]), _abi.ptr(self._d["node_center"]),
@@ -906,6 +919,26 @@ class FlatQuadtree:
                       _abi.ptr(coeff), _abi.ptr(lre), _abi.ptr(lim), _abi.ptr(ws), nbytes,
                       _abi.stream())
         return lre, lim
+
+    def _row_dst(self, mrow, mcol):
+        """Each M2L entry's row (its destination cell), made from the row starts when the
+        rows were built without it (the mixed FMM's rows: only the lane-per-pair M2L reads
+        it), and kept with the rows."""
+        cache = self.__dict__.setdefault("_row_dst_cache", {})
+        hit = cache.get(mcol.data_ptr())
+        if hit is not None and hit[0] is mcol:
+            return hit[1]
+        total = int(mrow[-1].item())
+        dev = mrow.device
+        if total == 0:
+            dst = torch.zeros(1, dtype=torch.int32, device=dev)
+        else:
+            counts = (mrow[1:] - mrow[:-1]).to(torch.int64)
+            dst = torch.repeat_interleave(
+                torch.arange(mrow.shape[0] - 1, dtype=torch.int32, device=dev), counts,
+                output_size=total)
+        cache[mcol.data_ptr()] = (mcol, dst)
+        return dst
 
     def _l2l(self, target, lre, lim) -> None:
         """L2L down ``target``, level by level, in place."""
