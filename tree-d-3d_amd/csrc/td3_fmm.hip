@@ -740,91 +740,147 @@ __global__ void l2l16_kernel(const double* __restrict__ center, const double* __
 // tskip skipped).  FP64: every term in FP64.  Mixed: differences in FP64 (exact for nearby
 // particles, whatever cells hold them), the kernel in FP32, FP32 sums over the leaf, then
 // FP64.  A pair at distance 0 (softening 0) sets bad instead of adding a term.
+// One pair's term: the source record sv (index ks) at the target.  FP64: into the FP64 sums;
+// mixed: into the FP32 partial sums fp, fx, fy of the partner leaf (the FP64 sums when the
+// FP32 distance is zero or below range).  The FP64 sums are explicit fma calls, so every
+// kernel built on these helpers rounds each term the same way.
+template <int KERNEL, int PREC, int MODE>
+__device__ __forceinline__ void p2p_term(double tx, double ty, double tq, i64 k, i64 tskip,
+                                         int ks, const double4& sv,
+                                         const i64* __restrict__ sperm, double eps2,
+                                         int with_potential, double& acc_p, double& acc_x,
+                                         double& acc_y, float& fp, float& fx, float& fy,
+                                         bool& bad) {
+    const double sq = sv.z;
+    if (MODE == 0 && ks == k) return;
+    if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) return;
+    if (PREC == TD3_FP64) {
+        const double dx = tx - sv.x, dy = ty - sv.y;
+        const double r2 = fma(dx, dx, fma(dy, dy, eps2));
+        if (r2 == 0.0) {
+            if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
+            return;
+        }
+        if (KERNEL == TD3_COULOMB) {
+            const double inv_r = 1.0 / sqrt(r2);
+            const double cr3 = sq * inv_r * inv_r * inv_r;
+            acc_p = fma(sq, inv_r, acc_p);
+            acc_x = fma(cr3, dx, acc_x);
+            acc_y = fma(cr3, dy, acc_y);
+        } else {
+            if (with_potential) acc_p = fma(-sq, log(r2), acc_p);
+            const double inv = 2.0 * sq / r2;
+            acc_x = fma(inv, dx, acc_x);
+            acc_y = fma(inv, dy, acc_y);
+        }
+    } else {
+        const double ddx = tx - sv.x, ddy = ty - sv.y;
+        const float dx = (float)ddx, dy = (float)ddy, qf = (float)sq;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, (float)eps2));
+        if (r2 == 0.0f) {
+            // zero or below FP32 range: decide on the FP64 pair
+            const double dr2 = fma(ddx, ddx, fma(ddy, ddy, eps2));
+            if (dr2 == 0.0) {
+                if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
+                return;
+            }
+            if (KERNEL == TD3_COULOMB) {
+                const double inv_r = 1.0 / sqrt(dr2);
+                const double cr3 = sq * inv_r * inv_r * inv_r;
+                acc_p = fma(sq, inv_r, acc_p);
+                acc_x = fma(cr3, ddx, acc_x);
+                acc_y = fma(cr3, ddy, acc_y);
+            } else {
+                if (with_potential) acc_p = fma(-sq, log(dr2), acc_p);
+                const double inv = 2.0 * sq / dr2;
+                acc_x = fma(inv, ddx, acc_x);
+                acc_y = fma(inv, ddy, acc_y);
+            }
+            return;
+        }
+        if (KERNEL == TD3_COULOMB) {
+            const float inv_r = __builtin_amdgcn_rsqf(r2);
+            const float c = qf * inv_r;
+            const float cr3 = c * inv_r * inv_r;
+            fp += c;
+            fx = fmaf(cr3, dx, fx);
+            fy = fmaf(cr3, dy, fy);
+        } else {
+            // -q ln r2 = -ln 2 q log2 r2 and 2 q d / r2; the constants at the end
+            if (with_potential) fp = fmaf(qf, __builtin_amdgcn_logf(r2), fp);
+            const float inv = qf * __builtin_amdgcn_rcpf(r2);
+            fx = fmaf(inv, dx, fx);
+            fy = fmaf(inv, dy, fy);
+        }
+    }
+}
+
+// The end of a partner leaf in mixed precision: its FP32 partial sums into the FP64 sums.
+template <int KERNEL, int PREC>
+__device__ __forceinline__ void p2p_fold(float fp, float fx, float fy, double& acc_p,
+                                         double& acc_x, double& acc_y) {
+    if (PREC == TD3_FP64) return;
+    if (KERNEL == TD3_COULOMB) {
+        acc_p += (double)fp;
+        acc_x += (double)fx;
+        acc_y += (double)fy;
+    } else {
+        acc_p = fma(-0.6931471805599453, (double)fp, acc_p);
+        acc_x += 2.0 * (double)fx;
+        acc_y += 2.0 * (double)fy;
+    }
+}
+
 template <int KERNEL, int PREC, int MODE>
 __device__ __forceinline__ void p2p_leaf(double tx, double ty, double tq, i64 k, i64 tskip,
                                          int4 sf, const double4* __restrict__ src4,
                                          const i64* __restrict__ sperm, double eps2,
                                          int with_potential, double& acc_p, double& acc_x,
                                          double& acc_y, bool& bad) {
-    if (PREC == TD3_FP64) {
-        for (int ks = sf.x; ks < sf.y; ++ks) {
-            const double4 sv = src4[ks];
-            const double sq = sv.z;
-            if (MODE == 0 && ks == k) continue;
-            if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
-            const double dx = tx - sv.x, dy = ty - sv.y;
-            const double r2 = dx * dx + dy * dy + eps2;
-            if (r2 == 0.0) {
-                if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
-                continue;
+    float fp = 0.0f, fx = 0.0f, fy = 0.0f;
+    for (int ks = sf.x; ks < sf.y; ++ks)
+        p2p_term<KERNEL, PREC, MODE>(tx, ty, tq, k, tskip, ks, src4[ks], sperm, eps2,
+                                     with_potential, acc_p, acc_x, acc_y, fp, fx, fy, bad);
+    p2p_fold<KERNEL, PREC>(fp, fx, fy, acc_p, acc_x, acc_y);
+}
+
+// p2p_leaf over a whole P2P row (entries r0 .. r1 - 1 of pcol) as one loop over its sources:
+// a lane moves to its next partner leaf when it has run out of the current one, so the
+// lanes of a wavefront, which walk different rows, stay busy until the longest row ends
+// rather than until each partner position's largest leaf ends.  Same terms, same order and
+// same per-leaf partial sums as p2p_leaf called partner by partner.  Some lane of a
+// wavefront moves to its next partner at almost every step, so the move is paid on every
+// iteration: the FP64 terms outweigh it (the near field of 1M clustered and uniform particles,
+// leaf 8, theta 0.5: 0.80 against 0.93 ms), the mixed ones do not (0.49 against 0.40 ms), so
+// only FP64 takes this loop.
+template <int KERNEL, int PREC>
+__device__ __forceinline__ void p2p_row(double tx, double ty, double tq, i64 k, u32 r0, u32 r1,
+                                        const int* __restrict__ pcol,
+                                        const int4* __restrict__ sinfo,
+                                        const double4* __restrict__ src4, double eps2,
+                                        int with_potential, double& acc_p, double& acc_x,
+                                        double& acc_y, bool& bad) {
+    float fp = 0.0f, fx = 0.0f, fy = 0.0f;
+    u32 r = r0;
+    int ks = 0, ke = 0;
+    bool started = false;
+    int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
+    for (;;) {
+        while (ks >= ke) {
+            if (started) {
+                p2p_fold<KERNEL, PREC>(fp, fx, fy, acc_p, acc_x, acc_y);
+                fp = fx = fy = 0.0f;
             }
-            if (KERNEL == TD3_COULOMB) {
-                const double inv_r = 1.0 / sqrt(r2);
-                const double cr3 = sq * inv_r * inv_r * inv_r;
-                acc_p += sq * inv_r;
-                acc_x += cr3 * dx;
-                acc_y += cr3 * dy;
-            } else {
-                if (with_potential) acc_p -= sq * log(r2);
-                const double inv = 2.0 * sq / r2;
-                acc_x += inv * dx;
-                acc_y += inv * dy;
-            }
+            if (r >= r1) return;
+            started = true;
+            ks = sf_next.x;
+            ke = sf_next.y;
+            ++r;
+            if (r < r1) sf_next = sinfo[pcol[r]];
         }
-    } else {
-        const float eps2f = (float)eps2;
-        float fp = 0.0f, fx = 0.0f, fy = 0.0f;
-        for (int ks = sf.x; ks < sf.y; ++ks) {
-            const double4 sv = src4[ks];
-            const double sq = sv.z;
-            if (MODE == 0 && ks == k) continue;
-            if (MODE == 1 && (sq == 0.0 || (tskip >= 0 && sperm[ks] == tskip))) continue;
-            const double ddx = tx - sv.x, ddy = ty - sv.y;
-            const float dx = (float)ddx, dy = (float)ddy, qf = (float)sq;
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
-            if (r2 == 0.0f) {
-                // zero or below FP32 range: decide on the FP64 pair
-                const double dr2 = ddx * ddx + ddy * ddy + eps2;
-                if (dr2 == 0.0) {
-                    if (MODE == 1 || (tq != 0.0 && sq != 0.0)) bad = true;
-                    continue;
-                }
-                if (KERNEL == TD3_COULOMB) {
-                    const double inv_r = 1.0 / sqrt(dr2);
-                    acc_p += sq * inv_r;
-                    acc_x += sq * inv_r * inv_r * inv_r * ddx;
-                    acc_y += sq * inv_r * inv_r * inv_r * ddy;
-                } else {
-                    if (with_potential) acc_p -= sq * log(dr2);
-                    acc_x += 2.0 * sq / dr2 * ddx;
-                    acc_y += 2.0 * sq / dr2 * ddy;
-                }
-                continue;
-            }
-            if (KERNEL == TD3_COULOMB) {
-                const float inv_r = __builtin_amdgcn_rsqf(r2);
-                const float c = qf * inv_r;
-                const float cr3 = c * inv_r * inv_r;
-                fp += c;
-                fx = fmaf(cr3, dx, fx);
-                fy = fmaf(cr3, dy, fy);
-            } else {
-                // -q ln r2 = -ln 2 q log2 r2 and 2 q d / r2; the constants at the end
-                if (with_potential) fp = fmaf(qf, __builtin_amdgcn_logf(r2), fp);
-                const float inv = qf * __builtin_amdgcn_rcpf(r2);
-                fx = fmaf(inv, dx, fx);
-                fy = fmaf(inv, dy, fy);
-            }
-        }
-        if (KERNEL == TD3_COULOMB) {
-            acc_p += (double)fp;
-            acc_x += (double)fx;
-            acc_y += (double)fy;
-        } else {
-            acc_p -= 0.6931471805599453 * (double)fp;
-            acc_x += 2.0 * (double)fx;
-            acc_y += 2.0 * (double)fy;
-        }
+        p2p_term<KERNEL, PREC, 0>(tx, ty, tq, k, -1, ks, src4[ks], nullptr, eps2,
+                                  with_potential, acc_p, acc_x, acc_y, fp, fx, fy, bad);
+        ++ks;
     }
 }
 
@@ -904,12 +960,17 @@ __global__ __launch_bounds__(256) void eval_kernel(
     double acc_p = 0.0, acc_x = 0.0, acc_y = 0.0;
     bool bad = false;
     const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
-    int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
-    for (u32 r = r0; r < r1; ++r) {
-        const int4 sf = sf_next;
-        if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
-        p2p_leaf<KERNEL, PREC, MODE>(tx, ty, tq, k, tskip, sf, src4, sperm, eps2, with_potential,
-                                     acc_p, acc_x, acc_y, bad);
+    if (MODE == 0 && PREC == TD3_FP64) {
+        p2p_row<KERNEL, PREC>(tx, ty, tq, k, r0, r1, pcol, sinfo, src4, eps2, with_potential,
+                              acc_p, acc_x, acc_y, bad);
+    } else {
+        int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
+        for (u32 r = r0; r < r1; ++r) {
+            const int4 sf = sf_next;
+            if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
+            p2p_leaf<KERNEL, PREC, MODE>(tx, ty, tq, k, tskip, sf, src4, sperm, eps2,
+                                         with_potential, acc_p, acc_x, acc_y, bad);
+        }
     }
     const i64 i = tperm[k];
     if (bad) td3_report(status, i + 1);
@@ -936,12 +997,17 @@ __global__ __launch_bounds__(256) void eval_near_kernel(
     double acc_p = 0.0, acc_x = 0.0, acc_y = 0.0;
     bool bad = false;
     const u32 r0 = prow[leaf], r1 = prow[leaf + 1];
-    int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
-    for (u32 r = r0; r < r1; ++r) {
-        const int4 sf = sf_next;
-        if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
-        p2p_leaf<KERNEL, PREC, 0>(tx, ty, tq, k, -1, sf, src4, nullptr, eps2, with_potential,
-                                  acc_p, acc_x, acc_y, bad);
+    if (PREC == TD3_FP64) {
+        p2p_row<KERNEL, PREC>(tx, ty, tq, k, r0, r1, pcol, sinfo, src4, eps2, with_potential,
+                              acc_p, acc_x, acc_y, bad);
+    } else {
+        int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
+        for (u32 r = r0; r < r1; ++r) {
+            const int4 sf = sf_next;
+            if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
+            p2p_leaf<KERNEL, PREC, 0>(tx, ty, tq, k, -1, sf, src4, nullptr, eps2,
+                                      with_potential, acc_p, acc_x, acc_y, bad);
+        }
     }
     if (bad) td3_report(status, tperm[k] + 1);
     near[k] = acc_p;
