@@ -864,7 +864,10 @@ __device__ __forceinline__ void p2p_row(double tx, double ty, double tq, i64 k, 
     u32 r = r0;
     int ks = 0, ke = 0;
     bool started = false;
+    // partner r's range and partner r + 1's column are loaded ahead, so taking a partner
+    // never waits on a column before its range can be loaded
     int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
+    int pc_next = r0 + 1 < r1 ? pcol[r0 + 1] : 0;
     for (;;) {
         while (ks >= ke) {
             if (started) {
@@ -876,7 +879,8 @@ __device__ __forceinline__ void p2p_row(double tx, double ty, double tq, i64 k, 
             ks = sf_next.x;
             ke = sf_next.y;
             ++r;
-            if (r < r1) sf_next = sinfo[pcol[r]];
+            if (r < r1) sf_next = sinfo[pc_next];
+            if (r + 1 < r1) pc_next = pcol[r + 1];
         }
         p2p_term<KERNEL, PREC, 0>(tx, ty, tq, k, -1, ks, src4[ks], nullptr, eps2,
                                   with_potential, acc_p, acc_x, acc_y, fp, fx, fy, bad);
@@ -1001,10 +1005,13 @@ __global__ __launch_bounds__(256) void eval_near_kernel(
         p2p_row<KERNEL, PREC>(tx, ty, tq, k, r0, r1, pcol, sinfo, src4, eps2, with_potential,
                               acc_p, acc_x, acc_y, bad);
     } else {
+        // the next partner's range and the one after's column loaded ahead, as in p2p_row
         int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
+        int pc_next = r0 + 1 < r1 ? pcol[r0 + 1] : 0;
         for (u32 r = r0; r < r1; ++r) {
             const int4 sf = sf_next;
-            if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
+            if (r + 1 < r1) sf_next = sinfo[pc_next];
+            if (r + 2 < r1) pc_next = pcol[r + 2];
             p2p_leaf<KERNEL, PREC, 0>(tx, ty, tq, k, -1, sf, src4, nullptr, eps2,
                                       with_potential, acc_p, acc_x, acc_y, bad);
         }
