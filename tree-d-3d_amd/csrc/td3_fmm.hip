@@ -851,8 +851,8 @@ __device__ __forceinline__ void p2p_leaf(double tx, double ty, double tq, i64 k,
 // same per-leaf partial sums as p2p_leaf called partner by partner.  Some lane of a
 // wavefront moves to its next partner at almost every step, so the move is paid on every
 // iteration: the FP64 terms outweigh it (the near field of 1M clustered and uniform particles,
-// leaf 8, theta 0.5: 0.80 against 0.93 ms), the mixed ones do not (0.49 against 0.40 ms), so
-// only FP64 takes this loop.
+// leaf 8, theta 0.5: 0.80 against 0.93 ms), the mixed ones do not (0.44 against 0.39 ms, both
+// loading columns ahead), so only FP64 takes this loop.
 template <int KERNEL, int PREC>
 __device__ __forceinline__ void p2p_row(double tx, double ty, double tq, i64 k, u32 r0, u32 r1,
                                         const int* __restrict__ pcol,
