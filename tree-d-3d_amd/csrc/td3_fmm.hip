@@ -968,10 +968,13 @@ __global__ __launch_bounds__(256) void eval_kernel(
         p2p_row<KERNEL, PREC>(tx, ty, tq, k, r0, r1, pcol, sinfo, src4, eps2, with_potential,
                               acc_p, acc_x, acc_y, bad);
     } else {
+        // the next partner's range and the one after's column loaded ahead, as in p2p_row
         int4 sf_next = r0 < r1 ? sinfo[pcol[r0]] : make_int4(0, 0, 0, 0);
+        int pc_next = r0 + 1 < r1 ? pcol[r0 + 1] : 0;
         for (u32 r = r0; r < r1; ++r) {
             const int4 sf = sf_next;
-            if (r + 1 < r1) sf_next = sinfo[pcol[r + 1]];
+            if (r + 1 < r1) sf_next = sinfo[pc_next];
+            if (r + 2 < r1) pc_next = pcol[r + 2];
             p2p_leaf<KERNEL, PREC, MODE>(tx, ty, tq, k, tskip, sf, src4, sperm, eps2,
                                          with_potential, acc_p, acc_x, acc_y, bad);
         }
